@@ -1,0 +1,112 @@
+"""ctypes mirrors of the C structs declared in include/hpgl.h.
+
+Same field names and order as the reference's geo_bsd/hpgl_wrap.py:6-139,
+kept in a module of their own so that code which only needs the layouts
+(the oracle binding of the tests) does not load the GPU library.
+"""
+import ctypes as C
+
+hpgl_output_handler = C.CFUNCTYPE(C.c_int, C.c_char_p, C.py_object)
+hpgl_progress_handler = C.CFUNCTYPE(C.c_int, C.c_char_p, C.c_int, C.py_object)
+
+
+class _HPGL_MEAN_KIND:
+    stationary_auto = 0
+    stationary = 1
+    varying = 2
+
+
+class _HPGL_KRIGING_KIND:
+    ordinary = 0
+    simple = 1
+
+
+class _HPGL_SHAPE(C.Structure):
+    _fields_ = [("data", C.c_int * 3), ("strides", C.c_int * 3)]
+
+
+class _HPGL_CONT_MASKED_ARRAY(C.Structure):
+    _fields_ = [("data", C.POINTER(C.c_float)), ("mask", C.POINTER(C.c_ubyte)), ("shape", _HPGL_SHAPE)]
+
+
+class _HPGL_IND_MASKED_ARRAY(C.Structure):
+    _fields_ = [
+        ("data", C.POINTER(C.c_ubyte)),
+        ("mask", C.POINTER(C.c_ubyte)),
+        ("shape", _HPGL_SHAPE),
+        ("indicator_count", C.c_int),
+    ]
+
+
+class _HPGL_UBYTE_ARRAY(C.Structure):
+    _fields_ = [("data", C.POINTER(C.c_ubyte)), ("shape", _HPGL_SHAPE)]
+
+
+class _HPGL_FLOAT_ARRAY(C.Structure):
+    _fields_ = [("data", C.POINTER(C.c_float)), ("shape", _HPGL_SHAPE)]
+
+
+_COV_FIELDS = [
+    ("covariance_type", C.c_int),
+    ("ranges", C.c_double * 3),
+    ("angles", C.c_double * 3),
+    ("sill", C.c_double),
+    ("nugget", C.c_double),
+]
+_NB_FIELDS = [("radiuses", C.c_int * 3), ("max_neighbours", C.c_int)]
+
+
+class _HPGL_OK_PARAMS(C.Structure):
+    _fields_ = _COV_FIELDS + _NB_FIELDS
+
+
+class _HPGL_SK_PARAMS(C.Structure):
+    _fields_ = _COV_FIELDS + _NB_FIELDS + [("automatic_mean", C.c_ubyte), ("mean", C.c_double)]
+
+
+class _HPGL_SGS_PARAMS(C.Structure):
+    _fields_ = _COV_FIELDS + _NB_FIELDS + [
+        ("kriging_kind", C.c_int),
+        ("seed", C.c_long),
+        ("min_neighbours", C.c_int),
+    ]
+
+
+class _HPGL_MEDIAN_IK_PARAMS(C.Structure):
+    _fields_ = _COV_FIELDS + _NB_FIELDS + [("marginal_probs", C.c_double * 2)]
+
+
+class _HPGL_IK_PARAMS(C.Structure):
+    _fields_ = _COV_FIELDS + _NB_FIELDS + [("marginal_prob", C.c_double)]
+
+
+class __hpgl_cov_params_t(C.Structure):
+    _fields_ = list(_COV_FIELDS)
+
+
+class hpgl_non_parametric_cdf_t(C.Structure):
+    _fields_ = [("values", C.POINTER(C.c_float)), ("probs", C.POINTER(C.c_float)), ("size", C.c_longlong)]
+
+
+class _HPGL_AMD_STATS(C.Structure):
+    _fields_ = [
+        ("nodes_target", C.c_longlong),
+        ("nodes_kriged", C.c_longlong),
+        ("nodes_no_neighbours", C.c_longlong),
+        ("nodes_singular", C.c_longlong),
+        ("solves_fp32", C.c_longlong),
+        ("solves_fp64", C.c_longlong),
+        ("nodes_template_path", C.c_longlong),
+        ("nodes_cluster_path", C.c_longlong),
+        ("ms_setup_host", C.c_double),
+        ("ms_search", C.c_double),
+        ("ms_solve", C.c_double),
+        ("ms_device_total", C.c_double),
+        ("ms_wall", C.c_double),
+    ]
+
+
+# Module-level alias without name mangling (the reference relies on the
+# double-underscore name only inside this module).
+_HPGL_COV_PARAMS = __hpgl_cov_params_t
+

@@ -1,0 +1,1161 @@
+// hpgl_oracle.cpp - CPU restatement of the reference's kriging hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the correctness oracle and the
+// timed CPU baseline.  Only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg load it (oracle/_build/liboracle.so), always as the
+// checker, never as the thing measured or shipped.  The product library
+// (hpgl_amd/hpgl.so) neither links nor loads it.
+//
+// Every function restates one reference routine; the citation is given as
+// file:line relative to the reference's src/geo_bsd/hpgl/ directory unless
+// a longer path is written.  Arithmetic types follow the reference exactly
+// (float values and means, double covariances and weights, float
+// accumulation in combine) and the file is compiled with -ffp-contract=off,
+// so no multiply-add is fused where the reference rounds twice.
+//
+// Third-party arithmetic restated here:
+//   * CLAPACK 3.1.1.1 dpotrf_/dpotrs_ ('U', n <= 64 so dpotrf runs the
+//     unblocked dpotf2) over f2c'd reference BLAS ddot/dgemv/dscal/dtrsm,
+//     in netlib loop order (call sites my_kriging_weights.h:93,100,200,210).
+//   * boost::mt11213b (Boost 1.38), as std::mersenne_twister_engine with
+//     the same parameters (bs_random_generator.h:11).
+//   * std::sort is called directly: tie order is pinned to this
+//     toolchain's libstdc++ (GCC 11), the same header the product uses.
+//
+// Parity status per function is recorded in the header comment of each
+// section ("pinned by ...").
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+#include <omp.h>
+
+#include "../include/hpgl.h"
+
+namespace orc {
+
+typedef int node_index_t;
+typedef float mean_t;
+typedef double covariance_t;
+
+// ---------------------------------------------------------------------------
+// Covariance model.  cov_model.h:9-182, TNT product tnt_126/tnt_cmat.h:471-497.
+// Pinned by: closed form; GsTL parta known answers; problem 7.2 SK vector.
+// ---------------------------------------------------------------------------
+struct CovParams {
+    int type;
+    double ranges[3];
+    double angles[3];
+    double sill;
+    double nugget;
+};
+
+static const double kPi = 3.14159265358979323846264338327950288419;
+
+static double grad_to_rad(double g) { return (g * kPi) / 180; }  // cov_model.h:11-14
+
+// 3x3 product in TNT's row-by-column order: sum = 0; sum = sum + a*b.
+static void matmul3(const double a[9], const double b[9], double c[9]) {
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) {
+            double sum = 0;
+            for (int j = 0; j < 3; ++j) sum = sum + a[3 * i + j] * b[3 * j + k];
+            c[3 * i + k] = sum;
+        }
+}
+
+// create_transform (cov_model.h:16-61): T = S * ((Rx * Ry) * Rz).
+static void create_transform(const double ranges[3], const double angles[3], double t[9]) {
+    for (int i = 0; i < 3; ++i)
+        if (ranges[i] == 0) throw std::runtime_error("create_transform: All covariance ranges should be non-zero");
+    double s[9] = {0}, rz[9] = {0}, ry[9] = {0}, rx[9] = {0};
+    s[0] = 1.0;
+    s[4] = ranges[0] / ranges[1];
+    s[8] = ranges[0] / ranges[2];
+    rz[0] = cos(grad_to_rad(angles[0]));
+    rz[4] = cos(grad_to_rad(angles[0]));
+    rz[1] = sin(grad_to_rad(angles[0]));
+    rz[3] = -sin(grad_to_rad(angles[0]));
+    rz[8] = 1;
+    ry[0] = cos(grad_to_rad(angles[1]));
+    ry[8] = cos(grad_to_rad(angles[1]));
+    ry[2] = -sin(grad_to_rad(angles[1]));
+    ry[6] = sin(grad_to_rad(angles[1]));
+    ry[4] = 1;
+    rx[4] = cos(grad_to_rad(angles[2]));
+    rx[8] = cos(grad_to_rad(angles[2]));
+    rx[5] = sin(grad_to_rad(angles[2]));
+    rx[7] = -sin(grad_to_rad(angles[2]));
+    rx[0] = 1;
+    double xy[9], xyz[9];
+    matmul3(rx, ry, xy);
+    matmul3(xy, rz, xyz);
+    matmul3(s, xyz, t);
+}
+
+struct CovModel {
+    CovParams p;
+    double t[9];
+    explicit CovModel(const CovParams &params) : p(params) { create_transform(p.ranges, p.angles, t); }
+
+    // transfrom_and_norm (cov_model.h:63-77)
+    double norm(const double v[3]) const {
+        double result = 0.0;
+        for (int i = 0; i < 3; ++i) {
+            double d = 0.0;
+            for (int j = 0; j < 3; ++j) d += v[j] * t[3 * i + j];
+            result += d * d;
+        }
+        return sqrt(result);
+    }
+    // cov_model.h:144-181
+    double of_h(double h) const {
+        const double c0 = p.sill - p.nugget;
+        if (h < 0.0001) return c0;
+        switch (p.type) {
+            case 0:
+                if (h > p.ranges[0]) return 0;
+                return c0 * (1 - 1.5 * h / p.ranges[0] + 0.5 * pow(h / p.ranges[0], 3));
+            case 1:
+                return c0 * exp(-3 * h / p.ranges[0]);
+            case 2:
+                return c0 * exp(-3 * pow(h / p.ranges[0], 2));
+        }
+        return 0;
+    }
+    // cov_model.h:131-142: v = c1 - c2
+    double operator()(const double c1[3], const double c2[3]) const {
+        double v[3] = {c1[0] - c2[0], c1[1] - c2[1], c1[2] - c2[2]};
+        return of_h(norm(v));
+    }
+    double at_int(int x1, int y1, int z1, int x2, int y2, int z2) const {
+        double v[3] = {(double)(x1 - x2), (double)(y1 - y2), (double)(z1 - z2)};
+        return of_h(norm(v));
+    }
+};
+
+// precalculated_covariances_t / covariance_field_t: double table over the
+// search box [-r, r]^3, 0 outside (precalculated_covariance.h:13-75,
+// covariance_field.h:108-169).  Both store C(offset) of the analytic model.
+struct CovTable {
+    int rx, ry, rz, sx, sy, sz;
+    std::vector<double> v;
+    CovTable(const CovModel &m, int r0, int r1, int r2) : rx(r0), ry(r1), rz(r2) {
+        sx = 2 * rx + 1;
+        sy = 2 * ry + 1;
+        sz = 2 * rz + 1;
+        v.resize((size_t)sx * sy * sz);
+        for (int z = 0; z < sz; ++z)
+            for (int y = 0; y < sy; ++y)
+                for (int x = 0; x < sx; ++x)
+                    v[((size_t)z * sy + y) * sx + x] = m.at_int(0, 0, 0, x - rx, y - ry, z - rz);
+    }
+    // operator()(c1, c2): vec = c2 - c1 (precalculated_covariance.h:60-75)
+    double lookup(int dx, int dy, int dz) const {
+        if (dx < -rx || dx > rx || dy < -ry || dy > ry || dz < -rz || dz > rz) return 0;
+        return v[((size_t)(dz + rz) * sy + (dy + ry)) * sx + (dx + rx)];
+    }
+};
+
+// A job's covariance as seen by search and solve: either the analytic model
+// (ordinary kriging, ordinary_kriging.cpp:27-34) or the box table
+// (SK/LVM/IK/SGS/median IK).
+struct JobCov {
+    const CovModel *model;
+    const CovTable *table;  // null => analytic
+    double operator()(int x1, int y1, int z1, int x2, int y2, int z2) const {
+        if (table) return table->lookup(x2 - x1, y2 - y1, z2 - z1);
+        return model->at_int(x1, y1, z1, x2, y2, z2);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Grid.  sugarbox_grid.h:51-57, sugarbox_grid.cpp:58-74.
+// ---------------------------------------------------------------------------
+struct Grid {
+    int X, Y, Z;
+    int size() const { return X * Y * Z; }
+    void loc(int idx, int &x, int &y, int &z) const {
+        x = idx % X;
+        y = idx % (X * Y) / X;
+        z = idx / (X * Y);
+    }
+    int index(int x, int y, int z) const {
+        if ((unsigned)x < (unsigned)X && (unsigned)y < (unsigned)Y && (unsigned)z < (unsigned)Z)
+            return z * X * Y + y * X + x;
+        return -1;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Template build: calc_cov_field (covariance_field.h:63-106), also
+// covariance_field_t::init (covariance_field.cpp:54-103).  Offsets are
+// v = -(x,y,z) generated in z,y,x order, kept when C > C0/100, sorted by C
+// descending with std::sort (unstable; libstdc++ order), then truncated at
+// the first C < C0/100.  Pinned by: problem 7.2 SK vector (19/20 nodes;
+// node 10 is a tie witness) and brute force on tiny grids.
+// ---------------------------------------------------------------------------
+struct Off {
+    int c[3];
+};
+
+static void build_template(const JobCov &cov, int rx, int ry, int rz, std::vector<Off> &out) {
+    std::vector<double> data;
+    out.clear();
+    const double threshold = cov(0, 0, 0, 0, 0, 0) / 100;
+    for (int z = -rz; z <= rz; ++z)
+        for (int y = -ry; y <= ry; ++y)
+            for (int x = -rx; x <= rx; ++x) {
+                double value = cov(0, 0, 0, x, y, z);
+                data.push_back(value);
+                if (value > threshold) out.push_back(Off{{-x, -y, -z}});
+            }
+    const int xd = 2 * rx + 1, yd = 2 * ry + 1;
+    auto key = [&](const Off &o) {
+        return data[(size_t)(o.c[2] + rz) * yd * xd + (size_t)(o.c[1] + ry) * xd + (o.c[0] + rx)];
+    };
+    std::sort(out.begin(), out.end(), [&](const Off &a, const Off &b) { return key(a) > key(b); });
+    for (size_t i = 0; i < out.size(); ++i) {
+        if (cov(0, 0, 0, out[i].c[0], out[i].c[1], out[i].c[2]) < threshold) {
+            out.resize(i);
+            break;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Cell-binned index: clusterizer_t (clusterizer.cpp:10-184) and
+// indexed_neighbour_lookup_t (sugarbox_indexed_neighbour_lookup.h:42-131).
+// Bin size = radii, bin coordinate = loc / r (double division), box of
+// (g/r + 2) bins per axis, per-bin limit = template_size / 250; a bin that
+// receives more than `limit` nodes is "exceeded".  Pinned by: brute force on
+// tiny grids (no reference fixture reaches this path).
+// ---------------------------------------------------------------------------
+struct Clusterizer {
+    double r[3];
+    int mx, my, mz;
+    size_t limit;
+    std::vector<std::vector<int>> nodes;
+    std::vector<char> exceeded;
+    const Grid *g;
+    Clusterizer(const Grid *grid, const double radii[3], size_t lim) : limit(lim), g(grid) {
+        for (int i = 0; i < 3; ++i) r[i] = radii[i];
+        mx = (int)(grid->X / r[0] + 2);
+        my = (int)(grid->Y / r[1] + 2);
+        mz = (int)(grid->Z / r[2] + 2);
+        nodes.resize((size_t)mx * my * mz);
+        exceeded.assign(nodes.size(), 0);
+    }
+    int bin(int cx, int cy, int cz) const {
+        // rect_3d_t::has is inclusive at both ends (geometry.h:21-24,82-85)
+        if (cx < 0 || cx > mx || cy < 0 || cy > my || cz < 0 || cz > mz) return -1;
+        return cz * mx * my + cy * mx + cx;
+    }
+    void coords(int idx, int &cx, int &cy, int &cz) const {
+        int x, y, z;
+        g->loc(idx, x, y, z);
+        cx = (int)(x / r[0]);
+        cy = (int)(y / r[1]);
+        cz = (int)(z / r[2]);
+    }
+    void add_node(int idx) {  // clusterizer.cpp:293-298, cluster_t::add_node :209-220
+        int cx, cy, cz;
+        coords(idx, cx, cy, cz);
+        int b = bin(cx, cy, cz);
+        if (exceeded[b]) return;
+        if (nodes[b].size() >= limit) {
+            exceeded[b] = 1;
+            nodes[b].clear();
+            return;
+        }
+        nodes[b].push_back(idx);
+    }
+    bool limit_exceeded(int idx) const {  // clusterizer.cpp:350-369
+        int cx, cy, cz;
+        coords(idx, cx, cy, cz);
+        for (int k = cz - 1; k <= cz + 1; ++k)
+            for (int j = cy - 1; j <= cy + 1; ++j)
+                for (int i = cx - 1; i <= cx + 1; ++i) {
+                    int b = bin(i, j, k);
+                    if (b >= 0 && exceeded[b]) return true;
+                }
+        return false;
+    }
+    void nearby(int idx, std::vector<int> &out) const {  // clusterizer.cpp:325-348
+        int cx, cy, cz;
+        coords(idx, cx, cy, cz);
+        out.clear();
+        for (int k = cz - 1; k <= cz + 1; ++k)
+            for (int j = cy - 1; j <= cy + 1; ++j)
+                for (int i = cx - 1; i <= cx + 1; ++i) {
+                    int b = bin(i, j, k);
+                    if (b >= 0) out.insert(out.end(), nodes[b].begin(), nodes[b].end());
+                }
+    }
+};
+
+// Search state of one job: template + clusterizer (one per IK category).
+struct Lookup {
+    const Grid *g;
+    JobCov cov;
+    std::vector<Off> tmpl;
+    Clusterizer *cl;
+    int max_n;
+    bool use_index;  // false for SGS (template only, sequential_simulation.h:198)
+    Lookup(const Grid *grid, const JobCov &c, const int radii[3], int max_neighbours, bool indexed)
+        : g(grid), cov(c), cl(nullptr), max_n(max_neighbours), use_index(indexed) {
+        // search_area_t holds the radii as double and calc_cov_field reads
+        // them back into int (covariance_field.h:66-68).
+        build_template(cov, radii[0], radii[1], radii[2], tmpl);
+        if (use_index) {
+            double r[3] = {(double)radii[0], (double)radii[1], (double)radii[2]};
+            cl = new Clusterizer(grid, r, tmpl.size() / 250);  // MAGIC_NUMBER_2
+        }
+    }
+    ~Lookup() { delete cl; }
+    Lookup(const Lookup &) = delete;
+    Lookup &operator=(const Lookup &) = delete;
+
+    // neighbour_lookup_t::find (sugarbox_neighbour_lookup.h:36-61)
+    template <typename Pred>
+    void find_template(int node, const Pred &pred, std::vector<int> &idx) const {
+        idx.clear();
+        int x, y, z;
+        g->loc(node, x, y, z);
+        for (size_t t = 0; t < tmpl.size() && (int)idx.size() < max_n; ++t) {
+            int p = g->index(x + tmpl[t].c[0], y + tmpl[t].c[1], z + tmpl[t].c[2]);
+            if (pred(p)) idx.push_back(p);
+        }
+    }
+    struct Entry {
+        int index;
+        double cov;
+    };
+    // indexed_neighbour_lookup_t::find (sugarbox_indexed_neighbour_lookup.h:66-123)
+    template <typename Pred>
+    int find(int node, const Pred &pred, std::vector<int> &idx) const {
+        if (!use_index || cl->limit_exceeded(node)) {
+            find_template(node, pred, idx);
+            return 0;
+        }
+        std::vector<int> cand;
+        cl->nearby(node, cand);
+        int x, y, z;
+        g->loc(node, x, y, z);
+        const double threshold = cov(x, y, z, x, y, z) / 100;
+        std::vector<Entry> v;
+        for (size_t i = 0; i < cand.size(); ++i) {
+            int cx, cy, cz;
+            g->loc(cand[i], cx, cy, cz);
+            double c = cov(x, y, z, cx, cy, cz);
+            if (c > threshold) v.push_back(Entry{cand[i], c});
+        }
+        std::sort(v.begin(), v.end(), [](const Entry &a, const Entry &b) { return a.cov > b.cov; });
+        size_t n = std::min(v.size(), (size_t)max_n);
+        idx.resize(n);
+        for (size_t i = 0; i < n; ++i) idx[i] = v[i].index;
+        return 1;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Per-node solve: sk_kriging_weights_3 / ok_kriging_weights_3
+// (my_kriging_weights.h:26-255) over restated dpotf2 + dpotrs.
+// Pinned by: scipy LAPACK dpotrf/dpotrs, GsTL/GSLIB parta weights and
+// estimates, the OK Lagrange identity, problem 7.2.
+// ---------------------------------------------------------------------------
+
+// dpotf2 'U' on column-major a (lda = n): LAPACK 3.1.1 dpotf2.f.
+// Returns info (0 = success, j+1 = leading minor j+1 not positive).
+static int dpotf2_upper(int n, double *a) {
+    for (int j = 0; j < n; ++j) {
+        // ajj = a(j,j) - ddot(j, a(0,j), 1, a(0,j), 1)
+        double dot = 0;
+        for (int k = 0; k < j; ++k) dot = dot + a[k + j * n] * a[k + j * n];
+        double ajj = a[j + j * n] - dot;
+        if (ajj <= 0 || std::isnan(ajj)) {
+            a[j + j * n] = ajj;
+            return j + 1;
+        }
+        ajj = sqrt(ajj);
+        a[j + j * n] = ajj;
+        if (j < n - 1) {
+            // dgemv('T', j, n-j-1, -1, a(0,j+1), n, a(0,j), 1, 1, a(j,j+1), n)
+            for (int c = j + 1; c < n; ++c) {
+                double temp = 0;
+                for (int k = 0; k < j; ++k) temp = temp + a[k + c * n] * a[k + j * n];
+                a[j + c * n] = a[j + c * n] + (-1.0) * temp;
+            }
+            // dscal(n-j-1, 1/ajj, a(j,j+1), n)
+            const double r = 1.0 / ajj;
+            for (int c = j + 1; c < n; ++c) a[j + c * n] = r * a[j + c * n];
+        }
+    }
+    return 0;
+}
+
+// dpotrs 'U', one right-hand side: dtrsm('L','U','T','N') then
+// dtrsm('L','U','N','N') in reference BLAS loop order.
+static void dpotrs_upper(int n, const double *a, double *b) {
+    for (int i = 0; i < n; ++i) {
+        double temp = b[i];
+        for (int k = 0; k < i; ++k) temp = temp - a[k + i * n] * b[k];
+        temp = temp / a[i + i * n];
+        b[i] = temp;
+    }
+    for (int k = n - 1; k >= 0; --k) {
+        if (b[k] != 0) {
+            b[k] = b[k] / a[k + k * n];
+            for (int i = 0; i < k; ++i) b[i] = b[i] - b[k] * a[i + k * n];
+        }
+    }
+}
+
+struct Coord {
+    int c[3];
+};
+
+// The reference fills A(i,j) = C(x_i, x_j) row by row (A[i*size+j]) and
+// hands it to Fortran as column major; A is symmetric so both agree.
+// dpotrs is called even when dpotrf failed (the call order is kept so
+// weights are the same garbage), success needs both infos == 0; dpotrs
+// never reports an error for valid arguments.
+static bool sk_weights(const Coord &center, const std::vector<Coord> &c, const JobCov &cov,
+                       std::vector<double> &w, double *variance) {
+    const int n = (int)c.size();
+    w.resize(n);
+    if (n <= 0) return false;
+    std::vector<double> A((size_t)n * n), b(n);
+    for (int i = 0; i < n; ++i) {
+        for (int j = i; j < n; ++j) {
+            A[(size_t)i * n + j] = cov(c[i].c[0], c[i].c[1], c[i].c[2], c[j].c[0], c[j].c[1], c[j].c[2]);
+            A[(size_t)j * n + i] = A[(size_t)i * n + j];
+        }
+        b[i] = cov(c[i].c[0], c[i].c[1], c[i].c[2], center.c[0], center.c[1], center.c[2]);
+    }
+    int info = dpotf2_upper(n, A.data());
+    for (int i = 0; i < n; ++i) w[i] = b[i];
+    dpotrs_upper(n, A.data(), w.data());
+    bool ok = info == 0;
+    if (variance) {
+        if (ok) {
+            double v = cov(center.c[0], center.c[1], center.c[2], center.c[0], center.c[1], center.c[2]);
+            for (int i = 0; i < n; ++i) v -= w[i] * b[i];
+            *variance = v;
+        } else {
+            *variance = -1;
+        }
+    }
+    return ok;
+}
+
+static bool ok_weights(const Coord &center, const std::vector<Coord> &c, const JobCov &cov,
+                       std::vector<double> &w, double *variance) {
+    const int n = (int)c.size();
+    if (n <= 0) return false;
+    w.resize(n);
+    std::vector<double> A((size_t)n * n), b(n);
+    for (int i = 0; i < n; ++i) {
+        for (int j = i; j < n; ++j) {
+            A[(size_t)i * n + j] = cov(c[i].c[0], c[i].c[1], c[i].c[2], c[j].c[0], c[j].c[1], c[j].c[2]);
+            A[(size_t)j * n + i] = A[(size_t)i * n + j];
+        }
+        b[i] = cov(c[i].c[0], c[i].c[1], c[i].c[2], center.c[0], center.c[1], center.c[2]);
+    }
+    int info = dpotf2_upper(n, A.data());
+    std::vector<double> skw(b), ones(n, 1.0);
+    dpotrs_upper(n, A.data(), skw.data());
+    dpotrs_upper(n, A.data(), ones.data());
+    bool ok = info == 0;
+    double sum_sk = 0, sum_ones = 0;
+    for (int k = 0; k < n; ++k) {
+        sum_sk += skw[k];
+        sum_ones += ones[k];
+    }
+    double mu = (sum_sk - 1) / sum_ones;
+    for (int k = 0; k < n; ++k) w[k] = skw[k] - mu * ones[k];
+    if (variance) {
+        if (ok) {
+            double v = cov(center.c[0], center.c[1], center.c[2], center.c[0], center.c[1], center.c[2]);
+            for (int i = 0; i < n; ++i) v -= w[i] * b[i];
+            v -= w[n - 1];  // my_kriging_weights.h:246, kept as in the reference
+            *variance = v;
+        } else {
+            *variance = -1;
+        }
+    }
+    return ok;
+}
+
+// combine (combiner.h:10-24): float start, (float)(double w * float(v - m)).
+template <typename V, typename R>
+static R combine(const std::vector<V> &values, const std::vector<double> &w, const std::vector<mean_t> &means,
+                 mean_t center_mean) {
+    R result = center_mean;
+    for (size_t i = 0; i < values.size(); ++i) result += static_cast<R>(w[i] * (values[i] - means[i]));
+    return result;
+}
+
+// ---------------------------------------------------------------------------
+// Neighbour dump (test hook; the product has the same hook).
+// ---------------------------------------------------------------------------
+static int *g_dump_idx = nullptr;
+static int *g_dump_cnt = nullptr;
+static long long g_dump_cells = 0;
+static int g_dump_maxn = 0;
+static int g_dump_offset = 0;  // cell offset for IK category blocks
+
+static void dump_node(int node, const std::vector<int> &idx) {
+    if (!g_dump_idx) return;
+    long long cell = (long long)g_dump_offset + node;
+    if (cell >= g_dump_cells) return;
+    g_dump_cnt[cell] = (int)idx.size();
+    for (int i = 0; i < (int)idx.size() && i < g_dump_maxn; ++i) g_dump_idx[cell * g_dump_maxn + i] = idx[i];
+}
+
+static int g_progress_tick = 1;
+static unsigned long g_progress_counter = 0;
+static std::string g_last_error;
+
+// ---------------------------------------------------------------------------
+// cont_kriging (cont_kriging.h:39-121): OpenMP static loop over all cells.
+// mean_kind: 0 none (OK, no_mean_t), 1 single double mean, 2 per-cell float.
+// ---------------------------------------------------------------------------
+struct Means {
+    int kind;
+    double single;
+    const float *cells;
+    double operator[](int i) const {
+        if (kind == 0) return 0;
+        if (kind == 1) return single;
+        return cells[i];
+    }
+};
+
+static void cont_kriging(const float *in, const unsigned char *in_mask, const Grid &g, const Lookup &nl,
+                         const Means &means, bool ok_system, bool mean_on_failure, float *out,
+                         unsigned char *out_mask) {
+    const int N = g.size();
+    for (int node = 0; node < N; ++node)
+        if (in_mask[node] != 0) nl.cl->add_node(node);
+    auto informed = [&](int idx) { return idx < 0 || idx >= N ? false : in_mask[idx] != 0; };
+#pragma omp parallel
+    {
+        std::vector<int> idx;
+        std::vector<Coord> coords;
+        std::vector<double> w;
+        std::vector<mean_t> ms;
+        std::vector<float> vals;
+#pragma omp for schedule(static)
+        for (int node = 0; node < N; ++node) {
+            if (in_mask[node] == 0) {
+                nl.find(node, informed, idx);
+                dump_node(node, idx);
+                int result;  // 0 success, 1 no neighbours, 2 singular
+                float value = 0;
+                if (idx.empty()) {
+                    result = 1;
+                } else {
+                    Coord center;
+                    g.loc(node, center.c[0], center.c[1], center.c[2]);
+                    coords.resize(idx.size());
+                    ms.resize(idx.size());
+                    for (size_t i = 0; i < idx.size(); ++i) {
+                        g.loc(idx[i], coords[i].c[0], coords[i].c[1], coords[i].c[2]);
+                        ms[i] = (mean_t)means[idx[i]];
+                    }
+                    bool ok = ok_system ? ok_weights(center, coords, nl.cov, w, nullptr)
+                                        : sk_weights(center, coords, nl.cov, w, nullptr);
+                    if (ok) {
+                        vals.resize(idx.size());
+                        for (size_t i = 0; i < idx.size(); ++i) vals[i] = in[idx[i]];
+                        value = combine<float, float>(vals, w, ms, (mean_t)means[node]);
+                        result = 0;
+                    } else {
+                        result = 2;
+                    }
+                }
+                if (result == 0) {
+                    out[node] = value;
+                    out_mask[node] = 1;
+                } else if (mean_on_failure) {
+                    out[node] = (float)means[node];
+                    out_mask[node] = 1;
+                }
+            } else {
+                out[node] = in[node];
+                out_mask[node] = 1;
+            }
+            if (g_progress_tick) {
+#pragma omp critical
+                { ++g_progress_counter; }
+            }
+        }
+    }
+}
+
+static CovParams cov_from(int type, const double r[3], const double a[3], double sill, double nugget) {
+    CovParams p;
+    p.type = type;
+    for (int i = 0; i < 3; ++i) {
+        p.ranges[i] = r[i];
+        p.angles[i] = a[i];
+    }
+    p.sill = sill;
+    p.nugget = nugget;
+    return p;
+}
+
+// ---------------------------------------------------------------------------
+// Indicator kriging (indicator_kriging.h:58-124, cdf_utils.cpp:9-24) and
+// median IK (median_ik.cpp:24-93).  Pinned by: closed form + brute force.
+// ---------------------------------------------------------------------------
+static unsigned char most_probable_category(const std::vector<float> &p) {
+    double max_prob = p[0];
+    unsigned char max_idx = 0;
+    for (unsigned char i = 1; i < p.size(); i++) {
+        double prob = p[i];
+        if (prob > max_prob) {
+            max_prob = prob;
+            max_idx = i;
+        }
+    }
+    return max_idx;
+}
+
+// ---------------------------------------------------------------------------
+// SGS chain.  Pinned by: problem 7.2 SGS vector (20/20), Boost's mt11213b
+// validation value, the Hull-Dobell full-period permutation property.
+// ---------------------------------------------------------------------------
+typedef std::mersenne_twister_engine<uint32_t, 32, 351, 175, 19, 0xccab8ee7u, 11, 0xffffffffu, 7, 0x31b6ab00u, 15,
+                                     0xffe50000u, 17, 1812433253u>
+    mt11213b;
+
+// primes.txt is absent from the reference tree; it is the ascending list of
+// primes from 2.  A sieve up to `limit` reproduces it.
+static void primes_upto(long limit, std::vector<long> &out) {
+    out.clear();
+    if (limit < 2) return;
+    std::vector<char> comp((size_t)limit + 1, 0);
+    for (long i = 2; i <= limit; ++i) {
+        if (comp[i]) continue;
+        out.push_back(i);
+        for (long j = i * i; j <= limit; j += i) comp[j] = 1;
+    }
+}
+
+struct PathGen {  // path_random_generator.cpp:13-94
+    long A, C, M, R;
+    PathGen(long size, long seed) {
+        std::vector<long> primes;
+        primes_upto(std::max(size + 2, 16L) * 1 + 64, primes);  // include a prime > size
+        long idx = 0;
+        A = 1;
+        M = size;
+        while (idx < (long)primes.size()) {
+            if (primes[idx] > size) break;
+            if (size % primes[idx] == 0) A *= primes[idx];
+            ++idx;
+        }
+        if (M % 4 == 0) A *= 2;
+        A += 1;
+        idx = 0;
+        while (idx + 1 < (long)primes.size() && primes[idx + 1] <= size / 3) ++idx;
+        for (long k = idx; k >= 0; k--) {
+            C = primes[k];
+            if (floor(double(size) / double(C)) != (double(size) / double(C))) break;
+        }
+        R = seed % size;
+    }
+    int next() {
+        int result = (int)R;
+        R = ((long long)A * (long long)R + (long long)C) % (long long)M;
+        return result;
+    }
+};
+
+// gaussian_cdf_t (gaussian_distribution.h:8-64)
+struct Gauss {
+    double mean, var;
+    Gauss() : mean(0), var(1) {}
+    Gauss(double m, double v) : mean(m), var(v) {}
+    double prob(double x) const {
+        double p = 0.2316419;
+        double b[] = {0.319381530, -0.356563782, 1.781477937, -1.821255978, 1.330274429};
+        double sx = (x - mean) / sqrt(var);
+        bool neg = (sx < 0);
+        if (neg) sx = -sx;
+        double t = 1 / (1 + p * sx);
+        double zx = 1 / sqrt(2 * 3.14159265358) * exp(-sx * sx / 2);
+        double result = 1 - zx * ((((b[4] * t + b[3]) * t + b[2]) * t + b[1]) * t + b[0]) * t;
+        if (neg) result = 1 - result;
+        return result;
+    }
+    double inverse(double p) const {
+        double ps[] = {-0.322232431088, -1.0, -0.342242088547, -0.0204231210245, -0.453642210148e-4};
+        double qs[] = {0.0993484626060, 0.588581570495, 0.531103462366, 0.103537752850, 0.38560700634e-2};
+        double p2 = (p > 0.5) ? (1 - p) : p;
+        double y = sqrt(log(1 / (p2 * p2)));
+        double xp = y + ((((ps[4] * y + ps[3]) * y + ps[2]) * y + ps[1]) * y + ps[0]) /
+                            ((((qs[4] * y + qs[3]) * y + qs[2]) * y + qs[1]) * y + qs[0]);
+        if (p < 0.5) xp = -xp;
+        return xp * sqrt(var) + mean;
+    }
+};
+
+// non_parametric_cdf_2_t (non_parametric_cdf.h:181-262), float arithmetic.
+struct NpCdf {
+    const float *values;
+    const float *probs;
+    long long size;
+    float prob(float value) const {
+        float result;
+        const float *it = std::lower_bound(values, values + size, value);
+        if (it == values + size)
+            result = probs[size - 1];
+        else if (it == values)
+            result = probs[0];
+        else {
+            long long i2 = it - values, i1 = i2 - 1;
+            float x1 = values[i1], x2 = values[i2];
+            float y1 = probs[i1], y2 = probs[i2];
+            result = y1 + (y2 - y1) / (x2 - x1) * (value - x1);
+        }
+        return result;
+    }
+    float inverse(float prob) const {
+        float result;
+        if (size <= 0) return 0;
+        const float *it = std::lower_bound(probs, probs + size, prob);
+        if (it == probs)
+            result = values[0];
+        else if (it == probs + size)
+            result = values[size - 1];
+        else {
+            long long i2 = it - probs, i1 = i2 - 1;
+            float x1 = values[i1], x2 = values[i2];
+            float y1 = probs[i1], y2 = probs[i2];
+            result = x1 + (x2 - x1) / (y2 - y1) * (prob - y1);
+        }
+        return result;
+    }
+};
+
+// do_sequential_gausian_simulation (sequential_simulation.h:51-110) for one
+// realisation, in place.  mean_kind as in Means; ok_system selects
+// ok_weight_calculator_t.
+static void sgs_core(float *data, unsigned char *mask, const Grid &g, const hpgl_sgs_params_t *sp,
+                     const std::vector<float> *lvm_means, const unsigned char *sim_mask) {
+    const int N = g.size();
+    CovParams cp = cov_from(sp->m_covariance_type, sp->m_ranges, sp->m_angles, sp->m_sill, sp->m_nugget);
+    CovModel model(cp);
+    mt11213b gen((uint32_t)sp->m_seed);
+    PathGen path((long)N, (long)(int)sp->m_seed);
+    CovTable table(model, sp->m_radiuses[0], sp->m_radiuses[1], sp->m_radiuses[2]);
+    JobCov cov{&model, &table};
+    Lookup nl(&g, cov, sp->m_radiuses, sp->m_max_neighbours, false);
+    const bool ok_system = lvm_means == nullptr && sp->m_kriging_kind == 0;
+    auto informed = [&](int idx) { return idx < 0 || idx >= N ? false : mask[idx] != 0; };
+    std::vector<int> idx;
+    std::vector<Coord> coords;
+    std::vector<double> w;
+    std::vector<mean_t> ms;
+    std::vector<float> vals;
+    int node = path.next();
+    for (int counter = 0; counter < N; ++counter, node = path.next()) {
+        if (mask[node] != 0) continue;
+        if (sim_mask && sim_mask[node] != 1) continue;
+        nl.find_template(node, informed, idx);
+        dump_node(node, idx);
+        bool ok = false;
+        float mean = 0;
+        double variance = 0;
+        if (!idx.empty()) {
+            Coord center;
+            g.loc(node, center.c[0], center.c[1], center.c[2]);
+            coords.resize(idx.size());
+            ms.resize(idx.size());
+            for (size_t i = 0; i < idx.size(); ++i) {
+                g.loc(idx[i], coords[i].c[0], coords[i].c[1], coords[i].c[2]);
+                ms[i] = lvm_means ? (*lvm_means)[idx[i]] : (mean_t)0.0;
+            }
+            ok = ok_system ? ok_weights(center, coords, cov, w, &variance)
+                           : sk_weights(center, coords, cov, w, &variance);
+            if (ok) {
+                vals.resize(idx.size());
+                for (size_t i = 0; i < idx.size(); ++i) vals[i] = data[idx[i]];
+                mean = combine<float, float>(vals, w, ms, lvm_means ? (*lvm_means)[node] : (mean_t)0.0);
+            }
+        }
+        double u = double(gen()) / double(mt11213b::max());
+        double value = ok ? Gauss(mean, variance).inverse(u) : Gauss().inverse(u);
+        data[node] = (float)value;
+        mask[node] = 1;
+    }
+}
+
+static void transform_forward(float *data, const unsigned char *mask, int N, const NpCdf &cdf) {
+    Gauss to;
+    for (int i = 0; i < N; ++i)
+        if (mask[i]) {
+            double P = cdf.prob(data[i]);
+            data[i] = (float)to.inverse(P);
+        }
+}
+static void transform_back(float *data, unsigned char *mask, int N, const NpCdf &cdf) {
+    Gauss from;
+    for (int i = 0; i < N; ++i)
+        if (mask[i]) {
+            double P = from.prob(data[i]);
+            data[i] = cdf.inverse((float)P);
+            mask[i] = 1;
+        }
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+#define ORACLE_API extern "C" __attribute__((visibility("default")))
+#define GUARD_BEGIN try {
+#define GUARD_END                        \
+    }                                    \
+    catch (const std::exception &e) {    \
+        g_last_error = e.what();         \
+        return;                          \
+    }
+
+ORACLE_API const char *oracle_last_error() { return g_last_error.c_str(); }
+ORACLE_API void oracle_set_threads(int n) {
+    if (n > 0) omp_set_num_threads(n);
+}
+ORACLE_API int oracle_get_threads() { return omp_get_max_threads(); }
+ORACLE_API void oracle_set_progress_tick(int on) { g_progress_tick = on; }
+ORACLE_API void oracle_set_neighbour_dump(int *indices, int *counts, long long n_cells, int max_n) {
+    g_dump_idx = indices;
+    g_dump_cnt = counts;
+    g_dump_cells = n_cells;
+    g_dump_maxn = max_n;
+    g_dump_offset = 0;
+}
+
+// hpgl_ordinary_kriging (api.cpp:195-231 -> ordinary_kriging.cpp:14-46)
+ORACLE_API void oracle_ordinary_kriging(hpgl_cont_masked_array_t *in, hpgl_ok_params_t *p,
+                                        hpgl_cont_masked_array_t *out) {
+    g_last_error.clear();
+    GUARD_BEGIN
+    Grid g{in->m_shape.m_data[0], in->m_shape.m_data[1], in->m_shape.m_data[2]};
+    CovModel model(cov_from(p->m_covariance_type, p->m_ranges, p->m_angles, p->m_sill, p->m_nugget));
+    JobCov cov{&model, nullptr};
+    Lookup nl(&g, cov, p->m_radiuses, p->m_max_neighbours, true);
+    Means m{0, 0, nullptr};
+    cont_kriging(in->m_data, in->m_mask, g, nl, m, true, false, out->m_data, out->m_mask);
+    GUARD_END
+}
+
+// hpgl_simple_kriging (api.cpp:261-290 -> simple_kriging.cpp:87-120)
+ORACLE_API void oracle_simple_kriging(float *in, unsigned char *in_mask, hpgl_shape_t *shape, hpgl_sk_params_t *p,
+                                      float *out, unsigned char *out_mask, hpgl_shape_t *) {
+    g_last_error.clear();
+    GUARD_BEGIN
+    Grid g{shape->m_data[0], shape->m_data[1], shape->m_data[2]};
+    double mean;
+    if (p->m_automatic_mean) {  // calc_mean.cpp:6-25
+        int count = 0;
+        double sum = 0;
+        for (int i = 0; i < g.size(); ++i)
+            if (in_mask[i] != 0) {
+                sum += in[i];
+                count += 1;
+            }
+        mean = count == 0 ? 0 : sum / count;
+    } else {
+        mean = p->m_mean;
+    }
+    CovModel model(cov_from(p->m_covariance_type, p->m_ranges, p->m_angles, p->m_sill, p->m_nugget));
+    CovTable table(model, p->m_radiuses[0], p->m_radiuses[1], p->m_radiuses[2]);
+    JobCov cov{&model, &table};
+    Lookup nl(&g, cov, p->m_radiuses, p->m_max_neighbours, true);
+    Means m{1, mean, nullptr};
+    cont_kriging(in, in_mask, g, nl, m, false, true, out, out_mask);
+    GUARD_END
+}
+
+// hpgl_lvm_kriging (api.cpp:344-383 -> lvm_kriging.cpp:16-37)
+ORACLE_API void oracle_lvm_kriging(float *in, unsigned char *in_mask, hpgl_shape_t *shape, float *mean_data,
+                                   hpgl_shape_t *, hpgl_ok_params_t *p, float *out, unsigned char *out_mask,
+                                   hpgl_shape_t *) {
+    g_last_error.clear();
+    GUARD_BEGIN
+    Grid g{shape->m_data[0], shape->m_data[1], shape->m_data[2]};
+    CovModel model(cov_from(p->m_covariance_type, p->m_ranges, p->m_angles, p->m_sill, p->m_nugget));
+    CovTable table(model, p->m_radiuses[0], p->m_radiuses[1], p->m_radiuses[2]);
+    JobCov cov{&model, &table};
+    Lookup nl(&g, cov, p->m_radiuses, p->m_max_neighbours, true);
+    Means m{2, 0, mean_data};
+    cont_kriging(in, in_mask, g, nl, m, false, true, out, out_mask);
+    GUARD_END
+}
+
+// hpgl_indicator_kriging (api.cpp:385-406 -> indicator_kriging.h:58-124)
+ORACLE_API void oracle_indicator_kriging(hpgl_ind_masked_array_t *in, hpgl_ind_masked_array_t *out,
+                                         hpgl_ik_params_t *params, int K) {
+    g_last_error.clear();
+    GUARD_BEGIN
+    Grid g{in->m_shape.m_data[0], in->m_shape.m_data[1], in->m_shape.m_data[2]};
+    const int N = g.size();
+    std::vector<CovModel *> models;
+    std::vector<CovTable *> tables;
+    std::vector<Lookup *> nls;
+    for (int k = 0; k < K; ++k) {
+        const hpgl_ik_params_t &p = params[k];
+        models.push_back(
+            new CovModel(cov_from(p.m_covariance_type, p.m_ranges, p.m_angles, p.m_sill, p.m_nugget)));
+        tables.push_back(new CovTable(*models[k], p.m_radiuses[0], p.m_radiuses[1], p.m_radiuses[2]));
+        nls.push_back(new Lookup(&g, JobCov{models[k], tables[k]}, p.m_radiuses, p.m_max_neighbours, true));
+    }
+    for (int node = 0; node < N; ++node)
+        if (in->m_mask[node] != 0)
+            for (int k = 0; k < K; ++k) nls[k]->cl->add_node(node);
+    const unsigned char *data = in->m_data;
+    const unsigned char *mask = in->m_mask;
+    auto informed = [&](int idx) { return idx < 0 || idx >= N ? false : mask[idx] != 0; };
+    int *const dump_idx = g_dump_idx;
+#pragma omp parallel
+    {
+        std::vector<int> idx;
+        std::vector<Coord> coords;
+        std::vector<double> w;
+        std::vector<mean_t> ms;
+        std::vector<unsigned char> vals;
+        std::vector<float> probs;
+#pragma omp for schedule(static)
+        for (int node = 0; node < N; ++node) {
+            probs.clear();
+            for (int k = 0; k < K; ++k) {
+                const mean_t mk = (mean_t)(double)(float)params[k].m_marginal_prob;
+                nls[k]->find(node, informed, idx);
+                if (dump_idx) {
+                    long long cell = (long long)k * N + node;
+                    if (cell < g_dump_cells) {
+                        g_dump_cnt[cell] = (int)idx.size();
+                        for (int i = 0; i < (int)idx.size() && i < g_dump_maxn; ++i)
+                            g_dump_idx[cell * g_dump_maxn + i] = idx[i];
+                    }
+                }
+                float prob = mk;
+                if (!idx.empty()) {
+                    Coord center;
+                    g.loc(node, center.c[0], center.c[1], center.c[2]);
+                    coords.resize(idx.size());
+                    ms.assign(idx.size(), mk);
+                    for (size_t i = 0; i < idx.size(); ++i)
+                        g.loc(idx[i], coords[i].c[0], coords[i].c[1], coords[i].c[2]);
+                    if (sk_weights(center, coords, nls[k]->cov, w, nullptr)) {
+                        vals.resize(idx.size());
+                        for (size_t i = 0; i < idx.size(); ++i) vals[i] = data[idx[i]] == k ? 1 : 0;
+                        prob = combine<unsigned char, float>(vals, w, ms, mk);
+                    }
+                }
+                probs.push_back(prob);
+            }
+            out->m_data[node] = most_probable_category(probs);
+            out->m_mask[node] = 1;
+            if (g_progress_tick) {
+#pragma omp critical
+                { ++g_progress_counter; }
+            }
+        }
+    }
+    for (int k = 0; k < K; ++k) {
+        delete nls[k];
+        delete tables[k];
+        delete models[k];
+    }
+    GUARD_END
+}
+
+// hpgl_median_ik (api.cpp:469-515 -> median_ik.cpp:24-93).  On a failed
+// solve the reference reads an uninitialised `prob`; here it is the
+// marginal probability m_marginal_probs[1].
+ORACLE_API void oracle_median_ik(hpgl_ind_masked_array_t *in, hpgl_median_ik_params_t *p,
+                                 hpgl_ind_masked_array_t *out) {
+    g_last_error.clear();
+    GUARD_BEGIN
+    Grid g{in->m_shape.m_data[0], in->m_shape.m_data[1], in->m_shape.m_data[2]};
+    const int N = g.size();
+    CovModel model(cov_from(p->m_covariance_type, p->m_ranges, p->m_angles, p->m_sill, p->m_nugget));
+    CovTable table(model, p->m_radiuses[0], p->m_radiuses[1], p->m_radiuses[2]);
+    JobCov cov{&model, &table};
+    Lookup nl(&g, cov, p->m_radiuses, p->m_max_neighbours, true);
+    for (int node = 0; node < N; ++node)
+        if (in->m_mask[node] != 0) nl.cl->add_node(node);
+    const unsigned char *mask = in->m_mask;
+    auto informed = [&](int idx) { return idx < 0 || idx >= N ? false : mask[idx] != 0; };
+    const mean_t m1 = (mean_t)p->m_marginal_probs[1];
+#pragma omp parallel
+    {
+        std::vector<int> idx;
+        std::vector<Coord> coords;
+        std::vector<double> w;
+        std::vector<mean_t> ms;
+        std::vector<unsigned char> vals;
+#pragma omp for schedule(static)
+        for (int node = 0; node < N; ++node) {
+            nl.find(node, informed, idx);
+            dump_node(node, idx);
+            double prob = m1;
+            if (!idx.empty()) {
+                Coord center;
+                g.loc(node, center.c[0], center.c[1], center.c[2]);
+                coords.resize(idx.size());
+                ms.assign(idx.size(), m1);
+                for (size_t i = 0; i < idx.size(); ++i) g.loc(idx[i], coords[i].c[0], coords[i].c[1], coords[i].c[2]);
+                if (sk_weights(center, coords, cov, w, nullptr)) {
+                    vals.resize(idx.size());
+                    for (size_t i = 0; i < idx.size(); ++i) vals[i] = in->m_data[idx[i]] == 1 ? 1 : 0;
+                    prob = combine<unsigned char, double>(vals, w, ms, m1);
+                }
+            }
+            out->m_data[node] = prob <= 0.5 ? 0 : 1;  // choose_indicator, median_ik.cpp:15-22
+            out->m_mask[node] = 1;
+        }
+    }
+    GUARD_END
+}
+
+// hpgl_sgs_simulation (api.cpp:408-438 -> sequential_gaussian_simulation.cpp:18-75).
+// The SK mean is fixed at 0 (sequential_gaussian_simulation.cpp:39); the
+// `mean` argument only changes a logged parameter in the reference.
+ORACLE_API void oracle_sgs_simulation(hpgl_cont_masked_array_t *prop, hpgl_sgs_params_t *p,
+                                      hpgl_non_parametric_cdf_t *cdf, double *, hpgl_ubyte_array_t *sim_mask) {
+    g_last_error.clear();
+    GUARD_BEGIN
+    Grid g{prop->m_shape.m_data[0], prop->m_shape.m_data[1], prop->m_shape.m_data[2]};
+    const int N = g.size();
+    NpCdf c{cdf ? cdf->m_values : nullptr, cdf ? cdf->m_probs : nullptr, cdf ? cdf->m_size : 0};
+    if (cdf) transform_forward(prop->m_data, prop->m_mask, N, c);
+    sgs_core(prop->m_data, prop->m_mask, g, p, nullptr, sim_mask ? sim_mask->m_data : nullptr);
+    if (cdf) transform_back(prop->m_data, prop->m_mask, N, c);
+    GUARD_END
+}
+
+// hpgl_sgs_lvm_simulation (api.cpp:440-467 -> sequential_gaussian_simulation.cpp:77-128)
+ORACLE_API void oracle_sgs_lvm_simulation(hpgl_cont_masked_array_t *prop, hpgl_sgs_params_t *p,
+                                          hpgl_non_parametric_cdf_t *cdf, hpgl_float_array_t *means,
+                                          hpgl_ubyte_array_t *sim_mask) {
+    g_last_error.clear();
+    GUARD_BEGIN
+    Grid g{prop->m_shape.m_data[0], prop->m_shape.m_data[1], prop->m_shape.m_data[2]};
+    const int N = g.size();
+    std::vector<float> mv(means->m_data, means->m_data + N);
+    NpCdf c{cdf ? cdf->m_values : nullptr, cdf ? cdf->m_probs : nullptr, cdf ? cdf->m_size : 0};
+    if (cdf) {
+        transform_forward(prop->m_data, prop->m_mask, N, c);
+        Gauss to;
+        for (int i = 0; i < N; ++i) mv[i] = (float)to.inverse(c.prob(means->m_data[i]));
+    }
+    sgs_core(prop->m_data, prop->m_mask, g, p, &mv, sim_mask ? sim_mask->m_data : nullptr);
+    if (cdf) transform_back(prop->m_data, prop->m_mask, N, c);
+    GUARD_END
+}
+
+// hpgl_simple_kriging_weights (api.cpp:292-342 -> simple_kriging.cpp:65-85):
+// analytic covariance on real coordinates.
+ORACLE_API int oracle_simple_kriging_weights(float *center, float *xs, float *ys, float *zs, int n,
+                                             hpgl_cov_params_t *p, float *weights, double *variance_out) {
+    g_last_error.clear();
+    try {
+        CovModel model(cov_from(p->m_covariance_type, p->m_ranges, p->m_angles, p->m_sill, p->m_nugget));
+        std::vector<double> A((size_t)n * n), b(n), w(n);
+        auto C = [&](double x1, double y1, double z1, double x2, double y2, double z2) {
+            double a[3] = {x1, y1, z1}, c[3] = {x2, y2, z2};
+            return model(a, c);
+        };
+        if (n <= 0) return 1;
+        for (int i = 0; i < n; ++i) {
+            for (int j = i; j < n; ++j) {
+                A[(size_t)i * n + j] = C(xs[i], ys[i], zs[i], xs[j], ys[j], zs[j]);
+                A[(size_t)j * n + i] = A[(size_t)i * n + j];
+            }
+            b[i] = C(xs[i], ys[i], zs[i], center[0], center[1], center[2]);
+        }
+        int info = dpotf2_upper(n, A.data());
+        for (int i = 0; i < n; ++i) w[i] = b[i];
+        dpotrs_upper(n, A.data(), w.data());
+        if (info != 0) return 1;
+        double v = C(center[0], center[1], center[2], center[0], center[1], center[2]);
+        for (int i = 0; i < n; ++i) v -= w[i] * b[i];
+        if (variance_out) *variance_out = v;
+        for (int i = 0; i < n; ++i) weights[i] = (float)w[i];
+        return 0;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return -1;
+    }
+}
+
+// ---- introspection hooks used by the tests ----
+
+// Ordinary-kriging weights on integer coordinates with the analytic model
+// (my_kriging_weights.h:132-255).  Returns 1 on success.
+ORACLE_API int oracle_ok_weights_int(const int *center, const int *coords, int n, hpgl_cov_params_t *p, double *w,
+                                     double *variance) {
+    CovModel model(cov_from(p->m_covariance_type, p->m_ranges, p->m_angles, p->m_sill, p->m_nugget));
+    JobCov cov{&model, nullptr};
+    Coord c0{{center[0], center[1], center[2]}};
+    std::vector<Coord> cs(n);
+    for (int i = 0; i < n; ++i) cs[i] = Coord{{coords[3 * i], coords[3 * i + 1], coords[3 * i + 2]}};
+    std::vector<double> ww;
+    bool ok = ok_weights(c0, cs, cov, ww, variance);
+    for (int i = 0; i < n; ++i) w[i] = ww[i];
+    return ok ? 1 : 0;
+}
+
+// dpotf2 + dpotrs on a dense column-major SPD matrix (for the scipy check).
+ORACLE_API int oracle_cholesky_solve(int n, const double *a, const double *b, double *x, double *factor) {
+    std::vector<double> A(a, a + (size_t)n * n);
+    int info = dpotf2_upper(n, A.data());
+    for (int i = 0; i < n; ++i) x[i] = b[i];
+    dpotrs_upper(n, A.data(), x);
+    if (factor) std::copy(A.begin(), A.end(), factor);
+    return info;
+}
+
+// Covariance C(offset) of the analytic model.
+ORACLE_API double oracle_covariance(hpgl_cov_params_t *p, double dx, double dy, double dz) {
+    CovModel model(cov_from(p->m_covariance_type, p->m_ranges, p->m_angles, p->m_sill, p->m_nugget));
+    double a[3] = {0, 0, 0}, c[3] = {dx, dy, dz};
+    return model(a, c);
+}
+
+// Search template (offsets in search order).  table != 0 uses the box
+// table (SK/LVM/IK/SGS); 0 uses the analytic model (OK).  Returns the size.
+ORACLE_API int oracle_template(hpgl_cov_params_t *p, const int *radii, int table, int *out_xyz, int cap) {
+    CovModel model(cov_from(p->m_covariance_type, p->m_ranges, p->m_angles, p->m_sill, p->m_nugget));
+    CovTable *t = table ? new CovTable(model, radii[0], radii[1], radii[2]) : nullptr;
+    JobCov cov{&model, t};
+    std::vector<Off> out;
+    build_template(cov, radii[0], radii[1], radii[2], out);
+    for (int i = 0; i < (int)out.size() && i < cap; ++i)
+        for (int c = 0; c < 3; ++c) out_xyz[3 * i + c] = out[i].c[c];
+    delete t;
+    return (int)out.size();
+}
+
+// The SGS random path (first `count` cells) and mt11213b output.
+ORACLE_API void oracle_sgs_path(long size, long seed, int *out, long count) {
+    PathGen pg(size, (long)(int)seed);
+    for (long i = 0; i < count; ++i) out[i] = pg.next();
+}
+ORACLE_API void oracle_mt11213b(unsigned int seed, unsigned int *out, long count) {
+    mt11213b gen(seed);
+    for (long i = 0; i < count; ++i) out[i] = gen();
+}
+ORACLE_API double oracle_gauss_inverse(double p) { return Gauss().inverse(p); }
+ORACLE_API double oracle_gauss_prob(double x) { return Gauss().prob(x); }

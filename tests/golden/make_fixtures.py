@@ -1,0 +1,105 @@
+"""Writes the golden fixtures of tests/golden/ from the reference's own files.
+
+Run in the container that mounts the reference (it reads /root/reference);
+the JSON it writes is committed and the tests read only the JSON.  Nothing
+of the reference is imported or executed: the script parses printed result
+files and the literal data in them.
+
+Sources:
+  * sample-scripts/solved_problems/7.2. Conditioning by Kriging/
+      transect.txt (input) and Result/problem_7.2.txt (printed outputs of
+      problem_7_2.py: SK, SGS, second SK);
+  * GsTL-1.3/test/kriging/test_kriging.cpp (8 data points, lines 150-158),
+      test_kriging.result (SK and OK weights, combined estimates),
+      gslib/kriging/OK_kt3d.out and SK_kt3d.out (GSLIB estimates).
+"""
+import json
+import os
+import re
+
+REF = "/root/reference/src"
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _floats(text):
+    return [float(t) for t in re.findall(r"[-+]?\d*\.\d+(?:[eE][-+]?\d+)?|[-+]?\d+(?:[eE][-+]?\d+)?", text)]
+
+
+def problem_7_2():
+    d = os.path.join(REF, "sample-scripts", "solved_problems", "7.2. Conditioning by Kriging")
+    lines = open(os.path.join(d, "transect.txt")).read().split("\n")
+    n_props = int(lines[1])
+    pts = [list(map(float, ln.split())) for ln in lines[2 + n_props:] if ln.strip()]
+    # problem_7_2.py:25-60: 20 cells, x0 = min(x) - 5, cell size 1 + (max-min)/20,
+    # cell i = int((x - x0) / size).
+    xs = [p[0] for p in pts]
+    x0 = min(xs) - 5
+    size = 1 + (max(xs) - min(xs)) / 20
+    cells = [int((p[0] - x0) / size) for p in pts]
+    vals = [p[1] for p in pts]
+    res = open(os.path.join(d, "Result", "problem_7.2.txt")).read()
+    blocks = re.split(r"(SK result:|SGS result:|Simple Kriging result:|Error Field:|Cond\.Sim:)", res)
+    out = {}
+    for k in range(1, len(blocks), 2):
+        out[blocks[k].rstrip(":")] = _floats(blocks[k + 1])
+    return {
+        "source": "sample-scripts/solved_problems/7.2. Conditioning by Kriging (transect.txt, Result/problem_7.2.txt)",
+        "grid": [20, 1, 1],
+        "cells": cells,
+        "values": vals,
+        "sk": {"cov_type": 1, "ranges": [5, 1, 1], "sill": 1.0, "nugget": 0.0, "radiuses": [5, 1, 1],
+               "max_neighbours": 2, "mean": None},
+        "sgs": {"cov_type": 1, "ranges": [5, 1, 1], "sill": 1.0, "nugget": 0.0, "radiuses": [2, 1, 1],
+                "max_neighbours": 2, "seed": 3244759, "kriging_type": "sk",
+                "cdf_from": "informed values of the SK input (calc_cdf)"},
+        "expected_sk": out["SK result"],
+        "expected_sgs": out["SGS result"],
+        "expected_sk2": out["Simple Kriging result"],
+    }
+
+
+def gstl_parta():
+    d = os.path.join(REF, "GsTL-1.3", "test", "kriging")
+    src = open(os.path.join(d, "test_kriging.cpp")).read()
+    pts = {}
+    for name, x, y, v in re.findall(r"Node (P\d)\((\d+),(\d+),([-\d.]+)\)", src):
+        pts[name] = (int(x), int(y), float(v))
+    order = ["P1", "P2", "P3", "P4", "P5", "P6", "P7", "P8"]  # voisin.add_node order
+    res = open(os.path.join(d, "test_kriging.result")).read()
+    sk_block = res.split("Simple kriging")[1].split("Ordinary kriging")[0]
+    ok_block = res.split("Ordinary kriging")[1].split("Kriging with Trend")[0]
+    sk_w = _floats(sk_block.split("Here are the weights:")[1])[1:]
+    ok_w = _floats(ok_block.split("Here are the weights:")[1])[1:]
+    comb = res.split("Combining the weights")[1]
+    sk_mean, sk_var = _floats(comb.split("SK mean:")[1])[:2]
+    ok_mean, ok_var = _floats(comb.split("OK mean:")[1])[:2]
+    gs_ok = _floats(open(os.path.join(d, "gslib", "kriging", "OK_kt3d.out")).read().split("EstimationVariance")[1])
+    gs_sk = _floats(open(os.path.join(d, "gslib", "kriging", "SK_kt3d.out")).read().split("EstimationVariance")[1])
+    return {
+        "source": "GsTL-1.3/test/kriging (test_kriging.cpp, test_kriging.result, gslib/kriging/*_kt3d.out)",
+        "covariance": "exponential, range 10, sill 1, isotropic, 2D",
+        "center": [41, 29],
+        "points": [[pts[n][0], pts[n][1]] for n in order],
+        "values": [pts[n][2] for n in order],
+        "sk_weights": sk_w[:8],
+        "ok_weights": ok_w[:8],
+        "ok_lagrange": ok_w[8],
+        "sk_mean_for_combination": 9.0,
+        "sk_estimate_mean9": sk_mean,
+        "sk_variance": sk_var,
+        "ok_estimate": ok_mean,
+        "ok_variance": ok_var,
+        "gslib_ok": gs_ok[:2],
+        "gslib_sk_mean0": gs_sk[:2],
+    }
+
+
+def main():
+    for name, fn in [("problem_7_2.json", problem_7_2), ("gstl_parta.json", gstl_parta)]:
+        with open(os.path.join(HERE, name), "w") as f:
+            json.dump(fn(), f, indent=1)
+        print("wrote", name)
+
+
+if __name__ == "__main__":
+    main()
