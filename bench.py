@@ -1,0 +1,244 @@
+"""Benchmark of the MI355X kriging path: kriged nodes/s on one GPU.
+
+Workload (default): simple + indicator kriging on a 256x256x128 grid
+(8,388,608 cells), 1% of cells informed (uniform, seed 1002), values
+standard normal.
+  * SK: mean 0, exponential covariance, ranges 40/40/8, sill 1, nugget 0.05,
+    search radii 40/40/8 (radii are not given by the workload; chosen equal
+    to the ranges), max_neighbours 24.
+  * IK: 5 categories from thresholds at the data quantiles 0.1/0.3/0.5/0.7/0.9
+    (category = min(searchsorted(t, v), 4)), marginal probabilities = the
+    category frequencies; category k has a spherical covariance with ranges
+    (30+5k, 30+5k, (30+5k)/5), sill 1, radii = ranges, max_neighbours 24.
+One step = one SK job + one IK job on data already resident in HBM, through
+the device-pointer C ABI (the output arrays are re-initialised from the
+input inside the step, as the Python API clones the input).  Kriged nodes
+per step = uninformed cells x (1 + 5).
+
+Prints one JSON line.  --gpus N > 1 exits with a message: this build
+measures one GPU.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def make_cfg2(seed=1002):
+    from cases import Cov, Grid, categories, scattered
+
+    g = Grid(256, 256, 128)
+    data, mask = scattered(g.shape, 0.01, seed, law="normal")
+    cat, marg = categories(data, mask)
+    sk = dict(cov=Cov(1, (40, 40, 8), sill=1.0, nugget=0.05), radii=(40, 40, 8), max_n=24, mean=0.0)
+    ik = [dict(cov=Cov(0, (30 + 5 * k, 30 + 5 * k, (30 + 5 * k) // 5)),
+               radii=(30 + 5 * k, 30 + 5 * k, (30 + 5 * k) // 5), max_n=24, marginal=marg[k]) for k in range(5)]
+    return g, data, mask, cat, marg, sk, ik
+
+
+def _params(abi, sk, ik):
+    def cov_kw(c):
+        return dict(covariance_type=c.type, ranges=(C.c_double * 3)(*c.ranges), angles=(C.c_double * 3)(*c.angles),
+                    sill=c.sill, nugget=c.nugget)
+
+    skp = abi._HPGL_SK_PARAMS(radiuses=(C.c_int * 3)(*sk["radii"]), max_neighbours=sk["max_n"], automatic_mean=0,
+                              mean=sk["mean"], **cov_kw(sk["cov"]))
+    ikp = (abi._HPGL_IK_PARAMS * len(ik))()
+    for i, p in enumerate(ik):
+        ikp[i] = abi._HPGL_IK_PARAMS(radiuses=(C.c_int * 3)(*p["radii"]), max_neighbours=p["max_n"],
+                                     marginal_prob=p["marginal"], **cov_kw(p["cov"]))
+    return skp, ikp
+
+
+def cpu_baseline(g, data, mask, cat, marg, sk, ik, budget_s=15.0, threads=16):
+    """The CPU oracle (a C++/OpenMP restatement of the reference's CPU path)
+    on a bounded sample of the same workload: every s-th cell of the grid is
+    estimated, s chosen from a short pilot so the sample takes ~budget_s."""
+    import oracle_binding as O
+
+    O.set_threads(threads)
+    n_unk = int(np.sum(mask == 0))
+    ikd = [{"cov_model": p["cov"], "radiuses": p["radii"], "max_neighbours": p["max_n"]} for p in ik]
+
+    def run(stride):
+        O.set_sample_stride(stride)
+        t0 = time.perf_counter()
+        O.simple_kriging((data, mask), g, sk["radii"], sk["max_n"], sk["cov"], mean=sk["mean"])
+        t1 = time.perf_counter()
+        O.indicator_kriging((cat, mask, 5), g, ikd, marg)
+        t2 = time.perf_counter()
+        O.set_sample_stride(1)
+        return t1 - t0, t2 - t1
+
+    pilot = 4096
+    a, b = run(pilot)
+    per_sample = (a + b) / (g.size / pilot)
+    stride = max(1, int(g.size * per_sample / budget_s))
+    a, b = run(stride)
+    sampled_unk = n_unk / stride  # uniform sampling of cells
+    nodes = sampled_unk * (1 + 5)
+    return {"value": nodes / (a + b), "unit": "kriged nodes/s", "cores": threads, "kind": "port",
+            "sample": "every %d-th cell of the same SK+IK workload (%.0f kriged nodes, SK %.1f s + IK %.1f s)"
+                      % (stride, nodes, a, b)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64-compare", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the kriging path has no multi-GPU mode in this build "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from hpgl_amd.geo_bsd import hpgl_wrap
+    from hpgl_amd.geo_bsd import _abi as abi
+
+    lib = hpgl_wrap._hpgl_so
+    vp = C.c_void_p
+    lib.hpgl_amd_simple_kriging_dev.argtypes = [vp, vp, C.POINTER(abi._HPGL_SHAPE), C.POINTER(abi._HPGL_SK_PARAMS),
+                                                vp, vp, vp]
+    lib.hpgl_amd_indicator_kriging_dev.argtypes = [vp, vp, C.POINTER(abi._HPGL_SHAPE),
+                                                   C.POINTER(abi._HPGL_IK_PARAMS), C.c_int, vp, vp, vp]
+
+    g, data, mask, cat, marg, sk, ik = make_cfg2()
+    n_unk = int(np.sum(mask == 0))
+    nodes_per_step = n_unk * (1 + len(ik))
+
+    dev = torch.device("cuda:0")
+    d_in = torch.from_numpy(data.ravel(order="F").copy()).to(dev)
+    d_mask = torch.from_numpy(mask.ravel(order="F").copy()).to(dev)
+    d_cat = torch.from_numpy(cat.ravel(order="F").copy()).to(dev)
+    d_out = torch.empty_like(d_in)
+    d_out_mask = torch.empty_like(d_mask)
+    d_iout = torch.empty_like(d_cat)
+    d_iout_mask = torch.empty_like(d_mask)
+    shape = abi._HPGL_SHAPE(data=(C.c_int * 3)(*g.shape), strides=(C.c_int * 3)(1, g.x, g.x * g.y))
+    skp, ikp = _params(abi, sk, ik)
+    stream = torch.cuda.current_stream(dev)
+    sp = vp(stream.cuda_stream)
+
+    kernel_keys = ["ms_k_template", "ms_k_cluster", "ms_k_solve_mixed", "ms_k_solve_fp64"]
+    count_keys = ["n_k_template", "n_k_cluster", "n_k_solve_mixed", "n_k_solve_fp64"]
+    acc = {}
+
+    def step(record):
+        d_out.copy_(d_in)
+        d_out_mask.copy_(d_mask)
+        lib.hpgl_amd_simple_kriging_dev(d_in.data_ptr(), d_mask.data_ptr(), C.byref(shape), C.byref(skp),
+                                        d_out.data_ptr(), d_out_mask.data_ptr(), sp)
+        hpgl_wrap.check_last_error()
+        s1 = hpgl_wrap.last_stats()
+        d_iout.copy_(d_cat)
+        d_iout_mask.copy_(d_mask)
+        lib.hpgl_amd_indicator_kriging_dev(d_cat.data_ptr(), d_mask.data_ptr(), C.byref(shape), ikp, len(ik),
+                                           d_iout.data_ptr(), d_iout_mask.data_ptr(), sp)
+        hpgl_wrap.check_last_error()
+        s2 = hpgl_wrap.last_stats()
+        if record:
+            for s in (s1, s2):
+                for k, v in s.items():
+                    acc[k] = acc.get(k, 0) + v
+
+    hpgl_wrap.set_precision(args.precision)
+    for _ in range(args.warmup):
+        step(False)
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        step(True)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1000.0 / args.steps
+    value = nodes_per_step * args.steps / wall
+
+    # dominant kernel and its roofline (HBM-bound search / gather work)
+    dom = max(range(4), key=lambda i: acc.get(kernel_keys[i], 0.0))
+    dom_name = ["k_template_scan", "k_cluster", "k_solve_mixed", "k_solve_fp64"][dom]
+    launches = max(1, acc.get(count_keys[dom], 0))
+    avg_ms = acc.get(kernel_keys[dom], 0.0) / launches
+    if dom == 1:
+        # per target: target cell (4 B) + per candidate its cell index and its
+        # rank-table entry (4 + 4 B) + the neighbour list and count written (4 B each)
+        alg_bytes = (4 * acc["nodes_cluster_path"] + 8 * acc["cluster_candidates"] +
+                     4 * (acc["neighbours_total"] + acc["nodes_cluster_path"])) / launches
+    else:
+        # per target: value+mask read and written (5 + 5 B) + 8 B per neighbour (index + value)
+        alg_bytes = (10 * acc["nodes_target"] + 8 * acc["neighbours_total"]) / launches
+    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None, "kernel": dom_name,
+                "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": int(alg_bytes)}
+
+    out = {
+        "metric": "kriged nodes/s, mixed-prec vs plain FP64 at stated err; p50/p99 job latency",
+        "value": value,
+        "unit": "kriged nodes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if args.precision == "mixed" else "f64",
+        "data": "synthetic",
+        "config": {"workload": "SK + 5-category IK, 256x256x128 grid, 1% informed, max_neighbours 24 "
+                               "(radii = ranges)", "cells": g.size, "kriged_nodes_per_step": nodes_per_step,
+                   "precision": args.precision},
+        "roofline": roofline,
+        "kernel_ms_per_step": {k: round(acc.get(k, 0.0) / args.steps, 3) for k in kernel_keys},
+        "solves": {"fp32_accepted": int(acc.get("solves_fp32", 0)), "fp64": int(acc.get("solves_fp64", 0)),
+                   "singular": int(acc.get("nodes_singular", 0)),
+                   "no_neighbours": int(acc.get("nodes_no_neighbours", 0))},
+        "paths": {"template": int(acc.get("nodes_template_path", 0)), "cluster": int(acc.get("nodes_cluster_path", 0))},
+        "host_setup_ms_per_step": round(acc.get("ms_setup_host", 0.0) / args.steps, 2),
+        "device_ms_per_step": round(float(ev0.elapsed_time(ev1)) / args.steps, 3),
+    }
+
+    if not args.no_fp64_compare and args.precision == "mixed":
+        mixed_sk = d_out.cpu().numpy()
+        mixed_ik = d_iout.cpu().numpy()
+        hpgl_wrap.set_precision("fp64")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        step(False)
+        torch.cuda.synchronize()
+        t64 = time.perf_counter() - t0
+        hpgl_wrap.set_precision("mixed")
+        ref = d_out.cpu().numpy().astype(np.float64)
+        m = mask.ravel(order="F") == 0
+        rel = np.abs(mixed_sk[m].astype(np.float64) - ref[m]) / np.maximum(np.abs(ref[m]), 1e-3)
+        out["fp64_compare"] = {"fp64_value": nodes_per_step / t64, "fp64_ms_per_step": t64 * 1000,
+                               "sk_max_rel_err_vs_fp64": float(rel.max()),
+                               "sk_rms_rel_err_vs_fp64": float(np.sqrt(np.mean(rel ** 2))),
+                               "ik_category_agreement": float(np.mean(mixed_ik == d_iout.cpu().numpy()))}
+
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(g, data, mask, cat, marg, sk, ik)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

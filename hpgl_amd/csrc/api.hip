@@ -1,0 +1,528 @@
+// api.hip - the extern "C" boundary (include/hpgl.h).
+//
+// Host-pointer entry points copy the caller's arrays to HBM, run the job on
+// the calling thread's HIP stream and copy the outputs back; the *_dev
+// variants take device pointers and a stream.  Every exception is caught
+// here and reported through hpgl_get_last_exception_message().
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/hpgl.h"
+#include "engine.h"
+#include "kernels.h"
+
+using namespace hpgl_amd;
+
+namespace {
+
+thread_local std::string t_error;
+thread_local int t_failed = 0;
+thread_local Stats t_stats;
+thread_local Dump t_dump;
+thread_local hipStream_t t_stream = nullptr;
+int g_precision = HPGL_AMD_PRECISION_MIXED;
+int g_threads = 0;
+int (*g_output_handler)(char *, void *) = nullptr;
+void *g_output_param = nullptr;
+int (*g_progress_handler)(char *, int, void *) = nullptr;
+void *g_progress_param = nullptr;
+
+hipStream_t thread_stream() {
+    if (!t_stream) hip_check(hipStreamCreateWithFlags(&t_stream, hipStreamNonBlocking), "hipStreamCreate");
+    return t_stream;
+}
+
+void begin_call() {
+    t_error.clear();
+    t_failed = 0;
+    t_stats = Stats();
+}
+
+void fail(const char *msg) {
+    t_error = msg;
+    t_failed = 1;
+}
+
+Dump *take_dump(Dump &local) {
+    if (!t_dump.indices) return nullptr;
+    local = t_dump;
+    t_dump = Dump();
+    return &local;
+}
+
+void progress(const char *stage, int pct) {
+    if (g_progress_handler) g_progress_handler(const_cast<char *>(stage), pct, g_progress_param);
+}
+
+CovSpec cov_of(int type, const double *ranges, const double *angles, double sill, double nugget) {
+    CovSpec c;
+    c.type = type;
+    for (int i = 0; i < 3; ++i) {
+        c.ranges[i] = ranges[i];
+        c.angles[i] = angles[i];
+    }
+    c.sill = sill;
+    c.nugget = nugget;
+    return c;
+}
+
+long long volume(const hpgl_shape_t *s) { return (long long)s->m_data[0] * s->m_data[1] * s->m_data[2]; }
+
+template <typename T>
+struct DevArray {
+    T *p = nullptr;
+    size_t n = 0;
+    DevArray(size_t count) : n(count) { p = (T *)pool_alloc(std::max<size_t>(count, 1) * sizeof(T)); }
+    ~DevArray() { pool_free(p); }
+    void put(const T *h, hipStream_t s) {
+        if (n) hip_check(hipMemcpyAsync(p, h, n * sizeof(T), hipMemcpyHostToDevice, s), "H2D");
+    }
+    void get(T *h, hipStream_t s) {
+        if (n) hip_check(hipMemcpyAsync(h, p, n * sizeof(T), hipMemcpyDeviceToHost, s), "D2H");
+    }
+};
+
+// One SK system on real coordinates (simple_kriging.cpp:65-85), analytic
+// covariance, dpotf2/dpotrs order; run by one lane: the call is a utility
+// on a handful of points, not a grid job.
+__global__ void k_sk_weights_real(const float *center, const float *xs, const float *ys, const float *zs, int n,
+                                  DevCov c, double *A, double *w, int *info_out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    auto C = [&](double x1, double y1, double z1, double x2, double y2, double z2) {
+        const double v0 = x1 - x2, v1 = y1 - y2, v2 = z1 - z2;
+        double r = 0.0;
+        for (int i = 0; i < 3; ++i) {
+            double d = 0.0;
+            d += v0 * c.t[3 * i + 0];
+            d += v1 * c.t[3 * i + 1];
+            d += v2 * c.t[3 * i + 2];
+            r += d * d;
+        }
+        const double h = sqrt(r);
+        if (h < 0.0001) return c.c0;
+        const double a = c.range0;
+        if (c.type == 0) {
+            if (h > a) return 0.0;
+            return c.c0 * (1 - 1.5 * h / a + 0.5 * pow(h / a, 3.0));
+        }
+        if (c.type == 1) return c.c0 * exp(-3 * h / a);
+        return c.c0 * exp(-3 * pow(h / a, 2.0));
+    };
+    for (int i = 0; i < n; ++i) {
+        for (int j = i; j < n; ++j) {
+            A[(size_t)i * n + j] = C(xs[i], ys[i], zs[i], xs[j], ys[j], zs[j]);
+            A[(size_t)j * n + i] = A[(size_t)i * n + j];
+        }
+        w[i] = C(xs[i], ys[i], zs[i], center[0], center[1], center[2]);
+    }
+    int info = 0;
+    for (int j = 0; j < n && info == 0; ++j) {  // dpotf2 'U', column-major
+        double dot = 0;
+        for (int k = 0; k < j; ++k) dot = dot + A[k + j * n] * A[k + j * n];
+        double ajj = A[j + j * n] - dot;
+        if (ajj <= 0 || ajj != ajj) {
+            info = j + 1;
+            break;
+        }
+        ajj = sqrt(ajj);
+        A[j + j * n] = ajj;
+        const double r = 1.0 / ajj;
+        for (int col = j + 1; col < n; ++col) {
+            double temp = 0;
+            for (int k = 0; k < j; ++k) temp = temp + A[k + col * n] * A[k + j * n];
+            A[j + col * n] = r * (A[j + col * n] + (-1.0) * temp);
+        }
+    }
+    if (info == 0) {  // dpotrs
+        for (int i = 0; i < n; ++i) {
+            double temp = w[i];
+            for (int k = 0; k < i; ++k) temp = temp - A[k + i * n] * w[k];
+            w[i] = temp / A[i + i * n];
+        }
+        for (int k = n - 1; k >= 0; --k)
+            if (w[k] != 0) {
+                w[k] = w[k] / A[k + k * n];
+                for (int i = 0; i < k; ++i) w[i] = w[i] - w[k] * A[i + k * n];
+            }
+    }
+    *info_out = info;
+}
+
+}  // namespace
+
+#define API extern "C" __attribute__((visibility("default")))
+#define GUARD_BEGIN \
+    begin_call();   \
+    try {
+#define GUARD_END                          \
+    }                                      \
+    catch (const std::exception &e) {      \
+        fail(e.what());                    \
+    }                                      \
+    catch (...) {                          \
+        fail("unknown error in hpgl.so");  \
+    }
+
+API char *hpgl_get_last_exception_message(void) { return const_cast<char *>(t_error.c_str()); }
+
+API void hpgl_set_output_handler(int (*handler)(char *, void *), void *param) {
+    g_output_handler = handler;
+    g_output_param = param;
+}
+
+API void hpgl_set_progress_handler(int (*handler)(char *, int, void *), void *param) {
+    g_progress_handler = handler;
+    g_progress_param = param;
+}
+
+API void hpgl_set_thread_num(int n) { g_threads = n; }
+API int hpgl_get_thread_num(void) { return g_threads; }
+
+#define NOT_ON_PATH(name)                                                                  \
+    {                                                                                      \
+        begin_call();                                                                      \
+        fail(name ": property file I/O is not part of this library");                     \
+    }
+API void hpgl_read_inc_file_float(char *, float, int, float *, unsigned char *) NOT_ON_PATH("hpgl_read_inc_file_float")
+API void hpgl_read_inc_file_byte(char *, int, int, unsigned char *, unsigned char *, unsigned char *, int)
+    NOT_ON_PATH("hpgl_read_inc_file_byte")
+API void hpgl_write_inc_file_float(char *, hpgl_cont_masked_array_t *, float, char *)
+    NOT_ON_PATH("hpgl_write_inc_file_float")
+API void hpgl_write_inc_file_byte(char *, hpgl_ind_masked_array_t *, int, char *, unsigned char *, int)
+    NOT_ON_PATH("hpgl_write_inc_file_byte")
+API void hpgl_write_gslib_cont_property(hpgl_cont_masked_array_t *, const char *, const char *, double)
+    NOT_ON_PATH("hpgl_write_gslib_cont_property")
+API void hpgl_write_gslib_byte_property(hpgl_ind_masked_array_t *, const char *, const char *, double,
+                                        unsigned char *, int) NOT_ON_PATH("hpgl_write_gslib_byte_property")
+
+API void hpgl_amd_set_precision(int mode) {
+    g_precision = mode == HPGL_AMD_PRECISION_FP64 ? HPGL_AMD_PRECISION_FP64 : HPGL_AMD_PRECISION_MIXED;
+}
+API int hpgl_amd_get_precision(void) { return g_precision; }
+API int hpgl_amd_get_last_error(void) { return t_failed; }
+
+API void hpgl_amd_get_last_stats(hpgl_amd_stats_t *o) {
+    if (!o) return;
+    o->nodes_target = t_stats.nodes_target;
+    o->nodes_kriged = t_stats.nodes_kriged;
+    o->nodes_no_neighbours = t_stats.nodes_no_neighbours;
+    o->nodes_singular = t_stats.nodes_singular;
+    o->solves_fp32 = t_stats.solves_fp32;
+    o->solves_fp64 = t_stats.solves_fp64;
+    o->nodes_template_path = t_stats.nodes_template_path;
+    o->nodes_cluster_path = t_stats.nodes_cluster_path;
+    o->ms_setup_host = t_stats.ms_setup_host;
+    o->ms_search = t_stats.ms_search;
+    o->ms_solve = t_stats.ms_solve;
+    o->ms_device_total = t_stats.ms_device_total;
+    o->ms_wall = t_stats.ms_wall;
+    o->ms_k_template = t_stats.ms_k_template;
+    o->ms_k_cluster = t_stats.ms_k_cluster;
+    o->ms_k_solve_mixed = t_stats.ms_k_solve_mixed;
+    o->ms_k_solve_fp64 = t_stats.ms_k_solve_fp64;
+    o->n_k_template = t_stats.n_k_template;
+    o->n_k_cluster = t_stats.n_k_cluster;
+    o->n_k_solve_mixed = t_stats.n_k_solve_mixed;
+    o->n_k_solve_fp64 = t_stats.n_k_solve_fp64;
+    o->cluster_candidates = t_stats.cluster_candidates;
+    o->neighbours_total = t_stats.neighbours_total;
+}
+
+API void hpgl_amd_set_neighbour_dump(int *indices, int *counts, long long n_cells, int max_n) {
+    t_dump.indices = indices;
+    t_dump.counts = counts;
+    t_dump.n_cells = n_cells;
+    t_dump.max_n = max_n;
+}
+
+API int hpgl_amd_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+// ---- device-pointer variants ----
+
+static void ok_dev(const float *in, const unsigned char *in_mask, const hpgl_shape_t *shape,
+                   const hpgl_ok_params_t *p, float *out, unsigned char *out_mask, hipStream_t s) {
+    ContJob j;
+    std::memset(&j, 0, sizeof(j));
+    for (int i = 0; i < 3; ++i) {
+        j.grid[i] = shape->m_data[i];
+        j.radii[i] = p->m_radiuses[i];
+    }
+    j.cov = cov_of(p->m_covariance_type, p->m_ranges, p->m_angles, p->m_sill, p->m_nugget);
+    j.max_n = p->m_max_neighbours;
+    j.ordinary = true;
+    j.mean_kind = 0;
+    j.mean_on_failure = false;
+    j.d_in = in;
+    j.d_in_mask = in_mask;
+    j.d_out = out;
+    j.d_out_mask = out_mask;
+    Dump d;
+    progress("ordinary kriging", 0);
+    run_cont(j, s, g_precision, take_dump(d), t_stats);
+    progress("ordinary kriging", 100);
+}
+
+static void sk_dev(const float *in, const unsigned char *in_mask, const hpgl_shape_t *shape,
+                   const hpgl_sk_params_t *p, float *out, unsigned char *out_mask, hipStream_t s) {
+    ContJob j;
+    std::memset(&j, 0, sizeof(j));
+    for (int i = 0; i < 3; ++i) {
+        j.grid[i] = shape->m_data[i];
+        j.radii[i] = p->m_radiuses[i];
+    }
+    j.cov = cov_of(p->m_covariance_type, p->m_ranges, p->m_angles, p->m_sill, p->m_nugget);
+    j.max_n = p->m_max_neighbours;
+    j.ordinary = false;
+    j.mean_kind = 1;
+    j.mean_auto = p->m_automatic_mean != 0;
+    j.mean_single = p->m_mean;
+    j.mean_on_failure = true;
+    j.d_in = in;
+    j.d_in_mask = in_mask;
+    j.d_out = out;
+    j.d_out_mask = out_mask;
+    Dump d;
+    progress("simple kriging", 0);
+    run_cont(j, s, g_precision, take_dump(d), t_stats);
+    progress("simple kriging", 100);
+}
+
+static void lvm_dev(const float *in, const unsigned char *in_mask, const hpgl_shape_t *shape, const float *mean,
+                    const hpgl_ok_params_t *p, float *out, unsigned char *out_mask, hipStream_t s) {
+    ContJob j;
+    std::memset(&j, 0, sizeof(j));
+    for (int i = 0; i < 3; ++i) {
+        j.grid[i] = shape->m_data[i];
+        j.radii[i] = p->m_radiuses[i];
+    }
+    j.cov = cov_of(p->m_covariance_type, p->m_ranges, p->m_angles, p->m_sill, p->m_nugget);
+    j.max_n = p->m_max_neighbours;
+    j.ordinary = false;
+    j.mean_kind = 2;
+    j.d_mean = mean;
+    j.mean_on_failure = true;
+    j.d_in = in;
+    j.d_in_mask = in_mask;
+    j.d_out = out;
+    j.d_out_mask = out_mask;
+    Dump d;
+    progress("lvm kriging", 0);
+    run_cont(j, s, g_precision, take_dump(d), t_stats);
+    progress("lvm kriging", 100);
+}
+
+static void ik_dev(const unsigned char *in, const unsigned char *in_mask, const hpgl_shape_t *shape,
+                   const hpgl_ik_params_t *params, int K, unsigned char *out, unsigned char *out_mask,
+                   hipStream_t s) {
+    IndJob j;
+    for (int i = 0; i < 3; ++i) j.grid[i] = shape->m_data[i];
+    for (int k = 0; k < K; ++k) {
+        const hpgl_ik_params_t &p = params[k];
+        IndicatorParams c;
+        c.cov = cov_of(p.m_covariance_type, p.m_ranges, p.m_angles, p.m_sill, p.m_nugget);
+        for (int i = 0; i < 3; ++i) c.radii[i] = p.m_radiuses[i];
+        c.max_n = p.m_max_neighbours;
+        c.marginal = (double)(float)p.m_marginal_prob;  // ik_params_t keeps it as float
+        j.cats.push_back(c);
+    }
+    j.median = false;
+    j.d_in = in;
+    j.d_in_mask = in_mask;
+    j.d_out = out;
+    j.d_out_mask = out_mask;
+    Dump d;
+    progress("indicator kriging", 0);
+    run_ind(j, s, g_precision, take_dump(d), t_stats);
+    progress("indicator kriging", 100);
+}
+
+API void hpgl_amd_ordinary_kriging_dev(const float *in, const unsigned char *in_mask, const hpgl_shape_t *shape,
+                                       const hpgl_ok_params_t *p, float *out, unsigned char *out_mask, void *stream) {
+    GUARD_BEGIN
+    ok_dev(in, in_mask, shape, p, out, out_mask, (hipStream_t)stream);
+    GUARD_END
+}
+
+API void hpgl_amd_simple_kriging_dev(const float *in, const unsigned char *in_mask, const hpgl_shape_t *shape,
+                                     const hpgl_sk_params_t *p, float *out, unsigned char *out_mask, void *stream) {
+    GUARD_BEGIN
+    sk_dev(in, in_mask, shape, p, out, out_mask, (hipStream_t)stream);
+    GUARD_END
+}
+
+API void hpgl_amd_lvm_kriging_dev(const float *in, const unsigned char *in_mask, const hpgl_shape_t *shape,
+                                  const float *mean, const hpgl_ok_params_t *p, float *out, unsigned char *out_mask,
+                                  void *stream) {
+    GUARD_BEGIN
+    lvm_dev(in, in_mask, shape, mean, p, out, out_mask, (hipStream_t)stream);
+    GUARD_END
+}
+
+API void hpgl_amd_indicator_kriging_dev(const unsigned char *in, const unsigned char *in_mask,
+                                        const hpgl_shape_t *shape, const hpgl_ik_params_t *params, int K,
+                                        unsigned char *out, unsigned char *out_mask, void *stream) {
+    GUARD_BEGIN
+    ik_dev(in, in_mask, shape, params, K, out, out_mask, (hipStream_t)stream);
+    GUARD_END
+}
+
+// ---- host-pointer drop-in entry points ----
+
+API void hpgl_ordinary_kriging(hpgl_cont_masked_array_t *in, hpgl_ok_params_t *p, hpgl_cont_masked_array_t *out) {
+    GUARD_BEGIN
+    const long long N = volume(&in->m_shape);
+    if (volume(&out->m_shape) != N) throw std::runtime_error("input and output sizes differ");
+    hipStream_t s = thread_stream();
+    DevArray<float> di(N), dout(N);
+    DevArray<unsigned char> dm(N), dom(N);
+    di.put(in->m_data, s);
+    dm.put(in->m_mask, s);
+    dout.put(out->m_data, s);
+    dom.put(out->m_mask, s);
+    ok_dev(di.p, dm.p, &in->m_shape, p, dout.p, dom.p, s);
+    dout.get(out->m_data, s);
+    dom.get(out->m_mask, s);
+    hip_check(hipStreamSynchronize(s), "sync");
+    GUARD_END
+}
+
+API void hpgl_simple_kriging(float *input_data, unsigned char *input_mask, hpgl_shape_t *in_shape,
+                             hpgl_sk_params_t *p, float *output_data, unsigned char *output_mask,
+                             hpgl_shape_t *out_shape) {
+    GUARD_BEGIN
+    const long long N = volume(in_shape);
+    if (volume(out_shape) != N) throw std::runtime_error("input and output sizes differ");
+    hipStream_t s = thread_stream();
+    DevArray<float> di(N), dout(N);
+    DevArray<unsigned char> dm(N), dom(N);
+    di.put(input_data, s);
+    dm.put(input_mask, s);
+    dout.put(output_data, s);
+    dom.put(output_mask, s);
+    sk_dev(di.p, dm.p, in_shape, p, dout.p, dom.p, s);
+    dout.get(output_data, s);
+    dom.get(output_mask, s);
+    hip_check(hipStreamSynchronize(s), "sync");
+    GUARD_END
+}
+
+API void hpgl_lvm_kriging(float *input_data, unsigned char *input_mask, hpgl_shape_t *in_shape, float *mean_data,
+                          hpgl_shape_t *mean_shape, hpgl_ok_params_t *p, float *output_data,
+                          unsigned char *output_mask, hpgl_shape_t *out_shape) {
+    GUARD_BEGIN
+    const long long N = volume(in_shape);
+    if (volume(out_shape) != N || volume(mean_shape) != N) throw std::runtime_error("array sizes differ");
+    hipStream_t s = thread_stream();
+    DevArray<float> di(N), dmean(N), dout(N);
+    DevArray<unsigned char> dm(N), dom(N);
+    di.put(input_data, s);
+    dm.put(input_mask, s);
+    dmean.put(mean_data, s);
+    dout.put(output_data, s);
+    dom.put(output_mask, s);
+    lvm_dev(di.p, dm.p, in_shape, dmean.p, p, dout.p, dom.p, s);
+    dout.get(output_data, s);
+    dom.get(output_mask, s);
+    hip_check(hipStreamSynchronize(s), "sync");
+    GUARD_END
+}
+
+API void hpgl_indicator_kriging(hpgl_ind_masked_array_t *in, hpgl_ind_masked_array_t *out, hpgl_ik_params_t *params,
+                                int K) {
+    GUARD_BEGIN
+    const long long N = volume(&in->m_shape);
+    if (volume(&out->m_shape) != N) throw std::runtime_error("input and output sizes differ");
+    hipStream_t s = thread_stream();
+    DevArray<unsigned char> di(N), dm(N), dout(N), dom(N);
+    di.put(in->m_data, s);
+    dm.put(in->m_mask, s);
+    dout.put(out->m_data, s);
+    dom.put(out->m_mask, s);
+    ik_dev(di.p, dm.p, &in->m_shape, params, K, dout.p, dom.p, s);
+    dout.get(out->m_data, s);
+    dom.get(out->m_mask, s);
+    hip_check(hipStreamSynchronize(s), "sync");
+    GUARD_END
+}
+
+API void hpgl_median_ik(hpgl_ind_masked_array_t *in, hpgl_median_ik_params_t *p, hpgl_ind_masked_array_t *out) {
+    GUARD_BEGIN
+    const long long N = volume(&in->m_shape);
+    if (volume(&out->m_shape) != N)
+        throw std::runtime_error("median_ik_for_two_indicators: Input property size is not equal output property size");
+    hipStream_t s = thread_stream();
+    DevArray<unsigned char> di(N), dm(N), dout(N), dom(N);
+    di.put(in->m_data, s);
+    dm.put(in->m_mask, s);
+    dout.put(out->m_data, s);
+    dom.put(out->m_mask, s);
+    IndJob j;
+    for (int i = 0; i < 3; ++i) j.grid[i] = in->m_shape.m_data[i];
+    IndicatorParams c;
+    c.cov = cov_of(p->m_covariance_type, p->m_ranges, p->m_angles, p->m_sill, p->m_nugget);
+    for (int i = 0; i < 3; ++i) c.radii[i] = p->m_radiuses[i];
+    c.max_n = p->m_max_neighbours;
+    c.marginal = p->m_marginal_probs[1];
+    j.cats.push_back(c);
+    j.median = true;
+    j.d_in = di.p;
+    j.d_in_mask = dm.p;
+    j.d_out = dout.p;
+    j.d_out_mask = dom.p;
+    Dump d;
+    progress("median ik", 0);
+    run_ind(j, s, g_precision, take_dump(d), t_stats);
+    progress("median ik", 100);
+    dout.get(out->m_data, s);
+    dom.get(out->m_mask, s);
+    hip_check(hipStreamSynchronize(s), "sync");
+    GUARD_END
+}
+
+API int hpgl_simple_kriging_weights(float *center, float *xs, float *ys, float *zs, int n, hpgl_cov_params_t *p,
+                                    float *weights) {
+    GUARD_BEGIN
+    if (n <= 0) return 0;
+    CovModel m(cov_of(p->m_covariance_type, p->m_ranges, p->m_angles, p->m_sill, p->m_nugget));
+    DevCov c;
+    std::memset(&c, 0, sizeof(c));
+    for (int i = 0; i < 9; ++i) c.t[i] = m.t[i];
+    c.range0 = m.s.ranges[0];
+    c.c0 = m.c0();
+    c.type = m.s.type;
+    hipStream_t s = thread_stream();
+    DevArray<float> dc(3), dx(n), dy(n), dz(n);
+    DevArray<double> A((size_t)n * n), w(n);
+    DevArray<int> info(1);
+    dc.put(center, s);
+    dx.put(xs, s);
+    dy.put(ys, s);
+    dz.put(zs, s);
+    hipLaunchKernelGGL(k_sk_weights_real, dim3(1), dim3(64), 0, s, dc.p, dx.p, dy.p, dz.p, n, c, A.p, w.p, info.p);
+    hip_check(hipGetLastError(), "sk weights");
+    std::vector<double> hw(n);
+    int hinfo = 0;
+    w.get(hw.data(), s);
+    info.get(&hinfo, s);
+    hip_check(hipStreamSynchronize(s), "sync");
+    if (hinfo != 0) throw std::runtime_error("simple_kriging_weights: singular system");
+    for (int i = 0; i < n; ++i) weights[i] = (float)hw[i];
+    GUARD_END
+    return 0;
+}
+
+// SGS entry points are declared in include/hpgl.h; they are implemented in sgs.hip.
+
+API void hpgl_amd_report_error(const char *msg) {
+    begin_call();
+    fail(msg);
+}

@@ -1,0 +1,95 @@
+// engine.h - orchestration of one kriging job on the device.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "model.h"
+
+namespace hpgl_amd {
+
+struct Stats {
+    long long nodes_target = 0;
+    long long nodes_kriged = 0;
+    long long nodes_no_neighbours = 0;
+    long long nodes_singular = 0;
+    long long solves_fp32 = 0;
+    long long solves_fp64 = 0;
+    long long nodes_template_path = 0;
+    long long nodes_cluster_path = 0;
+    double ms_setup_host = 0;
+    double ms_search = 0;
+    double ms_solve = 0;
+    double ms_device_total = 0;
+    double ms_wall = 0;
+    // per-kernel device time (HIP events around each launch) and launch counts
+    double ms_k_template = 0;
+    double ms_k_cluster = 0;
+    double ms_k_solve_mixed = 0;
+    double ms_k_solve_fp64 = 0;
+    long long n_k_template = 0;
+    long long n_k_cluster = 0;
+    long long n_k_solve_mixed = 0;
+    long long n_k_solve_fp64 = 0;
+    long long cluster_candidates = 0;  // candidate cells gathered by the cluster path
+    long long neighbours_total = 0;    // sum of neighbour counts over solved systems
+};
+
+// Neighbour dump requested by the caller (host buffers).
+struct Dump {
+    int *indices = nullptr;
+    int *counts = nullptr;
+    long long n_cells = 0;
+    int max_n = 0;
+};
+
+// A continuous-variable kriging job (OK, SK, LVM) on device buffers.
+struct ContJob {
+    int grid[3];
+    CovSpec cov;
+    int radii[3];
+    int max_n;
+    bool ordinary;          // OK system and analytic covariance
+    int mean_kind;          // 0 none, 1 single, 2 per-cell
+    double mean_single;     // mean_kind 1 (SK: given or computed from the data)
+    bool mean_auto;         // SK automatic mean (calc_mean.cpp:6-25)
+    const float *d_mean = nullptr;
+    bool mean_on_failure;
+    const float *d_in;
+    const uint8_t *d_in_mask;
+    float *d_out;
+    uint8_t *d_out_mask;
+};
+
+struct IndicatorParams {
+    CovSpec cov;
+    int radii[3];
+    int max_n;
+    double marginal;  // IK: marginal_prob of the category; median IK: marginal_probs[1]
+};
+
+// Indicator kriging (K >= 1 categories) or median IK (median = true, one
+// parameter set, indicator value 1).
+struct IndJob {
+    int grid[3];
+    std::vector<IndicatorParams> cats;
+    bool median;
+    const uint8_t *d_in;
+    const uint8_t *d_in_mask;
+    uint8_t *d_out;
+    uint8_t *d_out_mask;
+};
+
+void run_cont(const ContJob &job, hipStream_t s, int precision, const Dump *dump, Stats &st);
+void run_ind(const IndJob &job, hipStream_t s, int precision, const Dump *dump, Stats &st);
+
+// Device memory held by the library between calls (size-class cache).
+void *pool_alloc(size_t bytes);
+void pool_free(void *p);
+
+void hip_check(hipError_t e, const char *what);
+
+}  // namespace hpgl_amd

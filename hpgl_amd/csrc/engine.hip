@@ -1,0 +1,612 @@
+// engine.hip - one kriging job on the device: host set-up, cell bins,
+// search, solve, epilogue, statistics.
+#include "engine.h"
+
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+
+namespace hpgl_amd {
+
+int cluster_lds_capacity();
+
+void hip_check(hipError_t e, const char *what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ---------------------------------------------------------------------------
+// Device memory cache: freed blocks are kept per size class (powers of two
+// from 256 B) and reused, so repeated jobs do not call hipMalloc.
+// ---------------------------------------------------------------------------
+namespace {
+std::mutex g_pool_mu;
+std::multimap<size_t, void *> g_free;
+std::map<void *, size_t> g_live;
+
+size_t size_class(size_t bytes) {
+    size_t c = 256;
+    while (c < bytes) c <<= 1;
+    return c;
+}
+}  // namespace
+
+void *pool_alloc(size_t bytes) {
+    if (bytes == 0) bytes = 1;
+    const size_t c = size_class(bytes);
+    {
+        std::lock_guard<std::mutex> lk(g_pool_mu);
+        auto it = g_free.find(c);
+        if (it != g_free.end()) {
+            void *p = it->second;
+            g_free.erase(it);
+            g_live[p] = c;
+            return p;
+        }
+    }
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, c);
+    if (e != hipSuccess) {
+        // release cached blocks and retry once
+        std::lock_guard<std::mutex> lk(g_pool_mu);
+        for (auto &kv : g_free) (void)hipFree(kv.second);
+        g_free.clear();
+        e = hipMalloc(&p, c);
+    }
+    hip_check(e, "hipMalloc");
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    g_live[p] = c;
+    return p;
+}
+
+void pool_free(void *p) {
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    auto it = g_live.find(p);
+    if (it == g_live.end()) return;
+    g_free.emplace(it->second, p);
+    g_live.erase(it);
+}
+
+namespace {
+
+template <typename T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    explicit DBuf(size_t count) { alloc(count); }
+    void alloc(size_t count) {
+        release();
+        n = count;
+        p = (T *)pool_alloc(std::max<size_t>(count, 1) * sizeof(T));
+    }
+    void release() {
+        pool_free(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DBuf() { release(); }
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+};
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct Events {
+    hipEvent_t e[8];
+    Events() {
+        for (auto &x : e) hip_check(hipEventCreate(&x), "hipEventCreate");
+    }
+    ~Events() {
+        for (auto &x : e) (void)hipEventDestroy(x);
+    }
+    float ms(int a, int b) {
+        float v = 0;
+        hip_check(hipEventElapsedTime(&v, e[a], e[b]), "hipEventElapsedTime");
+        return v;
+    }
+};
+
+// Device time of one launch sequence on stream s.
+struct KTimer {
+    hipEvent_t a, b;
+    hipStream_t s;
+    explicit KTimer(hipStream_t st) : s(st) {
+        hip_check(hipEventCreate(&a), "hipEventCreate");
+        hip_check(hipEventCreate(&b), "hipEventCreate");
+        hip_check(hipEventRecord(a, s), "hipEventRecord");
+    }
+    double stop() {
+        hip_check(hipEventRecord(b, s), "hipEventRecord");
+        hip_check(hipEventSynchronize(b), "hipEventSynchronize");
+        float ms = 0;
+        hip_check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+        return ms;
+    }
+    ~KTimer() {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+    }
+};
+
+struct IsZero {
+    const uint8_t *m;
+    __host__ __device__ bool operator()(const int &i) const { return m[i] == 0; }
+};
+struct IsNonZero {
+    const uint8_t *m;
+    __host__ __device__ bool operator()(const int &i) const { return m[i] != 0; }
+};
+struct FlagEq {
+    const uint8_t *f;
+    uint8_t v;
+    __host__ __device__ bool operator()(const int &i) const { return f[i] == v; }
+};
+
+// Ascending list of i in [0, n) with pred(i), returned with its length.
+template <typename Pred>
+int select_indices(Pred pred, int n, DBuf<int> &out, hipStream_t s) {
+    out.alloc(n);
+    DBuf<int> d_num(1);
+    hipcub::CountingInputIterator<int> it(0);
+    size_t bytes = 0;
+    hip_check(hipcub::DeviceSelect::If(nullptr, bytes, it, out.p, d_num.p, n, pred, s), "select");
+    DBuf<uint8_t> tmp(bytes);
+    hip_check(hipcub::DeviceSelect::If(tmp.p, bytes, it, out.p, d_num.p, n, pred, s), "select");
+    int num = 0;
+    hip_check(hipMemcpyAsync(&num, d_num.p, sizeof(int), hipMemcpyDeviceToHost, s), "copy count");
+    hip_check(hipStreamSynchronize(s), "sync");
+    return num;
+}
+
+__global__ void k_iota(int *p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = i;
+}
+
+template <typename T>
+void upload(DBuf<T> &d, const std::vector<T> &h, hipStream_t s) {
+    d.alloc(h.size());
+    if (!h.empty()) hip_check(hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s), "upload");
+}
+
+int solve_group(int max_n) {
+    int g = 8;
+    while (g < max_n) g <<= 1;
+    return g;
+}
+
+DevCov make_devcov(const CovModel &m, const BoxTable *table, const double *d_table) {
+    DevCov c;
+    std::memset(&c, 0, sizeof(c));
+    c.table = table ? d_table : nullptr;
+    for (int i = 0; i < 3; ++i) {
+        c.tr[i] = table ? table->r[i] : 0;
+        c.td[i] = table ? table->d[i] : 1;
+    }
+    for (int i = 0; i < 9; ++i) c.t[i] = m.t[i];
+    c.range0 = m.s.ranges[0];
+    c.c0 = m.c0();
+    c.type = m.s.type;
+    return c;
+}
+
+void validate_search(const int grid[3], const int radii[3], int max_n) {
+    long long n = 1;
+    for (int i = 0; i < 3; ++i) {
+        if (grid[i] <= 0) throw std::runtime_error("grid dimensions must be positive");
+        if (radii[i] <= 0) throw std::runtime_error("search radiuses must be positive");
+        n *= grid[i];
+    }
+    if (n >= (1ll << 31)) throw std::runtime_error("grid has more than 2^31-1 cells");
+    if (max_n < 0 || max_n > kMaxNeighbours)
+        throw std::runtime_error("max_neighbours must be in [0, " + std::to_string(kMaxNeighbours) + "]");
+}
+
+// Everything the search of one covariance needs on the device.
+struct SearchSetup {
+    CovModel model;
+    bool use_table;
+    BoxTable table;
+    Template tmpl;
+    BinGeometry bins;
+    DBuf<double> d_table;
+    DBuf<int32_t> d_tmpl;
+    explicit SearchSetup(const CovSpec &c) : model(c), use_table(false) {}
+};
+
+// Neighbour search for targets[0..T) over the informed cells inf[0..I).
+// Fills nbr[T][max_n] and cnt[T]; returns (template-path, cluster-path) counts.
+void search(SearchSetup &ss, const int grid[3], const int radii[3], int max_n, const uint8_t *d_mask,
+            const int *d_targets, int T, const int *d_inf, int I, int *d_nbr, int *d_cnt, hipStream_t s, Stats &st) {
+    double t0 = now_ms();
+    if (ss.use_table) ss.table.build(ss.model, radii);
+    build_template(ss.model, ss.use_table ? &ss.table : nullptr, radii, ss.tmpl);
+    ss.bins.init(grid, radii, ss.tmpl.size());
+    st.ms_setup_host += now_ms() - t0;
+    if (ss.use_table) upload(ss.d_table, ss.table.v, s);
+    upload(ss.d_tmpl, ss.tmpl.xyz, s);
+    if (T == 0) return;
+
+    BinArgs ba;
+    ba.X = grid[0];
+    ba.Y = grid[1];
+    ba.Z = grid[2];
+    for (int i = 0; i < 3; ++i) {
+        ba.r[i] = ss.bins.r[i];
+        ba.m[i] = ss.bins.m[i];
+    }
+    ba.limit = ss.bins.limit;
+    DBuf<uint32_t> bin_count(ss.bins.nbins);
+    DBuf<int> bin_of(std::max(I, 1));
+    launch_fill_u32(bin_count.p, 0, ss.bins.nbins, s);
+    launch_bin_ids(d_inf, I, ba, bin_of.p, bin_count.p, s);
+    DBuf<uint8_t> flag(T);
+    launch_path_flags(d_targets, T, ba, bin_count.p, flag.p, s);
+    hip_check(hipGetLastError(), "bin kernels");
+
+    DBuf<int> tlist, clist;
+    const int n_tmpl = select_indices(FlagEq{flag.p, 0}, T, tlist, s);
+    const int n_clu = T - n_tmpl;
+    st.nodes_template_path += n_tmpl;
+    st.nodes_cluster_path += n_clu;
+
+    ScanArgs sa;
+    sa.mask = d_mask;
+    sa.key = nullptr;
+    sa.target_key = nullptr;
+    sa.X = grid[0];
+    sa.Y = grid[1];
+    sa.Z = grid[2];
+    sa.tmpl = (const int4 *)ss.d_tmpl.p;
+    sa.tsize = ss.tmpl.size();
+    sa.max_n = max_n;
+    sa.targets = d_targets;
+    sa.list = tlist.p;
+    sa.n_list = n_tmpl;
+    sa.nbr = d_nbr;
+    sa.cnt = d_cnt;
+    if (n_tmpl > 0) {
+        KTimer kt(s);
+        launch_template_scan(sa, 0, s);
+        hip_check(hipGetLastError(), "template scan");
+        st.ms_k_template += kt.stop();
+        st.n_k_template += 1;
+    }
+
+    if (n_clu > 0) {
+        select_indices(FlagEq{flag.p, 1}, T, clist, s);
+        // bin members: informed cells sorted by (bin, cell); radix sort is stable
+        DBuf<int> keys_out(std::max(I, 1)), cells_out(std::max(I, 1));
+        size_t bytes = 0;
+        int end_bit = 1;
+        while ((1ll << end_bit) < ss.bins.nbins) ++end_bit;
+        hip_check(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, bin_of.p, keys_out.p, d_inf, cells_out.p, I, 0,
+                                                     end_bit, s),
+                  "radix sort");
+        DBuf<uint8_t> tmp(bytes);
+        hip_check(hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, bin_of.p, keys_out.p, d_inf, cells_out.p, I, 0,
+                                                     end_bit, s),
+                  "radix sort");
+        DBuf<uint32_t> bin_start(ss.bins.nbins);
+        bytes = 0;
+        hip_check(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, bin_count.p, bin_start.p, (int)ss.bins.nbins, s),
+                  "scan");
+        DBuf<uint8_t> tmp2(bytes);
+        hip_check(hipcub::DeviceScan::ExclusiveSum(tmp2.p, bytes, bin_count.p, bin_start.p, (int)ss.bins.nbins, s),
+                  "scan");
+        // sort keys as ranks over the offsets a candidate can have
+        t0 = now_ms();
+        RankTable rt;
+        int span[3];
+        for (int i = 0; i < 3; ++i) span[i] = ss.use_table ? radii[i] : 2 * radii[i] - 1;
+        for (int i = 0; i < 3; ++i) span[i] = std::max(span[i], 0);
+        rt.build(ss.model, ss.use_table ? &ss.table : nullptr, span);
+        st.ms_setup_host += now_ms() - t0;
+        DBuf<uint32_t> d_rank;
+        upload(d_rank, rt.rank, s);
+
+        ClusterArgs ca;
+        ca.X = grid[0];
+        ca.Y = grid[1];
+        ca.Z = grid[2];
+        ca.bins = ba;
+        ca.bin_start = bin_start.p;
+        ca.bin_count = bin_count.p;
+        ca.bin_cells = cells_out.p;
+        ca.rank = d_rank.p;
+        for (int i = 0; i < 3; ++i) {
+            ca.span[i] = rt.span[i];
+            ca.rd[i] = rt.d[i];
+        }
+        ca.max_n = max_n;
+        ca.targets = d_targets;
+        ca.list = clist.p;
+        ca.n_list = n_clu;
+        ca.nbr = d_nbr;
+        ca.cnt = d_cnt;
+        // candidates per target are at most 27 bins x limit cells
+        const long long max_cand = 27ll * std::max(ss.bins.limit, 0);
+        const int waves = 8192;
+        DBuf<uint64_t> scratch;
+        if (max_cand > cluster_lds_capacity()) {
+            ca.scratch_per_wave = (int)max_cand;
+            ca.n_waves_scratch = waves;
+            scratch.alloc((size_t)waves * max_cand * 2);
+            ca.scratch = scratch.p;
+        } else {
+            ca.scratch_per_wave = 0;
+            ca.n_waves_scratch = waves;
+            ca.scratch = nullptr;
+        }
+        DBuf<unsigned int> overflow(1);
+        DBuf<unsigned long long> cand(1);
+        hip_check(hipMemsetAsync(overflow.p, 0, sizeof(unsigned int), s), "memset");
+        hip_check(hipMemsetAsync(cand.p, 0, sizeof(unsigned long long), s), "memset");
+        ca.overflow = overflow.p;
+        ca.cand_total = cand.p;
+        {
+            KTimer kt(s);
+            launch_cluster_search(ca, s);
+            hip_check(hipGetLastError(), "cluster search");
+            st.ms_k_cluster += kt.stop();
+            st.n_k_cluster += 1;
+        }
+        unsigned long long nc = 0;
+        hip_check(hipMemcpyAsync(&nc, cand.p, sizeof(nc), hipMemcpyDeviceToHost, s), "copy cand");
+        unsigned int ov = 0;
+        hip_check(hipMemcpyAsync(&ov, overflow.p, sizeof(ov), hipMemcpyDeviceToHost, s), "copy overflow");
+        hip_check(hipStreamSynchronize(s), "sync");
+        st.cluster_candidates += (long long)nc;
+        if (ov & 1u) throw std::runtime_error("cluster search: candidate list exceeded the scratch capacity");
+        if (ov & 2u) throw std::runtime_error("cluster search: internal error in the sort replay");
+    }
+    {
+        DBuf<int> total(1);
+        size_t bytes = 0;
+        hip_check(hipcub::DeviceReduce::Sum(nullptr, bytes, d_cnt, total.p, T, s), "reduce");
+        DBuf<uint8_t> tmp(bytes);
+        hip_check(hipcub::DeviceReduce::Sum(tmp.p, bytes, d_cnt, total.p, T, s), "reduce");
+        int h = 0;
+        hip_check(hipMemcpyAsync(&h, total.p, sizeof(int), hipMemcpyDeviceToHost, s), "copy");
+        hip_check(hipStreamSynchronize(s), "sync");
+        st.neighbours_total += h;
+    }
+}
+
+void write_dump(const Dump *dump, long long cell_offset, const int *d_targets, int T, const int *d_nbr,
+                const int *d_cnt, int max_n, hipStream_t s) {
+    if (!dump || !dump->indices || T == 0) return;
+    std::vector<int> tg(T), cnt(T), nbr((size_t)T * max_n);
+    hip_check(hipMemcpyAsync(tg.data(), d_targets, T * sizeof(int), hipMemcpyDeviceToHost, s), "dump");
+    hip_check(hipMemcpyAsync(cnt.data(), d_cnt, T * sizeof(int), hipMemcpyDeviceToHost, s), "dump");
+    if (max_n > 0)
+        hip_check(hipMemcpyAsync(nbr.data(), d_nbr, nbr.size() * sizeof(int), hipMemcpyDeviceToHost, s), "dump");
+    hip_check(hipStreamSynchronize(s), "dump");
+    for (int t = 0; t < T; ++t) {
+        const long long cell = cell_offset + tg[t];
+        if (cell < 0 || cell >= dump->n_cells) continue;
+        dump->counts[cell] = cnt[t];
+        for (int k = 0; k < cnt[t] && k < dump->max_n; ++k) dump->indices[cell * dump->max_n + k] = nbr[(size_t)t * max_n + k];
+    }
+}
+
+// Solve all targets: mixed kernel then FP64 for the flagged ones, or FP64 only.
+void solve(SolveArgs sa, int epi, int precision, DBuf<uint8_t> &status, hipStream_t s, Stats &st) {
+    const int G = solve_group(sa.max_n);
+    sa.status = status.p;
+    sa.subset = nullptr;
+    sa.n_subset = 0;
+    if (precision == 0) {
+        {
+            KTimer kt(s);
+            launch_solve_mixed(sa, G, epi, s);
+            hip_check(hipGetLastError(), "solve mixed");
+            st.ms_k_solve_mixed += kt.stop();
+            st.n_k_solve_mixed += 1;
+        }
+        DBuf<int> sub;
+        const int nf = select_indices(FlagEq{status.p, (uint8_t)ST_FALLBACK}, sa.T, sub, s);
+        if (nf > 0) {
+            sa.subset = sub.p;
+            sa.n_subset = nf;
+            KTimer kt(s);
+            launch_solve_fp64(sa, G, epi, s);
+            hip_check(hipGetLastError(), "solve fp64");
+            st.ms_k_solve_fp64 += kt.stop();
+            st.n_k_solve_fp64 += 1;
+        }
+    } else {
+        KTimer kt(s);
+        launch_solve_fp64(sa, G, epi, s);
+        hip_check(hipGetLastError(), "solve fp64");
+        st.ms_k_solve_fp64 += kt.stop();
+        st.n_k_solve_fp64 += 1;
+    }
+}
+
+void collect_counters(const unsigned long long *d_counters, hipStream_t s, Stats &st) {
+    unsigned long long c[4];
+    hip_check(hipMemcpyAsync(c, d_counters, sizeof(c), hipMemcpyDeviceToHost, s), "counters");
+    hip_check(hipStreamSynchronize(s), "sync");
+    st.solves_fp32 += c[0];
+    st.solves_fp64 += c[1];
+    st.nodes_kriged += c[0] + c[1];
+    st.nodes_singular += c[2];
+    st.nodes_no_neighbours += c[3];
+}
+
+}  // namespace
+
+// calc_mean (calc_mean.cpp:6-25): sum of informed values in double, in
+// cell order, divided by their count (0 when there are none).
+static double device_calc_mean(const float *d_in, const uint8_t *d_mask, long long N, hipStream_t s) {
+    std::vector<float> v(N);
+    std::vector<uint8_t> m(N);
+    hip_check(hipMemcpyAsync(v.data(), d_in, N * sizeof(float), hipMemcpyDeviceToHost, s), "mean");
+    hip_check(hipMemcpyAsync(m.data(), d_mask, N, hipMemcpyDeviceToHost, s), "mean");
+    hip_check(hipStreamSynchronize(s), "sync");
+    double sum = 0;
+    int count = 0;
+    for (long long i = 0; i < N; ++i)
+        if (m[i] != 0) {
+            sum += v[i];
+            count += 1;
+        }
+    return count == 0 ? 0.0 : sum / count;
+}
+
+void run_cont(const ContJob &job, hipStream_t s, int precision, const Dump *dump, Stats &st) {
+    const double w0 = now_ms();
+    validate_search(job.grid, job.radii, job.max_n);
+    validate_cov(job.cov);
+    const long long N = (long long)job.grid[0] * job.grid[1] * job.grid[2];
+    Events ev;
+    hip_check(hipEventRecord(ev.e[0], s), "event");
+
+    double mean = job.mean_single;
+    if (job.mean_kind == 1 && job.mean_auto) mean = device_calc_mean(job.d_in, job.d_in_mask, N, s);
+
+    DBuf<int> targets, inf;
+    const int T = select_indices(IsZero{job.d_in_mask}, (int)N, targets, s);
+    const int I = select_indices(IsNonZero{job.d_in_mask}, (int)N, inf, s);
+    st.nodes_target += T;
+
+    SearchSetup ss(job.cov);
+    ss.use_table = !job.ordinary;
+    DBuf<int> nbr((size_t)T * std::max(job.max_n, 1)), cnt(T);
+    hip_check(hipEventRecord(ev.e[1], s), "event");
+    search(ss, job.grid, job.radii, job.max_n, job.d_in_mask, targets.p, T, inf.p, I, nbr.p, cnt.p, s, st);
+    hip_check(hipEventRecord(ev.e[2], s), "event");
+    write_dump(dump, 0, targets.p, T, nbr.p, cnt.p, job.max_n, s);
+
+    DBuf<unsigned long long> counters(4);
+    hip_check(hipMemsetAsync(counters.p, 0, 4 * sizeof(unsigned long long), s), "memset");
+    DBuf<uint8_t> status(T);
+    SolveArgs sa;
+    std::memset(&sa, 0, sizeof(sa));
+    sa.targets = targets.p;
+    sa.T = T;
+    sa.nbr = nbr.p;
+    sa.cnt = cnt.p;
+    sa.max_n = std::max(job.max_n, 1);
+    sa.X = job.grid[0];
+    sa.Y = job.grid[1];
+    sa.Z = job.grid[2];
+    sa.cov = make_devcov(ss.model, ss.use_table ? &ss.table : nullptr, ss.d_table.p);
+    sa.ok_system = job.ordinary ? 1 : 0;
+    sa.values_f = job.d_in;
+    sa.mean_kind = job.mean_kind;
+    sa.mean_single = (float)mean;
+    sa.mean_cells = job.d_mean;
+    sa.mean_on_failure = job.mean_on_failure ? 1 : 0;
+    sa.out_f = job.d_out;
+    sa.out_mask = job.d_out_mask;
+    sa.counters = counters.p;
+    hip_check(hipEventRecord(ev.e[3], s), "event");
+    if (T > 0) solve(sa, EPI_CONT, precision, status, s, st);
+    hip_check(hipEventRecord(ev.e[4], s), "event");
+    launch_copy_informed(job.d_in, job.d_in_mask, N, job.d_out, job.d_out_mask, s);
+    hip_check(hipEventRecord(ev.e[5], s), "event");
+    collect_counters(counters.p, s, st);
+    hip_check(hipStreamSynchronize(s), "sync");
+    st.ms_search += ev.ms(1, 2);
+    st.ms_solve += ev.ms(3, 4);
+    st.ms_device_total += ev.ms(0, 5);
+    st.ms_wall += now_ms() - w0;
+}
+
+void run_ind(const IndJob &job, hipStream_t s, int precision, const Dump *dump, Stats &st) {
+    const double w0 = now_ms();
+    const long long N = (long long)job.grid[0] * job.grid[1] * job.grid[2];
+    const int K = (int)job.cats.size();
+    if (K < 1) throw std::runtime_error("indicator kriging needs at least one indicator");
+    for (const auto &c : job.cats) {
+        validate_search(job.grid, c.radii, c.max_n);
+        validate_cov(c.cov);
+    }
+    Events ev;
+    hip_check(hipEventRecord(ev.e[0], s), "event");
+    // every cell is estimated, informed ones included (indicator_kriging.h:98-119)
+    DBuf<int> targets((size_t)N), inf;
+    hipLaunchKernelGGL(k_iota, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, targets.p, (int)N);
+    const int T = (int)N;
+    const int I = select_indices(IsNonZero{job.d_in_mask}, (int)N, inf, s);
+    st.nodes_target += (long long)T * K;
+
+    DBuf<float> probs(job.median ? 1 : (size_t)K * N);
+    DBuf<unsigned long long> counters(4);
+    hip_check(hipMemsetAsync(counters.p, 0, 4 * sizeof(unsigned long long), s), "memset");
+    DBuf<uint8_t> status(T);
+    int maxn_all = 1;
+    for (const auto &c : job.cats) maxn_all = std::max(maxn_all, c.max_n);
+    DBuf<int> nbr((size_t)T * maxn_all), cnt(T);
+    for (int k = 0; k < K; ++k) {
+        const IndicatorParams &c = job.cats[k];
+        SearchSetup ss(c.cov);
+        ss.use_table = true;
+        hipEvent_t a0, a1, b0, b1;
+        hip_check(hipEventCreate(&a0), "event");
+        hip_check(hipEventCreate(&a1), "event");
+        hip_check(hipEventCreate(&b0), "event");
+        hip_check(hipEventCreate(&b1), "event");
+        hip_check(hipEventRecord(a0, s), "event");
+        search(ss, job.grid, c.radii, c.max_n, job.d_in_mask, targets.p, T, inf.p, I, nbr.p, cnt.p, s, st);
+        hip_check(hipEventRecord(a1, s), "event");
+        write_dump(dump, (long long)k * N, targets.p, T, nbr.p, cnt.p, c.max_n, s);
+        SolveArgs sa;
+        std::memset(&sa, 0, sizeof(sa));
+        sa.targets = targets.p;
+        sa.T = T;
+        sa.nbr = nbr.p;
+        sa.cnt = cnt.p;
+        sa.max_n = std::max(c.max_n, 1);
+        sa.X = job.grid[0];
+        sa.Y = job.grid[1];
+        sa.Z = job.grid[2];
+        sa.cov = make_devcov(ss.model, &ss.table, ss.d_table.p);
+        sa.ok_system = 0;
+        sa.values_u8 = job.d_in;
+        sa.category = job.median ? 1 : k;
+        sa.mean_kind = 1;
+        // marginal probabilities pass through float (indicator_probability_t)
+        sa.mean_single = (float)c.marginal;
+        sa.out_f = job.median ? nullptr : probs.p + (size_t)k * N;
+        sa.out_u8 = job.d_out;
+        sa.out_mask = job.d_out_mask;
+        sa.counters = counters.p;
+        hip_check(hipEventRecord(b0, s), "event");
+        solve(sa, job.median ? EPI_MEDIAN : EPI_IK, precision, status, s, st);
+        hip_check(hipEventRecord(b1, s), "event");
+        hip_check(hipEventSynchronize(b1), "sync");
+        float ms = 0;
+        hip_check(hipEventElapsedTime(&ms, a0, a1), "elapsed");
+        st.ms_search += ms;
+        hip_check(hipEventElapsedTime(&ms, b0, b1), "elapsed");
+        st.ms_solve += ms;
+        (void)hipEventDestroy(a0);
+        (void)hipEventDestroy(a1);
+        (void)hipEventDestroy(b0);
+        (void)hipEventDestroy(b1);
+    }
+    if (!job.median) launch_ik_argmax(probs.p, K, N, job.d_out, job.d_out_mask, s);
+    hip_check(hipEventRecord(ev.e[5], s), "event");
+    collect_counters(counters.p, s, st);
+    hip_check(hipStreamSynchronize(s), "sync");
+    st.ms_device_total += ev.ms(0, 5);
+    st.ms_wall += now_ms() - w0;
+}
+
+}  // namespace hpgl_amd

@@ -1,0 +1,141 @@
+// kernels.h - launch interface of the HIP kernels (kernels.hip, search.hip).
+//
+// Layouts in HBM (all cell arrays are dense Fortran order, cell = x + X*y + X*Y*z):
+//   mask      uint8[N]                 informed flag
+//   values    float[N] / uint8[N]      property (continuous / categorical)
+//   targets   int32[T]                 cells to estimate, ascending
+//   nbr       int32[T][max_n]          neighbour cells of each target, search order
+//   cnt       int32[T]                 neighbour count (0..max_n)
+//   status    uint8[T]                 0 solved, 1 no neighbours, 2 singular,
+//                                      3 handed to the FP64 kernel
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace hpgl_amd {
+
+constexpr int kMaxNeighbours = 64;
+
+enum SolveStatus : uint8_t { ST_OK = 0, ST_NO_NEIGHBOURS = 1, ST_SINGULAR = 2, ST_FALLBACK = 3 };
+
+// Covariance as the solve kernels see it: either a box table (SK, LVM,
+// IK, median IK, SGS: precalculated_covariance.h) or the analytic model
+// (OK: cov_model.h) evaluated on the device.
+struct DevCov {
+    const double *table;  // nullptr => analytic
+    int tr[3];            // table radii
+    int td[3];            // table extents 2r+1
+    double t[9];          // transform (row-major)
+    double range0;
+    double c0;            // sill - nugget
+    int type;
+};
+
+// What the solve epilogue does with the weights.
+enum EpilogueKind : int {
+    EPI_CONT = 0,    // OK/SK/LVM: write estimate (or failure policy) to out
+    EPI_IK = 1,      // indicator kriging: write the category probability
+    EPI_MEDIAN = 2,  // median IK: double accumulation, threshold at 0.5
+    EPI_SGS = 3,     // SGS: write the kriged mean and variance
+};
+
+struct SolveArgs {
+    // system
+    const int *targets;
+    int T;
+    const int *nbr;
+    const int *cnt;
+    int max_n;
+    int X, Y, Z;
+    DevCov cov;
+    int ok_system;  // 1: ordinary kriging (2 RHS, Lagrange), 0: simple kriging
+    // values and means
+    const float *values_f;    // EPI_CONT, EPI_SGS
+    const uint8_t *values_u8;  // EPI_IK, EPI_MEDIAN
+    int category;              // indicator value tested against values_u8
+    int mean_kind;             // 0 none, 1 single, 2 per-cell float
+    float mean_single;
+    const float *mean_cells;
+    int mean_on_failure;       // EPI_CONT
+    // outputs
+    float *out_f;              // EPI_CONT estimate per cell; EPI_IK prob per target; EPI_SGS mean per target
+    uint8_t *out_mask;         // EPI_CONT, EPI_MEDIAN
+    uint8_t *out_u8;           // EPI_MEDIAN
+    double *out_var;           // EPI_SGS variance per target
+    uint8_t *status;
+    unsigned long long *counters;  // [0] fp32 accepted, [1] fp64 solved, [2] singular, [3] no neighbours
+    // FP64 pass: list of target positions to solve (nullptr = all T)
+    const int *subset;
+    int n_subset;
+};
+
+void launch_solve_mixed(const SolveArgs &a, int group, int epi, hipStream_t s);
+void launch_solve_fp64(const SolveArgs &a, int group, int epi, hipStream_t s);
+
+// Template scan (sugarbox_neighbour_lookup.h:36-61) over targets[list[i]].
+// pred_kind 0: informed iff mask[cell] != 0.
+// pred_kind 1 (SGS): informed iff key[cell] < key_of_target[i].
+struct ScanArgs {
+    const uint8_t *mask;
+    const int32_t *key;
+    const int32_t *target_key;
+    int X, Y, Z;
+    const int4 *tmpl;
+    int tsize;
+    int max_n;
+    const int *targets;
+    const int *list;  // positions into targets; nullptr = all
+    int n_list;
+    int *nbr;
+    int *cnt;
+};
+void launch_template_scan(const ScanArgs &a, int pred_kind, hipStream_t s);
+
+// Cell bins (clusterizer.cpp): per-bin counts of informed cells and the
+// decision per target whether any of its 27 bins is over the limit.
+struct BinArgs {
+    int X, Y, Z;
+    int r[3];
+    int m[3];
+    int limit;
+};
+void launch_bin_ids(const int *cells, int n, const BinArgs &b, int *bin_of, uint32_t *bin_count, hipStream_t s);
+void launch_path_flags(const int *targets, int T, const BinArgs &b, const uint32_t *bin_count, uint8_t *cluster_flag,
+                       hipStream_t s);
+
+// Cluster path (sugarbox_indexed_neighbour_lookup.h:66-123) over
+// targets[list[i]]: gather the 27 bins' cells, keep rank > 0, order exactly
+// as libstdc++ std::sort by descending key, keep max_n.
+struct ClusterArgs {
+    int X, Y, Z;
+    BinArgs bins;
+    const uint32_t *bin_start;
+    const uint32_t *bin_count;
+    const int *bin_cells;     // informed cells sorted by (bin, cell)
+    const uint32_t *rank;     // rank table over [-span, span]^3
+    int span[3];
+    int rd[3];
+    int max_n;
+    const int *targets;
+    const int *list;
+    int n_list;
+    int *nbr;
+    int *cnt;
+    // global scratch for candidate lists that do not fit in LDS
+    uint64_t *scratch;
+    int scratch_per_wave;     // capacity per wave, elements
+    int n_waves_scratch;
+    unsigned int *overflow;   // bit 0: a list exceeded scratch capacity; bit 1: sort replay error
+    unsigned long long *cand_total;  // += candidates gathered per target
+};
+void launch_cluster_search(const ClusterArgs &a, hipStream_t s);
+
+// Small helpers.
+void launch_copy_informed(const float *in, const uint8_t *in_mask, long long n, float *out, uint8_t *out_mask,
+                          hipStream_t s);
+void launch_ik_argmax(const float *probs, int K, long long n, uint8_t *out, uint8_t *out_mask, hipStream_t s);
+void launch_fill_u32(uint32_t *p, uint32_t v, long long n, hipStream_t s);
+
+}  // namespace hpgl_amd

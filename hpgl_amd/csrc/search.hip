@@ -1,0 +1,155 @@
+// search.hip - neighbour search kernels and small element-wise helpers.
+//
+// Template scan (sugarbox_neighbour_lookup.h:36-61): one lane per target
+// cell; the lanes of a wavefront hold neighbouring cells and walk the same
+// template offset in lockstep, so the template entry is a scalar load and
+// the 64 mask probes of one step hit one or two cache lines.  A lane stops
+// when it has max_n neighbours; the wave stops when all lanes have.
+//
+// Cell bins (clusterizer.cpp:70-184): counts per bin of informed cells and,
+// per target, whether any of its 27 bins holds more than `limit` cells (the
+// reference then uses the template scan, sugarbox_indexed_neighbour_lookup.h:68).
+#include "kernels.h"
+
+#include <hip/hip_runtime.h>
+
+namespace hpgl_amd {
+
+namespace {
+
+template <int PRED>
+__global__ __launch_bounds__(256) void k_template_scan(ScanArgs a) {
+    const int li = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = li < a.n_list;
+    const int t = active ? (a.list ? a.list[li] : li) : 0;
+    const int node = active ? a.targets[t] : 0;
+    const int X = a.X, Y = a.Y, Z = a.Z;
+    const int x = node % X, y = node % (X * Y) / X, z = node / (X * Y);
+    const int32_t my_key = (PRED == 1 && active) ? a.target_key[t] : 0;
+    int count = 0;
+    int *out = a.nbr + (size_t)t * a.max_n;
+    const int max_n = a.max_n;
+    bool want = active && count < max_n;
+    for (int k = 0; k < a.tsize; ++k) {
+        if ((k & 7) == 0 && !__any(want)) break;
+        const int4 o = a.tmpl[k];
+        if (want) {
+            const int px = x + o.x, py = y + o.y, pz = z + o.z;
+            if ((unsigned)px < (unsigned)X && (unsigned)py < (unsigned)Y && (unsigned)pz < (unsigned)Z) {
+                const int idx = (pz * Y + py) * X + px;
+                bool inf;
+                if (PRED == 0)
+                    inf = a.mask[idx] != 0;
+                else
+                    inf = a.key[idx] < my_key;
+                if (inf) {
+                    out[count++] = idx;
+                    want = count < max_n;
+                }
+            }
+        }
+    }
+    if (active) a.cnt[t] = count;
+}
+
+__global__ void k_bin_ids(const int *cells, int n, BinArgs b, int *bin_of, uint32_t *bin_count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = cells[i];
+    const int x = c % b.X, y = c % (b.X * b.Y) / b.X, z = c / (b.X * b.Y);
+    const int bx = x / b.r[0], by = y / b.r[1], bz = z / b.r[2];
+    const int bin = (bz * b.m[1] + by) * b.m[0] + bx;
+    bin_of[i] = bin;
+    atomicAdd(bin_count + bin, 1u);
+}
+
+__global__ void k_path_flags(const int *targets, int T, BinArgs b, const uint32_t *bin_count, uint8_t *flag) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    const int c = targets[t];
+    const int x = c % b.X, y = c % (b.X * b.Y) / b.X, z = c / (b.X * b.Y);
+    const int bx = x / b.r[0], by = y / b.r[1], bz = z / b.r[2];
+    bool exceeded = false;
+    for (int k = bz - 1; k <= bz + 1; ++k)
+        for (int j = by - 1; j <= by + 1; ++j)
+            for (int i = bx - 1; i <= bx + 1; ++i) {
+                // rect_3d_t::has is inclusive at both ends (geometry.h:21-24);
+                // i <= m-1 always holds for cells inside the grid.
+                if (i < 0 || j < 0 || k < 0 || i >= b.m[0] || j >= b.m[1] || k >= b.m[2]) continue;
+                if (bin_count[(k * b.m[1] + j) * b.m[0] + i] > (uint32_t)b.limit) exceeded = true;
+            }
+    flag[t] = exceeded ? 0 : 1;  // 1 = cluster path
+}
+
+__global__ void k_copy_informed(const float *in, const uint8_t *in_mask, long long n, float *out, uint8_t *out_mask) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (in_mask[i] != 0) {
+        out[i] = in[i];
+        out_mask[i] = 1;
+    }
+}
+
+// most_probable_category (cdf_utils.cpp:9-24): first index of the maximum,
+// compared in double.
+__global__ void k_ik_argmax(const float *probs, int K, long long n, uint8_t *out, uint8_t *out_mask) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double best = probs[i];
+    int idx = 0;
+    for (int k = 1; k < K; ++k) {
+        const double p = probs[(long long)k * n + i];
+        if (p > best) {
+            best = p;
+            idx = k;
+        }
+    }
+    out[i] = (uint8_t)idx;
+    out_mask[i] = 1;
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+inline unsigned blocks(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace
+
+void launch_template_scan(const ScanArgs &a, int pred_kind, hipStream_t s) {
+    if (a.n_list <= 0) return;
+    if (pred_kind == 0)
+        hipLaunchKernelGGL(k_template_scan<0>, dim3(blocks(a.n_list, 256)), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_template_scan<1>, dim3(blocks(a.n_list, 256)), dim3(256), 0, s, a);
+}
+
+void launch_bin_ids(const int *cells, int n, const BinArgs &b, int *bin_of, uint32_t *bin_count, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_bin_ids, dim3(blocks(n, 256)), dim3(256), 0, s, cells, n, b, bin_of, bin_count);
+}
+
+void launch_path_flags(const int *targets, int T, const BinArgs &b, const uint32_t *bin_count, uint8_t *flag,
+                       hipStream_t s) {
+    if (T <= 0) return;
+    hipLaunchKernelGGL(k_path_flags, dim3(blocks(T, 256)), dim3(256), 0, s, targets, T, b, bin_count, flag);
+}
+
+void launch_copy_informed(const float *in, const uint8_t *in_mask, long long n, float *out, uint8_t *out_mask,
+                          hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_copy_informed, dim3(blocks(n, 256)), dim3(256), 0, s, in, in_mask, n, out, out_mask);
+}
+
+void launch_ik_argmax(const float *probs, int K, long long n, uint8_t *out, uint8_t *out_mask, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_ik_argmax, dim3(blocks(n, 256)), dim3(256), 0, s, probs, K, n, out, out_mask);
+}
+
+void launch_fill_u32(uint32_t *p, uint32_t v, long long n, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks(n, 256)), dim3(256), 0, s, p, v, n);
+}
+
+}  // namespace hpgl_amd

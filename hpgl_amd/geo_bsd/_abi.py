@@ -103,6 +103,16 @@ class _HPGL_AMD_STATS(C.Structure):
         ("ms_solve", C.c_double),
         ("ms_device_total", C.c_double),
         ("ms_wall", C.c_double),
+        ("ms_k_template", C.c_double),
+        ("ms_k_cluster", C.c_double),
+        ("ms_k_solve_mixed", C.c_double),
+        ("ms_k_solve_fp64", C.c_double),
+        ("n_k_template", C.c_longlong),
+        ("n_k_cluster", C.c_longlong),
+        ("n_k_solve_mixed", C.c_longlong),
+        ("n_k_solve_fp64", C.c_longlong),
+        ("cluster_candidates", C.c_longlong),
+        ("neighbours_total", C.c_longlong),
     ]
 
 

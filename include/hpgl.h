@@ -233,6 +233,9 @@ void hpgl_median_ik(struct hpgl_ind_masked_array_t *in_data,
 void hpgl_amd_set_precision(int mode);
 int hpgl_amd_get_precision(void);
 
+/* Number of HIP devices visible to the library (0 without a GPU). */
+int hpgl_amd_device_count(void);
+
 /* Non-zero when the last call on this thread failed; the message is
  * available from hpgl_get_last_exception_message(). */
 int hpgl_amd_get_last_error(void);
@@ -252,6 +255,16 @@ struct hpgl_amd_stats_t {
     double ms_solve;                /* device time, build+solve+combine kernels */
     double ms_device_total;         /* device time of the whole call */
     double ms_wall;                 /* wall time of the call */
+    double ms_k_template;           /* device time of the template-scan kernel launches */
+    double ms_k_cluster;            /* ... cell-binned search kernel launches */
+    double ms_k_solve_mixed;        /* ... mixed-precision solve kernel launches */
+    double ms_k_solve_fp64;         /* ... FP64 solve kernel launches */
+    long long n_k_template;         /* launch counts of the same kernels */
+    long long n_k_cluster;
+    long long n_k_solve_mixed;
+    long long n_k_solve_fp64;
+    long long cluster_candidates;   /* candidate cells gathered by the cell-binned search */
+    long long neighbours_total;     /* neighbours over all searched targets */
 };
 void hpgl_amd_get_last_stats(struct hpgl_amd_stats_t *out);
 

@@ -519,6 +519,9 @@ static void dump_node(int node, const std::vector<int> &idx) {
 }
 
 static int g_progress_tick = 1;
+// CPU-baseline sampling: only cells with cell % g_sample_stride == 0 are
+// estimated (1 = all cells, the reference's behaviour).
+static int g_sample_stride = 1;
 static unsigned long g_progress_counter = 0;
 static std::string g_last_error;
 
@@ -553,6 +556,7 @@ static void cont_kriging(const float *in, const unsigned char *in_mask, const Gr
         std::vector<float> vals;
 #pragma omp for schedule(static)
         for (int node = 0; node < N; ++node) {
+            if (g_sample_stride > 1 && node % g_sample_stride != 0) continue;
             if (in_mask[node] == 0) {
                 nl.find(node, informed, idx);
                 dump_node(node, idx);
@@ -836,6 +840,7 @@ ORACLE_API void oracle_set_threads(int n) {
 }
 ORACLE_API int oracle_get_threads() { return omp_get_max_threads(); }
 ORACLE_API void oracle_set_progress_tick(int on) { g_progress_tick = on; }
+ORACLE_API void oracle_set_sample_stride(int stride) { g_sample_stride = stride > 1 ? stride : 1; }
 ORACLE_API void oracle_set_neighbour_dump(int *indices, int *counts, long long n_cells, int max_n) {
     g_dump_idx = indices;
     g_dump_cnt = counts;
@@ -936,6 +941,7 @@ ORACLE_API void oracle_indicator_kriging(hpgl_ind_masked_array_t *in, hpgl_ind_m
         std::vector<float> probs;
 #pragma omp for schedule(static)
         for (int node = 0; node < N; ++node) {
+            if (g_sample_stride > 1 && node % g_sample_stride != 0) continue;
             probs.clear();
             for (int k = 0; k < K; ++k) {
                 const mean_t mk = (mean_t)(double)(float)params[k].m_marginal_prob;

@@ -14,9 +14,9 @@ def pytest_configure(config):
 
 def _gpu_available():
     try:
-        import torch  # noqa: F401 - device discovery only
+        from hpgl_amd.geo_bsd import hpgl_wrap
 
-        return torch.cuda.is_available()
+        return hpgl_wrap._hpgl_so.hpgl_amd_device_count() > 0
     except Exception:
         return False
 

@@ -75,6 +75,7 @@ lib.oracle_gauss_prob.argtypes = [C.c_double]
 lib.oracle_set_neighbour_dump.argtypes = [C.c_void_p, C.c_void_p, C.c_longlong, C.c_int]
 lib.oracle_set_threads.argtypes = [C.c_int]
 lib.oracle_set_progress_tick.argtypes = [C.c_int]
+lib.oracle_set_sample_stride.argtypes = [C.c_int]
 
 
 class OracleError(RuntimeError):
@@ -295,6 +296,11 @@ def gauss_prob(x):
 
 def set_threads(n):
     lib.oracle_set_threads(n)
+
+
+def set_sample_stride(stride):
+    """Estimate only cells with cell % stride == 0 (CPU-baseline sampling)."""
+    lib.oracle_set_sample_stride(int(stride))
 
 
 def set_progress_tick(on):

@@ -1,0 +1,69 @@
+"""The C ABI library loads, exports every symbol include/hpgl.h declares,
+and its struct layouts match the ctypes mirrors (no GPU calls)."""
+import ctypes as C
+import os
+import re
+
+import oracle_binding as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_functions():
+    src = open(os.path.join(ROOT, "include", "hpgl.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(hpgl_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    from hpgl_amd.geo_bsd import hpgl_wrap
+
+    lib = hpgl_wrap._hpgl_so
+    names = _declared_functions()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_struct_sizes_match_reference_layouts():
+    # LP64 layouts of geo_bsd/hpgl/api.h measured for the reference
+    a = O.abi
+    sizes = {"_HPGL_SHAPE": 24, "_HPGL_CONT_MASKED_ARRAY": 40, "_HPGL_IND_MASKED_ARRAY": 48, "_HPGL_FLOAT_ARRAY": 32,
+             "_HPGL_UBYTE_ARRAY": 32, "_HPGL_OK_PARAMS": 88, "_HPGL_SK_PARAMS": 104, "_HPGL_COV_PARAMS": 72,
+             "_HPGL_SGS_PARAMS": 112, "_HPGL_MEDIAN_IK_PARAMS": 104, "_HPGL_IK_PARAMS": 96,
+             "hpgl_non_parametric_cdf_t": 24}
+    for name, size in sizes.items():
+        assert C.sizeof(getattr(a, name)) == size, name
+    assert a._HPGL_SK_PARAMS.automatic_mean.offset == 88 and a._HPGL_SK_PARAMS.mean.offset == 96
+    assert a._HPGL_SGS_PARAMS.kriging_kind.offset == 88 and a._HPGL_SGS_PARAMS.seed.offset == 96
+    assert a._HPGL_OK_PARAMS.radiuses.offset == 72 and a._HPGL_OK_PARAMS.max_neighbours.offset == 84
+
+
+def test_errors_are_reported_not_thrown():
+    """File I/O is off the kriging path: the call returns and sets the error."""
+    from hpgl_amd.geo_bsd import hpgl_wrap
+
+    lib = hpgl_wrap._hpgl_so
+    lib.hpgl_read_inc_file_float.argtypes = [C.c_char_p, C.c_float, C.c_int, C.c_void_p, C.c_void_p]
+    lib.hpgl_read_inc_file_float(b"x.inc", -99.0, 0, None, None)
+    assert lib.hpgl_amd_get_last_error() == 1
+    assert b"not part of this library" in lib.hpgl_get_last_exception_message()
+
+
+def test_python_api_surface():
+    """The geo_bsd names a reference user calls on the kriging path exist."""
+    import hpgl_amd.geo_bsd as g
+
+    for name in ["ordinary_kriging", "simple_kriging", "lvm_kriging", "indicator_kriging", "median_ik",
+                 "sgs_simulation", "simple_kriging_weights", "CovarianceModel", "SugarboxGrid", "ContProperty",
+                 "IndProperty", "CdfData", "calc_cdf", "covariance", "set_thread_num", "get_thread_num",
+                 "set_output_handler", "set_progress_handler", "normed_cov_model", "calc_mean"]:
+        assert hasattr(g, name), name
+    cm = g.CovarianceModel(type=g.covariance.exponential, ranges=(5, 1, 1), sill=2.0, nugget=0.5)
+    n = g.normed_cov_model(cm)
+    assert abs(n.sill - 2.0 / 1.5) < 1e-15 and abs(n.nugget - 0.5 / 1.5) < 1e-15
+    try:
+        g.CovarianceModel(sill=1.0, nugget=2.0)
+        assert False
+    except Exception as e:
+        assert "Nugget" in str(e)
