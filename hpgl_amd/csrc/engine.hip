@@ -337,18 +337,26 @@ void search(SearchSetup &ss, const int grid[3], const int radii[3], int max_n, c
         ca.n_list = n_clu;
         ca.nbr = d_nbr;
         ca.cnt = d_cnt;
-        // candidates per target are at most 27 bins x limit cells
+        // candidates per target are at most 27 bins x limit cells; an element
+        // packs (rank << lb) | raw index into 32 bits
         const long long max_cand = 27ll * std::max(ss.bins.limit, 0);
+        int lb = 1;
+        while ((1ll << lb) <= max_cand) ++lb;
+        uint32_t max_rank = 0;
+        for (uint32_t r : rt.rank) max_rank = std::max(max_rank, r);
+        if (lb >= 32 || (unsigned long long)max_rank >= (1ull << (32 - lb)))
+            throw std::runtime_error("cluster search: covariance has too many distinct values for 32-bit sort keys");
+        ca.local_bits = lb;
         const int waves = 8192;
-        DBuf<uint64_t> scratch;
+        ca.n_waves = waves;
+        DBuf<uint32_t> scratch;
         if (max_cand > cluster_lds_capacity()) {
             ca.scratch_per_wave = (int)max_cand;
-            ca.n_waves_scratch = waves;
-            scratch.alloc((size_t)waves * max_cand * 2);
+            scratch.alloc((size_t)std::min<long long>(waves, n_clu) * max_cand * 3);
             ca.scratch = scratch.p;
+            ca.n_waves = (int)std::min<long long>(waves, n_clu);
         } else {
             ca.scratch_per_wave = 0;
-            ca.n_waves_scratch = waves;
             ca.scratch = nullptr;
         }
         DBuf<unsigned int> overflow(1);
@@ -403,37 +411,85 @@ void write_dump(const Dump *dump, long long cell_offset, const int *d_targets, i
     }
 }
 
-// Solve all targets: mixed kernel then FP64 for the flagged ones, or FP64 only.
+__global__ void k_bucket_keys(const int *cnt, int T, uint8_t *key) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    const int n = cnt[t];
+    // 0..7: quad buckets of order 4, 8, ..., 32; 8: wider systems
+    key[t] = n > kQuadMaxMixed ? 8 : (uint8_t)(n <= 4 ? 0 : (n + 3) / 4 - 1);
+}
+
+struct KeyEq {
+    const uint8_t *k;
+    uint8_t v;
+    __host__ __device__ bool operator()(const int &i) const { return k[i] == v; }
+};
+struct KeyEqStatus {
+    const uint8_t *k;
+    uint8_t v;
+    const uint8_t *st;
+    uint8_t sv;
+    __host__ __device__ bool operator()(const int &i) const { return k[i] == v && st[i] == sv; }
+};
+
+// FP64 solves of targets whose bucket key is k (and, when `only_fallback`,
+// whose status is ST_FALLBACK): quad kernel up to order 16, lane-per-row
+// kernel above.
+void solve_fp64_bucket(SolveArgs sa, const DBuf<uint8_t> &keys, uint8_t k, bool only_fallback,
+                       const DBuf<uint8_t> &status, hipStream_t s, Stats &st) {
+    DBuf<int> list;
+    int n;
+    if (only_fallback)
+        n = select_indices(KeyEqStatus{keys.p, k, status.p, (uint8_t)ST_FALLBACK}, sa.T, list, s);
+    else
+        n = select_indices(KeyEq{keys.p, k}, sa.T, list, s);
+    if (n == 0) return;
+    sa.subset = list.p;
+    sa.n_subset = n;
+    const int order = k == 8 ? 64 : 4 * (k + 1);
+    KTimer kt(s);
+    if (order <= kQuadMaxFp64)
+        launch_solve_quad(sa, order, true, s);
+    else
+        launch_solve_fp64(sa, order, s);
+    hip_check(hipGetLastError(), "solve fp64");
+    st.ms_k_solve_fp64 += kt.stop();
+    st.n_k_solve_fp64 += 1;
+}
+
+// Solve all targets, bucketed by neighbour count: the mixed kernels then
+// FP64 for the systems they hand over, or FP64 only.
 void solve(SolveArgs sa, int epi, int precision, DBuf<uint8_t> &status, hipStream_t s, Stats &st) {
-    const int G = solve_group(sa.max_n);
     sa.status = status.p;
     sa.subset = nullptr;
     sa.n_subset = 0;
+    sa.epi = epi;
+    DBuf<uint8_t> keys(sa.T);
+    hipLaunchKernelGGL(k_bucket_keys, dim3((sa.T + 255) / 256), dim3(256), 0, s, sa.cnt, sa.T, keys.p);
+    hip_check(hipGetLastError(), "bucket keys");
+    // the neighbour limit bounds the buckets that can be non-empty
+    const int maxn = sa.max_n;
+    const int kmax = maxn > kQuadMaxMixed ? 8 : std::max(0, (maxn + 3) / 4 - 1);
     if (precision == 0) {
-        {
+        for (int k = 0; k <= kmax; ++k) {
+            SolveArgs b = sa;
+            DBuf<int> list;
+            const int n = select_indices(KeyEq{keys.p, (uint8_t)k}, sa.T, list, s);
+            if (n == 0) continue;
+            b.subset = list.p;
+            b.n_subset = n;
             KTimer kt(s);
-            launch_solve_mixed(sa, G, epi, s);
+            if (k < 8)
+                launch_solve_quad(b, 4 * (k + 1), false, s);
+            else
+                launch_solve_mixed(b, 64, s);
             hip_check(hipGetLastError(), "solve mixed");
             st.ms_k_solve_mixed += kt.stop();
             st.n_k_solve_mixed += 1;
         }
-        DBuf<int> sub;
-        const int nf = select_indices(FlagEq{status.p, (uint8_t)ST_FALLBACK}, sa.T, sub, s);
-        if (nf > 0) {
-            sa.subset = sub.p;
-            sa.n_subset = nf;
-            KTimer kt(s);
-            launch_solve_fp64(sa, G, epi, s);
-            hip_check(hipGetLastError(), "solve fp64");
-            st.ms_k_solve_fp64 += kt.stop();
-            st.n_k_solve_fp64 += 1;
-        }
+        for (int k = 0; k <= kmax; ++k) solve_fp64_bucket(sa, keys, (uint8_t)k, true, status, s, st);
     } else {
-        KTimer kt(s);
-        launch_solve_fp64(sa, G, epi, s);
-        hip_check(hipGetLastError(), "solve fp64");
-        st.ms_k_solve_fp64 += kt.stop();
-        st.n_k_solve_fp64 += 1;
+        for (int k = 0; k <= kmax; ++k) solve_fp64_bucket(sa, keys, (uint8_t)k, false, status, s, st);
     }
 }
 

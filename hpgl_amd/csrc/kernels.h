@@ -66,13 +66,20 @@ struct SolveArgs {
     double *out_var;           // EPI_SGS variance per target
     uint8_t *status;
     unsigned long long *counters;  // [0] fp32 accepted, [1] fp64 solved, [2] singular, [3] no neighbours
-    // FP64 pass: list of target positions to solve (nullptr = all T)
+    // list of target positions to solve (nullptr = all T)
     const int *subset;
     int n_subset;
+    int epi;  // EpilogueKind
 };
 
-void launch_solve_mixed(const SolveArgs &a, int group, int epi, hipStream_t s);
-void launch_solve_fp64(const SolveArgs &a, int group, int epi, hipStream_t s);
+// One wavefront per 64/G systems, lane = row (any order up to G = 64).
+void launch_solve_mixed(const SolveArgs &a, int group, hipStream_t s);
+void launch_solve_fp64(const SolveArgs &a, int group, hipStream_t s);
+// Four lanes per system (solve_quad.hip), order = bucket size: a multiple
+// of 4 up to kQuadMaxMixed (mixed) or kQuadMaxFp64 (FP64).
+constexpr int kQuadMaxMixed = 32;
+constexpr int kQuadMaxFp64 = 16;
+void launch_solve_quad(const SolveArgs &a, int order, bool fp64, hipStream_t s);
 
 // Template scan (sugarbox_neighbour_lookup.h:36-61) over targets[list[i]].
 // pred_kind 0: informed iff mask[cell] != 0.
@@ -123,10 +130,12 @@ struct ClusterArgs {
     int n_list;
     int *nbr;
     int *cnt;
-    // global scratch for candidate lists that do not fit in LDS
-    uint64_t *scratch;
-    int scratch_per_wave;     // capacity per wave, elements
-    int n_waves_scratch;
+    int local_bits;           // element = (rank << local_bits) | raw candidate index
+    int n_waves;              // grid size (waves loop over the list)
+    // global scratch for candidate lists that do not fit in LDS:
+    // per wave 3 x scratch_per_wave words (elements, left stops, right stops)
+    uint32_t *scratch;
+    int scratch_per_wave;
     unsigned int *overflow;   // bit 0: a list exceeded scratch capacity; bit 1: sort replay error
     unsigned long long *cand_total;  // += candidates gathered per target
 };

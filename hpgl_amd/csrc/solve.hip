@@ -144,8 +144,7 @@ __device__ inline int stage_coords(const SolveArgs &a, int t, int n, int g, int 
     return nb;
 }
 
-// Value/mean of neighbour nb and the target's mean, per epilogue.
-template <int EPI>
+// Mean of neighbour nb.
 __device__ inline float nb_mean(const SolveArgs &a, int nb) {
     if (a.mean_kind == 2) return a.mean_cells[nb];
     if (a.mean_kind == 1) return a.mean_single;
@@ -156,10 +155,11 @@ __device__ inline float nb_mean(const SolveArgs &a, int nb) {
 // no neighbours); w is this lane's weight, term buffers are the group's.
 // Every lane of the wave calls this (it holds a barrier); `write` is false
 // for groups without a target or whose system went to the FP64 kernel.
-template <int EPI, int G>
+template <int G>
 __device__ inline void epilogue(const SolveArgs &a, bool write, int t, int n, int g, int i, int nb, bool ok, double w,
                                 double variance, float (*term)[G], double (*dterm)[G]) {
     const int node = a.targets[t];
+    const int EPI = a.epi;
     float m0;
     if (a.mean_kind == 2)
         m0 = a.mean_cells[node];
@@ -170,11 +170,11 @@ __device__ inline void epilogue(const SolveArgs &a, bool write, int t, int n, in
     if (write && ok && i < n) {
         if (EPI == EPI_CONT || EPI == EPI_SGS) {
             const float v = a.values_f[nb];
-            const float m = nb_mean<EPI>(a, nb);
+            const float m = nb_mean(a, nb);
             term[g][i] = (float)(w * (double)(v - m));
         } else {
             const float v = (a.values_u8[nb] == (uint8_t)a.category) ? 1.0f : 0.0f;
-            const float m = nb_mean<EPI>(a, nb);
+            const float m = nb_mean(a, nb);
             if (EPI == EPI_MEDIAN)
                 dterm[g][i] = (double)(w * (double)(v - m));
             else
@@ -220,15 +220,17 @@ __device__ inline void count(unsigned long long *c, int which) {
 // ---------------------------------------------------------------------------
 // Mixed precision kernel.
 // ---------------------------------------------------------------------------
-template <int G, int EPI>
+template <int G>
 __global__ __launch_bounds__(64) void k_solve_mixed(SolveArgs a) {
+    const int EPI = a.epi;
     constexpr int NG = 64 / G;
     __shared__ Shared<G> sh;
     const int lane = threadIdx.x;
     const int g = lane / G, i = lane % G;
     const int pos = blockIdx.x * NG + g;
-    const bool valid = pos < a.T;
-    const int t = valid ? pos : 0;
+    const int n_items = a.subset ? a.n_subset : a.T;
+    const bool valid = pos < n_items;
+    const int t = valid ? (a.subset ? a.subset[pos] : pos) : 0;
     const int n = valid ? min(a.cnt[t], G) : 0;
 
     int cx, cy, cz, xi, yi, zi;
@@ -387,15 +389,16 @@ __global__ __launch_bounds__(64) void k_solve_mixed(SolveArgs a) {
         if (st == ST_NO_NEIGHBOURS) count(a.counters, 3);
         if (st == ST_OK) count(a.counters, 0);
     }
-    epilogue<EPI, G>(a, valid && (n == 0 || !bad), t, n, g, i, nb, n > 0, w, variance, sh.term, sh.dterm);
+    epilogue<G>(a, valid && (n == 0 || !bad), t, n, g, i, nb, n > 0, w, variance, sh.term, sh.dterm);
 }
 
 // ---------------------------------------------------------------------------
 // FP64 kernel in the reference's operation order (contraction is off for
 // the whole translation unit; every product and sum rounds separately).
 // ---------------------------------------------------------------------------
-template <int G, int EPI>
+template <int G>
 __global__ __launch_bounds__(64) void k_solve_fp64(SolveArgs a) {
+    const int EPI = a.epi;
     constexpr int NG = 64 / G;
     __shared__ Shared64<G> sh;
     const int lane = threadIdx.x;
@@ -548,44 +551,33 @@ __global__ __launch_bounds__(64) void k_solve_fp64(SolveArgs a) {
         a.status[t] = n == 0 ? ST_NO_NEIGHBOURS : (ok ? ST_OK : ST_SINGULAR);
         count(a.counters, n == 0 ? 3 : (ok ? 1 : 2));
     }
-    epilogue<EPI, G>(a, valid, t, n, g, i, nb, ok, w, variance, sh.term, sh.dterm);
+    epilogue<G>(a, valid, t, n, g, i, nb, ok, w, variance, sh.term, sh.dterm);
 }
 
 }  // namespace
 
-#define HPGL_LAUNCH(KERNEL, G, EPI) hipLaunchKernelGGL((KERNEL<G, EPI>), grid, dim3(64), 0, s, a)
-
-#define HPGL_SWITCH_EPI(KERNEL, G)                   \
-    switch (epi) {                                   \
-        case EPI_CONT: HPGL_LAUNCH(KERNEL, G, EPI_CONT); break;     \
-        case EPI_IK: HPGL_LAUNCH(KERNEL, G, EPI_IK); break;         \
-        case EPI_MEDIAN: HPGL_LAUNCH(KERNEL, G, EPI_MEDIAN); break; \
-        default: HPGL_LAUNCH(KERNEL, G, EPI_SGS); break;            \
-    }
-
-void launch_solve_mixed(const SolveArgs &a, int group, int epi, hipStream_t s) {
-    if (a.T <= 0) return;
-    const int ng = 64 / group;
-    dim3 grid((a.T + ng - 1) / ng);
-    switch (group) {
-        case 8: HPGL_SWITCH_EPI(k_solve_mixed, 8); break;
-        case 16: HPGL_SWITCH_EPI(k_solve_mixed, 16); break;
-        case 32: HPGL_SWITCH_EPI(k_solve_mixed, 32); break;
-        default: HPGL_SWITCH_EPI(k_solve_mixed, 64); break;
-    }
-}
-
-void launch_solve_fp64(const SolveArgs &a, int group, int epi, hipStream_t s) {
+void launch_solve_mixed(const SolveArgs &a, int group, hipStream_t s) {
     const int items = a.subset ? a.n_subset : a.T;
     if (items <= 0) return;
-    const int ng = 64 / group;
+    const int g = group <= 32 ? 32 : 64;
+    const int ng = 64 / g;
     dim3 grid((items + ng - 1) / ng);
-    switch (group) {
-        case 8: HPGL_SWITCH_EPI(k_solve_fp64, 8); break;
-        case 16: HPGL_SWITCH_EPI(k_solve_fp64, 16); break;
-        case 32: HPGL_SWITCH_EPI(k_solve_fp64, 32); break;
-        default: HPGL_SWITCH_EPI(k_solve_fp64, 64); break;
-    }
+    if (g == 32)
+        hipLaunchKernelGGL(k_solve_mixed<32>, grid, dim3(64), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_solve_mixed<64>, grid, dim3(64), 0, s, a);
+}
+
+void launch_solve_fp64(const SolveArgs &a, int group, hipStream_t s) {
+    const int items = a.subset ? a.n_subset : a.T;
+    if (items <= 0) return;
+    const int g = group <= 32 ? 32 : 64;
+    const int ng = 64 / g;
+    dim3 grid((items + ng - 1) / ng);
+    if (g == 32)
+        hipLaunchKernelGGL(k_solve_fp64<32>, grid, dim3(64), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_solve_fp64<64>, grid, dim3(64), 0, s, a);
 }
 
 }  // namespace hpgl_amd

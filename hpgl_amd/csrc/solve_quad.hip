@@ -1,0 +1,640 @@
+// solve_quad.hip - per-node kriging systems of order n <= 32, four lanes
+// per system.
+//
+// A wavefront holds 16 systems; the four lanes of a quad share one system,
+// lane q owning rows i = 4m + q (m = 0 .. N/4-1) of A and of its Cholesky
+// factor L, in VGPRs.  Every value a row needs from another row of the
+// same system moves inside the quad with one DPP quad_perm (a full-rate
+// VALU op, no LDS, no barrier).  N is the bucket order (a multiple of 4);
+// rows n..N-1 are padded with the identity.
+//
+// k_solve_quad_mixed: FP32 right-looking Cholesky, forward and backward
+//   substitution, one or two refinement steps whose residual b - A x is
+//   formed in FP64 with A re-evaluated from the covariance (table lookup or
+//   analytic model).  A system is handed to the FP64 kernel when a pivot is
+//   not positive, when the pivot-based condition estimate max(A_kk) /
+//   min(pivot) exceeds 1e4 (then FP32 rounding could decide the sign of a
+//   pivot the FP64 factorisation sees), or when the last correction is
+//   larger than 1e-9 of the solution.
+// k_solve_quad_fp64: the reference's dpotf2 + dpotrs operation order
+//   (sequential dot products, reciprocal scaling, division in the
+//   triangular solves, no fused multiply-add; the translation unit is
+//   compiled with contraction off), so with the same A and b the weights
+//   and the success/failure decision are the reference's bit for bit.
+#include "kernels.h"
+
+#include <hip/hip_runtime.h>
+
+namespace hpgl_amd {
+
+namespace {
+
+__device__ inline int dpp_bcast_i(int v, int src) {
+    switch (src) {
+        case 0: return __builtin_amdgcn_mov_dpp(v, 0x00, 0xF, 0xF, false);
+        case 1: return __builtin_amdgcn_mov_dpp(v, 0x55, 0xF, 0xF, false);
+        case 2: return __builtin_amdgcn_mov_dpp(v, 0xAA, 0xF, 0xF, false);
+        default: return __builtin_amdgcn_mov_dpp(v, 0xFF, 0xF, 0xF, false);
+    }
+}
+// value of lane `src` of this lane's quad
+__device__ inline float qb(float v, int src) { return __int_as_float(dpp_bcast_i(__float_as_int(v), src)); }
+__device__ inline double qb(double v, int src) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_bcast_i((int)(b & 0xffffffffll), src);
+    const int hi = dpp_bcast_i((int)(b >> 32), src);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ inline int qb(int v, int src) { return dpp_bcast_i(v, src); }
+
+// quad-wide reductions (the quad holds one system)
+__device__ inline float qswap1(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
+}
+__device__ inline float qswap2(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));
+}
+__device__ inline double qswap1(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffll), 0xB1, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0xB1, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ inline double qswap2(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffll), 0x4E, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0x4E, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ inline double qsum(double v) {
+    v += qswap1(v);
+    v += qswap2(v);
+    return v;
+}
+__device__ inline double qmax(double v) {
+    v = fmax(v, qswap1(v));
+    v = fmax(v, qswap2(v));
+    return v;
+}
+__device__ inline int qmaxi(int v) {
+    v = max(v, __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false));
+    v = max(v, __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false));
+    return v;
+}
+
+// Analytic model kept out of line: it is called from fully unrolled loops.
+__device__ __noinline__ double cov_model_at(const DevCov &c, int dx, int dy, int dz);
+
+__device__ inline double cov_at(const DevCov &c, int dx, int dy, int dz) {
+    if (c.table) {
+        if (dx < -c.tr[0] || dx > c.tr[0] || dy < -c.tr[1] || dy > c.tr[1] || dz < -c.tr[2] || dz > c.tr[2])
+            return 0.0;
+        return c.table[((size_t)(dz + c.tr[2]) * c.td[1] + (dy + c.tr[1])) * c.td[0] + (dx + c.tr[0])];
+    }
+    return cov_model_at(c, dx, dy, dz);
+}
+
+// cov_model.h:63-77 transfrom_and_norm, then :144-181
+__device__ __noinline__ double cov_model_at(const DevCov &c, int dx, int dy, int dz) {
+    const double v0 = dx, v1 = dy, v2 = dz;
+    double r = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        double d = 0.0;
+        d += v0 * c.t[3 * i + 0];
+        d += v1 * c.t[3 * i + 1];
+        d += v2 * c.t[3 * i + 2];
+        r += d * d;
+    }
+    const double h = sqrt(r);
+    if (h < 0.0001) return c.c0;
+    const double a = c.range0;
+    if (c.type == 0) {
+        if (h > a) return 0.0;
+        return c.c0 * (1 - 1.5 * h / a + 0.5 * pow(h / a, 3.0));
+    }
+    if (c.type == 1) return c.c0 * exp(-3 * h / a);
+    return c.c0 * exp(-3 * pow(h / a, 2.0));
+}
+
+constexpr int off(int m) { return 2 * m * (m + 1); }  // start of my m-th row (row length 4m+4)
+
+template <int N>
+struct QShared {
+    int xyz[16][3][N];
+    double x1[16][N];
+    double x2[16][N];
+    float tf[16][N];
+    double td[16][N];
+};
+
+// Common set-up of one quad: target, neighbour count, coordinates in LDS.
+template <int N>
+struct QCtx {
+    int t, n, node, q, slot;
+    bool valid;
+    int cx, cy, cz;
+    int nb[N / 4];
+};
+
+template <int N>
+__device__ inline void q_setup(const SolveArgs &a, QShared<N> &sh, QCtx<N> &c) {
+    const int lane = threadIdx.x & 63;
+    c.q = lane & 3;
+    c.slot = lane >> 2;
+    const int pos = blockIdx.x * 16 + c.slot;
+    const int items = a.subset ? a.n_subset : a.T;
+    c.valid = pos < items;
+    c.t = c.valid ? (a.subset ? a.subset[pos] : pos) : 0;
+    c.n = c.valid ? min(a.cnt[c.t], N) : 0;
+    c.node = a.targets[c.t];
+    c.cx = c.node % a.X;
+    c.cy = c.node % (a.X * a.Y) / a.X;
+    c.cz = c.node / (a.X * a.Y);
+#pragma unroll
+    for (int m = 0; m < N / 4; ++m) {
+        const int i = 4 * m + c.q;
+        int x = 0, y = 0, z = 0, cell = -1;
+        if (i < c.n) {
+            cell = a.nbr[(size_t)c.t * a.max_n + i];
+            x = cell % a.X;
+            y = cell % (a.X * a.Y) / a.X;
+            z = cell / (a.X * a.Y);
+        }
+        c.nb[m] = cell;
+        sh.xyz[c.slot][0][i] = x;
+        sh.xyz[c.slot][1][i] = y;
+        sh.xyz[c.slot][2][i] = z;
+    }
+    __syncthreads();
+}
+
+// A(i, j) of the padded system
+template <int N>
+__device__ inline double q_aij(const SolveArgs &a, const QShared<N> &sh, int slot, int n, int i, int j) {
+    if (i < n && j < n)
+        return cov_at(a.cov, sh.xyz[slot][0][j] - sh.xyz[slot][0][i], sh.xyz[slot][1][j] - sh.xyz[slot][1][i],
+                      sh.xyz[slot][2][j] - sh.xyz[slot][2][i]);
+    return i == j ? 1.0 : 0.0;
+}
+
+template <int N>
+__device__ inline double q_bi(const SolveArgs &a, const QShared<N> &sh, const QCtx<N> &c, int i) {
+    if (i >= c.n) return 0.0;
+    return cov_at(a.cov, c.cx - sh.xyz[c.slot][0][i], c.cy - sh.xyz[c.slot][1][i], c.cz - sh.xyz[c.slot][2][i]);
+}
+
+__device__ inline void bump(unsigned long long *c, int which) {
+    if (c) atomicAdd(c + which, 1ull);
+}
+
+// Writes one system's result (combine.h order) from the quad's weights.
+// w[m] is the weight of my row 4m+q; `write` false skips the output.
+template <int N>
+__device__ inline void q_epilogue(const SolveArgs &a, QShared<N> *shp, const QCtx<N> &c, bool write, bool ok,
+                                  const double *w, double variance) {
+    constexpr int M = N / 4;
+    const int EPI = a.epi;
+    float m0;
+    if (a.mean_kind == 2)
+        m0 = a.mean_cells[c.node];
+    else if (a.mean_kind == 1)
+        m0 = a.mean_single;
+    else
+        m0 = 0.0f;
+    float tf[M];
+    double td[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const int i = 4 * m + c.q;
+        tf[m] = 0.0f;
+        td[m] = 0.0;
+        if (ok && i < c.n) {
+            const int nb = c.nb[m];
+            const float mi = a.mean_kind == 2 ? a.mean_cells[nb] : (a.mean_kind == 1 ? a.mean_single : 0.0f);
+            float v;
+            if (EPI == EPI_CONT || EPI == EPI_SGS)
+                v = a.values_f[nb];
+            else
+                v = (a.values_u8[nb] == (uint8_t)a.category) ? 1.0f : 0.0f;
+            if (EPI == EPI_MEDIAN)
+                td[m] = (double)(w[m] * (double)(v - mi));
+            else
+                tf[m] = (float)(w[m] * (double)(v - mi));
+        }
+    }
+    // sequential sum in neighbour order by the quad's first lane
+    QShared<N> &sh = *shp;
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        sh.tf[c.slot][4 * m + c.q] = tf[m];
+        sh.td[c.slot][4 * m + c.q] = td[m];
+    }
+    __syncthreads();
+    float rf = m0;
+    double rd = (double)m0;
+    if (ok && c.q == 0) {
+        for (int i = 0; i < c.n; ++i) {
+            if (EPI == EPI_MEDIAN)
+                rd += sh.td[c.slot][i];
+            else
+                rf += sh.tf[c.slot][i];
+        }
+    }
+    if (!write || c.q != 0) return;
+    if (EPI == EPI_CONT) {
+        if (ok) {
+            a.out_f[c.node] = rf;
+            a.out_mask[c.node] = 1;
+        } else if (a.mean_on_failure) {
+            a.out_f[c.node] = m0;
+            a.out_mask[c.node] = 1;
+        }
+    } else if (EPI == EPI_IK) {
+        a.out_f[c.t] = rf;
+    } else if (EPI == EPI_MEDIAN) {
+        a.out_u8[c.node] = rd <= 0.5 ? 0 : 1;
+        a.out_mask[c.node] = 1;
+    } else {
+        a.out_f[c.t] = rf;
+        a.out_var[c.t] = ok ? variance : -1.0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Mixed precision.
+// ---------------------------------------------------------------------------
+template <int N>
+__device__ inline void q_solve32(const float *L, int q, float *v) {
+    constexpr int M = N / 4;
+    // forward: L y = v (column sweep, y_k broadcast from its row owner)
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        const int mk = k >> 2, ko = k & 3;
+        if (q == ko) v[mk] = v[mk] / L[off(mk) + k];
+        const float yk = qb(v[mk], ko);
+#pragma unroll
+        for (int m = 0; m < M; ++m)
+            if (4 * m + 3 > k) {
+                if (4 * m + q > k) v[m] = fmaf(-L[off(m) + k], yk, v[m]);
+            }
+    }
+    // backward: L^T x = y (row sweep, partial sums reduced in the quad)
+#pragma unroll
+    for (int i = N - 1; i >= 0; --i) {
+        const int mi = i >> 2, io = i & 3;
+        float p = 0.0f;
+#pragma unroll
+        for (int m = 0; m < M; ++m)
+            if (4 * m + 3 > i) {
+                if (4 * m + q > i) p = fmaf(L[off(m) + i], v[m], p);
+            }
+        p += qswap1(p);
+        p += qswap2(p);
+        if (q == io) v[mi] = (v[mi] - p) / L[off(mi) + i];
+    }
+}
+
+template <int N, bool TWO>
+__global__ __launch_bounds__(64) void k_solve_quad_mixed(SolveArgs a) {
+    const int EPI = a.epi;
+    constexpr int M = N / 4;
+    __shared__ QShared<N> sh;
+    QCtx<N> c;
+    q_setup<N>(a, sh, c);
+    const int q = c.q;
+
+    // build L <- A (FP32), b (FP64)
+    float L[off(M)];
+    double b[M];
+    double maxdiag = 0.0;
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const int i = 4 * m + q;
+        b[m] = q_bi<N>(a, sh, c, i);
+#pragma unroll
+        for (int j = 0; j < 4 * m + 4; ++j) {
+            const double aij = (j <= i) ? q_aij<N>(a, sh, c.slot, c.n, i, j) : 0.0;
+            L[off(m) + j] = (float)aij;
+            if (j == i && i < c.n) maxdiag = fmax(maxdiag, aij);
+        }
+    }
+    maxdiag = qmax(maxdiag);
+
+    // right-looking Cholesky in place
+    float minpiv = 3.0e38f;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        const int mk = k >> 2, ko = k & 3;
+        const float d = qb(L[off(mk) + k], ko);
+        if (k < c.n) minpiv = fminf(minpiv, d);
+        const float s = sqrtf(fmaxf(d, 1.0e-30f));
+        const float inv = 1.0f / s;
+#pragma unroll
+        for (int m = 0; m < M; ++m)
+            if (4 * m + 3 >= k) {
+                const int i = 4 * m + q;
+                const float v = L[off(m) + k];
+                L[off(m) + k] = (i > k) ? v * inv : (i == k ? s : v);
+            }
+#pragma unroll
+        for (int j = k + 1; j < N; ++j) {
+            const float cj = qb(L[off(j >> 2) + k], j & 3);
+#pragma unroll
+            for (int m = 0; m < M; ++m)
+                if (4 * m + 3 >= j) L[off(m) + j] = fmaf(-L[off(m) + k], cj, L[off(m) + j]);
+        }
+    }
+    bool bad = !(minpiv > 1.0e-4f * (float)maxdiag);
+
+    constexpr bool two = TWO;
+    float v1[M], v2[M];
+    double x1[M], x2[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const int i = 4 * m + q;
+        v1[m] = (float)b[m];
+        v2[m] = (two && i < c.n) ? 1.0f : 0.0f;
+    }
+    q_solve32<N>(L, q, v1);
+    if (two) q_solve32<N>(L, q, v2);
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        x1[m] = v1[m];
+        x2[m] = v2[m];
+    }
+    double rel = 0.0;
+    for (int step = 0; step < 2; ++step) {
+        // r = b - A x in FP64 for my rows; x goes through LDS
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            sh.x1[c.slot][4 * m + q] = x1[m];
+            if (TWO) sh.x2[c.slot][4 * m + q] = x2[m];
+        }
+        __syncthreads();
+        double r1[M], r2[M];
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            r1[m] = b[m];
+            r2[m] = (TWO && 4 * m + q < c.n) ? 1.0 : 0.0;
+        }
+#pragma unroll 2
+        for (int j = 0; j < N; ++j) {
+            const double xj1 = sh.x1[c.slot][j];
+            const double xj2 = TWO ? sh.x2[c.slot][j] : 0.0;
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                const double aij = q_aij<N>(a, sh, c.slot, c.n, 4 * m + q, j);
+                r1[m] = fma(-aij, xj1, r1[m]);
+                if (TWO) r2[m] = fma(-aij, xj2, r2[m]);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            v1[m] = (float)r1[m];
+            v2[m] = (float)r2[m];
+        }
+        q_solve32<N>(L, q, v1);
+        if (two) q_solve32<N>(L, q, v2);
+        double dmax = 0.0, xmax = 0.0;
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            x1[m] += (double)v1[m];
+            x2[m] += (double)v2[m];
+            dmax = fmax(dmax, fmax(fabs((double)v1[m]), fabs((double)v2[m])));
+            xmax = fmax(xmax, fmax(fabs(x1[m]), fabs(x2[m])));
+        }
+        dmax = qmax(dmax);
+        xmax = qmax(xmax);
+        rel = dmax / (xmax > 0 ? xmax : 1.0);
+        if (rel <= 1.0e-12) break;  // converged to FP64 accuracy after one step
+    }
+    if (!(rel <= 1.0e-9)) bad = true;
+
+    // weights
+    double w[M];
+    double variance = 0.0;
+    if (two) {
+        double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+        for (int m = 0; m < M; ++m)
+            if (4 * m + q < c.n) {
+                s1 += x1[m];
+                s2 += x2[m];
+            }
+        s1 = qsum(s1);
+        s2 = qsum(s2);
+        const double mu = (s1 - 1.0) / s2;
+#pragma unroll
+        for (int m = 0; m < M; ++m) w[m] = x1[m] - mu * x2[m];
+    } else {
+#pragma unroll
+        for (int m = 0; m < M; ++m) w[m] = x1[m];
+    }
+    if (EPI == EPI_SGS) {
+        double sv = 0.0, wl = 0.0;
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            const int i = 4 * m + q;
+            if (i < c.n) sv += w[m] * b[m];
+            if (i == c.n - 1) wl = w[m];
+        }
+        variance = a.cov.c0 - qsum(sv);
+        if (two) variance -= qsum(wl);  // my_kriging_weights.h:246
+    }
+
+    if (c.valid && q == 0) {
+        const uint8_t st = c.n == 0 ? ST_NO_NEIGHBOURS : (bad ? ST_FALLBACK : ST_OK);
+        a.status[c.t] = st;
+        if (st == ST_NO_NEIGHBOURS) bump(a.counters, 3);
+        if (st == ST_OK) bump(a.counters, 0);
+    }
+    q_epilogue<N>(a, &sh, c, c.valid && (c.n == 0 || !bad), c.n > 0, w, variance);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 in the reference's operation order.
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ __launch_bounds__(64) void k_solve_quad_fp64(SolveArgs a) {
+    const int EPI = a.epi;
+    constexpr int M = N / 4;
+    __shared__ QShared<N> sh;
+    QCtx<N> c;
+    q_setup<N>(a, sh, c);
+    const int q = c.q;
+
+    double L[off(M)];
+    double b[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const int i = 4 * m + q;
+        b[m] = q_bi<N>(a, sh, c, i);
+#pragma unroll
+        for (int j = 0; j < 4 * m + 4; ++j) L[off(m) + j] = (j <= i) ? q_aij<N>(a, sh, c.slot, c.n, i, j) : 0.0;
+    }
+
+    // dpotf2 'U' on U = L^T: for each j, row j of L (entries < j) is
+    // broadcast, the pivot is A_jj - sum_k L_jk^2 (sequential), and every
+    // row below gets (A_ij + (-1) * sum_k L_ik L_jk) * (1 / L_jj).
+    int info = 0;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        const int mj = j >> 2, jo = j & 3;
+        // every accumulator sums over k = 0..j-1 in order, as ddot/dgemv do
+        double dot = 0.0;
+        double temp[M];
+#pragma unroll
+        for (int m = 0; m < M; ++m) temp[m] = 0.0;
+#pragma unroll
+        for (int k = 0; k < j; ++k) {
+            const double rjk = qb(L[off(mj) + k], jo);
+            dot = dot + rjk * rjk;
+#pragma unroll
+            for (int m = 0; m < M; ++m)
+                if (4 * m + 3 > j) temp[m] = temp[m] + L[off(m) + k] * rjk;
+        }
+        const double ajj0 = qb(L[off(mj) + j], jo);
+        double ajj = ajj0 - dot;
+        if (info == 0 && j < c.n && (ajj <= 0.0 || ajj != ajj)) info = j + 1;
+        if (!(ajj > 0.0)) ajj = 1.0;  // failed system or padding: keep the arithmetic finite
+        ajj = sqrt(ajj);
+        const double rcp = 1.0 / ajj;
+#pragma unroll
+        for (int m = 0; m < M; ++m)
+            if (4 * m + 3 >= j) {
+                const int i = 4 * m + q;
+                if (i == j) L[off(m) + j] = ajj;
+                if (i > j) L[off(m) + j] = rcp * (L[off(m) + j] + (-1.0) * temp[m]);
+            }
+    }
+
+    const bool two = a.ok_system != 0;
+    double y1[M], y2[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const int i = 4 * m + q;
+        y1[m] = b[m];
+        y2[m] = (two && i < c.n) ? 1.0 : 0.0;
+    }
+    // dtrsm('L','U','T','N'): temp = b_i; temp -= U(k,i) b_k for k < i; temp / U(i,i)
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        const int mk = k >> 2, ko = k & 3;
+        if (q == ko) {
+            y1[mk] = y1[mk] / L[off(mk) + k];
+            y2[mk] = y2[mk] / L[off(mk) + k];
+        }
+        const double a1 = qb(y1[mk], ko), a2 = qb(y2[mk], ko);
+#pragma unroll
+        for (int m = 0; m < M; ++m)
+            if (4 * m + 3 > k && 4 * m + q > k) {
+                y1[m] = y1[m] - L[off(m) + k] * a1;
+                y2[m] = y2[m] - L[off(m) + k] * a2;
+            }
+    }
+    // dtrsm('L','U','N','N'): for k descending, if b_k != 0: b_k /= U(k,k);
+    // b_i -= b_k U(i,k) for i < k, with U(i,k) = L(k,i) from row k's owner.
+#pragma unroll
+    for (int k = N - 1; k >= 0; --k) {
+        const int mk = k >> 2, ko = k & 3;
+        int nz1 = 0, nz2 = 0;
+        if (q == ko) {
+            nz1 = y1[mk] != 0.0;
+            nz2 = y2[mk] != 0.0;
+            if (nz1) y1[mk] = y1[mk] / L[off(mk) + k];
+            if (nz2) y2[mk] = y2[mk] / L[off(mk) + k];
+        }
+        nz1 = qb(nz1, ko);
+        nz2 = qb(nz2, ko);
+        const double xk1 = qb(y1[mk], ko), xk2 = qb(y2[mk], ko);
+#pragma unroll
+        for (int i = 0; i < k; ++i) {
+            const double lki = qb(L[off(mk) + i], ko);  // row k, column i
+            if ((i & 3) == q) {
+                const int m = i >> 2;
+                if (nz1) y1[m] = y1[m] - xk1 * lki;
+                if (nz2) y2[m] = y2[m] - xk2 * lki;
+            }
+        }
+    }
+
+    // weights and variance, sums in neighbour order (my_kriging_weights.h:217-252)
+    double w[M];
+    double variance = 0.0;
+    {
+        double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            const double t1 = qb(y1[i >> 2], i & 3), t2 = qb(y2[i >> 2], i & 3);
+            if (i < c.n) {
+                s1 += t1;
+                s2 += t2;
+            }
+        }
+        if (two) {
+            const double mu = (s1 - 1) / s2;
+#pragma unroll
+            for (int m = 0; m < M; ++m) w[m] = y1[m] - mu * y2[m];
+        } else {
+#pragma unroll
+            for (int m = 0; m < M; ++m) w[m] = y1[m];
+        }
+    }
+    if (EPI == EPI_SGS) {
+        double v = a.cov.c0;
+        double wl = 0.0;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            const double wi = qb(w[i >> 2], i & 3), bi = qb(b[i >> 2], i & 3);
+            if (i < c.n) v -= wi * bi;
+            if (i == c.n - 1) wl = wi;
+        }
+        if (two && c.n > 0) v -= wl;
+        variance = v;
+    }
+
+    const bool ok = c.n > 0 && info == 0;
+    if (c.valid && q == 0) {
+        a.status[c.t] = c.n == 0 ? ST_NO_NEIGHBOURS : (ok ? ST_OK : ST_SINGULAR);
+        bump(a.counters, c.n == 0 ? 3 : (ok ? 1 : 2));
+    }
+    q_epilogue<N>(a, &sh, c, c.valid, ok, w, variance);
+}
+
+}  // namespace
+
+#define HPGL_Q(K, NN) hipLaunchKernelGGL(K<NN>, grid, dim3(64), 0, s, a)
+#define HPGL_QM(NN)                                                                     \
+    if (a.ok_system)                                                                    \
+        hipLaunchKernelGGL((k_solve_quad_mixed<NN, true>), grid, dim3(64), 0, s, a);   \
+    else                                                                                \
+        hipLaunchKernelGGL((k_solve_quad_mixed<NN, false>), grid, dim3(64), 0, s, a)
+
+void launch_solve_quad(const SolveArgs &a, int order, bool fp64, hipStream_t s) {
+    const int items = a.subset ? a.n_subset : a.T;
+    if (items <= 0) return;
+    dim3 grid((items + 15) / 16);
+    if (fp64) {
+        switch (order) {
+            case 4: HPGL_Q(k_solve_quad_fp64, 4); break;
+            case 8: HPGL_Q(k_solve_quad_fp64, 8); break;
+            case 12: HPGL_Q(k_solve_quad_fp64, 12); break;
+            default: HPGL_Q(k_solve_quad_fp64, 16); break;
+        }
+    } else {
+        switch (order) {
+            case 4: HPGL_QM(4); break;
+            case 8: HPGL_QM(8); break;
+            case 12: HPGL_QM(12); break;
+            case 16: HPGL_QM(16); break;
+            case 20: HPGL_QM(20); break;
+            case 24: HPGL_QM(24); break;
+            case 28: HPGL_QM(28); break;
+            default: HPGL_QM(32); break;
+        }
+    }
+}
+
+}  // namespace hpgl_amd

@@ -244,3 +244,27 @@ def test_mixed_vs_fp64_error_report(geo):
     assert s64["solves_fp32"] == 0 and s64["solves_fp64"] > 0
     assert smx["solves_fp32"] + smx["solves_fp64"] == s64["solves_fp64"]
     assert rel_err(mix.data, ref.data, 1.0) <= 1e-6
+
+
+@pytest.mark.parametrize("max_n", [7, 19, 32, 40])
+def test_uneven_neighbour_counts_buckets(geo, max_n):
+    """Sparse data near the grid edges gives neighbour counts across the
+    solve buckets (4..32 on four lanes per system, wider systems on one
+    lane per row); mixed within 1e-5 of the oracle, FP64 bit-exact."""
+    shape = (36, 30, 10)
+    data, mask = scattered(shape, 0.01, 50 + max_n, law="normal")
+    g = Grid(*shape)
+    cov = Cov(0, (8, 8, 3), nugget=0.2)
+    r = (8, 8, 3)
+    od, om = O.simple_kriging((data, mask), g, r, max_n, cov, mean=0.0)
+    for precision in ("mixed", "fp64"):
+        geo.set_precision(precision)
+        pidx, pcnt = _dump_product(g.size, max_n)
+        out = geo.simple_kriging((data, mask), g, r, max_n, cov, mean=0.0)
+        counts = pcnt[pcnt >= 0]
+        assert len(np.unique(counts)) > 3
+        assert np.array_equal(out.mask, om)
+        if precision == "fp64":
+            assert np.array_equal(out.data, od)
+        else:
+            assert rel_err(out.data, od, 1.0) <= TOL
