@@ -306,8 +306,18 @@ __global__ __launch_bounds__(64) void k_cluster(ClusterArgs a) {
                 const int j = jb + lane;
                 uint32_t rk = 0;
                 if (j < cnt) {
-                    const int c = a.bin_cells[start + j];
-                    const int dx = c % X - x, dy = c % (X * Y) / X - y, dz = c / (X * Y) - z;
+                    int dx, dy, dz;
+                    if (a.bin_xyz) {
+                        const uint32_t v = a.bin_xyz[start + j];
+                        dx = (int)(v & 2047u) - x;
+                        dy = (int)((v >> 11) & 2047u) - y;
+                        dz = (int)(v >> 22) - z;
+                    } else {
+                        const int c = a.bin_cells[start + j];
+                        dx = c % X - x;
+                        dy = c % (X * Y) / X - y;
+                        dz = c / (X * Y) - z;
+                    }
                     if (abs(dx) <= a.span[0] && abs(dy) <= a.span[1] && abs(dz) <= a.span[2])
                         rk = a.rank[((size_t)(dz + a.span[2]) * a.rd[1] + (dy + a.span[1])) * a.rd[0] +
                                     (dx + a.span[0])];

@@ -326,6 +326,13 @@ void search(SearchSetup &ss, const int grid[3], const int radii[3], int max_n, c
         ca.bin_start = bin_start.p;
         ca.bin_count = bin_count.p;
         ca.bin_cells = cells_out.p;
+        DBuf<uint32_t> xyz;
+        ca.bin_xyz = nullptr;
+        if (grid[0] <= 2048 && grid[1] <= 2048 && grid[2] <= 1024) {
+            xyz.alloc(std::max(I, 1));
+            launch_pack_xyz(cells_out.p, I, grid[0], grid[1], xyz.p, s);
+            ca.bin_xyz = xyz.p;
+        }
         ca.rank = d_rank.p;
         for (int i = 0; i < 3; ++i) {
             ca.span[i] = rt.span[i];

@@ -121,6 +121,7 @@ struct ClusterArgs {
     const uint32_t *bin_start;
     const uint32_t *bin_count;
     const int *bin_cells;     // informed cells sorted by (bin, cell)
+    const uint32_t *bin_xyz;  // their coordinates packed x | y << 11 | z << 22 (nullptr: decode the cell)
     const uint32_t *rank;     // rank table over [-span, span]^3
     int span[3];
     int rd[3];
@@ -146,5 +147,6 @@ void launch_copy_informed(const float *in, const uint8_t *in_mask, long long n, 
                           hipStream_t s);
 void launch_ik_argmax(const float *probs, int K, long long n, uint8_t *out, uint8_t *out_mask, hipStream_t s);
 void launch_fill_u32(uint32_t *p, uint32_t v, long long n, hipStream_t s);
+void launch_pack_xyz(const int *cells, int n, int X, int Y, uint32_t *out, hipStream_t s);
 
 }  // namespace hpgl_amd

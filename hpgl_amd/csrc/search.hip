@@ -113,6 +113,14 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, long long n) {
     if (i < n) p[i] = v;
 }
 
+__global__ void k_pack_xyz(const int *cells, int n, int X, int Y, uint32_t *out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = cells[i];
+    const uint32_t x = c % X, y = c % (X * Y) / X, z = c / (X * Y);
+    out[i] = x | (y << 11) | (z << 22);
+}
+
 inline unsigned blocks(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 }  // namespace
@@ -145,6 +153,11 @@ void launch_copy_informed(const float *in, const uint8_t *in_mask, long long n, 
 void launch_ik_argmax(const float *probs, int K, long long n, uint8_t *out, uint8_t *out_mask, hipStream_t s) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_ik_argmax, dim3(blocks(n, 256)), dim3(256), 0, s, probs, K, n, out, out_mask);
+}
+
+void launch_pack_xyz(const int *cells, int n, int X, int Y, uint32_t *out, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_pack_xyz, dim3(blocks(n, 256)), dim3(256), 0, s, cells, n, X, Y, out);
 }
 
 void launch_fill_u32(uint32_t *p, uint32_t v, long long n, hipStream_t s) {
