@@ -520,7 +520,125 @@ API int hpgl_simple_kriging_weights(float *center, float *xs, float *ys, float *
     return 0;
 }
 
-// SGS entry points are declared in include/hpgl.h; they are implemented in sgs.hip.
+// ---- sequential Gaussian simulation ----
+
+static SgsJob sgs_job(const hpgl_shape_t *shape, const hpgl_sgs_params_t *p, const hpgl_non_parametric_cdf_t *cdf,
+                      bool lvm) {
+    SgsJob j;
+    for (int i = 0; i < 3; ++i) {
+        j.grid[i] = shape->m_data[i];
+        j.radii[i] = p->m_radiuses[i];
+    }
+    j.cov = cov_of(p->m_covariance_type, p->m_ranges, p->m_angles, p->m_sill, p->m_nugget);
+    j.max_n = p->m_max_neighbours;
+    // kriging kind 0 is OK (sgs_params.h:10); the LVM variant always uses SK
+    // with the local mean (sequential_gaussian_simulation.cpp:77-128)
+    j.ordinary = !lvm && p->m_kriging_kind == 0;
+    if (cdf) {
+        if (cdf->m_size <= 0 || !cdf->m_values || !cdf->m_probs)
+            throw std::runtime_error("the non-parametric cdf is empty");
+        j.has_cdf = true;
+        j.cdf_values.assign(cdf->m_values, cdf->m_values + cdf->m_size);
+        j.cdf_probs.assign(cdf->m_probs, cdf->m_probs + cdf->m_size);
+    }
+    return j;
+}
+
+// Runs the job with conditioning data from host arrays and writes R
+// realisations to host arrays out[R][N], out_mask[R][N].
+static void sgs_host(SgsJob &j, const float *data, const unsigned char *mask, const float *lvm_mean,
+                     const unsigned char *sim_mask, float *out, unsigned char *out_mask) {
+    const long long N = (long long)j.grid[0] * j.grid[1] * j.grid[2];
+    const int R = (int)j.seeds.size();
+    hipStream_t s = thread_stream();
+    DevArray<float> di(N), dlvm(lvm_mean ? N : 0), dout((size_t)R * N);
+    DevArray<unsigned char> dm(N), dsim(sim_mask ? N : 0), dom((size_t)R * N);
+    di.put(data, s);
+    dm.put(mask, s);
+    if (lvm_mean) dlvm.put(lvm_mean, s);
+    if (sim_mask) dsim.put(sim_mask, s);
+    j.d_in = di.p;
+    j.d_in_mask = dm.p;
+    j.d_lvm = lvm_mean ? dlvm.p : nullptr;
+    j.d_sim_mask = sim_mask ? dsim.p : nullptr;
+    j.d_out = dout.p;
+    j.d_out_mask = dom.p;
+    Dump d;
+    progress("sgs", 0);
+    run_sgs(j, s, g_precision, take_dump(d), t_stats);
+    dout.get(out, s);
+    dom.get(out_mask, s);
+    hip_check(hipStreamSynchronize(s), "sync");
+    progress("sgs", 100);
+}
+
+static void check_same_size(const hpgl_shape_t *a, long long n, const char *what) {
+    if (volume(a) != n) throw std::runtime_error(std::string(what) + " has a different size from the property");
+}
+
+API void hpgl_sgs_simulation(hpgl_cont_masked_array_t *prop, hpgl_sgs_params_t *params, hpgl_non_parametric_cdf_t *cdf,
+                             double *mean, hpgl_ubyte_array_t *simulation_mask) {
+    GUARD_BEGIN
+    (void)mean;  // only logged by the reference (sequential_gaussian_simulation.cpp:39 uses mean 0)
+    const long long N = volume(&prop->m_shape);
+    if (simulation_mask) check_same_size(&simulation_mask->m_shape, N, "the simulation mask");
+    SgsJob j = sgs_job(&prop->m_shape, params, cdf, false);
+    j.seeds = {params->m_seed};
+    sgs_host(j, prop->m_data, prop->m_mask, nullptr, simulation_mask ? simulation_mask->m_data : nullptr,
+             prop->m_data, prop->m_mask);
+    GUARD_END
+}
+
+API void hpgl_sgs_lvm_simulation(hpgl_cont_masked_array_t *data, hpgl_sgs_params_t *params,
+                                 hpgl_non_parametric_cdf_t *cdf, hpgl_float_array_t *means,
+                                 hpgl_ubyte_array_t *simulation_mask) {
+    GUARD_BEGIN
+    const long long N = volume(&data->m_shape);
+    if (!means || !means->m_data) throw std::runtime_error("the local mean is missing");
+    check_same_size(&means->m_shape, N, "the local mean");
+    if (simulation_mask) check_same_size(&simulation_mask->m_shape, N, "the simulation mask");
+    SgsJob j = sgs_job(&data->m_shape, params, cdf, true);
+    j.seeds = {params->m_seed};
+    sgs_host(j, data->m_data, data->m_mask, means->m_data, simulation_mask ? simulation_mask->m_data : nullptr,
+             data->m_data, data->m_mask);
+    GUARD_END
+}
+
+API void hpgl_amd_sgs_simulation_batch(const hpgl_cont_masked_array_t *prop, const hpgl_sgs_params_t *params,
+                                       const hpgl_non_parametric_cdf_t *cdf, const hpgl_float_array_t *lvm_means,
+                                       const hpgl_ubyte_array_t *simulation_mask, const long *seeds, int R,
+                                       float *out, unsigned char *out_mask) {
+    GUARD_BEGIN
+    const long long N = volume(&prop->m_shape);
+    if (R < 1 || !seeds) throw std::runtime_error("at least one seed is needed");
+    if (lvm_means) check_same_size(&lvm_means->m_shape, N, "the local mean");
+    if (simulation_mask) check_same_size(&simulation_mask->m_shape, N, "the simulation mask");
+    SgsJob j = sgs_job(&prop->m_shape, params, cdf, lvm_means != nullptr);
+    j.seeds.assign(seeds, seeds + R);
+    sgs_host(j, prop->m_data, prop->m_mask, lvm_means ? lvm_means->m_data : nullptr,
+             simulation_mask ? simulation_mask->m_data : nullptr, out, out_mask);
+    GUARD_END
+}
+
+API void hpgl_amd_sgs_simulation_batch_dev(const float *d_in, const unsigned char *d_in_mask,
+                                           const hpgl_shape_t *shape, const hpgl_sgs_params_t *params,
+                                           const hpgl_non_parametric_cdf_t *cdf, const float *d_lvm_mean,
+                                           const unsigned char *d_sim_mask, const long *seeds, int R, float *d_out,
+                                           unsigned char *d_out_mask, void *stream) {
+    GUARD_BEGIN
+    if (R < 1 || !seeds) throw std::runtime_error("at least one seed is needed");
+    SgsJob j = sgs_job(shape, params, cdf, d_lvm_mean != nullptr);
+    j.seeds.assign(seeds, seeds + R);
+    j.d_in = d_in;
+    j.d_in_mask = d_in_mask;
+    j.d_lvm = d_lvm_mean;
+    j.d_sim_mask = d_sim_mask;
+    j.d_out = d_out;
+    j.d_out_mask = d_out_mask;
+    Dump d;
+    run_sgs(j, (hipStream_t)stream, g_precision, take_dump(d), t_stats);
+    GUARD_END
+}
 
 API void hpgl_amd_report_error(const char *msg) {
     begin_call();

@@ -27,6 +27,7 @@
 //    stable rank sort;
 //  * a range whose depth budget (2 floor(log2 n)) runs out is heap-sorted
 //    by one lane with libstdc++'s make_heap/sort_heap steps.
+#include "counters.h"
 #include "kernels.h"
 
 #include <hip/hip_runtime.h>
@@ -270,6 +271,7 @@ __global__ __launch_bounds__(64) void k_cluster(ClusterArgs a) {
     const BinArgs &b = a.bins;
     const Keyed key{a.local_bits};
     uint32_t *ge = a.scratch ? a.scratch + (size_t)blockIdx.x * a.scratch_per_wave * 3 : nullptr;
+    unsigned long long cand_acc = 0;
     for (int li = blockIdx.x; li < a.n_list; li += gridDim.x) {
         const int t = a.list ? a.list[li] : li;
         const int node = a.targets[t];
@@ -353,12 +355,13 @@ __global__ __launch_bounds__(64) void k_cluster(ClusterArgs a) {
             sane = false;
         }
         if (lane == 0) {
-            if (a.cand_total) atomicAdd(a.cand_total, (unsigned long long)total);
+            cand_acc += (unsigned long long)total;
             a.cnt[t] = sane ? min(total, a.max_n) : 0;
             if (!sane) atomicOr(a.overflow, 2u);
         }
         __syncthreads();
     }
+    if (lane == 0 && a.cand_total && cand_acc != 0) atomicAdd(counter_line(a.cand_total), cand_acc);
 }
 
 }  // namespace

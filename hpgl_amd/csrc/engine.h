@@ -83,8 +83,27 @@ struct IndJob {
     uint8_t *d_out_mask;
 };
 
+// R realisations of sequential Gaussian simulation on device buffers.
+struct SgsJob {
+    int grid[3];
+    CovSpec cov;
+    int radii[3];
+    int max_n;
+    bool ordinary;              // OK system (kriging kind 0, no LVM); SK with mean 0 otherwise
+    std::vector<long> seeds;    // one per realisation
+    const float *d_in;          // conditioning values (original units)
+    const uint8_t *d_in_mask;
+    const float *d_lvm = nullptr;       // per-cell mean (original units) or nullptr
+    const uint8_t *d_sim_mask = nullptr;  // cells with value 1 are simulated; nullptr = all
+    bool has_cdf = false;
+    std::vector<float> cdf_values, cdf_probs;
+    float *d_out;               // [R][N]
+    uint8_t *d_out_mask;        // [R][N]
+};
+
 void run_cont(const ContJob &job, hipStream_t s, int precision, const Dump *dump, Stats &st);
 void run_ind(const IndJob &job, hipStream_t s, int precision, const Dump *dump, Stats &st);
+void run_sgs(const SgsJob &job, hipStream_t s, int precision, const Dump *dump, Stats &st);
 
 // Device memory held by the library between calls (size-class cache).
 void *pool_alloc(size_t bytes);

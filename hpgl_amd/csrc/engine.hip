@@ -182,6 +182,10 @@ void upload(DBuf<T> &d, const std::vector<T> &h, hipStream_t s) {
     if (!h.empty()) hip_check(hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s), "upload");
 }
 
+// Largest covariance table built for the analytic model (entries, 256 MiB
+// of doubles); beyond it the solves evaluate the model on the device.
+constexpr long long kMaxPairTable = 1ll << 25;
+
 int solve_group(int max_n) {
     int g = 8;
     while (g < max_n) g <<= 1;
@@ -211,6 +215,9 @@ void validate_search(const int grid[3], const int radii[3], int max_n) {
         n *= grid[i];
     }
     if (n >= (1ll << 31)) throw std::runtime_error("grid has more than 2^31-1 cells");
+    long long box = 1;
+    for (int i = 0; i < 3; ++i) box *= 2ll * radii[i] + 1;
+    if (box >= (1ll << 31)) throw std::runtime_error("search radiuses too large for the covariance table");
     if (max_n < 0 || max_n > kMaxNeighbours)
         throw std::runtime_error("max_neighbours must be in [0, " + std::to_string(kMaxNeighbours) + "]");
 }
@@ -266,7 +273,6 @@ void search(SearchSetup &ss, const int grid[3], const int radii[3], int max_n, c
     ScanArgs sa;
     sa.mask = d_mask;
     sa.key = nullptr;
-    sa.target_key = nullptr;
     sa.X = grid[0];
     sa.Y = grid[1];
     sa.Z = grid[2];
@@ -367,9 +373,9 @@ void search(SearchSetup &ss, const int grid[3], const int radii[3], int max_n, c
             ca.scratch = nullptr;
         }
         DBuf<unsigned int> overflow(1);
-        DBuf<unsigned long long> cand(1);
+        DBuf<unsigned long long> cand(kCounterWords);
         hip_check(hipMemsetAsync(overflow.p, 0, sizeof(unsigned int), s), "memset");
-        hip_check(hipMemsetAsync(cand.p, 0, sizeof(unsigned long long), s), "memset");
+        hip_check(hipMemsetAsync(cand.p, 0, kCounterWords * sizeof(unsigned long long), s), "memset");
         ca.overflow = overflow.p;
         ca.cand_total = cand.p;
         {
@@ -379,12 +385,13 @@ void search(SearchSetup &ss, const int grid[3], const int radii[3], int max_n, c
             st.ms_k_cluster += kt.stop();
             st.n_k_cluster += 1;
         }
-        unsigned long long nc = 0;
-        hip_check(hipMemcpyAsync(&nc, cand.p, sizeof(nc), hipMemcpyDeviceToHost, s), "copy cand");
+        std::vector<unsigned long long> hc(kCounterWords);
+        hip_check(hipMemcpyAsync(hc.data(), cand.p, kCounterWords * sizeof(unsigned long long), hipMemcpyDeviceToHost, s),
+                  "copy cand");
         unsigned int ov = 0;
         hip_check(hipMemcpyAsync(&ov, overflow.p, sizeof(ov), hipMemcpyDeviceToHost, s), "copy overflow");
         hip_check(hipStreamSynchronize(s), "sync");
-        st.cluster_candidates += (long long)nc;
+        for (int l = 0; l < kCounterSlots; ++l) st.cluster_candidates += (long long)hc[(size_t)l * kCounterStride];
         if (ov & 1u) throw std::runtime_error("cluster search: candidate list exceeded the scratch capacity");
         if (ov & 2u) throw std::runtime_error("cluster search: internal error in the sort replay");
     }
@@ -455,10 +462,10 @@ void solve_fp64_bucket(SolveArgs sa, const DBuf<uint8_t> &keys, uint8_t k, bool 
     sa.n_subset = n;
     const int order = k == 8 ? 64 : 4 * (k + 1);
     KTimer kt(s);
-    if (order <= kQuadMaxFp64)
+    if (order <= kQuadMaxFp64 && sa.cov.table)
         launch_solve_quad(sa, order, true, s);
     else
-        launch_solve_fp64(sa, order, s);
+        launch_solve_fp64(sa, order <= 32 ? 32 : 64, s);
     hip_check(hipGetLastError(), "solve fp64");
     st.ms_k_solve_fp64 += kt.stop();
     st.n_k_solve_fp64 += 1;
@@ -486,10 +493,10 @@ void solve(SolveArgs sa, int epi, int precision, DBuf<uint8_t> &status, hipStrea
             b.subset = list.p;
             b.n_subset = n;
             KTimer kt(s);
-            if (k < 8)
+            if (k < 8 && sa.cov.table)
                 launch_solve_quad(b, 4 * (k + 1), false, s);
             else
-                launch_solve_mixed(b, 64, s);
+                launch_solve_mixed(b, k < 8 ? 32 : 64, s);
             hip_check(hipGetLastError(), "solve mixed");
             st.ms_k_solve_mixed += kt.stop();
             st.n_k_solve_mixed += 1;
@@ -501,9 +508,13 @@ void solve(SolveArgs sa, int epi, int precision, DBuf<uint8_t> &status, hipStrea
 }
 
 void collect_counters(const unsigned long long *d_counters, hipStream_t s, Stats &st) {
-    unsigned long long c[4];
-    hip_check(hipMemcpyAsync(c, d_counters, sizeof(c), hipMemcpyDeviceToHost, s), "counters");
+    std::vector<unsigned long long> h(kCounterWords);
+    hip_check(hipMemcpyAsync(h.data(), d_counters, kCounterWords * sizeof(unsigned long long), hipMemcpyDeviceToHost, s),
+              "counters");
     hip_check(hipStreamSynchronize(s), "sync");
+    unsigned long long c[4] = {0, 0, 0, 0};
+    for (int l = 0; l < kCounterSlots; ++l)
+        for (int w = 0; w < 4; ++w) c[w] += h[(size_t)l * kCounterStride + w];
     st.solves_fp32 += c[0];
     st.solves_fp64 += c[1];
     st.nodes_kriged += c[0] + c[1];
@@ -555,8 +566,8 @@ void run_cont(const ContJob &job, hipStream_t s, int precision, const Dump *dump
     hip_check(hipEventRecord(ev.e[2], s), "event");
     write_dump(dump, 0, targets.p, T, nbr.p, cnt.p, job.max_n, s);
 
-    DBuf<unsigned long long> counters(4);
-    hip_check(hipMemsetAsync(counters.p, 0, 4 * sizeof(unsigned long long), s), "memset");
+    DBuf<unsigned long long> counters(kCounterWords);
+    hip_check(hipMemsetAsync(counters.p, 0, kCounterWords * sizeof(unsigned long long), s), "memset");
     DBuf<uint8_t> status(T);
     SolveArgs sa;
     std::memset(&sa, 0, sizeof(sa));
@@ -568,7 +579,31 @@ void run_cont(const ContJob &job, hipStream_t s, int precision, const Dump *dump
     sa.X = job.grid[0];
     sa.Y = job.grid[1];
     sa.Z = job.grid[2];
-    sa.cov = make_devcov(ss.model, ss.use_table ? &ss.table : nullptr, ss.d_table.p);
+    // OK covariances are the analytic model (ordinary_kriging.cpp); the solve
+    // kernels read them from a table over every offset a neighbour pair can
+    // take: neighbours lie within 2r-1 of the target per axis (cluster path;
+    // r on the template path), so pairs within 4r-2.
+    BoxTable pair_table;
+    DBuf<double> d_pair;
+    if (job.ordinary) {
+        int pr[3];
+        long long entries = 1;
+        for (int i = 0; i < 3; ++i) {
+            pr[i] = std::max(2 * job.radii[i], 4 * job.radii[i] - 2);
+            entries *= 2 * pr[i] + 1;
+        }
+        if (entries <= kMaxPairTable) {
+            const double t0 = now_ms();
+            pair_table.build(ss.model, pr);
+            st.ms_setup_host += now_ms() - t0;
+            upload(d_pair, pair_table.v, s);
+            sa.cov = make_devcov(ss.model, &pair_table, d_pair.p);
+        } else {
+            sa.cov = make_devcov(ss.model, nullptr, nullptr);
+        }
+    } else {
+        sa.cov = make_devcov(ss.model, &ss.table, ss.d_table.p);
+    }
     sa.ok_system = job.ordinary ? 1 : 0;
     sa.values_f = job.d_in;
     sa.mean_kind = job.mean_kind;
@@ -610,8 +645,8 @@ void run_ind(const IndJob &job, hipStream_t s, int precision, const Dump *dump, 
     st.nodes_target += (long long)T * K;
 
     DBuf<float> probs(job.median ? 1 : (size_t)K * N);
-    DBuf<unsigned long long> counters(4);
-    hip_check(hipMemsetAsync(counters.p, 0, 4 * sizeof(unsigned long long), s), "memset");
+    DBuf<unsigned long long> counters(kCounterWords);
+    hip_check(hipMemsetAsync(counters.p, 0, kCounterWords * sizeof(unsigned long long), s), "memset");
     DBuf<uint8_t> status(T);
     int maxn_all = 1;
     for (const auto &c : job.cats) maxn_all = std::max(maxn_all, c.max_n);
@@ -665,6 +700,326 @@ void run_ind(const IndJob &job, hipStream_t s, int precision, const Dump *dump, 
         (void)hipEventDestroy(b1);
     }
     if (!job.median) launch_ik_argmax(probs.p, K, N, job.d_out, job.d_out_mask, s);
+    hip_check(hipEventRecord(ev.e[5], s), "event");
+    collect_counters(counters.p, s, st);
+    hip_check(hipStreamSynchronize(s), "sync");
+    st.ms_device_total += ev.ms(0, 5);
+    st.ms_wall += now_ms() - w0;
+}
+
+// ---------------------------------------------------------------------------
+// Sequential Gaussian simulation (sequential_gaussian_simulation.cpp:18-128,
+// sequential_simulation.h:51-110), R realisations per call.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+bool is_prime(long long p) {
+    if (p < 2) return false;
+    for (long long d = 2; d * d <= p; ++d)
+        if (p % d == 0) return false;
+    return true;
+}
+
+// Path LCG of path_random_generator.cpp:21-56 for N cells: A = 1 + the
+// product of the distinct primes dividing N (doubled when 4 | N); C = the
+// largest prime <= N/3 that does not divide N, else 2.  The path visits
+// every cell exactly once iff C does not divide N (Hull-Dobell: A - 1 is a
+// multiple of every prime factor of N, and of 4 when 4 | N).
+struct Lcg {
+    long long A, C;
+    bool full;
+};
+Lcg lcg_for(long long N) {
+    long long prod = 1, n = N;
+    for (long long p = 2; p * p <= n; ++p)
+        if (n % p == 0) {
+            prod *= p;
+            while (n % p == 0) n /= p;
+        }
+    if (n > 1) prod *= n;
+    if (N % 4 == 0) prod *= 2;
+    Lcg l;
+    l.A = prod + 1;
+    l.C = 2;
+    for (long long p = N / 3; p >= 2; --p)
+        if (is_prime(p) && N % p != 0) {
+            l.C = p;
+            break;
+        }
+    l.full = N == 1 || N % l.C != 0;
+    return l;
+}
+
+// The reference's loop runs N steps of the LCG and skips cells already
+// visited; when the period is short some cells are never visited.  Host
+// restatement for that case (grids of a few cells): path[k] = cell at step
+// k or -1 for a repeat, rank[cell] = first step or INT_MAX.
+void host_path(long long N, const Lcg &l, long long r0, int *path, int *rank) {
+    for (long long c = 0; c < N; ++c) rank[c] = 0x7fffffff;
+    long long R = r0;
+    for (long long k = 0; k < N; ++k) {
+        if (rank[R] == 0x7fffffff) {
+            rank[R] = (int)k;
+            path[k] = (int)R;
+        } else {
+            path[k] = -1;
+        }
+        R = (l.A * R + l.C) % N;
+    }
+}
+
+struct SgsTargetPred {
+    const int *key;
+    __host__ __device__ bool operator()(const int &cell) const {
+        if (cell < 0) return false;
+        const int k = key[cell];
+        return k >= 0 && k != 0x7fffffff;
+    }
+};
+
+// Cells of path[0..N) that are simulated, in path order.
+int select_path_targets(const int *path, long long N, const int *key, int *out, hipStream_t s) {
+    DBuf<int> d_num(1);
+    size_t bytes = 0;
+    hip_check(hipcub::DeviceSelect::If(nullptr, bytes, path, out, d_num.p, (int)N, SgsTargetPred{key}, s), "select");
+    DBuf<uint8_t> tmp(bytes);
+    hip_check(hipcub::DeviceSelect::If(tmp.p, bytes, path, out, d_num.p, (int)N, SgsTargetPred{key}, s), "select");
+    int num = 0;
+    hip_check(hipMemcpyAsync(&num, d_num.p, sizeof(int), hipMemcpyDeviceToHost, s), "copy count");
+    hip_check(hipStreamSynchronize(s), "sync");
+    return num;
+}
+
+__global__ void k_sgs_out_mask(const int *key, long long n, uint8_t *mask) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) mask[i] = key[i] != 0x7fffffff ? 1 : 0;
+}
+
+}  // namespace
+
+void run_sgs(const SgsJob &job, hipStream_t s, int precision, const Dump *dump, Stats &st) {
+    const double w0 = now_ms();
+    validate_search(job.grid, job.radii, job.max_n);
+    validate_cov(job.cov);
+    const int R = (int)job.seeds.size();
+    if (R < 1) throw std::runtime_error("sequential simulation needs at least one seed");
+    if (job.has_cdf && (job.cdf_values.empty() || job.cdf_values.size() != job.cdf_probs.size()))
+        throw std::runtime_error("the non-parametric cdf must have values and probabilities of one positive length");
+    const long long N = (long long)job.grid[0] * job.grid[1] * job.grid[2];
+    const int max_n = std::max(job.max_n, 1);
+    Events ev;
+    hip_check(hipEventRecord(ev.e[0], s), "event");
+
+    const Lcg lcg = lcg_for(N);
+    std::vector<long long> h_A(R, lcg.A), h_C(R, lcg.C), h_R0(R);
+    std::vector<uint32_t> h_mt_seed(R);
+    for (int r = 0; r < R; ++r) {
+        // PathGen takes the seed through int (sequential_gaussian_simulation.cpp:45)
+        const long long r0 = (long long)(int)job.seeds[r] % N;
+        if (r0 < 0) throw std::runtime_error("the seed, taken as int, must not be negative");
+        h_R0[r] = r0;
+        h_mt_seed[r] = (uint32_t)job.seeds[r];
+    }
+
+    // normal-score transform of the conditioning data and of the LVM mean
+    DBuf<float> cv, cp, data((size_t)N), lvm;
+    hip_check(hipMemcpyAsync(data.p, job.d_in, N * sizeof(float), hipMemcpyDeviceToDevice, s), "copy data");
+    if (job.d_lvm) {
+        lvm.alloc((size_t)N);
+        hip_check(hipMemcpyAsync(lvm.p, job.d_lvm, N * sizeof(float), hipMemcpyDeviceToDevice, s), "copy mean");
+    }
+    if (job.has_cdf) {
+        upload(cv, job.cdf_values, s);
+        upload(cp, job.cdf_probs, s);
+        const long long m = (long long)job.cdf_values.size();
+        launch_cdf_forward(data.p, job.d_in_mask, N, cv.p, cp.p, m, s);
+        if (job.d_lvm) launch_cdf_forward(lvm.p, nullptr, N, cv.p, cp.p, m, s);
+        hip_check(hipGetLastError(), "cdf forward");
+    }
+
+    // covariance table and search template (the reference's SGS searches
+    // with the template only, sequential_simulation.h:74)
+    SearchSetup ss(job.cov);
+    ss.use_table = true;
+    double t0 = now_ms();
+    ss.table.build(ss.model, job.radii);
+    build_template(ss.model, &ss.table, job.radii, ss.tmpl);
+    st.ms_setup_host += now_ms() - t0;
+    upload(ss.d_table, ss.table.v, s);
+    upload(ss.d_tmpl, ss.tmpl.xyz, s);
+
+    // realisations per chunk, from free device memory
+    size_t free_b = 0, total_b = 0;
+    hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+    const size_t per_real = (size_t)N * 17 + (size_t)N * (12 * (size_t)max_n + 32);
+    int RC = (int)std::max<size_t>(1, std::min<size_t>((size_t)R, (size_t)(free_b * 0.6) / per_real));
+
+    DBuf<long long> d_A, d_C, d_R0;
+    DBuf<uint32_t> d_mt_seed;
+    upload(d_A, h_A, s);
+    upload(d_C, h_C, s);
+    upload(d_R0, h_R0, s);
+    upload(d_mt_seed, h_mt_seed, s);
+    DBuf<unsigned long long> counters(kCounterWords);
+    hip_check(hipMemsetAsync(counters.p, 0, kCounterWords * sizeof(unsigned long long), s), "memset");
+    DBuf<unsigned int> err(1);
+
+    for (int r0 = 0; r0 < R; r0 += RC) {
+        const int rc = std::min(RC, R - r0);
+        DBuf<int> path((size_t)rc * N), rank((size_t)rc * N), key((size_t)rc * N);
+        if (lcg.full) {
+            SgsPathArgs pa;
+            pa.N = N;
+            pa.A = d_A.p + r0;
+            pa.C = d_C.p + r0;
+            pa.R0 = d_R0.p + r0;
+            pa.path = path.p;
+            pa.rank = rank.p;
+            launch_lcg_path(pa, rc, s);
+        } else {
+            std::vector<int> hp((size_t)rc * N), hr((size_t)rc * N);
+            for (int r = 0; r < rc; ++r) host_path(N, lcg, h_R0[r0 + r], hp.data() + (size_t)r * N, hr.data() + (size_t)r * N);
+            hip_check(hipMemcpyAsync(path.p, hp.data(), hp.size() * sizeof(int), hipMemcpyHostToDevice, s), "upload");
+            hip_check(hipMemcpyAsync(rank.p, hr.data(), hr.size() * sizeof(int), hipMemcpyHostToDevice, s), "upload");
+            hip_check(hipStreamSynchronize(s), "sync");
+        }
+        launch_sgs_key(job.d_in_mask, job.d_sim_mask, rank.p, N, rc, key.p, s);
+        hip_check(hipGetLastError(), "path kernels");
+        rank.release();
+
+        // simulated cells of each realisation in path order
+        std::vector<int> h_T(rc);
+        std::vector<long long> h_off(rc);
+        DBuf<int> targets((size_t)rc * N);
+        long long tot = 0;
+        for (int r = 0; r < rc; ++r) {
+            h_off[r] = tot;
+            h_T[r] = select_path_targets(path.p + (size_t)r * N, N, key.p + (size_t)r * N, targets.p + tot, s);
+            tot += h_T[r];
+        }
+        path.release();
+        if (tot >= (1ll << 31)) throw std::runtime_error("too many simulated cells in one batch");
+        st.nodes_target += tot;
+        int max_T = 0;
+        for (int r = 0; r < rc; ++r) max_T = std::max(max_T, h_T[r]);
+
+        // neighbours of every target among the cells informed before it
+        DBuf<int> nbr((size_t)std::max<long long>(tot, 1) * max_n), cnt((size_t)std::max<long long>(tot, 1));
+        hip_check(hipEventRecord(ev.e[1], s), "event");
+        for (int r = 0; r < rc; ++r) {
+            if (h_T[r] == 0) continue;
+            ScanArgs sa;
+            sa.mask = nullptr;
+            sa.key = key.p + (size_t)r * N;
+            sa.X = job.grid[0];
+            sa.Y = job.grid[1];
+            sa.Z = job.grid[2];
+            sa.tmpl = (const int4 *)ss.d_tmpl.p;
+            sa.tsize = ss.tmpl.size();
+            sa.max_n = job.max_n;
+            sa.targets = targets.p + h_off[r];
+            sa.list = nullptr;
+            sa.n_list = h_T[r];
+            sa.nbr = nbr.p + (size_t)h_off[r] * max_n;
+            sa.cnt = cnt.p + h_off[r];
+            launch_template_scan(sa, 1, s);
+            hip_check(hipGetLastError(), "template scan");
+            st.nodes_template_path += h_T[r];
+        }
+        hip_check(hipEventRecord(ev.e[2], s), "event");
+        hip_check(hipEventSynchronize(ev.e[2]), "sync");
+        st.ms_k_template += ev.ms(1, 2);
+        st.ms_search += ev.ms(1, 2);
+        st.n_k_template += rc;
+        if (dump)
+            for (int r = 0; r < rc; ++r)
+                write_dump(dump, (long long)(r0 + r) * N, targets.p + h_off[r], h_T[r],
+                           nbr.p + (size_t)h_off[r] * max_n, cnt.p + h_off[r], max_n, s);
+
+        // weights and variances
+        DBuf<double> w((size_t)std::max<long long>(tot, 1) * max_n), var((size_t)std::max<long long>(tot, 1));
+        DBuf<uint8_t> status((size_t)std::max<long long>(tot, 1));
+        if (tot > 0) {
+            SolveArgs sa;
+            std::memset(&sa, 0, sizeof(sa));
+            sa.targets = targets.p;
+            sa.T = (int)tot;
+            sa.nbr = nbr.p;
+            sa.cnt = cnt.p;
+            sa.max_n = max_n;
+            sa.X = job.grid[0];
+            sa.Y = job.grid[1];
+            sa.Z = job.grid[2];
+            sa.cov = make_devcov(ss.model, &ss.table, ss.d_table.p);
+            sa.ok_system = job.ordinary ? 1 : 0;
+            sa.mean_kind = 0;
+            sa.out_var = var.p;
+            sa.out_w = w.p;
+            sa.counters = counters.p;
+            hip_check(hipEventRecord(ev.e[3], s), "event");
+            solve(sa, EPI_SGS, precision, status, s, st);
+            hip_check(hipEventRecord(ev.e[4], s), "event");
+            hip_check(hipEventSynchronize(ev.e[4]), "sync");
+            st.ms_solve += ev.ms(3, 4);
+        }
+
+        // draws: one per simulated cell, in path order
+        DBuf<uint32_t> draws((size_t)rc * std::max(max_T, 1));
+        DBuf<long long> d_counts;
+        std::vector<long long> h_counts(h_T.begin(), h_T.end());
+        upload(d_counts, h_counts, s);
+        launch_mt11213b(d_mt_seed.p + r0, d_counts.p, std::max(max_T, 1), rc, draws.p, s);
+
+        // values: conditioning data everywhere, then the ordered sweep
+        float *vals = job.d_out + (size_t)r0 * N;
+        for (int r = 0; r < rc; ++r)
+            hip_check(hipMemcpyAsync(vals + (size_t)r * N, data.p, N * sizeof(float), hipMemcpyDeviceToDevice, s),
+                      "copy data");
+        DBuf<int> done((size_t)rc * N);
+        hip_check(hipMemsetAsync(done.p, 0, (size_t)rc * N * sizeof(int), s), "memset");
+        DBuf<unsigned int> ticket(rc);
+        hip_check(hipMemsetAsync(ticket.p, 0, rc * sizeof(unsigned int), s), "memset");
+        hip_check(hipMemsetAsync(err.p, 0, sizeof(unsigned int), s), "memset");
+        DBuf<int> d_T;
+        DBuf<long long> d_off;
+        upload(d_T, h_T, s);
+        upload(d_off, h_off, s);
+        if (tot > 0) {
+            SgsSweepArgs a;
+            a.N = N;
+            a.max_n = max_n;
+            a.T = d_T.p;
+            a.t_off = d_off.p;
+            a.targets = targets.p;
+            a.nbr = nbr.p;
+            a.cnt = cnt.p;
+            a.w = w.p;
+            a.var = var.p;
+            a.status = status.p;
+            a.draws = draws.p;
+            a.draw_stride = std::max(max_T, 1);
+            a.key = key.p;
+            a.lvm = job.d_lvm ? lvm.p : nullptr;
+            a.values = vals;
+            a.done = done.p;
+            a.ticket = ticket.p;
+            a.error = err.p;
+            a.spin_limit = 1ll << 24;
+            const int waves = (int)std::max<long long>(1, std::min<long long>((max_T + 63) / 64, 8192 / rc));
+            launch_sgs_sweep(a, rc, waves, s);
+            hip_check(hipGetLastError(), "sweep");
+        }
+        hipLaunchKernelGGL(k_sgs_out_mask, dim3((unsigned)(((long long)rc * N + 255) / 256)), dim3(256), 0, s, key.p,
+                           (long long)rc * N, job.d_out_mask + (size_t)r0 * N);
+        if (job.has_cdf)
+            launch_cdf_back(vals, job.d_out_mask + (size_t)r0 * N, (long long)rc * N, cv.p, cp.p,
+                            (long long)job.cdf_values.size(), s);
+        unsigned int h_err = 0;
+        hip_check(hipMemcpyAsync(&h_err, err.p, sizeof(h_err), hipMemcpyDeviceToHost, s), "copy error");
+        hip_check(hipStreamSynchronize(s), "sync");
+        if (h_err) throw std::runtime_error("sequential simulation: a cell waited on its neighbours past the limit");
+    }
     hip_check(hipEventRecord(ev.e[5], s), "event");
     collect_counters(counters.p, s, st);
     hip_check(hipStreamSynchronize(s), "sync");

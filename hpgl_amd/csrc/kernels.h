@@ -38,7 +38,7 @@ enum EpilogueKind : int {
     EPI_CONT = 0,    // OK/SK/LVM: write estimate (or failure policy) to out
     EPI_IK = 1,      // indicator kriging: write the category probability
     EPI_MEDIAN = 2,  // median IK: double accumulation, threshold at 0.5
-    EPI_SGS = 3,     // SGS: write the kriged mean and variance
+    EPI_SGS = 3,     // SGS: write the weights and the kriging variance
 };
 
 struct SolveArgs {
@@ -63,7 +63,8 @@ struct SolveArgs {
     float *out_f;              // EPI_CONT estimate per cell; EPI_IK prob per target; EPI_SGS mean per target
     uint8_t *out_mask;         // EPI_CONT, EPI_MEDIAN
     uint8_t *out_u8;           // EPI_MEDIAN
-    double *out_var;           // EPI_SGS variance per target
+    double *out_var;           // EPI_SGS variance per target (-1 on failure)
+    double *out_w;             // EPI_SGS weights per target [T][max_n]
     uint8_t *status;
     unsigned long long *counters;  // [0] fp32 accepted, [1] fp64 solved, [2] singular, [3] no neighbours
     // list of target positions to solve (nullptr = all T)
@@ -84,10 +85,12 @@ void launch_solve_quad(const SolveArgs &a, int order, bool fp64, hipStream_t s);
 // Template scan (sugarbox_neighbour_lookup.h:36-61) over targets[list[i]].
 // pred_kind 0: informed iff mask[cell] != 0.
 // pred_kind 1 (SGS): informed iff key[cell] < key_of_target[i].
+// pred_kind 0: cell informed iff mask[cell] != 0.
+// pred_kind 1: cell informed iff key[cell] < key[target] (the target's own
+// key is its path rank in sequential simulation).
 struct ScanArgs {
     const uint8_t *mask;
     const int32_t *key;
-    const int32_t *target_key;
     int X, Y, Z;
     const int4 *tmpl;
     int tsize;
@@ -141,6 +144,51 @@ struct ClusterArgs {
     unsigned long long *cand_total;  // += candidates gathered per target
 };
 void launch_cluster_search(const ClusterArgs &a, hipStream_t s);
+
+// Sequential Gaussian simulation (sgs.hip), R realisations at once.
+struct SgsPathArgs {
+    long long N;
+    const long long *A, *C, *R0;  // LCG per realisation (path_random_generator.cpp:21-56)
+    int *path;                    // [R][N] cell at each path position
+    int *rank;                    // [R][N] path position of each cell
+};
+void launch_lcg_path(const SgsPathArgs &p, int R, hipStream_t s);
+void launch_sgs_key(const uint8_t *hard, const uint8_t *sim_mask, const int *rank, long long N, int R, int *key,
+                    hipStream_t s);
+void launch_mt11213b(const uint32_t *seeds, const long long *counts, long long stride, int R, uint32_t *out,
+                     hipStream_t s);
+void launch_cdf_forward(float *v, const uint8_t *mask, long long n, const float *cv, const float *cp, long long m,
+                        hipStream_t s);
+void launch_cdf_back(float *v, const uint8_t *mask, long long n, const float *cv, const float *cp, long long m,
+                     hipStream_t s);
+struct SgsSweepArgs {
+    long long N;
+    int max_n;
+    const int *T;          // [R] targets per realisation
+    const long long *t_off;  // [R] offset of each realisation's targets
+    const int *targets;    // cells in path order
+    const int *nbr;
+    const int *cnt;
+    const double *w;
+    const double *var;
+    const uint8_t *status;  // ST_OK when the system was solved
+    const uint32_t *draws;  // [R][draw_stride]
+    long long draw_stride;
+    const int *key;         // [R][N]
+    const float *lvm;       // per-cell mean (transformed) or nullptr
+    float *values;          // [R][N]
+    int *done;              // [R][N], 1 once the cell's value is written
+    unsigned int *ticket;   // [R]
+    unsigned int *error;
+    long long spin_limit;
+};
+void launch_sgs_sweep(const SgsSweepArgs &a, int R, int waves_per_realisation, hipStream_t s);
+
+// Statistics counters (counters.h): kCounterSlots lines of kCounterStride
+// words; word w of every line accumulates counter w.
+constexpr int kCounterSlots = 1024;
+constexpr int kCounterStride = 8;
+constexpr int kCounterWords = kCounterSlots * kCounterStride;
 
 // Small helpers.
 void launch_copy_informed(const float *in, const uint8_t *in_mask, long long n, float *out, uint8_t *out_mask,

@@ -25,7 +25,7 @@ __global__ __launch_bounds__(256) void k_template_scan(ScanArgs a) {
     const int node = active ? a.targets[t] : 0;
     const int X = a.X, Y = a.Y, Z = a.Z;
     const int x = node % X, y = node % (X * Y) / X, z = node / (X * Y);
-    const int32_t my_key = (PRED == 1 && active) ? a.target_key[t] : 0;
+    const int32_t my_key = (PRED == 1 && active) ? a.key[node] : 0;
     int count = 0;
     int *out = a.nbr + (size_t)t * a.max_n;
     const int max_n = a.max_n;

@@ -1,23 +1,307 @@
-// sgs.hip - sequential Gaussian simulation entry points.
+// sgs.hip - device kernels of sequential Gaussian simulation
+// (sequential_gaussian_simulation.cpp:18-128, sequential_simulation.h:51-110).
+//
+// The reference visits the cells along a random path (a full-period LCG
+// over all cells, path_random_generator.cpp:21-94) and, at each cell,
+// searches neighbours among the cells informed so far, solves the SK/OK
+// system with its variance, draws u from mt11213b and writes
+// mean + sqrt(var) * Phi^-1(u).  Which cells are informed when a cell is
+// visited depends only on the path: hard data, plus simulated cells of
+// smaller path rank.  So every part except the value itself runs in
+// parallel over all cells of all realisations:
+//   * k_lcg_path: path rank of every cell (LCG jump-ahead per chunk);
+//   * k_sgs_key: key = -1 (hard data), rank (cell to simulate) or INT_MAX
+//     (never informed); cell c is informed for the target of rank p iff
+//     key[c] < p, which the template scan tests;
+//   * k_mt11213b: the draws, one workgroup per realisation, 351-word
+//     twists in three parallel phases;
+//   * weights and variances by the solve kernels (EPI_SGS);
+//   * k_sgs_sweep: the values, in path order.  Each wavefront claims 64
+//     consecutive path positions from a per-realisation ticket and writes
+//     a cell once all its simulated neighbours (all of smaller rank,
+//     claimed earlier by running wavefronts) are marked done.  The wait is
+//     bounded: past the limit the kernel sets an error flag and exits.
+#include "kernels.h"
+
 #include <hip/hip_runtime.h>
 
-#include <stdexcept>
+namespace hpgl_amd {
 
-#include "../../include/hpgl.h"
+namespace {
 
-extern "C" __attribute__((visibility("default"))) void hpgl_amd_report_error(const char *msg);
-
-extern "C" __attribute__((visibility("default"))) void hpgl_sgs_simulation(hpgl_cont_masked_array_t *,
-                                                                           hpgl_sgs_params_t *,
-                                                                           hpgl_non_parametric_cdf_t *, double *,
-                                                                           hpgl_ubyte_array_t *) {
-    hpgl_amd_report_error("hpgl_sgs_simulation: not available in this build");
+// (a, c) o (a, c): x -> a (a x + c) + c, modulo m
+__device__ inline void affine_sq(unsigned long long &a, unsigned long long &c, unsigned long long m) {
+    c = (a * c + c) % m;
+    a = (a * a) % m;
 }
 
-extern "C" __attribute__((visibility("default"))) void hpgl_sgs_lvm_simulation(hpgl_cont_masked_array_t *,
-                                                                               hpgl_sgs_params_t *,
-                                                                               hpgl_non_parametric_cdf_t *,
-                                                                               hpgl_float_array_t *,
-                                                                               hpgl_ubyte_array_t *) {
-    hpgl_amd_report_error("hpgl_sgs_lvm_simulation: not available in this build");
+__global__ void k_lcg_path(SgsPathArgs p) {
+    const long long chunk = 256;
+    const long long start = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * chunk;
+    const int r = blockIdx.y;
+    if (start >= p.N) return;
+    const unsigned long long M = (unsigned long long)p.N;
+    const unsigned long long A = (unsigned long long)p.A[r] % M, C = (unsigned long long)p.C[r] % M;
+    // R_start = f^start(R_0) by squaring the affine map
+    unsigned long long R = (unsigned long long)p.R0[r];
+    unsigned long long a = A, c = C;
+    for (long long e = start; e > 0; e >>= 1) {
+        if (e & 1) R = (a * R + c) % M;
+        affine_sq(a, c, M);
+    }
+    int *path = p.path + (size_t)r * p.N;
+    int *rank = p.rank + (size_t)r * p.N;
+    const long long end = start + chunk < p.N ? start + chunk : p.N;
+    for (long long k = start; k < end; ++k) {
+        path[k] = (int)R;
+        rank[R] = (int)k;
+        R = (A * R + C) % M;
+    }
 }
+
+__global__ void k_sgs_key(const uint8_t *hard, const uint8_t *sim_mask, const int *rank, long long N, int R,
+                          int *key) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N * R) return;
+    const long long c = i % N;
+    int k;
+    if (hard[c] != 0)
+        k = -1;
+    else if (sim_mask && sim_mask[c] != 1)
+        k = 0x7fffffff;
+    else
+        k = rank[i];
+    key[i] = k;
+}
+
+// std::mersenne_twister_engine<uint32_t, 32, 351, 175, 19, 0xccab8ee7, 11,
+// 0xffffffff, 7, 0x31b6ab00, 15, 0xffe50000, 17, 1812433253> (mt11213b)
+constexpr int kMtN = 351, kMtM = 175;
+constexpr uint32_t kMtA = 0xccab8ee7u, kUpper = 0xfff80000u, kLower = 0x0007ffffu;
+
+__device__ inline uint32_t mt_mix(uint32_t hi_word, uint32_t lo_word) {
+    const uint32_t y = (hi_word & kUpper) | (lo_word & kLower);
+    return (y >> 1) ^ ((y & 1u) ? kMtA : 0u);
+}
+
+__global__ __launch_bounds__(384) void k_mt11213b(const uint32_t *seeds, const long long *counts, long long stride,
+                                                 uint32_t *out) {
+    __shared__ uint32_t x[kMtN];
+    const int r = blockIdx.x;
+    const int k = threadIdx.x;
+    const long long count = counts[r];
+    uint32_t *o = out + (size_t)r * stride;
+    if (k == 0) {
+        uint32_t v = seeds[r];
+        x[0] = v;
+        for (int i = 1; i < kMtN; ++i) {
+            v = 1812433253u * (v ^ (v >> 30)) + (uint32_t)i;
+            x[i] = v;
+        }
+    }
+    __syncthreads();
+    for (long long base = 0; base < count; base += kMtN) {
+        // phase A: k < n-m, old values only
+        uint32_t nv = 0;
+        if (k < kMtN - kMtM) nv = x[k + kMtM] ^ mt_mix(x[k], x[k + 1]);
+        __syncthreads();
+        if (k < kMtN - kMtM) x[k] = nv;
+        __syncthreads();
+        // phase B: n-m <= k < n-1, x[k+m-n] from phase A
+        if (k >= kMtN - kMtM && k < kMtN - 1) nv = x[k + kMtM - kMtN] ^ mt_mix(x[k], x[k + 1]);
+        __syncthreads();
+        if (k >= kMtN - kMtM && k < kMtN - 1) x[k] = nv;
+        __syncthreads();
+        if (k == 0) x[kMtN - 1] = x[kMtM - 1] ^ mt_mix(x[kMtN - 1], x[0]);
+        __syncthreads();
+        if (k < kMtN && base + k < count) {
+            uint32_t z = x[k];
+            z ^= (z >> 11) & 0xffffffffu;
+            z ^= (z << 7) & 0x31b6ab00u;
+            z ^= (z << 15) & 0xffe50000u;
+            z ^= (z >> 17);
+            o[base + k] = z;
+        }
+        __syncthreads();
+    }
+}
+
+// gaussian_cdf_t (gaussian_distribution.h:26-63)
+__device__ inline double gauss_prob(double x, double mean, double var) {
+    const double p = 0.2316419;
+    const double b0 = 0.319381530, b1 = -0.356563782, b2 = 1.781477937, b3 = -1.821255978, b4 = 1.330274429;
+    double sx = (x - mean) / sqrt(var);
+    const bool neg = sx < 0;
+    if (neg) sx = -sx;
+    const double t = 1 / (1 + p * sx);
+    const double zx = 1 / sqrt(2 * 3.14159265358) * exp(-sx * sx / 2);
+    double result = 1 - zx * ((((b4 * t + b3) * t + b2) * t + b1) * t + b0) * t;
+    if (neg) result = 1 - result;
+    return result;
+}
+__device__ inline double gauss_inverse(double p, double mean, double var) {
+    const double p0 = -0.322232431088, p1 = -1.0, p2 = -0.342242088547, p3 = -0.0204231210245,
+                 p4 = -0.453642210148e-4;
+    const double q0 = 0.0993484626060, q1 = 0.588581570495, q2 = 0.531103462366, q3 = 0.103537752850,
+                 q4 = 0.38560700634e-2;
+    const double pp = (p > 0.5) ? (1 - p) : p;
+    const double y = sqrt(log(1 / (pp * pp)));
+    double xp = y + ((((p4 * y + p3) * y + p2) * y + p1) * y + p0) / ((((q4 * y + q3) * y + q2) * y + q1) * y + q0);
+    if (p < 0.5) xp = -xp;
+    return xp * sqrt(var) + mean;
+}
+
+// non_parametric_cdf_2_t (non_parametric_cdf.h:181-262), float arithmetic
+__device__ inline int lower_bound_f(const float *a, long long n, float v) {
+    long long lo = 0, hi = n;
+    while (lo < hi) {
+        const long long mid = (lo + hi) / 2;
+        if (a[mid] < v)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return (int)lo;
+}
+__device__ inline float np_prob(const float *values, const float *probs, long long n, float value) {
+    const int it = lower_bound_f(values, n, value);
+    if (it == n) return probs[n - 1];
+    if (it == 0) return probs[0];
+    const float x1 = values[it - 1], x2 = values[it], y1 = probs[it - 1], y2 = probs[it];
+    return y1 + (y2 - y1) / (x2 - x1) * (value - x1);
+}
+__device__ inline float np_inverse(const float *values, const float *probs, long long n, float prob) {
+    if (n <= 0) return 0;
+    const int it = lower_bound_f(probs, n, prob);
+    if (it == 0) return values[0];
+    if (it == n) return values[n - 1];
+    const float x1 = values[it - 1], x2 = values[it], y1 = probs[it - 1], y2 = probs[it];
+    return x1 + (x2 - x1) / (y2 - y1) * (prob - y1);
+}
+
+// transform_cdf_p (cdf_utils.h:14-24) forward: v -> Phi^-1(F_np(v)) on informed cells
+__global__ void k_cdf_forward(float *v, const uint8_t *mask, long long n, const float *cv, const float *cp,
+                              long long m) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || mask[i] == 0) return;
+    const double P = np_prob(cv, cp, m, v[i]);
+    v[i] = (float)gauss_inverse(P, 0.0, 1.0);
+}
+// forward transform of every cell (the LVM mean, sequential_gaussian_simulation.cpp:104)
+__global__ void k_cdf_forward_all(float *v, long long n, const float *cv, const float *cp, long long m) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double P = np_prob(cv, cp, m, v[i]);
+    v[i] = (float)gauss_inverse(P, 0.0, 1.0);
+}
+// back-transform of informed cells: v -> F_np^-1((float) Phi(v))
+__global__ void k_cdf_back(float *v, const uint8_t *mask, long long n, const float *cv, const float *cp,
+                           long long m) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || mask[i] == 0) return;
+    const double P = gauss_prob(v[i], 0.0, 1.0);
+    v[i] = np_inverse(cv, cp, m, (float)P);
+}
+
+__global__ __launch_bounds__(64) void k_sgs_sweep(SgsSweepArgs a) {
+    const int r = blockIdx.y;
+    const int lane = threadIdx.x;
+    const int T = a.T[r];
+    const int *targets = a.targets + a.t_off[r];
+    const int *nbr = a.nbr + (size_t)a.t_off[r] * a.max_n;
+    const int *cnt = a.cnt + a.t_off[r];
+    const double *wts = a.w + (size_t)a.t_off[r] * a.max_n;
+    const uint8_t *status = a.status + a.t_off[r];
+    const uint32_t *draws = a.draws + (size_t)r * a.draw_stride;
+    const int *key = a.key + (size_t)r * a.N;
+    float *val = a.values + (size_t)r * a.N;
+    int *done = a.done + (size_t)r * a.N;
+    unsigned int *ticket = a.ticket + r;
+    while (true) {
+        if (__hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+        int base = 0;
+        if (lane == 0) base = (int)atomicAdd(ticket, 64u);
+        base = __shfl(base, 0);
+        if (base >= T) break;
+        const int k = base + lane;
+        bool pending = k < T;
+        long long spins = 0;
+        while (__any(pending)) {
+            if (pending) {
+                const int n = cnt[k];
+                bool ready = true;
+                for (int i = 0; i < n && ready; ++i) {
+                    const int nb = nbr[(size_t)k * a.max_n + i];
+                    if (key[nb] >= 0 && __hip_atomic_load(done + nb, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0)
+                        ready = false;
+                }
+                if (ready) {
+                    const int cell = targets[k];
+                    const bool ok = status[k] == ST_OK;
+                    const double v0 = a.var[a.t_off[r] + k];
+                    const float m0 = a.lvm ? a.lvm[cell] : 0.0f;
+                    // combine (combiner.h:10-24) in neighbour order
+                    float mean = m0;
+                    if (ok) {
+                        for (int i = 0; i < n; ++i) {
+                            const int nb = nbr[(size_t)k * a.max_n + i];
+                            const float mi = a.lvm ? a.lvm[nb] : 0.0f;
+                            mean += (float)(wts[(size_t)k * a.max_n + i] * (double)(val[nb] - mi));
+                        }
+                    }
+                    // u = x / (2^32 - 1) (bs_random_generator.cpp:134-137)
+                    const double u = (double)draws[k] / 4294967295.0;
+                    const double value = ok ? gauss_inverse(u, (double)mean, v0) : gauss_inverse(u, 0.0, 1.0);
+                    val[cell] = (float)value;
+                    __hip_atomic_store(done + cell, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                    pending = false;
+                }
+            }
+            if (++spins > a.spin_limit) {
+                if (pending) atomicOr(a.error, 1u);
+                break;
+            }
+            if (__any(pending)) __builtin_amdgcn_s_sleep(2);
+        }
+    }
+}
+
+inline unsigned blocks(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace
+
+void launch_lcg_path(const SgsPathArgs &p, int R, hipStream_t s) {
+    const long long threads = (p.N + 255) / 256;
+    hipLaunchKernelGGL(k_lcg_path, dim3(blocks(threads, 64), R), dim3(64), 0, s, p);
+}
+
+void launch_sgs_key(const uint8_t *hard, const uint8_t *sim_mask, const int *rank, long long N, int R, int *key,
+                    hipStream_t s) {
+    hipLaunchKernelGGL(k_sgs_key, dim3(blocks(N * R, 256)), dim3(256), 0, s, hard, sim_mask, rank, N, R, key);
+}
+
+void launch_mt11213b(const uint32_t *seeds, const long long *counts, long long stride, int R, uint32_t *out,
+                     hipStream_t s) {
+    hipLaunchKernelGGL(k_mt11213b, dim3(R), dim3(384), 0, s, seeds, counts, stride, out);
+}
+
+void launch_cdf_forward(float *v, const uint8_t *mask, long long n, const float *cv, const float *cp, long long m,
+                        hipStream_t s) {
+    if (n <= 0) return;
+    if (mask)
+        hipLaunchKernelGGL(k_cdf_forward, dim3(blocks(n, 256)), dim3(256), 0, s, v, mask, n, cv, cp, m);
+    else
+        hipLaunchKernelGGL(k_cdf_forward_all, dim3(blocks(n, 256)), dim3(256), 0, s, v, n, cv, cp, m);
+}
+
+void launch_cdf_back(float *v, const uint8_t *mask, long long n, const float *cv, const float *cp, long long m,
+                     hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_cdf_back, dim3(blocks(n, 256)), dim3(256), 0, s, v, mask, n, cv, cp, m);
+}
+
+void launch_sgs_sweep(const SgsSweepArgs &a, int R, int waves_per_realisation, hipStream_t s) {
+    hipLaunchKernelGGL(k_sgs_sweep, dim3(waves_per_realisation, R), dim3(64), 0, s, a);
+}
+
+}  // namespace hpgl_amd

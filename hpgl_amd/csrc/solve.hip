@@ -25,6 +25,7 @@
 // The combine (combiner.h:10-24) is evaluated exactly as the reference:
 // each term (float)(w_i * (double)(v_i - m_i)) is formed per lane, then
 // summed in float in neighbour order by the group's first lane.
+#include "counters.h"
 #include "kernels.h"
 
 #include <hip/hip_runtime.h>
@@ -160,6 +161,12 @@ __device__ inline void epilogue(const SolveArgs &a, bool write, int t, int n, in
                                 double variance, float (*term)[G], double (*dterm)[G]) {
     const int node = a.targets[t];
     const int EPI = a.epi;
+    if (EPI == EPI_SGS) {
+        if (write && ok && i < n) a.out_w[(size_t)t * a.max_n + i] = w;
+        if (write && i == 0) a.out_var[t] = ok ? variance : -1.0;
+        __syncthreads();
+        return;
+    }
     float m0;
     if (a.mean_kind == 2)
         m0 = a.mean_cells[node];
@@ -168,7 +175,7 @@ __device__ inline void epilogue(const SolveArgs &a, bool write, int t, int n, in
     else
         m0 = 0.0f;
     if (write && ok && i < n) {
-        if (EPI == EPI_CONT || EPI == EPI_SGS) {
+        if (EPI == EPI_CONT) {
             const float v = a.values_f[nb];
             const float m = nb_mean(a, nb);
             term[g][i] = (float)(w * (double)(v - m));
@@ -204,17 +211,7 @@ __device__ inline void epilogue(const SolveArgs &a, bool write, int t, int n, in
             for (int k = 0; k < n; ++k) r += dterm[g][k];
         a.out_u8[node] = r <= 0.5 ? 0 : 1;
         a.out_mask[node] = 1;
-    } else {  // EPI_SGS
-        float r = m0;
-        if (ok)
-            for (int k = 0; k < n; ++k) r += term[g][k];
-        a.out_f[t] = r;
-        a.out_var[t] = ok ? variance : -1.0;
     }
-}
-
-__device__ inline void count(unsigned long long *c, int which) {
-    if (c) atomicAdd(c + which, 1ull);
 }
 
 // ---------------------------------------------------------------------------
@@ -383,12 +380,10 @@ __global__ __launch_bounds__(64) void k_solve_mixed(SolveArgs a) {
         }
     }
 
-    if (valid && i == 0) {
-        const uint8_t st = n == 0 ? ST_NO_NEIGHBOURS : (bad ? ST_FALLBACK : ST_OK);
-        a.status[t] = st;
-        if (st == ST_NO_NEIGHBOURS) count(a.counters, 3);
-        if (st == ST_OK) count(a.counters, 0);
-    }
+    const uint8_t st = n == 0 ? ST_NO_NEIGHBOURS : (bad ? ST_FALLBACK : ST_OK);
+    if (valid && i == 0) a.status[t] = st;
+    wave_count(a.counters, 3, valid && i == 0 && st == ST_NO_NEIGHBOURS);
+    wave_count(a.counters, 0, valid && i == 0 && st == ST_OK);
     epilogue<G>(a, valid && (n == 0 || !bad), t, n, g, i, nb, n > 0, w, variance, sh.term, sh.dterm);
 }
 
@@ -547,10 +542,10 @@ __global__ __launch_bounds__(64) void k_solve_fp64(SolveArgs a) {
     }
 
     const bool ok = n > 0 && info == 0;
-    if (valid && i == 0) {
-        a.status[t] = n == 0 ? ST_NO_NEIGHBOURS : (ok ? ST_OK : ST_SINGULAR);
-        count(a.counters, n == 0 ? 3 : (ok ? 1 : 2));
-    }
+    if (valid && i == 0) a.status[t] = n == 0 ? ST_NO_NEIGHBOURS : (ok ? ST_OK : ST_SINGULAR);
+    wave_count(a.counters, 3, valid && i == 0 && n == 0);
+    wave_count(a.counters, 1, valid && i == 0 && ok);
+    wave_count(a.counters, 2, valid && i == 0 && n > 0 && !ok);
     epilogue<G>(a, valid, t, n, g, i, nb, ok, w, variance, sh.term, sh.dterm);
 }
 

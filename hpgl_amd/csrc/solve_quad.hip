@@ -21,6 +21,7 @@
 //   triangular solves, no fused multiply-add; the translation unit is
 //   compiled with contraction off), so with the same A and b the weights
 //   and the success/failure decision are the reference's bit for bit.
+#include "counters.h"
 #include "kernels.h"
 
 #include <hip/hip_runtime.h>
@@ -82,39 +83,17 @@ __device__ inline int qmaxi(int v) {
     return v;
 }
 
-// Analytic model kept out of line: it is called from fully unrolled loops.
-__device__ __noinline__ double cov_model_at(const DevCov &c, int dx, int dy, int dz);
-
+// Covariance from the box table (every job that reaches these kernels has
+// one; the analytic OK model is tabulated on the host over the offsets its
+// neighbour pairs can take).  Branch-free: out-of-box offsets read entry 0
+// and are replaced by 0, so the unrolled build loops issue their loads
+// back to back.
 __device__ inline double cov_at(const DevCov &c, int dx, int dy, int dz) {
-    if (c.table) {
-        if (dx < -c.tr[0] || dx > c.tr[0] || dy < -c.tr[1] || dy > c.tr[1] || dz < -c.tr[2] || dz > c.tr[2])
-            return 0.0;
-        return c.table[((size_t)(dz + c.tr[2]) * c.td[1] + (dy + c.tr[1])) * c.td[0] + (dx + c.tr[0])];
-    }
-    return cov_model_at(c, dx, dy, dz);
-}
-
-// cov_model.h:63-77 transfrom_and_norm, then :144-181
-__device__ __noinline__ double cov_model_at(const DevCov &c, int dx, int dy, int dz) {
-    const double v0 = dx, v1 = dy, v2 = dz;
-    double r = 0.0;
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        double d = 0.0;
-        d += v0 * c.t[3 * i + 0];
-        d += v1 * c.t[3 * i + 1];
-        d += v2 * c.t[3 * i + 2];
-        r += d * d;
-    }
-    const double h = sqrt(r);
-    if (h < 0.0001) return c.c0;
-    const double a = c.range0;
-    if (c.type == 0) {
-        if (h > a) return 0.0;
-        return c.c0 * (1 - 1.5 * h / a + 0.5 * pow(h / a, 3.0));
-    }
-    if (c.type == 1) return c.c0 * exp(-3 * h / a);
-    return c.c0 * exp(-3 * pow(h / a, 2.0));
+    const bool in = dx >= -c.tr[0] && dx <= c.tr[0] && dy >= -c.tr[1] && dy <= c.tr[1] && dz >= -c.tr[2] &&
+                    dz <= c.tr[2];
+    const int idx = ((dz + c.tr[2]) * c.td[1] + (dy + c.tr[1])) * c.td[0] + (dx + c.tr[0]);
+    const double v = c.table[in ? idx : 0];
+    return in ? v : 0.0;
 }
 
 constexpr int off(int m) { return 2 * m * (m + 1); }  // start of my m-th row (row length 4m+4)
@@ -169,23 +148,21 @@ __device__ inline void q_setup(const SolveArgs &a, QShared<N> &sh, QCtx<N> &c) {
     __syncthreads();
 }
 
-// A(i, j) of the padded system
+// A(i, j) of the padded system.  Branch-free: padding rows hold
+// coordinates 0 in LDS, so the table read is always in bounds and the
+// result is selected afterwards.
 template <int N>
 __device__ inline double q_aij(const SolveArgs &a, const QShared<N> &sh, int slot, int n, int i, int j) {
-    if (i < n && j < n)
-        return cov_at(a.cov, sh.xyz[slot][0][j] - sh.xyz[slot][0][i], sh.xyz[slot][1][j] - sh.xyz[slot][1][i],
-                      sh.xyz[slot][2][j] - sh.xyz[slot][2][i]);
-    return i == j ? 1.0 : 0.0;
+    const double v = cov_at(a.cov, sh.xyz[slot][0][j] - sh.xyz[slot][0][i], sh.xyz[slot][1][j] - sh.xyz[slot][1][i],
+                            sh.xyz[slot][2][j] - sh.xyz[slot][2][i]);
+    return (i < n && j < n) ? v : (i == j ? 1.0 : 0.0);
 }
 
 template <int N>
 __device__ inline double q_bi(const SolveArgs &a, const QShared<N> &sh, const QCtx<N> &c, int i) {
-    if (i >= c.n) return 0.0;
-    return cov_at(a.cov, c.cx - sh.xyz[c.slot][0][i], c.cy - sh.xyz[c.slot][1][i], c.cz - sh.xyz[c.slot][2][i]);
-}
-
-__device__ inline void bump(unsigned long long *c, int which) {
-    if (c) atomicAdd(c + which, 1ull);
+    const double v =
+        cov_at(a.cov, c.cx - sh.xyz[c.slot][0][i], c.cy - sh.xyz[c.slot][1][i], c.cz - sh.xyz[c.slot][2][i]);
+    return i < c.n ? v : 0.0;
 }
 
 // Writes one system's result (combine.h order) from the quad's weights.
@@ -202,6 +179,19 @@ __device__ inline void q_epilogue(const SolveArgs &a, QShared<N> *shp, const QCt
         m0 = a.mean_single;
     else
         m0 = 0.0f;
+    if (EPI == EPI_SGS) {
+        // the values of simulated neighbours are not known yet: keep the
+        // weights and the variance for the ordered sweep
+        if (write) {
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                const int i = 4 * m + c.q;
+                if (ok && i < c.n) a.out_w[(size_t)c.t * a.max_n + i] = w[m];
+            }
+            if (c.q == 0) a.out_var[c.t] = ok ? variance : -1.0;
+        }
+        return;
+    }
     float tf[M];
     double td[M];
 #pragma unroll
@@ -213,7 +203,7 @@ __device__ inline void q_epilogue(const SolveArgs &a, QShared<N> *shp, const QCt
             const int nb = c.nb[m];
             const float mi = a.mean_kind == 2 ? a.mean_cells[nb] : (a.mean_kind == 1 ? a.mean_single : 0.0f);
             float v;
-            if (EPI == EPI_CONT || EPI == EPI_SGS)
+            if (EPI == EPI_CONT)
                 v = a.values_f[nb];
             else
                 v = (a.values_u8[nb] == (uint8_t)a.category) ? 1.0f : 0.0f;
@@ -255,9 +245,6 @@ __device__ inline void q_epilogue(const SolveArgs &a, QShared<N> *shp, const QCt
     } else if (EPI == EPI_MEDIAN) {
         a.out_u8[c.node] = rd <= 0.5 ? 0 : 1;
         a.out_mask[c.node] = 1;
-    } else {
-        a.out_f[c.t] = rf;
-        a.out_var[c.t] = ok ? variance : -1.0;
     }
 }
 
@@ -314,7 +301,8 @@ __global__ __launch_bounds__(64) void k_solve_quad_mixed(SolveArgs a) {
         b[m] = q_bi<N>(a, sh, c, i);
 #pragma unroll
         for (int j = 0; j < 4 * m + 4; ++j) {
-            const double aij = (j <= i) ? q_aij<N>(a, sh, c.slot, c.n, i, j) : 0.0;
+            const double e = q_aij<N>(a, sh, c.slot, c.n, i, j);
+            const double aij = (j <= i) ? e : 0.0;
             L[off(m) + j] = (float)aij;
             if (j == i && i < c.n) maxdiag = fmax(maxdiag, aij);
         }
@@ -364,8 +352,7 @@ __global__ __launch_bounds__(64) void k_solve_quad_mixed(SolveArgs a) {
         x2[m] = v2[m];
     }
     // Refinement: a second step runs unless every system of the wave already
-    // has a first correction below 1e-5 of its solution (the refined error is
-    // then of the order of that correction squared times cond(A) eps_32).
+    // has a first correction below 1e-9 of its solution.
     bool conv = false;
     for (int step = 0; step < 2; ++step) {
         // r = b - A x in FP64 for my rows; x goes through LDS
@@ -412,7 +399,7 @@ __global__ __launch_bounds__(64) void k_solve_quad_mixed(SolveArgs a) {
         xmax = qmax(xmax);
         const double rel = dmax / (xmax > 0 ? xmax : 1.0);
         if (step == 0) {
-            conv = rel <= 1.0e-5;
+            conv = rel <= 1.0e-9;
             if (__all(conv || !c.valid)) break;
         } else {
             conv = conv || rel <= 1.0e-9;
@@ -452,12 +439,10 @@ __global__ __launch_bounds__(64) void k_solve_quad_mixed(SolveArgs a) {
         if (two) variance -= qsum(wl);  // my_kriging_weights.h:246
     }
 
-    if (c.valid && q == 0) {
-        const uint8_t st = c.n == 0 ? ST_NO_NEIGHBOURS : (bad ? ST_FALLBACK : ST_OK);
-        a.status[c.t] = st;
-        if (st == ST_NO_NEIGHBOURS) bump(a.counters, 3);
-        if (st == ST_OK) bump(a.counters, 0);
-    }
+    const uint8_t st = c.n == 0 ? ST_NO_NEIGHBOURS : (bad ? ST_FALLBACK : ST_OK);
+    if (c.valid && q == 0) a.status[c.t] = st;
+    wave_count(a.counters, 3, c.valid && q == 0 && st == ST_NO_NEIGHBOURS);
+    wave_count(a.counters, 0, c.valid && q == 0 && st == ST_OK);
     q_epilogue<N>(a, &sh, c, c.valid && (c.n == 0 || !bad), c.n > 0, w, variance);
 }
 
@@ -480,7 +465,10 @@ __global__ __launch_bounds__(64) void k_solve_quad_fp64(SolveArgs a) {
         const int i = 4 * m + q;
         b[m] = q_bi<N>(a, sh, c, i);
 #pragma unroll
-        for (int j = 0; j < 4 * m + 4; ++j) L[off(m) + j] = (j <= i) ? q_aij<N>(a, sh, c.slot, c.n, i, j) : 0.0;
+        for (int j = 0; j < 4 * m + 4; ++j) {
+            const double e = q_aij<N>(a, sh, c.slot, c.n, i, j);
+            L[off(m) + j] = (j <= i) ? e : 0.0;
+        }
     }
 
     // dpotf2 'U' on U = L^T: for each j, row j of L (entries < j) is
@@ -604,10 +592,10 @@ __global__ __launch_bounds__(64) void k_solve_quad_fp64(SolveArgs a) {
     }
 
     const bool ok = c.n > 0 && info == 0;
-    if (c.valid && q == 0) {
-        a.status[c.t] = c.n == 0 ? ST_NO_NEIGHBOURS : (ok ? ST_OK : ST_SINGULAR);
-        bump(a.counters, c.n == 0 ? 3 : (ok ? 1 : 2));
-    }
+    if (c.valid && q == 0) a.status[c.t] = c.n == 0 ? ST_NO_NEIGHBOURS : (ok ? ST_OK : ST_SINGULAR);
+    wave_count(a.counters, 3, c.valid && q == 0 && c.n == 0);
+    wave_count(a.counters, 1, c.valid && q == 0 && ok);
+    wave_count(a.counters, 2, c.valid && q == 0 && c.n > 0 && !ok);
     q_epilogue<N>(a, &sh, c, c.valid, ok, w, variance);
 }
 

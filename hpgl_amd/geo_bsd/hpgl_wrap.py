@@ -81,6 +81,15 @@ _hpgl_so.hpgl_sgs_lvm_simulation.argtypes = [
     _P(_HPGL_CONT_MASKED_ARRAY), _P(_HPGL_SGS_PARAMS), _P(hpgl_non_parametric_cdf_t), _P(_HPGL_FLOAT_ARRAY),
     _P(_HPGL_UBYTE_ARRAY)]
 
+_hpgl_so.hpgl_amd_sgs_simulation_batch.restype = None
+_hpgl_so.hpgl_amd_sgs_simulation_batch.argtypes = [
+    _P(_HPGL_CONT_MASKED_ARRAY), _P(_HPGL_SGS_PARAMS), _P(hpgl_non_parametric_cdf_t), _P(_HPGL_FLOAT_ARRAY),
+    _P(_HPGL_UBYTE_ARRAY), _P(C.c_long), C.c_int, C.c_void_p, C.c_void_p]
+_hpgl_so.hpgl_amd_sgs_simulation_batch_dev.restype = None
+_hpgl_so.hpgl_amd_sgs_simulation_batch_dev.argtypes = [
+    C.c_void_p, C.c_void_p, _P(_HPGL_SHAPE), _P(_HPGL_SGS_PARAMS), _P(hpgl_non_parametric_cdf_t), C.c_void_p,
+    C.c_void_p, _P(C.c_long), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+
 _hpgl_so.hpgl_amd_set_precision.restype = None
 _hpgl_so.hpgl_amd_set_precision.argtypes = [C.c_int]
 _hpgl_so.hpgl_amd_get_precision.restype = C.c_int

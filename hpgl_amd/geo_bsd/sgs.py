@@ -66,6 +66,20 @@ def _as_cdf(cdf_data):
     return calc_cdf(cdf_data)
 
 
+def _sgs_params(cov_model, radiuses, max_neighbours, kriging_type, seed, min_neighbours):
+    return _HPGL_SGS_PARAMS(
+        covariance_type=cov_model.type,
+        ranges=_c_array(C.c_double, 3, cov_model.ranges),
+        angles=_c_array(C.c_double, 3, cov_model.angles),
+        sill=cov_model.sill,
+        nugget=cov_model.nugget,
+        radiuses=_c_array(C.c_int, 3, radiuses),
+        max_neighbours=max_neighbours,
+        kriging_kind={"sk": _HPGL_KRIGING_KIND.simple, "ok": _HPGL_KRIGING_KIND.ordinary}[kriging_type],
+        seed=seed,
+        min_neighbours=min_neighbours)
+
+
 @accepts_tuple("prop", 0)
 def sgs_simulation(prop, grid, cdf_data, radiuses, max_neighbours, cov_model, seed, kriging_type="sk", mean=None,
                    use_harddata=True, use_regions=False, region_size=None, mask=None, force_single_thread=False,
@@ -83,17 +97,7 @@ def sgs_simulation(prop, grid, cdf_data, radiuses, max_neighbours, cov_model, se
     prop.fix_shape(grid)
     cov_model = normed_cov_model(cov_model)
     out_prop, mean, mask = __prepare_sgs(prop=prop, mean=mean, use_harddata=use_harddata, mask=mask)
-    sgsp = _HPGL_SGS_PARAMS(
-        covariance_type=cov_model.type,
-        ranges=_c_array(C.c_double, 3, cov_model.ranges),
-        angles=_c_array(C.c_double, 3, cov_model.angles),
-        sill=cov_model.sill,
-        nugget=cov_model.nugget,
-        radiuses=_c_array(C.c_int, 3, radiuses),
-        max_neighbours=max_neighbours,
-        kriging_kind={"sk": _HPGL_KRIGING_KIND.simple, "ok": _HPGL_KRIGING_KIND.ordinary}[kriging_type],
-        seed=seed,
-        min_neighbours=min_neighbours)
+    sgsp = _sgs_params(cov_model, radiuses, max_neighbours, kriging_type, seed, min_neighbours)
     cdf = _as_cdf(cdf_data)
     hpgl_cdf = C.byref(_create_hpgl_nonparam_cdf(cdf)) if cdf is not None else None
     hpgl_mask = C.byref(_create_hpgl_ubyte_array(mask, grid)) if mask is not None else None
@@ -107,3 +111,39 @@ def sgs_simulation(prop, grid, cdf_data, radiuses, max_neighbours, cov_model, se
             C.byref(_create_hpgl_float_array(mean, grid)), hpgl_mask)
     check_last_error()
     return out_prop
+
+
+@accepts_tuple("prop", 0)
+def sgs_simulation_batch(prop, grid, cdf_data, radiuses, max_neighbours, cov_model, seeds, kriging_type="sk",
+                         mean=None, use_harddata=True, mask=None, min_neighbours=0, **params):
+    """Several realisations of ``sgs_simulation`` in one call, one per seed.
+
+    Takes the arguments of ``sgs_simulation`` with a sequence of seeds in
+    place of one; realisation r equals ``sgs_simulation(..., seed=seeds[r])``.
+    Returns a list of ContProperty, one per seed.  (No counterpart in the
+    reference, which runs one realisation per call.)
+    """
+    prop.fix_shape(grid)
+    cov_model = normed_cov_model(cov_model)
+    base, mean, mask = __prepare_sgs(prop=prop, mean=mean, use_harddata=use_harddata, mask=mask)
+    seeds = [int(s) for s in seeds]
+    if not seeds:
+        raise ValueError("at least one seed is needed")
+    sgsp = _sgs_params(cov_model, radiuses, max_neighbours, kriging_type, seeds[0], min_neighbours)
+    cdf = _as_cdf(cdf_data)
+    hpgl_cdf = C.byref(_create_hpgl_nonparam_cdf(cdf)) if cdf is not None else None
+    hpgl_mask = C.byref(_create_hpgl_ubyte_array(mask, grid)) if mask is not None else None
+    lvm = None
+    if mean is not None and not numpy.isscalar(mean):
+        lvm = C.byref(_create_hpgl_float_array(mean, grid))
+    R = len(seeds)
+    shape = base.data.shape
+    out = numpy.empty((R,) + tuple(shape[::-1]), dtype="float32")
+    out_mask = numpy.empty((R,) + tuple(shape[::-1]), dtype="uint8")
+    c_seeds = (C.c_long * R)(*seeds)
+    _hpgl_so.hpgl_amd_sgs_simulation_batch(
+        C.byref(_create_hpgl_cont_masked_array(base, grid)), C.byref(sgsp), hpgl_cdf, lvm, hpgl_mask, c_seeds, R,
+        out.ctypes.data, out_mask.ctypes.data)
+    check_last_error()
+    # realisation r holds its N cells in Fortran order
+    return [ContProperty(out[r].transpose(), out_mask[r].transpose()) for r in range(R)]

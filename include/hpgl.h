@@ -300,6 +300,31 @@ void hpgl_amd_indicator_kriging_dev(const unsigned char *d_in, const unsigned ch
                                     unsigned char *d_out, unsigned char *d_out_mask,
                                     void *stream);
 
+/* R realisations of sequential Gaussian simulation in one call.  Each
+ * realisation r is what hpgl_sgs_simulation (lvm_means == NULL) or
+ * hpgl_sgs_lvm_simulation would return with params->m_seed = seeds[r];
+ * prop is not modified, realisation r goes to out[r*N ...] and
+ * out_mask[r*N ...] (N cells each).  Realisations share one set of device
+ * launches: path, search and solve kernels run over all of them together
+ * and one ordered sweep writes all their values.  No reference counterpart:
+ * the reference runs one realisation per call. */
+void hpgl_amd_sgs_simulation_batch(const struct hpgl_cont_masked_array_t *prop,
+                                   const struct hpgl_sgs_params_t *params,
+                                   const struct hpgl_non_parametric_cdf_t *cdf,
+                                   const struct hpgl_float_array_t *lvm_means,
+                                   const struct hpgl_ubyte_array_t *simulation_mask,
+                                   const long *seeds, int R,
+                                   float *out, unsigned char *out_mask);
+/* Device-pointer variant of hpgl_amd_sgs_simulation_batch; cdf arrays and
+ * seeds stay on the host.  d_out and d_out_mask hold R*N entries. */
+void hpgl_amd_sgs_simulation_batch_dev(const float *d_in, const unsigned char *d_in_mask,
+                                       const struct hpgl_shape_t *shape,
+                                       const struct hpgl_sgs_params_t *params,
+                                       const struct hpgl_non_parametric_cdf_t *cdf,
+                                       const float *d_lvm_mean, const unsigned char *d_sim_mask,
+                                       const long *seeds, int R,
+                                       float *d_out, unsigned char *d_out_mask, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
