@@ -18,8 +18,10 @@
 //    k-th left stop (first elements with key <= pivot, scanning up) is
 //    swapped with the k-th right stop (key >= pivot, scanning down) while
 //    the left stop is below the right stop, and the cut is
-//    min(l_{s+1}, r_s) for s swaps.  One ballot pass records the stop
-//    positions, a second pass over k performs the (disjoint) swaps;
+//    min(l_{s+1}, r_s) for s swaps.  One ballot pass records, per chunk of
+//    64 positions, the masks of left and right stops and their running
+//    counts; the swap pass expands 64 left stops and the matching 64 right
+//    stops at a time from those masks and performs the (disjoint) swaps;
 //  * ranges that start at or beyond max_n are never partitioned: they
 //    cannot change the first max_n positions, which after the final
 //    insertion sort are the stable sort of the partition blocks they fall
@@ -36,23 +38,46 @@ namespace hpgl_amd {
 
 namespace {
 
-constexpr int kCap = 2048;  // candidates held in LDS per wavefront
+constexpr int kCap = 1024;  // candidates held in LDS per wavefront; longer lists go to global scratch
 constexpr int kStack = 64;
+
+// A wavefront orders its own accesses (LDS and the global scratch lists)
+// with a workgroup-scope fence, which waits for them to complete; the
+// workgroup is the wavefront itself, so no barrier is needed.
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
 
 __device__ inline uint64_t lanemask_lt() {
     const int lane = threadIdx.x & 63;
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
-struct WaveLds {
-    uint32_t e[kCap];
-    uint16_t L[kCap];
-    uint16_t R[kCap];
-    int stk_lo[kStack], stk_hi[kStack], stk_d[kStack];
-    int bin_start[27];
-    int bin_cnt[27];
-    int bin_pref[28];
+// Sort state of one list: elements, and per chunk of 64 positions of the
+// range being partitioned the masks of left / right stops (ml, mr) and the
+// number of stops before the chunk (pl, pr; entry nch holds the totals).
+struct SortBufs {
+    uint32_t *e;
+    uint64_t *ml, *mr;
+    int *pl, *pr;
 };
+
+template <int CAP>
+struct WaveLds {
+    uint32_t e[CAP];
+    uint64_t ml[CAP / 64 + 1], mr[CAP / 64 + 1];
+    int pl[CAP / 64 + 2], pr[CAP / 64 + 2];
+    int lbuf[64], rbuf[64];  // positions of the 64 stop pairs being swapped
+    int stk_lo[kStack], stk_hi[kStack], stk_d[kStack];
+    // the 27 bins as 9 rows (k, j) of three consecutive bins i-1, i, i+1,
+    // which are adjacent in the bin-sorted cell arrays
+    int seg_start[9];
+    int seg_cnt[9];
+    int seg_pref[10];
+};
+
+constexpr int kGatherUnroll = 4;  // 64-candidate groups whose loads are in flight together
 
 struct Keyed {
     int lb;
@@ -108,21 +133,82 @@ __device__ void heap_sort_(E *e, long len, Keyed key) {
     }
 }
 
-// One __unguarded_partition_pivot on e[lo, hi); returns the cut.
-// Lp[k] = position of the (k+1)-th left stop counted from lo+1 upwards;
-// Rq[k] = position of the (k+1)-th right stop counted from lo+1 upwards,
-// so the (k+1)-th right stop counted from hi-1 downwards is Rq[nR-1-k].
-template <typename E, typename P>
-__device__ int partition_(E *e, P *Lp, P *Rq, int lo, int hi, Keyed key) {
+// Position (0-63) of the j-th set bit of m (j < popcount(m)).
+__device__ inline int select_bit(uint64_t m, int j) {
+    int pos = 0;
+    int c = __popc((uint32_t)m);
+    if (j >= c) {
+        j -= c;
+        m >>= 32;
+        pos += 32;
+    }
+    c = __popc((uint32_t)m & 0xffffu);
+    if (j >= c) {
+        j -= c;
+        m >>= 16;
+        pos += 16;
+    }
+    c = __popc((uint32_t)m & 0xffu);
+    if (j >= c) {
+        j -= c;
+        m >>= 8;
+        pos += 8;
+    }
+    c = __popc((uint32_t)m & 0xfu);
+    if (j >= c) {
+        j -= c;
+        m >>= 4;
+        pos += 4;
+    }
+    c = __popc((uint32_t)m & 0x3u);
+    if (j >= c) {
+        j -= c;
+        m >>= 2;
+        pos += 2;
+    }
+    if (j >= (int)(m & 1u)) pos += 1;
+    return pos;
+}
+
+// Offset from the range start of the k-th stop (0-based, k < total) of the
+// masks m / counts p over nch chunks.  Called with the same k by every lane.
+__device__ inline int nth_stop(const uint64_t *m, const int *p, int nch, int k) {
+    int lo = 0, hi = nch;  // largest c < nch with p[c] <= k
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (p[mid] <= k)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return (lo << 6) + select_bit(m[lo], k - p[lo]);
+}
+
+// Writes buf[idx] = position of the stop with index idx0 + idx, for the
+// stops of index [idx0, idx0 + 64) (masks m / counts p, chunk c0 holds
+// index max(idx0, 0)).
+__device__ inline void expand_stops(const uint64_t *m, const int *p, int nch, int c0, int idx0, int first, int *buf) {
     const int lane = threadIdx.x & 63;
     const uint64_t lt = lanemask_lt();
+    for (int c = c0; c < nch && p[c] < idx0 + 64; ++c) {
+        const uint64_t mk = m[c];
+        const int idx = p[c] + __popcll(mk & lt) - idx0;
+        if (((mk >> lane) & 1u) && idx >= 0 && idx < 64) buf[idx] = first + (c << 6) + lane;
+    }
+}
+
+// One __unguarded_partition_pivot on e[lo, hi); returns the cut.
+template <typename S>
+__device__ int partition_(const SortBufs &b, S &sh, int lo, int hi, Keyed key) {
+    const int lane = threadIdx.x & 63;
+    uint32_t *e = b.e;
     {  // __move_median_to_first(lo, lo+1, mid, hi-1)
-        const int a = lo + 1, b = lo + (hi - lo) / 2, c = hi - 1;
-        const uint32_t ka = key(e[a]), kb = key(e[b]), kc = key(e[c]);
+        const int a = lo + 1, bm = lo + (hi - lo) / 2, c = hi - 1;
+        const uint32_t ka = key(e[a]), kb = key(e[bm]), kc = key(e[c]);
         int sw;
         if (ka > kb) {
             if (kb > kc)
-                sw = b;
+                sw = bm;
             else if (ka > kc)
                 sw = c;
             else
@@ -132,72 +218,84 @@ __device__ int partition_(E *e, P *Lp, P *Rq, int lo, int hi, Keyed key) {
         } else if (kb > kc) {
             sw = c;
         } else {
-            sw = b;
+            sw = bm;
         }
-        __syncthreads();
+        wave_sync();
         if (lane == 0) {
-            const E t0 = e[lo];
+            const uint32_t t0 = e[lo];
             e[lo] = e[sw];
             e[sw] = t0;
         }
-        __syncthreads();
+        wave_sync();
     }
     const uint32_t kp = key(e[lo]);
+    const int first = lo + 1;
+    const int nch = (hi - first + 63) >> 6;
+    // left stops: key <= pivot; right stops: key >= pivot
     int nL = 0, nR = 0;
-    int base = lo + 1;
-    for (; base + 64 <= hi; base += 64) {
-        const uint32_t k = key(e[base + lane]);
-        const bool lf = k <= kp, rf = k >= kp;
-        const uint64_t mL = __ballot(lf), mR = __ballot(rf);
-        if (lf) Lp[nL + __popcll(mL & lt)] = (P)(base + lane);
-        if (rf) Rq[nR + __popcll(mR & lt)] = (P)(base + lane);
-        nL += __popcll(mL);
-        nR += __popcll(mR);
-    }
-    if (base < hi) {
-        const int p = base + lane;
+    for (int c = 0; c < nch; ++c) {
+        const int p = first + (c << 6) + lane;
         const bool v = p < hi;
         const uint32_t k = v ? key(e[p]) : 0u;
-        const bool lf = v && k <= kp, rf = v && k >= kp;
-        const uint64_t mL = __ballot(lf), mR = __ballot(rf);
-        if (lf) Lp[nL + __popcll(mL & lt)] = (P)p;
-        if (rf) Rq[nR + __popcll(mR & lt)] = (P)p;
+        const uint64_t mL = __ballot(v && k <= kp), mR = __ballot(v && k >= kp);
+        if (lane == 0) {
+            b.ml[c] = mL;
+            b.mr[c] = mR;
+            b.pl[c] = nL;
+            b.pr[c] = nR;
+        }
         nL += __popcll(mL);
         nR += __popcll(mR);
     }
-    __syncthreads();
-    // swap l_k <-> r_k while l_k < r_k; s = number of swaps (a prefix of k)
+    if (lane == 0) {
+        b.pl[nch] = nL;
+        b.pr[nch] = nR;
+    }
+    wave_sync();
+    // swap the k-th left stop with the k-th right stop from the end while
+    // the former is below the latter; s = number of swaps (a prefix of k)
     const int m = min(nL, nR);
     int s = 0;
+    int cl = 0, cr = nch - 1;
     for (int kb = 0; kb < m; kb += 64) {
+        while (cl + 1 < nch && b.pl[cl + 1] <= kb) ++cl;
+        expand_stops(b.ml, b.pl, nch, cl, kb, first, sh.lbuf);
+        // right stops kb .. kb+63 from the end have indices
+        // nR-1-kb .. nR-64-kb counted from the start
+        const int base = nR - 64 - kb;
+        const int from = base > 0 ? base : 0;
+        while (cr > 0 && b.pr[cr] > from) --cr;
+        expand_stops(b.mr, b.pr, nch, cr, base, first, sh.rbuf);
+        wave_sync();
         const int k = kb + lane;
         bool c = false;
         int pl = 0, pr = 0;
         if (k < m) {
-            pl = (int)Lp[k];
-            pr = (int)Rq[nR - 1 - k];
+            pl = sh.lbuf[lane];
+            pr = sh.rbuf[63 - lane];
             c = pl < pr;
         }
         if (c) {
-            const E tl = e[pl];
+            const uint32_t tl = e[pl];
             e[pl] = e[pr];
             e[pr] = tl;
         }
         const int cnt = __popcll(__ballot(c));
         s += cnt;
+        wave_sync();
         if (cnt < 64) break;
     }
-    const int a = s < nL ? (int)Lp[s] : 0x7fffffff;
-    const int b = s >= 1 ? (int)Rq[nR - s] : 0x7fffffff;
-    __syncthreads();
-    return min(a, b);
+    const int a = s < nL ? first + nth_stop(b.ml, b.pl, nch, s) : 0x7fffffff;
+    const int bq = s >= 1 ? first + nth_stop(b.mr, b.pr, nch, nR - s) : 0x7fffffff;
+    wave_sync();
+    return min(a, bq);
 }
 
 // Returns false if the replay reached a state libstdc++'s algorithm cannot
 // reach (a cut outside the range or a deeper pending stack than max_n
 // ranges); the caller then reports an internal error instead of looping.
-template <typename E, typename P>
-__device__ bool topk_sort_(E *e, P *Lp, P *Rq, WaveLds &sh, int n, int K, Keyed key) {
+template <typename S>
+__device__ bool topk_sort_(const SortBufs &b, S &sh, int n, int K, Keyed key) {
     const int lane = threadIdx.x & 63;
     if (n <= 1) return true;
     const int lg = 31 - __clz(n);
@@ -207,19 +305,19 @@ __device__ bool topk_sort_(E *e, P *Lp, P *Rq, WaveLds &sh, int n, int K, Keyed 
         sh.stk_d[0] = 2 * lg;
     }
     int top = 1;
-    __syncthreads();
+    wave_sync();
     while (top > 0) {
         --top;
         int lo = sh.stk_lo[top], hi = sh.stk_hi[top], d = sh.stk_d[top];
-        __syncthreads();
+        wave_sync();
         while (hi - lo > 16) {
             if (d == 0) {
-                if (lane == 0) heap_sort_(e + lo, hi - lo, key);
-                __syncthreads();
+                if (lane == 0) heap_sort_(b.e + lo, hi - lo, key);
+                wave_sync();
                 break;
             }
             --d;
-            const int cut = partition_(e, Lp, Rq, lo, hi, key);
+            const int cut = partition_(b, sh, lo, hi, key);
             if (cut <= lo || cut >= hi || top >= kStack) return false;
             if (cut < K) {
                 if (lane == 0) {
@@ -228,7 +326,7 @@ __device__ bool topk_sort_(E *e, P *Lp, P *Rq, WaveLds &sh, int n, int K, Keyed 
                     sh.stk_d[top] = d;
                 }
                 ++top;
-                __syncthreads();
+                wave_sync();
             }
             hi = cut;
         }
@@ -238,8 +336,8 @@ __device__ bool topk_sort_(E *e, P *Lp, P *Rq, WaveLds &sh, int n, int K, Keyed 
 
 // Stable rank sort of e[0, P) by descending key; writes the first K cells,
 // decoding each kept element's raw candidate index back to its cell.
-template <typename E>
-__device__ void finish_(const E *e, int n, int K, int *out, const WaveLds &sh, const int *bin_cells, Keyed key) {
+template <typename E, typename S>
+__device__ void finish_(const E *e, int n, int K, int *out, const S &sh, const int *bin_cells, Keyed key) {
     const int lane = threadIdx.x & 63;
     const int P = min(n, K + 16);
     const uint32_t lmask = (1u << key.lb) - 1u;
@@ -255,100 +353,157 @@ __device__ void finish_(const E *e, int n, int K, int *out, const WaveLds &sh, c
             }
             if (r < K) {
                 const int j = (int)(ep & lmask);
-                int b = 0;
-                while (sh.bin_pref[b + 1] <= j) ++b;
-                out[r] = bin_cells[sh.bin_start[b] + (j - sh.bin_pref[b])];
+                int g = 0;
+                while (sh.seg_pref[g + 1] <= j) ++g;
+                out[r] = bin_cells[sh.seg_start[g] + (j - sh.seg_pref[g])];
             }
         }
     }
 }
 
+// One wavefront per workgroup; the wavefronts loop over the target list.
+// PACKED: candidate coordinates come from bin_xyz (x | y << 11 | z << 22),
+// otherwise they are decoded from the cell index.
+template <bool PACKED>
 __global__ __launch_bounds__(64) void k_cluster(ClusterArgs a) {
-    __shared__ WaveLds sh;
-    const int lane = threadIdx.x;
+    __shared__ WaveLds<kCap> sh;
+    constexpr int CAP = kCap;
+    const int wave_id = blockIdx.x, n_wave_ids = gridDim.x;
+    const int lane = threadIdx.x & 63;
     const uint64_t lt = lanemask_lt();
     const int X = a.X, Y = a.Y;
     const BinArgs &b = a.bins;
     const Keyed key{a.local_bits};
-    uint32_t *ge = a.scratch ? a.scratch + (size_t)blockIdx.x * a.scratch_per_wave * 3 : nullptr;
+    // global scratch of this wave for lists longer than CAP: masks first
+    // (8-byte aligned), then elements and counts
+    SortBufs gb{nullptr, nullptr, nullptr, nullptr, nullptr};
+    uint32_t *ge = nullptr;
+    if (a.scratch) {
+        const int nchm = a.scratch_per_wave / 64 + 2;
+        uint32_t *base = a.scratch + (size_t)wave_id * cluster_scratch_words(a.scratch_per_wave);
+        gb.ml = reinterpret_cast<uint64_t *>(base);
+        gb.mr = gb.ml + nchm;
+        gb.e = reinterpret_cast<uint32_t *>(gb.mr + nchm);
+        gb.pl = reinterpret_cast<int *>(gb.e + a.scratch_per_wave);
+        gb.pr = gb.pl + nchm;
+        ge = gb.e;
+    }
+    const SortBufs lb{sh.e, sh.ml, sh.mr, sh.pl, sh.pr};
     unsigned long long cand_acc = 0;
-    for (int li = blockIdx.x; li < a.n_list; li += gridDim.x) {
+    for (int li = wave_id; li < a.n_list; li += n_wave_ids) {
         const int t = a.list ? a.list[li] : li;
         const int node = a.targets[t];
         const int x = node % X, y = node % (X * Y) / X, z = node / (X * Y);
         const int bx = x / b.r[0], by = y / b.r[1], bz = z / b.r[2];
-        // the 27 bins in gather order (k, j, i ascending) and their offsets
-        // in the raw concatenation; bins outside the bin box hold 0 cells
-        if (lane < 27) {
-            const int ii = bx - 1 + lane % 3, jj = by - 1 + (lane / 3) % 3, kk = bz - 1 + lane / 9;
+        // the 27 bins in gather order (k, j, i ascending) as 9 contiguous
+        // segments of the bin-sorted arrays; bins outside the bin box hold
+        // no cells (bx + 1 <= m0 - 1 always holds, bx - 1 may be -1)
+        if (lane < 9) {
+            const int jj = by - 1 + lane % 3, kk = bz - 1 + lane / 3;
             int start = 0, cnt = 0;
-            if (ii >= 0 && jj >= 0 && kk >= 0 && ii < b.m[0] && jj < b.m[1] && kk < b.m[2]) {
-                const int bin = (kk * b.m[1] + jj) * b.m[0] + ii;
-                start = (int)a.bin_start[bin];
-                cnt = (int)a.bin_count[bin];
+            if (jj >= 0 && kk >= 0 && jj < b.m[1] && kk < b.m[2]) {
+                const int row = (kk * b.m[1] + jj) * b.m[0];
+                const int i0 = bx > 0 ? bx - 1 : 0, i1 = bx + 1;
+                start = (int)a.bin_start[row + i0];
+                cnt = (int)(a.bin_start[row + i1] + a.bin_count[row + i1]) - start;
             }
-            sh.bin_start[lane] = start;
-            sh.bin_cnt[lane] = cnt;
+            sh.seg_start[lane] = start;
+            sh.seg_cnt[lane] = cnt;
         }
-        __syncthreads();
-        if (lane == 0) {
-            int acc = 0;
-            for (int q = 0; q < 27; ++q) {
-                sh.bin_pref[q] = acc;
-                acc += sh.bin_cnt[q];
-            }
-            sh.bin_pref[27] = acc;
+        wave_sync();
+        int pref = 0;
+#pragma unroll
+        for (int g = 0; g < 9; ++g) {
+            if (lane == g) sh.seg_pref[g] = pref;
+            pref += sh.seg_cnt[g];
         }
-        __syncthreads();
-        // gather: keep candidates with rank > 0, in raw order
+        if (lane == 9) sh.seg_pref[9] = pref;
+        const int raw_total = __builtin_amdgcn_readfirstlane(pref);
+        wave_sync();
+        // gather: keep candidates with rank > 0, in raw order.  Each pass
+        // issues the loads of kGatherUnroll groups of 64 before compacting.
+        // A group normally lies in one segment (a scalar cursor gives its
+        // offset); a group that crosses segment ends resolves them per lane.
+        int seg = 0;
+        int seg_end = __builtin_amdgcn_readfirstlane(sh.seg_pref[1]);
+        int seg_off = __builtin_amdgcn_readfirstlane(sh.seg_start[0]);
+        const int s0 = a.span[0], s1 = a.span[1], s2 = a.span[2];
+        const int rd0 = a.rd[0], rd1 = a.rd[1], rd2 = a.rd[2];
         int pos = 0;
-        for (int q = 0; q < 27; ++q) {
-            const int start = sh.bin_start[q], cnt = sh.bin_cnt[q], j0 = sh.bin_pref[q];
-            for (int jb = 0; jb < cnt; jb += 64) {
-                const int j = jb + lane;
-                uint32_t rk = 0;
-                if (j < cnt) {
-                    int dx, dy, dz;
-                    if (a.bin_xyz) {
-                        const uint32_t v = a.bin_xyz[start + j];
-                        dx = (int)(v & 2047u) - x;
-                        dy = (int)((v >> 11) & 2047u) - y;
-                        dz = (int)(v >> 22) - z;
-                    } else {
-                        const int c = a.bin_cells[start + j];
-                        dx = c % X - x;
-                        dy = c % (X * Y) / X - y;
-                        dz = c / (X * Y) - z;
-                    }
-                    if (abs(dx) <= a.span[0] && abs(dy) <= a.span[1] && abs(dz) <= a.span[2])
-                        rk = a.rank[((size_t)(dz + a.span[2]) * a.rd[1] + (dy + a.span[1])) * a.rd[0] +
-                                    (dx + a.span[0])];
+        for (int base = 0; base < raw_total; base += 64 * kGatherUnroll) {
+            // positions in the bin-sorted arrays (no memory reads but LDS)
+            int gi[kGatherUnroll];
+#pragma unroll
+            for (int u = 0; u < kGatherUnroll; ++u) {
+                const int gb = base + u * 64;
+                const int p = gb + lane;
+                while (seg < 8 && gb >= seg_end) {
+                    ++seg;
+                    seg_end = __builtin_amdgcn_readfirstlane(sh.seg_pref[seg + 1]);
+                    seg_off = __builtin_amdgcn_readfirstlane(sh.seg_start[seg] - sh.seg_pref[seg]);
                 }
-                const uint64_t mk = __ballot(rk != 0);
-                if (rk != 0) {
-                    const int p = pos + __popcll(mk & lt);
-                    const uint32_t ev = (rk << key.lb) | (uint32_t)(j0 + j);
-                    if (p < kCap)
-                        sh.e[p] = ev;
-                    else if (ge && p < a.scratch_per_wave)
-                        ge[p] = ev;
+                int g0 = seg_off + p;
+                if (seg < 8 && gb + 63 >= seg_end) {
+                    for (int g = seg + 1; g < 9; ++g) {
+                        const int pg = __builtin_amdgcn_readfirstlane(sh.seg_pref[g]);
+                        if (pg > gb + 63) break;
+                        if (p >= pg) g0 = sh.seg_start[g] - pg + p;
+                    }
+                }
+                gi[u] = p < raw_total ? g0 : 0;
+            }
+            // coordinates, then rank-table reads, each batch in flight together
+            int dx[kGatherUnroll], dy[kGatherUnroll], dz[kGatherUnroll];
+#pragma unroll
+            for (int u = 0; u < kGatherUnroll; ++u) {
+                if (PACKED) {
+                    const uint32_t w = a.bin_xyz[gi[u]];
+                    dx[u] = (int)(w & 2047u) - x;
+                    dy[u] = (int)((w >> 11) & 2047u) - y;
+                    dz[u] = (int)(w >> 22) - z;
+                } else {
+                    const int c = a.bin_cells[gi[u]];
+                    dx[u] = c % X - x;
+                    dy[u] = c % (X * Y) / X - y;
+                    dz[u] = c / (X * Y) - z;
+                }
+            }
+            uint32_t rk[kGatherUnroll];
+#pragma unroll
+            for (int u = 0; u < kGatherUnroll; ++u) {
+                // offsets within [-span, span] per axis index the rank table
+                const int ox = dx[u] + s0, oy = dy[u] + s1, oz = dz[u] + s2;
+                const bool in = base + u * 64 + lane < raw_total && (unsigned)ox < (unsigned)rd0 &&
+                                (unsigned)oy < (unsigned)rd1 && (unsigned)oz < (unsigned)rd2;
+                const int ri = in ? (oz * rd1 + oy) * rd0 + ox : 0;
+                const uint32_t r = a.rank[ri];
+                rk[u] = in ? r : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < kGatherUnroll; ++u) {
+                const uint64_t mk = __ballot(rk[u] != 0);
+                const int pp = pos + __popcll(mk & lt);
+                if (rk[u] != 0) {
+                    const uint32_t ev = (rk[u] << key.lb) | (uint32_t)(base + u * 64 + lane);
+                    if (pp < CAP)
+                        sh.e[pp] = ev;
+                    else if (ge && pp < a.scratch_per_wave)
+                        ge[pp] = ev;
                 }
                 pos += __popcll(mk);
             }
         }
         const int total = pos;
-        __syncthreads();
+        wave_sync();
         int *out = a.nbr + (size_t)t * a.max_n;
         bool sane = true;
-        if (total <= kCap) {
-            sane = topk_sort_(sh.e, sh.L, sh.R, sh, total, a.max_n, key);
+        if (total <= CAP) {
+            sane = topk_sort_(lb, sh, total, a.max_n, key);
             if (sane) finish_(sh.e, total, a.max_n, out, sh, a.bin_cells, key);
         } else if (ge && total <= a.scratch_per_wave) {
-            uint32_t *gL = ge + a.scratch_per_wave;
-            uint32_t *gR = gL + a.scratch_per_wave;
-            for (int p = lane; p < kCap; p += 64) ge[p] = sh.e[p];  // the head went to LDS
-            __syncthreads();
-            sane = topk_sort_(ge, gL, gR, sh, total, a.max_n, key);
+            for (int p = lane; p < CAP; p += 64) ge[p] = sh.e[p];  // the head went to LDS
+            wave_sync();
+            sane = topk_sort_(gb, sh, total, a.max_n, key);
             if (sane) finish_(ge, total, a.max_n, out, sh, a.bin_cells, key);
         } else {
             if (lane == 0) atomicOr(a.overflow, 1u);
@@ -359,7 +514,7 @@ __global__ __launch_bounds__(64) void k_cluster(ClusterArgs a) {
             a.cnt[t] = sane ? min(total, a.max_n) : 0;
             if (!sane) atomicOr(a.overflow, 2u);
         }
-        __syncthreads();
+        wave_sync();
     }
     if (lane == 0 && a.cand_total && cand_acc != 0) atomicAdd(counter_line(a.cand_total), cand_acc);
 }
@@ -371,7 +526,10 @@ int cluster_lds_capacity() { return kCap; }
 void launch_cluster_search(const ClusterArgs &a, hipStream_t s) {
     if (a.n_list <= 0) return;
     const int grid = a.n_list < a.n_waves ? a.n_list : a.n_waves;
-    hipLaunchKernelGGL(k_cluster, dim3(grid), dim3(64), 0, s, a);
+    if (a.bin_xyz)
+        hipLaunchKernelGGL(k_cluster<true>, dim3(grid), dim3(64), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_cluster<false>, dim3(grid), dim3(64), 0, s, a);
 }
 
 }  // namespace hpgl_amd

@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -18,7 +19,6 @@
 
 namespace hpgl_amd {
 
-int cluster_lds_capacity();
 
 void hip_check(hipError_t e, const char *what) {
     if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
@@ -319,6 +319,8 @@ void search(SearchSetup &ss, const int grid[3], const int radii[3], int max_n, c
         int span[3];
         for (int i = 0; i < 3; ++i) span[i] = ss.use_table ? radii[i] : 2 * radii[i] - 1;
         for (int i = 0; i < 3; ++i) span[i] = std::max(span[i], 0);
+        if ((2ll * span[0] + 1) * (2ll * span[1] + 1) * (2ll * span[2] + 1) >= (1ll << 31))
+            throw std::runtime_error("search radiuses too large for the cell-binned search");
         rt.build(ss.model, ss.use_table ? &ss.table : nullptr, span);
         st.ms_setup_host += now_ms() - t0;
         DBuf<uint32_t> d_rank;
@@ -365,7 +367,7 @@ void search(SearchSetup &ss, const int grid[3], const int radii[3], int max_n, c
         DBuf<uint32_t> scratch;
         if (max_cand > cluster_lds_capacity()) {
             ca.scratch_per_wave = (int)max_cand;
-            scratch.alloc((size_t)std::min<long long>(waves, n_clu) * max_cand * 3);
+            scratch.alloc((size_t)std::min<long long>(waves, n_clu) * cluster_scratch_words((int)max_cand));
             ca.scratch = scratch.p;
             ca.n_waves = (int)std::min<long long>(waves, n_clu);
         } else {

@@ -137,13 +137,23 @@ struct ClusterArgs {
     int local_bits;           // element = (rank << local_bits) | raw candidate index
     int n_waves;              // grid size (waves loop over the list)
     // global scratch for candidate lists that do not fit in LDS:
-    // per wave 3 x scratch_per_wave words (elements, left stops, right stops)
+    // cluster_scratch_words(scratch_per_wave) words per wave
     uint32_t *scratch;
     int scratch_per_wave;
     unsigned int *overflow;   // bit 0: a list exceeded scratch capacity; bit 1: sort replay error
     unsigned long long *cand_total;  // += candidates gathered per target
 };
 void launch_cluster_search(const ClusterArgs &a, hipStream_t s);
+// Scratch words per wave for lists of up to n candidates: stop masks and
+// counts per chunk of 64 and the elements (an even count keeps the masks
+// of the next wave 8-byte aligned).
+__host__ __device__ inline long long cluster_scratch_words(int n) {
+    const long long nchm = n / 64 + 2;
+    const long long w = 4 * nchm + n + 2 * nchm;
+    return w + (w & 1);
+}
+// Candidates a wavefront holds in LDS.
+int cluster_lds_capacity();
 
 // Sequential Gaussian simulation (sgs.hip), R realisations at once.
 struct SgsPathArgs {

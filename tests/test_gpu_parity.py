@@ -105,6 +105,31 @@ def test_simple_kriging_cluster_path(geo, precision):
         assert rel_err(out.data, od, 1.0) <= TOL
 
 
+@pytest.mark.parametrize("angle", [0.0, 30.0])
+def test_cluster_path_long_candidate_lists(geo, angle):
+    """Dense data and large radii: many candidate lists exceed what a
+    wavefront holds in LDS and are sorted in global scratch; with and
+    without rotation of the covariance model."""
+    shape = (90, 90, 24)
+    data, mask = scattered(shape, 0.03, 11, law="normal")
+    g = Grid(*shape)
+    cov = Cov(1, (30, 30, 8), angles=(angle, 0.0, 0.0), sill=1.0, nugget=0.05)
+    r, mx = (30, 30, 8), 24
+    pidx, pcnt = _dump_product(g.size, mx)
+    out = geo.simple_kriging((data, mask), g, r, mx, cov, mean=0.0)
+    st = geo.last_stats()
+    d = O.Dump(g.size, mx)
+    try:
+        od, om = O.simple_kriging((data, mask), g, r, mx, cov, mean=0.0)
+    finally:
+        d.close()
+    assert st["nodes_cluster_path"] > 0
+    assert st["cluster_candidates"] > 900 * st["nodes_cluster_path"]  # many lists beyond 1024
+    _same_lists(pidx, pcnt, d.idx, d.cnt)
+    assert np.array_equal(out.mask, om)
+    assert rel_err(out.data, od, 1.0) <= TOL
+
+
 def test_ordinary_kriging_cluster_path_analytic_keys(geo):
     """OK's cluster path orders by the analytic covariance, including
     candidates beyond the search radius."""
