@@ -100,6 +100,9 @@ def main():
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-compare", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain 1 GiB device copy (x * 1 -> y, 3 times), the reference kernel for "
+                         "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the kriging path has no multi-GPU mode in this build "
@@ -107,6 +110,17 @@ def main():
         sys.exit(2)
 
     import torch
+
+    if args.copy_calibration:
+        n = 256 * 1024 * 1024
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            torch.mul(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_kernel": "torch.mul(x, 1.0, out=y)", "bytes_read_per_launch": 4 * n,
+                          "bytes_written_per_launch": 4 * n, "launches": 3}), flush=True)
+        return
 
     from hpgl_amd.geo_bsd import hpgl_wrap
     from hpgl_amd.geo_bsd import _abi as abi
@@ -212,7 +226,9 @@ def main():
         "solves": {"fp32_accepted": int(acc.get("solves_fp32", 0)), "fp64": int(acc.get("solves_fp64", 0)),
                    "singular": int(acc.get("nodes_singular", 0)),
                    "no_neighbours": int(acc.get("nodes_no_neighbours", 0))},
-        "paths": {"template": int(acc.get("nodes_template_path", 0)), "cluster": int(acc.get("nodes_cluster_path", 0))},
+        "paths": {"template": int(acc.get("nodes_template_path", 0)), "cluster": int(acc.get("nodes_cluster_path", 0)),
+                  "cluster_candidates": int(acc.get("cluster_candidates", 0)),
+                  "cluster_lists_spilled": int(acc.get("cluster_lists_spilled", 0))},
         "host_setup_ms_per_step": round(acc.get("ms_setup_host", 0.0) / args.steps, 2),
         "device_ms_per_step": round(float(ev0.elapsed_time(ev1)) / args.steps, 3),
     }

@@ -231,6 +231,7 @@ API void hpgl_amd_get_last_stats(hpgl_amd_stats_t *o) {
     o->n_k_solve_fp64 = t_stats.n_k_solve_fp64;
     o->cluster_candidates = t_stats.cluster_candidates;
     o->neighbours_total = t_stats.neighbours_total;
+    o->cluster_lists_spilled = t_stats.cluster_lists_spilled;
 }
 
 API void hpgl_amd_set_neighbour_dump(int *indices, int *counts, long long n_cells, int max_n) {

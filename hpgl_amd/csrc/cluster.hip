@@ -38,8 +38,11 @@ namespace hpgl_amd {
 
 namespace {
 
-constexpr int kCap = 1024;  // candidates held in LDS per wavefront; longer lists go to global scratch
-constexpr int kStack = 64;
+// Candidates a wavefront holds in LDS: the kernel is built for 1024, 2048
+// and 4096 (about 6, 10 and 19 KB of LDS per wavefront) and the host picks
+// the smallest that the job's expected lists fit; longer lists go to global
+// scratch.
+constexpr int kCaps[3] = {1024, 2048, 4096};
 
 // A wavefront orders its own accesses (LDS and the global scratch lists)
 // with a workgroup-scope fence, which waits for them to complete; the
@@ -69,7 +72,6 @@ struct WaveLds {
     uint64_t ml[CAP / 64 + 1], mr[CAP / 64 + 1];
     int pl[CAP / 64 + 2], pr[CAP / 64 + 2];
     int lbuf[64], rbuf[64];  // positions of the 64 stop pairs being swapped
-    int stk_lo[kStack], stk_hi[kStack], stk_d[kStack];
     // the 27 bins as 9 rows (k, j) of three consecutive bins i-1, i, i+1,
     // which are adjacent in the bin-sorted cell arrays
     int seg_start[9];
@@ -291,46 +293,173 @@ __device__ int partition_(const SortBufs &b, S &sh, int lo, int hi, Keyed key) {
     return min(a, bq);
 }
 
+__device__ inline uint64_t readlane64(uint64_t v, int l) {
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, l);
+    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), l);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Offset from the range start of the k-th stop (0-based, k < total) when
+// lane c holds the mask m and the preceding count p of chunk c (c < nch).
+__device__ inline int nth_stop_reg(uint64_t m, int p, int nch, int k) {
+    int lo = 0, hi = nch;  // largest c < nch with p[c] <= k
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (__builtin_amdgcn_readlane(p, mid) <= k)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return (lo << 6) + select_bit(readlane64(m, lo), k - __builtin_amdgcn_readlane(p, lo));
+}
+
+// buf[idx] = position of the stop with index idx0 + idx for the stops of
+// index [idx0, idx0 + 64), from the chunk masks / counts held by the lanes.
+__device__ inline void expand_stops_reg(uint64_t m, int p, int nch, int c0, int idx0, int first, int *buf) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = lanemask_lt();
+    for (int c = c0; c < nch; ++c) {
+        const int pc = __builtin_amdgcn_readlane(p, c);
+        if (pc >= idx0 + 64) break;
+        const uint64_t mk = readlane64(m, c);
+        const int idx = pc + __popcll(mk & lt) - idx0;
+        if (((mk >> lane) & 1u) && idx >= 0 && idx < 64) buf[idx] = first + (c << 6) + lane;
+    }
+}
+
+// partition_ for ranges of at most 64 chunks (4097 elements): the chunk
+// masks and counts stay in registers, lane c holding chunk c.
+template <typename S>
+__device__ int partition_reg(uint32_t *e, S &sh, int lo, int hi, Keyed key) {
+    const int lane = threadIdx.x & 63;
+    {  // __move_median_to_first(lo, lo+1, mid, hi-1)
+        const int a = lo + 1, bm = lo + (hi - lo) / 2, c = hi - 1;
+        const uint32_t ka = key(e[a]), kb = key(e[bm]), kc = key(e[c]);
+        int sw;
+        if (ka > kb) {
+            if (kb > kc)
+                sw = bm;
+            else if (ka > kc)
+                sw = c;
+            else
+                sw = a;
+        } else if (ka > kc) {
+            sw = a;
+        } else if (kb > kc) {
+            sw = c;
+        } else {
+            sw = bm;
+        }
+        sw = __builtin_amdgcn_readfirstlane(sw);
+        wave_sync();
+        if (lane == 0) {
+            const uint32_t t0 = e[lo];
+            e[lo] = e[sw];
+            e[sw] = t0;
+        }
+        wave_sync();
+    }
+    const uint32_t kp = key(e[lo]);
+    const int first = lo + 1;
+    const int nch = (hi - first + 63) >> 6;
+    uint64_t my_ml = 0, my_mr = 0;
+    int my_pl = 0, my_pr = 0;
+    int nL = 0, nR = 0;
+    for (int c = 0; c < nch; c += 2) {
+        const int p0 = first + (c << 6) + lane, p1 = p0 + 64;
+        const bool v0 = p0 < hi, v1 = p1 < hi;
+        const uint32_t k0 = v0 ? key(e[p0]) : 0u;
+        const uint32_t k1 = v1 ? key(e[p1]) : 0u;
+        const uint64_t mL0 = __ballot(v0 && k0 <= kp), mR0 = __ballot(v0 && k0 >= kp);
+        const uint64_t mL1 = __ballot(v1 && k1 <= kp), mR1 = __ballot(v1 && k1 >= kp);
+        if (lane == c) {
+            my_ml = mL0;
+            my_mr = mR0;
+            my_pl = nL;
+            my_pr = nR;
+        }
+        nL += __popcll(mL0);
+        nR += __popcll(mR0);
+        if (lane == c + 1) {
+            my_ml = mL1;
+            my_mr = mR1;
+            my_pl = nL;
+            my_pr = nR;
+        }
+        nL += __popcll(mL1);
+        nR += __popcll(mR1);
+    }
+    const int m = min(nL, nR);
+    int s = 0;
+    int cl = 0, cr = nch - 1;
+    for (int kb = 0; kb < m; kb += 64) {
+        while (cl + 1 < nch && __builtin_amdgcn_readlane(my_pl, cl + 1) <= kb) ++cl;
+        expand_stops_reg(my_ml, my_pl, nch, cl, kb, first, sh.lbuf);
+        const int base = nR - 64 - kb;
+        const int from = base > 0 ? base : 0;
+        while (cr > 0 && __builtin_amdgcn_readlane(my_pr, cr) > from) --cr;
+        expand_stops_reg(my_mr, my_pr, nch, cr, base, first, sh.rbuf);
+        wave_sync();
+        const int k = kb + lane;
+        bool c = false;
+        int pl = 0, pr = 0;
+        if (k < m) {
+            pl = sh.lbuf[lane];
+            pr = sh.rbuf[63 - lane];
+            c = pl < pr;
+        }
+        if (c) {
+            const uint32_t tl = e[pl];
+            e[pl] = e[pr];
+            e[pr] = tl;
+        }
+        const int cnt = __popcll(__ballot(c));
+        s += cnt;
+        wave_sync();
+        if (cnt < 64) break;
+    }
+    const int a = s < nL ? first + nth_stop_reg(my_ml, my_pl, nch, s) : 0x7fffffff;
+    const int bq = s >= 1 ? first + nth_stop_reg(my_mr, my_pr, nch, nR - s) : 0x7fffffff;
+    return min(a, bq);
+}
+
 // Returns false if the replay reached a state libstdc++'s algorithm cannot
 // reach (a cut outside the range or a deeper pending stack than max_n
 // ranges); the caller then reports an internal error instead of looping.
-template <typename S>
+template <bool REG, typename S>
 __device__ bool topk_sort_(const SortBufs &b, S &sh, int n, int K, Keyed key) {
     const int lane = threadIdx.x & 63;
     if (n <= 1) return true;
     const int lg = 31 - __clz(n);
-    if (lane == 0) {
-        sh.stk_lo[0] = 0;
-        sh.stk_hi[0] = n;
-        sh.stk_d[0] = 2 * lg;
-    }
+    // pending ranges: lane i holds stack entry i
+    int s_lo = 0, s_hi = n, s_d = 2 * lg;
     int top = 1;
-    wave_sync();
     while (top > 0) {
         --top;
-        int lo = sh.stk_lo[top], hi = sh.stk_hi[top], d = sh.stk_d[top];
-        wave_sync();
+        int lo = __builtin_amdgcn_readlane(s_lo, top), hi = __builtin_amdgcn_readlane(s_hi, top),
+            d = __builtin_amdgcn_readlane(s_d, top);
         while (hi - lo > 16) {
             if (d == 0) {
+                wave_sync();
                 if (lane == 0) heap_sort_(b.e + lo, hi - lo, key);
                 wave_sync();
                 break;
             }
             --d;
-            const int cut = partition_(b, sh, lo, hi, key);
-            if (cut <= lo || cut >= hi || top >= kStack) return false;
+            const int cut = REG ? partition_reg(b.e, sh, lo, hi, key) : partition_(b, sh, lo, hi, key);
+            if (cut <= lo || cut >= hi || top >= 64) return false;
             if (cut < K) {
-                if (lane == 0) {
-                    sh.stk_lo[top] = cut;
-                    sh.stk_hi[top] = hi;
-                    sh.stk_d[top] = d;
+                if (lane == top) {
+                    s_lo = cut;
+                    s_hi = hi;
+                    s_d = d;
                 }
                 ++top;
-                wave_sync();
             }
             hi = cut;
         }
     }
+    wave_sync();
     return true;
 }
 
@@ -364,10 +493,9 @@ __device__ void finish_(const E *e, int n, int K, int *out, const S &sh, const i
 // One wavefront per workgroup; the wavefronts loop over the target list.
 // PACKED: candidate coordinates come from bin_xyz (x | y << 11 | z << 22),
 // otherwise they are decoded from the cell index.
-template <bool PACKED>
+template <bool PACKED, int CAP>
 __global__ __launch_bounds__(64) void k_cluster(ClusterArgs a) {
-    __shared__ WaveLds<kCap> sh;
-    constexpr int CAP = kCap;
+    __shared__ WaveLds<CAP> sh;
     const int wave_id = blockIdx.x, n_wave_ids = gridDim.x;
     const int lane = threadIdx.x & 63;
     const uint64_t lt = lanemask_lt();
@@ -389,7 +517,7 @@ __global__ __launch_bounds__(64) void k_cluster(ClusterArgs a) {
         ge = gb.e;
     }
     const SortBufs lb{sh.e, sh.ml, sh.mr, sh.pl, sh.pr};
-    unsigned long long cand_acc = 0;
+    unsigned long long cand_acc = 0, spill_acc = 0;
     for (int li = wave_id; li < a.n_list; li += n_wave_ids) {
         const int t = a.list ? a.list[li] : li;
         const int node = a.targets[t];
@@ -498,12 +626,13 @@ __global__ __launch_bounds__(64) void k_cluster(ClusterArgs a) {
         int *out = a.nbr + (size_t)t * a.max_n;
         bool sane = true;
         if (total <= CAP) {
-            sane = topk_sort_(lb, sh, total, a.max_n, key);
+            sane = topk_sort_<true>(lb, sh, total, a.max_n, key);
             if (sane) finish_(sh.e, total, a.max_n, out, sh, a.bin_cells, key);
         } else if (ge && total <= a.scratch_per_wave) {
+            spill_acc += 1;
             for (int p = lane; p < CAP; p += 64) ge[p] = sh.e[p];  // the head went to LDS
             wave_sync();
-            sane = topk_sort_(gb, sh, total, a.max_n, key);
+            sane = topk_sort_<false>(gb, sh, total, a.max_n, key);
             if (sane) finish_(ge, total, a.max_n, out, sh, a.bin_cells, key);
         } else {
             if (lane == 0) atomicOr(a.overflow, 1u);
@@ -517,19 +646,34 @@ __global__ __launch_bounds__(64) void k_cluster(ClusterArgs a) {
         wave_sync();
     }
     if (lane == 0 && a.cand_total && cand_acc != 0) atomicAdd(counter_line(a.cand_total), cand_acc);
+    if (lane == 0 && a.cand_total && spill_acc != 0) atomicAdd(counter_line(a.cand_total) + 1, spill_acc);
 }
 
 }  // namespace
 
-int cluster_lds_capacity() { return kCap; }
+int cluster_capacity(long long expected_max) {
+    for (int c : kCaps)
+        if (expected_max <= c) return c;
+    return kCaps[2];
+}
+
+template <bool PACKED>
+static void launch_cap(const ClusterArgs &a, int grid, hipStream_t s) {
+    if (a.cap <= kCaps[0])
+        hipLaunchKernelGGL((k_cluster<PACKED, kCaps[0]>), dim3(grid), dim3(64), 0, s, a);
+    else if (a.cap <= kCaps[1])
+        hipLaunchKernelGGL((k_cluster<PACKED, kCaps[1]>), dim3(grid), dim3(64), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_cluster<PACKED, kCaps[2]>), dim3(grid), dim3(64), 0, s, a);
+}
 
 void launch_cluster_search(const ClusterArgs &a, hipStream_t s) {
     if (a.n_list <= 0) return;
     const int grid = a.n_list < a.n_waves ? a.n_list : a.n_waves;
     if (a.bin_xyz)
-        hipLaunchKernelGGL(k_cluster<true>, dim3(grid), dim3(64), 0, s, a);
+        launch_cap<true>(a, grid, s);
     else
-        hipLaunchKernelGGL(k_cluster<false>, dim3(grid), dim3(64), 0, s, a);
+        launch_cap<false>(a, grid, s);
 }
 
 }  // namespace hpgl_amd

@@ -36,6 +36,7 @@ struct Stats {
     long long n_k_solve_fp64 = 0;
     long long cluster_candidates = 0;  // candidate cells gathered by the cluster path
     long long neighbours_total = 0;    // sum of neighbour counts over solved systems
+    long long cluster_lists_spilled = 0;  // cell-binned lists longer than the LDS capacity
 };
 
 // Neighbour dump requested by the caller (host buffers).

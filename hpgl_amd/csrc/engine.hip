@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -362,10 +363,16 @@ void search(SearchSetup &ss, const int grid[3], const int radii[3], int max_n, c
         if (lb >= 32 || (unsigned long long)max_rank >= (1ull << (32 - lb)))
             throw std::runtime_error("cluster search: covariance has too many distinct values for 32-bit sort keys");
         ca.local_bits = lb;
+        // expected list length: informed density x offsets with rank > 0,
+        // plus six standard deviations of a Poisson count
+        long long n_pos = 0;
+        for (uint32_t r : rt.rank) n_pos += r != 0;
+        const double expect = (double)I / ((double)grid[0] * grid[1] * grid[2]) * (double)n_pos;
+        ca.cap = cluster_capacity((long long)(expect + 6.0 * std::sqrt(expect) + 64.0));
         const int waves = 8192;
         ca.n_waves = waves;
         DBuf<uint32_t> scratch;
-        if (max_cand > cluster_lds_capacity()) {
+        if (max_cand > ca.cap) {
             ca.scratch_per_wave = (int)max_cand;
             scratch.alloc((size_t)std::min<long long>(waves, n_clu) * cluster_scratch_words((int)max_cand));
             ca.scratch = scratch.p;
@@ -393,7 +400,10 @@ void search(SearchSetup &ss, const int grid[3], const int radii[3], int max_n, c
         unsigned int ov = 0;
         hip_check(hipMemcpyAsync(&ov, overflow.p, sizeof(ov), hipMemcpyDeviceToHost, s), "copy overflow");
         hip_check(hipStreamSynchronize(s), "sync");
-        for (int l = 0; l < kCounterSlots; ++l) st.cluster_candidates += (long long)hc[(size_t)l * kCounterStride];
+        for (int l = 0; l < kCounterSlots; ++l) {
+            st.cluster_candidates += (long long)hc[(size_t)l * kCounterStride];
+            st.cluster_lists_spilled += (long long)hc[(size_t)l * kCounterStride + 1];
+        }
         if (ov & 1u) throw std::runtime_error("cluster search: candidate list exceeded the scratch capacity");
         if (ov & 2u) throw std::runtime_error("cluster search: internal error in the sort replay");
     }

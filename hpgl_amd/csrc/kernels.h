@@ -135,13 +135,14 @@ struct ClusterArgs {
     int *nbr;
     int *cnt;
     int local_bits;           // element = (rank << local_bits) | raw candidate index
+    int cap;                  // LDS list capacity, a value returned by cluster_capacity
     int n_waves;              // grid size (waves loop over the list)
     // global scratch for candidate lists that do not fit in LDS:
     // cluster_scratch_words(scratch_per_wave) words per wave
     uint32_t *scratch;
     int scratch_per_wave;
     unsigned int *overflow;   // bit 0: a list exceeded scratch capacity; bit 1: sort replay error
-    unsigned long long *cand_total;  // += candidates gathered per target
+    unsigned long long *cand_total;  // counter lines: word 0 candidates gathered, word 1 lists sorted in scratch
 };
 void launch_cluster_search(const ClusterArgs &a, hipStream_t s);
 // Scratch words per wave for lists of up to n candidates: stop masks and
@@ -152,8 +153,9 @@ __host__ __device__ inline long long cluster_scratch_words(int n) {
     const long long w = 4 * nchm + n + 2 * nchm;
     return w + (w & 1);
 }
-// Candidates a wavefront holds in LDS.
-int cluster_lds_capacity();
+// LDS list capacity of the cell-binned search kernel for lists of up to
+// expected_max candidates (the largest built capacity when none suffices).
+int cluster_capacity(long long expected_max);
 
 // Sequential Gaussian simulation (sgs.hip), R realisations at once.
 struct SgsPathArgs {

@@ -113,6 +113,7 @@ class _HPGL_AMD_STATS(C.Structure):
         ("n_k_solve_fp64", C.c_longlong),
         ("cluster_candidates", C.c_longlong),
         ("neighbours_total", C.c_longlong),
+        ("cluster_lists_spilled", C.c_longlong),
     ]
 
 

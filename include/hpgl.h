@@ -265,6 +265,7 @@ struct hpgl_amd_stats_t {
     long long n_k_solve_fp64;
     long long cluster_candidates;   /* candidate cells gathered by the cell-binned search */
     long long neighbours_total;     /* neighbours over all searched targets */
+    long long cluster_lists_spilled; /* cell-binned candidate lists sorted in global scratch */
 };
 void hpgl_amd_get_last_stats(struct hpgl_amd_stats_t *out);
 
