@@ -299,28 +299,34 @@ __device__ inline uint64_t readlane64(uint64_t v, int l) {
     return ((uint64_t)hi << 32) | lo;
 }
 
+// Highest lane whose predicate holds (the wavefront must hold at least one).
+__device__ inline int last_lane(bool pred) { return 63 - __clzll(__ballot(pred)); }
+
+// Position (0-63) of the j-th set bit of a wave-uniform mask m.
+__device__ inline int select_bit_wave(uint64_t m, int j) {
+    const int lane = threadIdx.x & 63;
+    const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+    return __ffsll((unsigned long long)__ballot(((m >> lane) & 1u) && below == j)) - 1;
+}
+
 // Offset from the range start of the k-th stop (0-based, k < total) when
 // lane c holds the mask m and the preceding count p of chunk c (c < nch).
 __device__ inline int nth_stop_reg(uint64_t m, int p, int nch, int k) {
-    int lo = 0, hi = nch;  // largest c < nch with p[c] <= k
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (__builtin_amdgcn_readlane(p, mid) <= k)
-            lo = mid;
-        else
-            hi = mid;
-    }
-    return (lo << 6) + select_bit(readlane64(m, lo), k - __builtin_amdgcn_readlane(p, lo));
+    const int lane = threadIdx.x & 63;
+    const int c = last_lane(lane < nch && p <= k);  // chunk 0 always qualifies
+    return (c << 6) + select_bit_wave(readlane64(m, c), k - __builtin_amdgcn_readlane(p, c));
 }
 
 // buf[idx] = position of the stop with index idx0 + idx for the stops of
 // index [idx0, idx0 + 64), from the chunk masks / counts held by the lanes.
-__device__ inline void expand_stops_reg(uint64_t m, int p, int nch, int c0, int idx0, int first, int *buf) {
+__device__ inline void expand_stops_reg(uint64_t m, int p, int nch, int idx0, int first, int *buf) {
     const int lane = threadIdx.x & 63;
     const uint64_t lt = lanemask_lt();
-    for (int c = c0; c < nch; ++c) {
+    const int from = idx0 > 0 ? idx0 : 0;
+    const int c0 = last_lane(lane < nch && p <= from);
+    const int c1 = last_lane(lane < nch && p < idx0 + 64);
+    for (int c = c0; c <= c1; ++c) {
         const int pc = __builtin_amdgcn_readlane(p, c);
-        if (pc >= idx0 + 64) break;
         const uint64_t mk = readlane64(m, c);
         const int idx = pc + __popcll(mk & lt) - idx0;
         if (((mk >> lane) & 1u) && idx >= 0 && idx < 64) buf[idx] = first + (c << 6) + lane;
@@ -391,14 +397,11 @@ __device__ int partition_reg(uint32_t *e, S &sh, int lo, int hi, Keyed key) {
     }
     const int m = min(nL, nR);
     int s = 0;
-    int cl = 0, cr = nch - 1;
     for (int kb = 0; kb < m; kb += 64) {
-        while (cl + 1 < nch && __builtin_amdgcn_readlane(my_pl, cl + 1) <= kb) ++cl;
-        expand_stops_reg(my_ml, my_pl, nch, cl, kb, first, sh.lbuf);
-        const int base = nR - 64 - kb;
-        const int from = base > 0 ? base : 0;
-        while (cr > 0 && __builtin_amdgcn_readlane(my_pr, cr) > from) --cr;
-        expand_stops_reg(my_mr, my_pr, nch, cr, base, first, sh.rbuf);
+        expand_stops_reg(my_ml, my_pl, nch, kb, first, sh.lbuf);
+        // right stops kb .. kb+63 from the end have indices
+        // nR-1-kb .. nR-64-kb counted from the start
+        expand_stops_reg(my_mr, my_pr, nch, nR - 64 - kb, first, sh.rbuf);
         wave_sync();
         const int k = kb + lane;
         bool c = false;
