@@ -32,6 +32,8 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+FP64_PEAK_TFLOPS = 78.6
 
 
 def make_cfg2(seed=1002):
@@ -187,23 +189,41 @@ def main():
     ms_per_step = wall * 1000.0 / args.steps
     value = nodes_per_step * args.steps / wall
 
-    # dominant kernel and its roofline (HBM-bound search / gather work)
+    # dominant kernel and its roofline.  Algorithmic bytes per kriged node:
+    # 5 B read (centre value + mask) + 5 B written + 8 B per neighbour
+    # (value + index); flops per SK system of n neighbours:
+    # n^3/3 + n^2/2 + n/6 (Cholesky) + 2n^2 (two triangular solves)
+    # + 2n + 2n (sums) + 20 (n(n+1)/2 + n + 1) (covariance entries).
     dom = max(range(4), key=lambda i: acc.get(kernel_keys[i], 0.0))
     dom_name = ["k_template_scan", "k_cluster", "k_solve_mixed", "k_solve_fp64"][dom]
     launches = max(1, acc.get(count_keys[dom], 0))
     avg_ms = acc.get(kernel_keys[dom], 0.0) / launches
-    if dom == 1:
-        # per target: target cell (4 B) + per candidate its cell index and its
-        # rank-table entry (4 + 4 B) + the neighbour list and count written (4 B each)
-        alg_bytes = (4 * acc["nodes_cluster_path"] + 8 * acc["cluster_candidates"] +
-                     4 * (acc["neighbours_total"] + acc["nodes_cluster_path"])) / launches
-    else:
-        # per target: value+mask read and written (5 + 5 B) + 8 B per neighbour (index + value)
-        alg_bytes = (10 * acc["nodes_target"] + 8 * acc["neighbours_total"]) / launches
+    n_avg = acc["neighbours_total"] / max(1, acc["nodes_target"])
+    bytes_per_node = 10.0 + 8.0 * n_avg
+    dom_nodes = {0: acc["nodes_template_path"], 1: acc["nodes_cluster_path"]}.get(
+        dom, acc["solves_fp32"] + acc["solves_fp64"])
+    alg_bytes = bytes_per_node * dom_nodes / launches
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_sk_ik_256x256x128.md")
+    if os.path.exists(tpath):
+        for line in open(tpath):
+            if line.startswith("bench-traffic %s bytes_per_launch=" % dom_name):
+                traffic = int(line.split("=", 1)[1])
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None, "kernel": dom_name,
-                "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": int(alg_bytes)}
+                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic, "kernel": dom_name,
+                "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": int(alg_bytes),
+                "bytes_per_node": round(bytes_per_node, 2)}
+    n = n_avg
+    flops_per_system = n ** 3 / 3 + n ** 2 / 2 + n / 6 + 2 * n ** 2 + 4 * n + 20 * (n * (n + 1) / 2 + n + 1)
+    solve_ms = acc.get("ms_k_solve_mixed", 0.0) + acc.get("ms_k_solve_fp64", 0.0)
+    systems = acc["solves_fp32"] + acc["solves_fp64"] + acc["nodes_singular"]
+    solve_tflops = flops_per_system * systems / (solve_ms * 1e-3) / 1e12 if solve_ms > 0 else 0.0
+    roofline_solve = {"bound": "fp32-valu" if args.precision == "mixed" else "fp64-valu",
+                      "achieved": round(solve_tflops, 3),
+                      "peak": FP32_PEAK_TFLOPS if args.precision == "mixed" else FP64_PEAK_TFLOPS,
+                      "unit": "TFLOP/s", "flops_per_system": round(flops_per_system, 1)}
+    roofline_solve["frac"] = round(solve_tflops / roofline_solve["peak"], 6)
 
     out = {
         "metric": "kriged nodes/s, mixed-prec vs plain FP64 at stated err; p50/p99 job latency",
@@ -222,6 +242,7 @@ def main():
                                "(radii = ranges)", "cells": g.size, "kriged_nodes_per_step": nodes_per_step,
                    "precision": args.precision},
         "roofline": roofline,
+        "roofline_solve": roofline_solve,
         "kernel_ms_per_step": {k: round(acc.get(k, 0.0) / args.steps, 3) for k in kernel_keys},
         "solves": {"fp32_accepted": int(acc.get("solves_fp32", 0)), "fp64": int(acc.get("solves_fp64", 0)),
                    "singular": int(acc.get("nodes_singular", 0)),

@@ -10,7 +10,9 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <list>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -223,30 +225,111 @@ void validate_search(const int grid[3], const int radii[3], int max_n) {
         throw std::runtime_error("max_neighbours must be in [0, " + std::to_string(kMaxNeighbours) + "]");
 }
 
-// Everything the search of one covariance needs on the device.
-struct SearchSetup {
+// Everything the search and the solves of one covariance need, built on
+// the host and uploaded once, then kept resident: jobs that repeat a
+// covariance model and radii (a serving loop, the categories of repeated IK
+// jobs) reuse them.  Keyed by device, model, radii and table use; the
+// cluster-path rank table and the OK pair table are built on first use.
+struct CovAssets {
     CovModel model;
-    bool use_table;
+    bool use_table = false;
+    int radii[3] = {0, 0, 0};
     BoxTable table;
     Template tmpl;
-    BinGeometry bins;
     DBuf<double> d_table;
     DBuf<int32_t> d_tmpl;
-    explicit SearchSetup(const CovSpec &c) : model(c), use_table(false) {}
+    std::mutex lazy;  // guards the members below
+    bool have_rank = false;
+    RankTable rank;
+    long long rank_positive = 0;  // offsets with rank > 0
+    DBuf<uint32_t> d_rank;
+    bool have_pair = false;
+    BoxTable pair;
+    DBuf<double> d_pair;
+    explicit CovAssets(const CovSpec &c) : model(c) {}
 };
+
+constexpr size_t kAssetCacheEntries = 32;
+
+struct AssetCache {
+    std::mutex mu;
+    std::list<std::pair<std::string, std::shared_ptr<CovAssets>>> lru;  // most recent first
+};
+AssetCache &asset_cache() {
+    static AssetCache c;
+    return c;
+}
+
+// Assets of (cov, radii, use_table) on the current device; builds them on a
+// miss (host time counted in ms_setup_host) and waits for the upload before
+// other streams may use them.
+std::shared_ptr<CovAssets> get_assets(const CovSpec &cov, const int radii[3], bool use_table, hipStream_t s,
+                                      Stats &st) {
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    std::string key(reinterpret_cast<const char *>(&dev), sizeof(dev));
+    key.append(reinterpret_cast<const char *>(&cov.type), sizeof(cov.type));
+    key.append(reinterpret_cast<const char *>(cov.ranges), sizeof(cov.ranges));
+    key.append(reinterpret_cast<const char *>(cov.angles), sizeof(cov.angles));
+    key.append(reinterpret_cast<const char *>(&cov.sill), sizeof(cov.sill));
+    key.append(reinterpret_cast<const char *>(&cov.nugget), sizeof(cov.nugget));
+    key.append(reinterpret_cast<const char *>(radii), 3 * sizeof(int));
+    key.push_back(use_table ? 1 : 0);
+    AssetCache &c = asset_cache();
+    {
+        std::lock_guard<std::mutex> g(c.mu);
+        for (auto it = c.lru.begin(); it != c.lru.end(); ++it)
+            if (it->first == key) {
+                auto e = it->second;
+                c.lru.splice(c.lru.begin(), c.lru, it);
+                return e;
+            }
+    }
+    auto e = std::make_shared<CovAssets>(cov);
+    e->use_table = use_table;
+    for (int i = 0; i < 3; ++i) e->radii[i] = radii[i];
+    const double t0 = now_ms();
+    if (use_table) e->table.build(e->model, radii);
+    build_template(e->model, use_table ? &e->table : nullptr, radii, e->tmpl);
+    st.ms_setup_host += now_ms() - t0;
+    if (use_table) upload(e->d_table, e->table.v, s);
+    upload(e->d_tmpl, e->tmpl.xyz, s);
+    hip_check(hipStreamSynchronize(s), "sync");
+    std::lock_guard<std::mutex> g(c.mu);
+    c.lru.emplace_front(key, e);
+    while (c.lru.size() > kAssetCacheEntries) c.lru.pop_back();
+    return e;
+}
+
+// Rank table of the cluster path over [-span, span]^3: span = r with a
+// table, 2r - 1 with the analytic model (candidates lie in the 27 bins).
+const CovAssets &assets_rank(CovAssets &a, hipStream_t s, Stats &st) {
+    std::lock_guard<std::mutex> g(a.lazy);
+    if (a.have_rank) return a;
+    int span[3];
+    for (int i = 0; i < 3; ++i) span[i] = std::max(a.use_table ? a.radii[i] : 2 * a.radii[i] - 1, 0);
+    if ((2ll * span[0] + 1) * (2ll * span[1] + 1) * (2ll * span[2] + 1) >= (1ll << 31))
+        throw std::runtime_error("search radiuses too large for the cell-binned search");
+    const double t0 = now_ms();
+    a.rank.build(a.model, a.use_table ? &a.table : nullptr, span);
+    a.rank_positive = 0;
+    for (uint32_t r : a.rank.rank) a.rank_positive += r != 0;
+    st.ms_setup_host += now_ms() - t0;
+    upload(a.d_rank, a.rank.rank, s);
+    hip_check(hipStreamSynchronize(s), "sync");
+    a.have_rank = true;
+    return a;
+}
 
 // Neighbour search for targets[0..T) over the informed cells inf[0..I).
 // Fills nbr[T][max_n] and cnt[T]; returns (template-path, cluster-path) counts.
-void search(SearchSetup &ss, const int grid[3], const int radii[3], int max_n, const uint8_t *d_mask,
+void search(CovAssets &as, const int grid[3], const int radii[3], int max_n, const uint8_t *d_mask,
             const int *d_targets, int T, const int *d_inf, int I, int *d_nbr, int *d_cnt, hipStream_t s, Stats &st) {
-    double t0 = now_ms();
-    if (ss.use_table) ss.table.build(ss.model, radii);
-    build_template(ss.model, ss.use_table ? &ss.table : nullptr, radii, ss.tmpl);
-    ss.bins.init(grid, radii, ss.tmpl.size());
-    st.ms_setup_host += now_ms() - t0;
-    if (ss.use_table) upload(ss.d_table, ss.table.v, s);
-    upload(ss.d_tmpl, ss.tmpl.xyz, s);
     if (T == 0) return;
+    struct {
+        BinGeometry bins;
+    } ss;
+    ss.bins.init(grid, radii, as.tmpl.size());
 
     BinArgs ba;
     ba.X = grid[0];
@@ -277,8 +360,8 @@ void search(SearchSetup &ss, const int grid[3], const int radii[3], int max_n, c
     sa.X = grid[0];
     sa.Y = grid[1];
     sa.Z = grid[2];
-    sa.tmpl = (const int4 *)ss.d_tmpl.p;
-    sa.tsize = ss.tmpl.size();
+    sa.tmpl = (const int4 *)as.d_tmpl.p;
+    sa.tsize = as.tmpl.size();
     sa.max_n = max_n;
     sa.targets = d_targets;
     sa.list = tlist.p;
@@ -315,17 +398,8 @@ void search(SearchSetup &ss, const int grid[3], const int radii[3], int max_n, c
         hip_check(hipcub::DeviceScan::ExclusiveSum(tmp2.p, bytes, bin_count.p, bin_start.p, (int)ss.bins.nbins, s),
                   "scan");
         // sort keys as ranks over the offsets a candidate can have
-        t0 = now_ms();
-        RankTable rt;
-        int span[3];
-        for (int i = 0; i < 3; ++i) span[i] = ss.use_table ? radii[i] : 2 * radii[i] - 1;
-        for (int i = 0; i < 3; ++i) span[i] = std::max(span[i], 0);
-        if ((2ll * span[0] + 1) * (2ll * span[1] + 1) * (2ll * span[2] + 1) >= (1ll << 31))
-            throw std::runtime_error("search radiuses too large for the cell-binned search");
-        rt.build(ss.model, ss.use_table ? &ss.table : nullptr, span);
-        st.ms_setup_host += now_ms() - t0;
-        DBuf<uint32_t> d_rank;
-        upload(d_rank, rt.rank, s);
+        const CovAssets &ra = assets_rank(as, s, st);
+        const RankTable &rt = ra.rank;
 
         ClusterArgs ca;
         ca.X = grid[0];
@@ -342,7 +416,7 @@ void search(SearchSetup &ss, const int grid[3], const int radii[3], int max_n, c
             launch_pack_xyz(cells_out.p, I, grid[0], grid[1], xyz.p, s);
             ca.bin_xyz = xyz.p;
         }
-        ca.rank = d_rank.p;
+        ca.rank = ra.d_rank.p;
         for (int i = 0; i < 3; ++i) {
             ca.span[i] = rt.span[i];
             ca.rd[i] = rt.d[i];
@@ -365,8 +439,7 @@ void search(SearchSetup &ss, const int grid[3], const int radii[3], int max_n, c
         ca.local_bits = lb;
         // expected list length: informed density x offsets with rank > 0,
         // plus six standard deviations of a Poisson count
-        long long n_pos = 0;
-        for (uint32_t r : rt.rank) n_pos += r != 0;
+        const long long n_pos = ra.rank_positive;
         const double expect = (double)I / ((double)grid[0] * grid[1] * grid[2]) * (double)n_pos;
         ca.cap = cluster_capacity((long long)(expect + 6.0 * std::sqrt(expect) + 64.0));
         const int waves = 8192;
@@ -570,11 +643,10 @@ void run_cont(const ContJob &job, hipStream_t s, int precision, const Dump *dump
     const int I = select_indices(IsNonZero{job.d_in_mask}, (int)N, inf, s);
     st.nodes_target += T;
 
-    SearchSetup ss(job.cov);
-    ss.use_table = !job.ordinary;
+    std::shared_ptr<CovAssets> as = get_assets(job.cov, job.radii, !job.ordinary, s, st);
     DBuf<int> nbr((size_t)T * std::max(job.max_n, 1)), cnt(T);
     hip_check(hipEventRecord(ev.e[1], s), "event");
-    search(ss, job.grid, job.radii, job.max_n, job.d_in_mask, targets.p, T, inf.p, I, nbr.p, cnt.p, s, st);
+    search(*as, job.grid, job.radii, job.max_n, job.d_in_mask, targets.p, T, inf.p, I, nbr.p, cnt.p, s, st);
     hip_check(hipEventRecord(ev.e[2], s), "event");
     write_dump(dump, 0, targets.p, T, nbr.p, cnt.p, job.max_n, s);
 
@@ -595,8 +667,6 @@ void run_cont(const ContJob &job, hipStream_t s, int precision, const Dump *dump
     // kernels read them from a table over every offset a neighbour pair can
     // take: neighbours lie within 2r-1 of the target per axis (cluster path;
     // r on the template path), so pairs within 4r-2.
-    BoxTable pair_table;
-    DBuf<double> d_pair;
     if (job.ordinary) {
         int pr[3];
         long long entries = 1;
@@ -605,16 +675,21 @@ void run_cont(const ContJob &job, hipStream_t s, int precision, const Dump *dump
             entries *= 2 * pr[i] + 1;
         }
         if (entries <= kMaxPairTable) {
-            const double t0 = now_ms();
-            pair_table.build(ss.model, pr);
-            st.ms_setup_host += now_ms() - t0;
-            upload(d_pair, pair_table.v, s);
-            sa.cov = make_devcov(ss.model, &pair_table, d_pair.p);
+            std::lock_guard<std::mutex> g(as->lazy);
+            if (!as->have_pair) {
+                const double t0 = now_ms();
+                as->pair.build(as->model, pr);
+                st.ms_setup_host += now_ms() - t0;
+                upload(as->d_pair, as->pair.v, s);
+                hip_check(hipStreamSynchronize(s), "sync");
+                as->have_pair = true;
+            }
+            sa.cov = make_devcov(as->model, &as->pair, as->d_pair.p);
         } else {
-            sa.cov = make_devcov(ss.model, nullptr, nullptr);
+            sa.cov = make_devcov(as->model, nullptr, nullptr);
         }
     } else {
-        sa.cov = make_devcov(ss.model, &ss.table, ss.d_table.p);
+        sa.cov = make_devcov(as->model, &as->table, as->d_table.p);
     }
     sa.ok_system = job.ordinary ? 1 : 0;
     sa.values_f = job.d_in;
@@ -665,15 +740,14 @@ void run_ind(const IndJob &job, hipStream_t s, int precision, const Dump *dump, 
     DBuf<int> nbr((size_t)T * maxn_all), cnt(T);
     for (int k = 0; k < K; ++k) {
         const IndicatorParams &c = job.cats[k];
-        SearchSetup ss(c.cov);
-        ss.use_table = true;
+        std::shared_ptr<CovAssets> as = get_assets(c.cov, c.radii, true, s, st);
         hipEvent_t a0, a1, b0, b1;
         hip_check(hipEventCreate(&a0), "event");
         hip_check(hipEventCreate(&a1), "event");
         hip_check(hipEventCreate(&b0), "event");
         hip_check(hipEventCreate(&b1), "event");
         hip_check(hipEventRecord(a0, s), "event");
-        search(ss, job.grid, c.radii, c.max_n, job.d_in_mask, targets.p, T, inf.p, I, nbr.p, cnt.p, s, st);
+        search(*as, job.grid, c.radii, c.max_n, job.d_in_mask, targets.p, T, inf.p, I, nbr.p, cnt.p, s, st);
         hip_check(hipEventRecord(a1, s), "event");
         write_dump(dump, (long long)k * N, targets.p, T, nbr.p, cnt.p, c.max_n, s);
         SolveArgs sa;
@@ -686,7 +760,7 @@ void run_ind(const IndJob &job, hipStream_t s, int precision, const Dump *dump, 
         sa.X = job.grid[0];
         sa.Y = job.grid[1];
         sa.Z = job.grid[2];
-        sa.cov = make_devcov(ss.model, &ss.table, ss.d_table.p);
+        sa.cov = make_devcov(as->model, &as->table, as->d_table.p);
         sa.ok_system = 0;
         sa.values_u8 = job.d_in;
         sa.category = job.median ? 1 : k;
@@ -852,14 +926,7 @@ void run_sgs(const SgsJob &job, hipStream_t s, int precision, const Dump *dump, 
 
     // covariance table and search template (the reference's SGS searches
     // with the template only, sequential_simulation.h:74)
-    SearchSetup ss(job.cov);
-    ss.use_table = true;
-    double t0 = now_ms();
-    ss.table.build(ss.model, job.radii);
-    build_template(ss.model, &ss.table, job.radii, ss.tmpl);
-    st.ms_setup_host += now_ms() - t0;
-    upload(ss.d_table, ss.table.v, s);
-    upload(ss.d_tmpl, ss.tmpl.xyz, s);
+    std::shared_ptr<CovAssets> as = get_assets(job.cov, job.radii, true, s, st);
 
     // realisations per chunk, from free device memory
     size_t free_b = 0, total_b = 0;
@@ -927,8 +994,8 @@ void run_sgs(const SgsJob &job, hipStream_t s, int precision, const Dump *dump, 
             sa.X = job.grid[0];
             sa.Y = job.grid[1];
             sa.Z = job.grid[2];
-            sa.tmpl = (const int4 *)ss.d_tmpl.p;
-            sa.tsize = ss.tmpl.size();
+            sa.tmpl = (const int4 *)as->d_tmpl.p;
+            sa.tsize = as->tmpl.size();
             sa.max_n = job.max_n;
             sa.targets = targets.p + h_off[r];
             sa.list = nullptr;
@@ -963,7 +1030,7 @@ void run_sgs(const SgsJob &job, hipStream_t s, int precision, const Dump *dump, 
             sa.X = job.grid[0];
             sa.Y = job.grid[1];
             sa.Z = job.grid[2];
-            sa.cov = make_devcov(ss.model, &ss.table, ss.d_table.p);
+            sa.cov = make_devcov(as->model, &as->table, as->d_table.p);
             sa.ok_system = job.ordinary ? 1 : 0;
             sa.mean_kind = 0;
             sa.out_var = var.p;
