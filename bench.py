@@ -94,6 +94,141 @@ def cpu_baseline(g, data, mask, cat, marg, sk, ik, budget_s=15.0, threads=16):
                       % (stride, nodes, a, b)}
 
 
+def make_serving_trace(n_jobs=60, rate=20.0, seed=1005):
+    """Poisson request stream of mixed OK / SK / IK / LVM jobs: grid axes
+    uniform in [32, 256] x [32, 256] x [32, 128], informed fraction
+    U(0.5%, 3%), covariance type U{0, 1, 2}, horizontal ranges U(5, 50) and
+    vertical ranges a quarter of that, radii = ranges, max_neighbours
+    U{8..32}; IK with 3 categories (data terciles); the LVM mean is a smooth
+    Gaussian field of correlation length 50 / 3 cells (range 50)."""
+    from cases import categories, scattered, smooth_field
+
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    jobs = []
+    for i in range(n_jobs):
+        t += rng.exponential(1.0 / rate)
+        kind = ["ok", "sk", "ik", "lvm"][int(rng.integers(4))]
+        shape = (int(rng.integers(32, 257)), int(rng.integers(32, 257)), int(rng.integers(32, 129)))
+        frac = float(rng.uniform(0.005, 0.03))
+        ctype = int(rng.integers(3))
+        h = float(rng.uniform(5, 50))
+        ranges = (h, h, max(h / 4, 1.0))
+        radii = tuple(max(1, int(round(v))) for v in ranges)
+        max_n = int(rng.integers(8, 33))
+        data, mask = scattered(shape, frac, int(rng.integers(1 << 30)), law="normal")
+        job = dict(t=t, kind=kind, shape=shape, ctype=ctype, ranges=ranges, radii=radii, max_n=max_n,
+                   data=data, mask=mask)
+        if kind == "ik":
+            job["cat"], job["marg"] = categories(data, mask, quantiles=(1 / 3, 2 / 3), k=3)
+        if kind == "lvm":
+            job["mean"] = smooth_field(shape, 50.0 / 3.0, int(rng.integers(1 << 30)))
+        jobs.append(job)
+    return jobs
+
+
+def _serving_params(abi, job):
+    def cov_kw():
+        return dict(covariance_type=job["ctype"], ranges=(C.c_double * 3)(*job["ranges"]),
+                    angles=(C.c_double * 3)(0.0, 0.0, 0.0), sill=1.0, nugget=0.1)
+
+    r = (C.c_int * 3)(*job["radii"])
+    if job["kind"] in ("ok", "lvm"):
+        return abi._HPGL_OK_PARAMS(radiuses=r, max_neighbours=job["max_n"], **cov_kw())
+    if job["kind"] == "sk":
+        return abi._HPGL_SK_PARAMS(radiuses=r, max_neighbours=job["max_n"], automatic_mean=1, mean=0.0, **cov_kw())
+    ikp = (abi._HPGL_IK_PARAMS * 3)()
+    for k in range(3):
+        ikp[k] = abi._HPGL_IK_PARAMS(radiuses=(C.c_int * 3)(*job["radii"]), max_neighbours=job["max_n"],
+                                     marginal_prob=job["marg"][k], **cov_kw())
+    return ikp
+
+
+def serving_bench(slots=4):
+    """Replays the request stream against the library's request loop:
+    jobs are submitted at their arrival times and overlap on the GPU."""
+    from hpgl_amd.geo_bsd import _abi as abi
+    from hpgl_amd.serving import Server
+
+    jobs = make_serving_trace()
+    kinds = {"ok": abi.JOB_OK, "sk": abi.JOB_SK, "lvm": abi.JOB_LVM}
+    with Server(slots=slots) as srv:
+        # warm the resident covariance assets and kernels with one small job
+        w = make_serving_trace(n_jobs=1, seed=7)[0]
+        srv.submit_cont(abi.JOB_SK, w["shape"], w["data"], w["mask"],
+                        _serving_params(abi, dict(w, kind="sk"))).wait()
+        handles = []
+        t0 = time.perf_counter()
+        for job in jobs:
+            delay = job["t"] - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            p = _serving_params(abi, job)
+            if job["kind"] == "ik":
+                handles.append(srv.submit_ik(job["shape"], job["cat"], job["mask"], p, 3))
+            else:
+                handles.append(srv.submit_cont(kinds[job["kind"]], job["shape"], job["data"], job["mask"], p,
+                                               mean=job.get("mean")))
+        results = [h.wait() for h in handles]
+        makespan = time.perf_counter() - t0
+    failed = [r["message"] for r in results if r["status"] != 0]
+    if failed:
+        raise RuntimeError("serving job failed: %s" % failed[0])
+    lat = np.array([r["latency_ms"] for r in results])
+    nodes = sum(r["nodes"] for r in results)
+    return jobs, {"jobs": len(jobs), "arrival_rate_per_s": 20.0, "slots": slots,
+                  "p50_latency_ms": round(float(np.percentile(lat, 50)), 2),
+                  "p99_latency_ms": round(float(np.percentile(lat, 99)), 2),
+                  "mean_queue_ms": round(float(np.mean([r["queue_ms"] for r in results])), 2),
+                  "kriged_nodes": int(nodes), "nodes_per_s": nodes / makespan, "makespan_s": round(makespan, 3),
+                  "trace": "Poisson 20 jobs/s, OK/SK/IK/LVM, grids 32..256 x 32..256 x 32..128, seed 1005"}
+
+
+def serving_cpu_baseline(jobs, threads=16, nodes_per_sample=20000):
+    """The same trace processed first-in first-out by the CPU oracle: each
+    job's service time is measured on a sample of its cells (every s-th
+    cell, about nodes_per_sample of them) and scaled to the job; latencies
+    follow from the arrival times of a single FIFO server."""
+    import oracle_binding as O
+    from cases import Cov, Grid
+
+    O.set_threads(threads)
+    service = []
+    nodes_all = 0
+    for job in jobs:
+        g = Grid(*job["shape"])
+        n_unk = int(np.sum(job["mask"] == 0))
+        nodes = g.size * 3 if job["kind"] == "ik" else n_unk
+        stride = max(1, g.size // nodes_per_sample)
+        cov = Cov(job["ctype"], job["ranges"], sill=1.0, nugget=0.1)
+        O.set_sample_stride(stride)
+        t0 = time.perf_counter()
+        if job["kind"] == "ok":
+            O.ordinary_kriging((job["data"], job["mask"]), g, job["radii"], job["max_n"], cov)
+        elif job["kind"] == "sk":
+            O.simple_kriging((job["data"], job["mask"]), g, job["radii"], job["max_n"], cov)
+        elif job["kind"] == "lvm":
+            O.lvm_kriging((job["data"], job["mask"]), g, job["mean"], job["radii"], job["max_n"], cov)
+        else:
+            ikd = [{"cov_model": cov, "radiuses": job["radii"], "max_neighbours": job["max_n"]} for _ in range(3)]
+            O.indicator_kriging((job["cat"], job["mask"], 3), g, ikd, job["marg"])
+        dt = time.perf_counter() - t0
+        O.set_sample_stride(1)
+        service.append(dt * stride)
+        nodes_all += nodes
+    end = 0.0
+    lat = []
+    for job, sv in zip(jobs, service):
+        start = max(job["t"], end)
+        end = start + sv
+        lat.append((end - job["t"]) * 1000.0)
+    return {"p50_latency_ms": round(float(np.percentile(lat, 50)), 1),
+            "p99_latency_ms": round(float(np.percentile(lat, 99)), 1),
+            "nodes_per_s": nodes_all / end, "cores": threads, "kind": "port",
+            "sample": "each job's cells sampled every s-th (about %d per job), service time scaled by s, "
+                      "FIFO single server over the arrival trace" % nodes_per_sample}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +237,7 @@ def main():
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-compare", action="store_true")
+    ap.add_argument("--no-serving", action="store_true", help="skip the request-stream (p50/p99) measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (x * 1 -> y, 3 times), the reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
@@ -272,6 +408,11 @@ def main():
                                "sk_rms_rel_err_vs_fp64": float(np.sqrt(np.mean(rel ** 2))),
                                "ik_category_agreement": float(np.mean(mixed_ik == d_iout.cpu().numpy()))}
 
+    if not args.no_serving:
+        trace, out["serving"] = serving_bench()
+        if not args.no_cpu_baseline:
+            out["serving"]["cpu_baseline"] = serving_cpu_baseline(trace)
+        del trace
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(g, data, mask, cat, marg, sk, ik)
     print(json.dumps(out), flush=True)

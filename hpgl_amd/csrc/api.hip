@@ -641,7 +641,9 @@ API void hpgl_amd_sgs_simulation_batch_dev(const float *d_in, const unsigned cha
     GUARD_END
 }
 
-API void hpgl_amd_report_error(const char *msg) {
+// Failure of a library component outside this file (the request loop):
+// sets the calling thread's last-error state as a failed entry point does.
+void hpgl_amd::report_error(const char *msg) {
     begin_call();
     fail(msg);
 }

@@ -112,4 +112,7 @@ void pool_free(void *p);
 
 void hip_check(hipError_t e, const char *what);
 
+// Sets the calling thread's last-error state (api.hip).
+void report_error(const char *msg);
+
 }  // namespace hpgl_amd

@@ -117,6 +117,43 @@ class _HPGL_AMD_STATS(C.Structure):
     ]
 
 
+class _HPGL_AMD_JOB(C.Structure):
+    """struct hpgl_amd_job_t (include/hpgl.h): one request of the request loop."""
+    _fields_ = [
+        ("kind", C.c_int),
+        ("shape", _HPGL_SHAPE),
+        ("cont_data", C.POINTER(C.c_float)),
+        ("cont_mask", C.POINTER(C.c_ubyte)),
+        ("ind_data", C.POINTER(C.c_ubyte)),
+        ("ind_mask", C.POINTER(C.c_ubyte)),
+        ("indicator_count", C.c_int),
+        ("mean", C.POINTER(C.c_float)),
+        ("params", C.c_void_p),
+        ("cdf", C.POINTER(hpgl_non_parametric_cdf_t)),
+        ("sim_mask", C.POINTER(C.c_ubyte)),
+        ("out_cont", C.POINTER(C.c_float)),
+        ("out_cont_mask", C.POINTER(C.c_ubyte)),
+        ("out_ind", C.POINTER(C.c_ubyte)),
+        ("out_ind_mask", C.POINTER(C.c_ubyte)),
+    ]
+
+
+class _HPGL_AMD_JOB_RESULT(C.Structure):
+    """struct hpgl_amd_job_result_t (include/hpgl.h)."""
+    _fields_ = [
+        ("status", C.c_int),
+        ("queue_ms", C.c_double),
+        ("latency_ms", C.c_double),
+        ("run_ms", C.c_double),
+        ("nodes", C.c_longlong),
+        ("hbm_bytes", C.c_longlong),
+        ("message", C.c_char * 256),
+    ]
+
+
+JOB_OK, JOB_SK, JOB_LVM, JOB_IK, JOB_MEDIAN_IK, JOB_SGS = range(6)
+
+
 # Module-level alias without name mangling (the reference relies on the
 # double-underscore name only inside this module).
 _HPGL_COV_PARAMS = __hpgl_cov_params_t

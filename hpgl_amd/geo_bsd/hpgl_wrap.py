@@ -15,7 +15,7 @@ from ._abi import (  # noqa: F401
     hpgl_output_handler, hpgl_progress_handler, _HPGL_MEAN_KIND, _HPGL_KRIGING_KIND, _HPGL_SHAPE,
     _HPGL_CONT_MASKED_ARRAY, _HPGL_IND_MASKED_ARRAY, _HPGL_UBYTE_ARRAY, _HPGL_FLOAT_ARRAY, _HPGL_OK_PARAMS,
     _HPGL_SK_PARAMS, _HPGL_SGS_PARAMS, _HPGL_MEDIAN_IK_PARAMS, _HPGL_IK_PARAMS, _HPGL_COV_PARAMS,
-    __hpgl_cov_params_t, hpgl_non_parametric_cdf_t, _HPGL_AMD_STATS)
+    __hpgl_cov_params_t, hpgl_non_parametric_cdf_t, _HPGL_AMD_STATS, _HPGL_AMD_JOB, _HPGL_AMD_JOB_RESULT)
 
 _PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIBRARY_PATH = os.path.join(_PKG_DIR, "hpgl.so")
@@ -89,6 +89,15 @@ _hpgl_so.hpgl_amd_sgs_simulation_batch_dev.restype = None
 _hpgl_so.hpgl_amd_sgs_simulation_batch_dev.argtypes = [
     C.c_void_p, C.c_void_p, _P(_HPGL_SHAPE), _P(_HPGL_SGS_PARAMS), _P(hpgl_non_parametric_cdf_t), C.c_void_p,
     C.c_void_p, _P(C.c_long), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+
+_hpgl_so.hpgl_amd_server_create.restype = C.c_void_p
+_hpgl_so.hpgl_amd_server_create.argtypes = [C.c_int, C.c_longlong]
+_hpgl_so.hpgl_amd_server_submit.restype = C.c_longlong
+_hpgl_so.hpgl_amd_server_submit.argtypes = [C.c_void_p, _P(_HPGL_AMD_JOB)]
+_hpgl_so.hpgl_amd_server_wait.restype = C.c_int
+_hpgl_so.hpgl_amd_server_wait.argtypes = [C.c_void_p, C.c_longlong, _P(_HPGL_AMD_JOB_RESULT)]
+_hpgl_so.hpgl_amd_server_destroy.restype = None
+_hpgl_so.hpgl_amd_server_destroy.argtypes = [C.c_void_p]
 
 _hpgl_so.hpgl_amd_set_precision.restype = None
 _hpgl_so.hpgl_amd_set_precision.argtypes = [C.c_int]

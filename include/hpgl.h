@@ -326,6 +326,58 @@ void hpgl_amd_sgs_simulation_batch_dev(const float *d_in, const unsigned char *d
                                        const long *seeds, int R,
                                        float *d_out, unsigned char *d_out_mask, void *stream);
 
+/* ---- Request loop with continuous batching (no reference counterpart:
+ * the reference runs one blocking call at a time).  Jobs wait in arrival
+ * order; a free worker slot (own HIP stream) starts the first waiting job
+ * whose estimated HBM footprint fits the unused budget, so jobs overlap on
+ * the device; a job larger than the budget runs alone.  A job's result is
+ * the one its entry point returns alone.  The caller keeps every buffer of
+ * a job alive until hpgl_amd_server_wait returns for it. ---- */
+#define HPGL_AMD_JOB_OK 0          /* hpgl_ordinary_kriging;  params: hpgl_ok_params_t */
+#define HPGL_AMD_JOB_SK 1          /* hpgl_simple_kriging;    params: hpgl_sk_params_t */
+#define HPGL_AMD_JOB_LVM 2         /* hpgl_lvm_kriging;       params: hpgl_ok_params_t, mean */
+#define HPGL_AMD_JOB_IK 3          /* hpgl_indicator_kriging; params: hpgl_ik_params_t[indicator_count] */
+#define HPGL_AMD_JOB_MEDIAN_IK 4   /* hpgl_median_ik;         params: hpgl_median_ik_params_t */
+#define HPGL_AMD_JOB_SGS 5         /* hpgl_sgs_simulation / _lvm_ (mean != NULL), in place on cont_* */
+
+struct hpgl_amd_job_t {
+    int kind;
+    struct hpgl_shape_t shape;
+    float *cont_data;               /* OK, SK, LVM input; SGS property (in place) */
+    unsigned char *cont_mask;
+    unsigned char *ind_data;        /* IK, median IK input */
+    unsigned char *ind_mask;
+    int indicator_count;
+    float *mean;                    /* LVM and LVM SGS per-cell mean, else NULL */
+    void *params;
+    struct hpgl_non_parametric_cdf_t *cdf;  /* SGS, may be NULL */
+    unsigned char *sim_mask;        /* SGS, may be NULL */
+    float *out_cont;                /* OK, SK, LVM output (prefilled as for the entry point) */
+    unsigned char *out_cont_mask;
+    unsigned char *out_ind;         /* IK, median IK output */
+    unsigned char *out_ind_mask;
+};
+
+struct hpgl_amd_job_result_t {
+    int status;                     /* 0 = success, else the call failed (message) */
+    double queue_ms;                /* submit to start */
+    double latency_ms;              /* submit to completion */
+    double run_ms;                  /* start to completion */
+    long long nodes;                /* cells estimated (x categories for IK) */
+    long long hbm_bytes;            /* footprint the job was admitted with */
+    char message[256];
+};
+
+/* slots = worker threads; hbm_budget_bytes <= 0 takes half the free HBM.
+ * Returns NULL on failure (message via hpgl_get_last_exception_message). */
+void *hpgl_amd_server_create(int slots, long long hbm_budget_bytes);
+/* Returns the job id, or -1 on failure. */
+long long hpgl_amd_server_submit(void *server, const struct hpgl_amd_job_t *job);
+/* Blocks until the job has finished; 0 on success, -1 for an unknown id. */
+int hpgl_amd_server_wait(void *server, long long id, struct hpgl_amd_job_result_t *result);
+/* Waits for every submitted job, then stops the workers. */
+void hpgl_amd_server_destroy(void *server);
+
 #ifdef __cplusplus
 }
 #endif

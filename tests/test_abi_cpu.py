@@ -39,6 +39,35 @@ def test_struct_sizes_match_reference_layouts():
     assert a._HPGL_OK_PARAMS.radiuses.offset == 72 and a._HPGL_OK_PARAMS.max_neighbours.offset == 84
 
 
+def test_extension_struct_layouts_match_the_header():
+    """hpgl_amd_job_t / hpgl_amd_job_result_t / hpgl_amd_stats_t as the C
+    compiler lays them out from include/hpgl.h."""
+    import subprocess
+    import tempfile
+
+    from hpgl_amd.geo_bsd import _abi as a
+
+    prog = r"""
+#include <stddef.h>
+#include <stdio.h>
+#include "hpgl.h"
+int main(void) {
+    printf("%zu %zu %zu %zu %zu %zu\n", sizeof(struct hpgl_amd_job_t), offsetof(struct hpgl_amd_job_t, params),
+           offsetof(struct hpgl_amd_job_t, out_ind_mask), sizeof(struct hpgl_amd_job_result_t),
+           offsetof(struct hpgl_amd_job_result_t, message), sizeof(struct hpgl_amd_stats_t));
+    return 0;
+}
+"""
+    with tempfile.TemporaryDirectory() as d:
+        src, exe = os.path.join(d, "l.c"), os.path.join(d, "l")
+        open(src, "w").write(prog)
+        subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), src, "-o", exe])
+        got = [int(v) for v in subprocess.check_output([exe]).split()]
+    want = [C.sizeof(a._HPGL_AMD_JOB), a._HPGL_AMD_JOB.params.offset, a._HPGL_AMD_JOB.out_ind_mask.offset,
+            C.sizeof(a._HPGL_AMD_JOB_RESULT), a._HPGL_AMD_JOB_RESULT.message.offset, C.sizeof(a._HPGL_AMD_STATS)]
+    assert got == want
+
+
 def test_errors_are_reported_not_thrown():
     """File I/O is off the kriging path: the call returns and sets the error."""
     from hpgl_amd.geo_bsd import hpgl_wrap
