@@ -368,6 +368,7 @@ void search(CovAssets &as, const int grid[3], const int radii[3], int max_n, con
     sa.n_list = n_tmpl;
     sa.nbr = d_nbr;
     sa.cnt = d_cnt;
+    sa.simbits = nullptr;
     if (n_tmpl > 0) {
         KTimer kt(s);
         launch_template_scan(sa, 0, s);
@@ -985,6 +986,7 @@ void run_sgs(const SgsJob &job, hipStream_t s, int precision, const Dump *dump, 
 
         // neighbours of every target among the cells informed before it
         DBuf<int> nbr((size_t)std::max<long long>(tot, 1) * max_n), cnt((size_t)std::max<long long>(tot, 1));
+        DBuf<uint64_t> simbits((size_t)std::max<long long>(tot, 1));
         hip_check(hipEventRecord(ev.e[1], s), "event");
         for (int r = 0; r < rc; ++r) {
             if (h_T[r] == 0) continue;
@@ -1002,6 +1004,7 @@ void run_sgs(const SgsJob &job, hipStream_t s, int precision, const Dump *dump, 
             sa.n_list = h_T[r];
             sa.nbr = nbr.p + (size_t)h_off[r] * max_n;
             sa.cnt = cnt.p + h_off[r];
+            sa.simbits = simbits.p + h_off[r];
             launch_template_scan(sa, 1, s);
             hip_check(hipGetLastError(), "template scan");
             st.nodes_template_path += h_T[r];
@@ -1078,13 +1081,13 @@ void run_sgs(const SgsJob &job, hipStream_t s, int precision, const Dump *dump, 
             a.status = status.p;
             a.draws = draws.p;
             a.draw_stride = std::max(max_T, 1);
-            a.key = key.p;
+            a.simbits = simbits.p;
             a.lvm = job.d_lvm ? lvm.p : nullptr;
             a.values = vals;
             a.done = done.p;
             a.ticket = ticket.p;
             a.error = err.p;
-            a.spin_limit = 1ll << 24;
+            a.spin_limit = 1ll << 26;
             const int waves = (int)std::max<long long>(1, std::min<long long>((max_T + 63) / 64, 8192 / rc));
             launch_sgs_sweep(a, rc, waves, s);
             hip_check(hipGetLastError(), "sweep");

@@ -100,6 +100,9 @@ struct ScanArgs {
     int n_list;
     int *nbr;
     int *cnt;
+    // pred_kind 1 only, may be nullptr: bit i set when neighbour i of the
+    // target is a simulated cell (key >= 0) rather than conditioning data
+    uint64_t *simbits;
 };
 void launch_template_scan(const ScanArgs &a, int pred_kind, hipStream_t s);
 
@@ -186,7 +189,7 @@ struct SgsSweepArgs {
     const uint8_t *status;  // ST_OK when the system was solved
     const uint32_t *draws;  // [R][draw_stride]
     long long draw_stride;
-    const int *key;         // [R][N]
+    const uint64_t *simbits;  // per target: neighbours that are simulated cells
     const float *lvm;       // per-cell mean (transformed) or nullptr
     float *values;          // [R][N]
     int *done;              // [R][N], 1 once the cell's value is written

@@ -30,6 +30,7 @@ __global__ __launch_bounds__(256) void k_template_scan(ScanArgs a) {
     int *out = a.nbr + (size_t)t * a.max_n;
     const int max_n = a.max_n;
     bool want = active && count < max_n;
+    uint64_t sim = 0;
     for (int k = 0; k < a.tsize; ++k) {
         if ((k & 7) == 0 && !__any(want)) break;
         const int4 o = a.tmpl[k];
@@ -38,10 +39,13 @@ __global__ __launch_bounds__(256) void k_template_scan(ScanArgs a) {
             if ((unsigned)px < (unsigned)X && (unsigned)py < (unsigned)Y && (unsigned)pz < (unsigned)Z) {
                 const int idx = (pz * Y + py) * X + px;
                 bool inf;
-                if (PRED == 0)
+                if (PRED == 0) {
                     inf = a.mask[idx] != 0;
-                else
-                    inf = a.key[idx] < my_key;
+                } else {
+                    const int32_t kc = a.key[idx];
+                    inf = kc < my_key;
+                    if (inf && kc >= 0) sim |= 1ull << count;
+                }
                 if (inf) {
                     out[count++] = idx;
                     want = count < max_n;
@@ -50,6 +54,7 @@ __global__ __launch_bounds__(256) void k_template_scan(ScanArgs a) {
         }
     }
     if (active) a.cnt[t] = count;
+    if (PRED == 1 && active && a.simbits) a.simbits[t] = sim;
 }
 
 __global__ void k_bin_ids(const int *cells, int n, BinArgs b, int *bin_of, uint32_t *bin_count) {

@@ -207,13 +207,14 @@ __global__ __launch_bounds__(64) void k_sgs_sweep(SgsSweepArgs a) {
     const int r = blockIdx.y;
     const int lane = threadIdx.x;
     const int T = a.T[r];
-    const int *targets = a.targets + a.t_off[r];
-    const int *nbr = a.nbr + (size_t)a.t_off[r] * a.max_n;
-    const int *cnt = a.cnt + a.t_off[r];
-    const double *wts = a.w + (size_t)a.t_off[r] * a.max_n;
-    const uint8_t *status = a.status + a.t_off[r];
+    const long long off = a.t_off[r];
+    const int *targets = a.targets + off;
+    const int *nbr = a.nbr + (size_t)off * a.max_n;
+    const int *cnt = a.cnt + off;
+    const double *wts = a.w + (size_t)off * a.max_n;
+    const uint8_t *status = a.status + off;
+    const uint64_t *simbits = a.simbits + off;
     const uint32_t *draws = a.draws + (size_t)r * a.draw_stride;
-    const int *key = a.key + (size_t)r * a.N;
     float *val = a.values + (size_t)r * a.N;
     int *done = a.done + (size_t)r * a.N;
     unsigned int *ticket = a.ticket + r;
@@ -225,26 +226,44 @@ __global__ __launch_bounds__(64) void k_sgs_sweep(SgsSweepArgs a) {
         if (base >= T) break;
         const int k = base + lane;
         bool pending = k < T;
+        // simulated neighbours not yet seen done (conditioning data never wait)
+        uint64_t waitset = pending ? simbits[k] : 0ull;
+        const int *row = nbr + (size_t)(pending ? k : 0) * a.max_n;
         long long spins = 0;
         while (__any(pending)) {
             if (pending) {
-                const int n = cnt[k];
-                bool ready = true;
-                for (int i = 0; i < n && ready; ++i) {
-                    const int nb = nbr[(size_t)k * a.max_n + i];
-                    if (key[nb] >= 0 && __hip_atomic_load(done + nb, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0)
-                        ready = false;
+                // poll up to eight outstanding flags per pass, loads in flight together
+                int idx[8];
+                int m = 0;
+                uint64_t ws = waitset;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    idx[q] = ws ? __ffsll((unsigned long long)ws) - 1 : -1;
+                    if (ws) ws &= ws - 1;
                 }
-                if (ready) {
+                int fl[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    fl[q] = idx[q] >= 0 ? __hip_atomic_load(done + row[idx[q]], __ATOMIC_RELAXED,
+                                                            __HIP_MEMORY_SCOPE_AGENT)
+                                        : 0;
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    if (idx[q] >= 0 && fl[q] != 0) waitset &= ~(1ull << idx[q]);
+                (void)m;
+                if (waitset == 0) {
+                    // every neighbour's value is published: order the reads after the flags
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                     const int cell = targets[k];
+                    const int n = cnt[k];
                     const bool ok = status[k] == ST_OK;
-                    const double v0 = a.var[a.t_off[r] + k];
+                    const double v0 = a.var[off + k];
                     const float m0 = a.lvm ? a.lvm[cell] : 0.0f;
                     // combine (combiner.h:10-24) in neighbour order
                     float mean = m0;
                     if (ok) {
                         for (int i = 0; i < n; ++i) {
-                            const int nb = nbr[(size_t)k * a.max_n + i];
+                            const int nb = row[i];
                             const float mi = a.lvm ? a.lvm[nb] : 0.0f;
                             mean += (float)(wts[(size_t)k * a.max_n + i] * (double)(val[nb] - mi));
                         }
@@ -261,7 +280,7 @@ __global__ __launch_bounds__(64) void k_sgs_sweep(SgsSweepArgs a) {
                 if (pending) atomicOr(a.error, 1u);
                 break;
             }
-            if (__any(pending)) __builtin_amdgcn_s_sleep(2);
+            if (__any(pending)) __builtin_amdgcn_s_sleep(1);
         }
     }
 }
