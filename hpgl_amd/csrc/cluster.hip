@@ -606,8 +606,8 @@ __global__ __launch_bounds__(64) void k_cluster(ClusterArgs a) {
                 const int ox = dx[u] + s0, oy = dy[u] + s1, oz = dz[u] + s2;
                 const bool in = base + u * 64 + lane < raw_total && (unsigned)ox < (unsigned)rd0 &&
                                 (unsigned)oy < (unsigned)rd1 && (unsigned)oz < (unsigned)rd2;
-                const int ri = in ? (oz * rd1 + oy) * rd0 + ox : 0;
-                const uint32_t r = a.rank[ri];
+                const int full = (oz * rd1 + oy) * rd0 + ox;
+                const uint32_t r = a.rank[in ? full : 0];
                 rk[u] = in ? r : 0u;
             }
 #pragma unroll

@@ -534,7 +534,8 @@ struct KeyEqStatus {
 
 // FP64 solves of targets whose bucket key is k (and, when `only_fallback`,
 // whose status is ST_FALLBACK): quad kernel up to order 16, lane-per-row
-// kernel above.
+// kernels above.
+void launch_fp64_subset(const SolveArgs &b, int order, hipStream_t s, Stats &st);
 void solve_fp64_bucket(SolveArgs sa, const DBuf<uint8_t> &keys, uint8_t k, bool only_fallback,
                        const DBuf<uint8_t> &status, hipStream_t s, Stats &st) {
     DBuf<int> list;
@@ -546,19 +547,38 @@ void solve_fp64_bucket(SolveArgs sa, const DBuf<uint8_t> &keys, uint8_t k, bool 
     if (n == 0) return;
     sa.subset = list.p;
     sa.n_subset = n;
-    const int order = k == 8 ? 64 : 4 * (k + 1);
+    launch_fp64_subset(sa, k == 8 ? 64 : 4 * (k + 1), s, st);
+}
+
+struct SubsetStatusEq {
+    const int *list;
+    const uint8_t *st;
+    uint8_t v;
+    __host__ __device__ bool operator()(const int &i) const { return st[list[i]] == v; }
+};
+
+// Systems per bucket solved first in mixed precision to decide whether the
+// bucket's model is too ill-conditioned for FP32 (Gaussian covariance with a
+// tiny nugget, near-duplicate samples): when more than half of them are
+// handed to FP64, the rest of the bucket goes to FP64 directly.
+constexpr int kMixedProbe = 16384;
+
+void launch_fp64_subset(const SolveArgs &b, int order, hipStream_t s, Stats &st) {
     KTimer kt(s);
-    if (order <= kQuadMaxFp64 && sa.cov.table)
-        launch_solve_quad(sa, order, true, s);
+    if (order <= kQuadMaxFp64 && b.cov.table)
+        launch_solve_quad(b, order, true, s);
     else
-        launch_solve_fp64(sa, order <= 32 ? 32 : 64, s);
+        launch_solve_fp64(b, order <= 32 ? 32 : 64, s);
     hip_check(hipGetLastError(), "solve fp64");
     st.ms_k_solve_fp64 += kt.stop();
     st.n_k_solve_fp64 += 1;
 }
 
-// Solve all targets, bucketed by neighbour count: the mixed kernels then
-// FP64 for the systems they hand over, or FP64 only.
+// Solve all targets, bucketed by neighbour count.  Mixed mode: the quad
+// kernels (orders <= 32 with a covariance table) in FP32 + FP64 refinement,
+// then FP64 for the systems they hand over; wider systems and the
+// analytic-covariance fallback go to the FP64 kernels directly.  FP64 mode:
+// every system through the FP64 kernels.
 void solve(SolveArgs sa, int epi, int precision, DBuf<uint8_t> &status, hipStream_t s, Stats &st) {
     sa.status = status.p;
     sa.subset = nullptr;
@@ -570,26 +590,43 @@ void solve(SolveArgs sa, int epi, int precision, DBuf<uint8_t> &status, hipStrea
     // the neighbour limit bounds the buckets that can be non-empty
     const int maxn = sa.max_n;
     const int kmax = maxn > kQuadMaxMixed ? 8 : std::max(0, (maxn + 3) / 4 - 1);
-    if (precision == 0) {
-        for (int k = 0; k <= kmax; ++k) {
-            SolveArgs b = sa;
-            DBuf<int> list;
-            const int n = select_indices(KeyEq{keys.p, (uint8_t)k}, sa.T, list, s);
-            if (n == 0) continue;
-            b.subset = list.p;
-            b.n_subset = n;
+    for (int k = 0; k <= kmax; ++k) {
+        const int order = k == 8 ? 64 : 4 * (k + 1);
+        const bool mixed = precision == 0 && k < 8 && sa.cov.table;
+        if (!mixed) {
+            solve_fp64_bucket(sa, keys, (uint8_t)k, false, status, s, st);
+            continue;
+        }
+        SolveArgs b = sa;
+        DBuf<int> list;
+        const int n = select_indices(KeyEq{keys.p, (uint8_t)k}, sa.T, list, s);
+        if (n == 0) continue;
+        const int probe = n > 4 * kMixedProbe ? kMixedProbe : n;
+        b.subset = list.p;
+        b.n_subset = probe;
+        {
             KTimer kt(s);
-            if (k < 8 && sa.cov.table)
-                launch_solve_quad(b, 4 * (k + 1), false, s);
-            else
-                launch_solve_mixed(b, k < 8 ? 32 : 64, s);
+            launch_solve_quad(b, order, false, s);
             hip_check(hipGetLastError(), "solve mixed");
             st.ms_k_solve_mixed += kt.stop();
             st.n_k_solve_mixed += 1;
         }
-        for (int k = 0; k <= kmax; ++k) solve_fp64_bucket(sa, keys, (uint8_t)k, true, status, s, st);
-    } else {
-        for (int k = 0; k <= kmax; ++k) solve_fp64_bucket(sa, keys, (uint8_t)k, false, status, s, st);
+        if (probe < n) {
+            DBuf<int> tmp;
+            const int fb = select_indices(SubsetStatusEq{list.p, status.p, (uint8_t)ST_FALLBACK}, probe, tmp, s);
+            b.subset = list.p + probe;
+            b.n_subset = n - probe;
+            if (2 * fb > probe) {
+                launch_fp64_subset(b, order, s, st);
+            } else {
+                KTimer kt(s);
+                launch_solve_quad(b, order, false, s);
+                hip_check(hipGetLastError(), "solve mixed");
+                st.ms_k_solve_mixed += kt.stop();
+                st.n_k_solve_mixed += 1;
+            }
+        }
+        solve_fp64_bucket(sa, keys, (uint8_t)k, true, status, s, st);
     }
 }
 

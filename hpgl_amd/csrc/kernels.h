@@ -73,8 +73,8 @@ struct SolveArgs {
     int epi;  // EpilogueKind
 };
 
-// One wavefront per 64/G systems, lane = row (any order up to G = 64).
-void launch_solve_mixed(const SolveArgs &a, int group, hipStream_t s);
+// FP64 solves (solve.hip): group <= 32 two systems per wavefront with rows
+// in registers, otherwise one system per wavefront with rows in LDS.
 void launch_solve_fp64(const SolveArgs &a, int group, hipStream_t s);
 // Four lanes per system (solve_quad.hip), order = bucket size: a multiple
 // of 4 up to kQuadMaxMixed (mixed) or kQuadMaxFp64 (FP64).

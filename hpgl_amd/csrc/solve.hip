@@ -1,26 +1,17 @@
-// solve.hip - per-node kriging systems on CDNA4: build, factor, solve,
-// combine, in one kernel per precision.
+// solve.hip - per-node kriging systems in FP64 in the reference's operation
+// order: the plain-FP64 path, the fallback of the mixed-precision quad
+// kernels (solve_quad.hip) and the path for systems wider than 32.
 //
-// Mapping: one wavefront (64 lanes, one workgroup) holds 64/G systems of
-// order n <= G; lane i of a group owns row i of A (and of L) in VGPRs.
-// Rows n..G-1 are padded with the identity so every group runs the same
-// unrolled code.  Cross-lane traffic (pivot column, right-hand sides,
-// transposed factor) goes through LDS; a workgroup is a single wavefront,
-// so __syncthreads() is only an LDS ordering point.
-//
-// k_solve_mixed: FP32 Cholesky (right-looking, explicit fmaf), two
-//   refinement steps whose residual b - A x is formed in FP64 from the FP64
-//   row of A kept in registers.  A system goes to the FP64 kernel when the
-//   FP32 factorisation breaks down, when the pivot-based condition estimate
-//   max(A_kk) / min(pivot) exceeds 1e4 (then FP32 rounding could decide
-//   the sign of a pivot the FP64 factorisation sees), or when the second
-//   correction is larger than 1e-9 of the solution (refinement did not
-//   converge to FP64 accuracy).
-// k_solve_fp64: the reference's dpotf2 + dpotrs operation order
-//   (left-looking sequential dot products, reciprocal scaling, division in
-//   the triangular solves, sequential subtraction), compiled with FMA
-//   contraction off: with the same A and b it returns the reference's
-//   weights bit for bit, and the same success/failure decision.
+// k_solve_fp64<G> (G = 32): two systems per wavefront, lane i of a group
+//   holds row i of the factor in registers; row j and the solution
+//   components are broadcast with v_readlane.
+// k_solve_fp64_lds (orders 33..64): one system per wavefront, rows of the
+//   factor in LDS (stride 65), row j read as a broadcast.
+// Both follow dpotf2 + dpotrs (left-looking sequential dot products,
+// reciprocal scaling, division in the triangular solves, sequential
+// subtraction) with FMA contraction off: with the same A and b they return
+// the reference's weights bit for bit, and the same success/failure
+// decision.  Rows n..G-1 of a group are padded with the identity.
 //
 // The combine (combiner.h:10-24) is evaluated exactly as the reference:
 // each term (float)(w_i * (double)(v_i - m_i)) is formed per lane, then
@@ -82,30 +73,10 @@ __device__ inline void cell_xyz(int cell, int X, int Y, int &x, int &y, int &z) 
 }
 
 template <int G>
-struct Shared {
-    static constexpr int NG = 64 / G;
-    float col[NG][G + 1];
-    float Lt[NG][G * (G + 1)];
-    double xb[NG][G];
-    float y1[NG][G];
-    float y2[NG][G];
-    float term[NG][G];
-    double dterm[NG][G];
-    int xyz[NG][3][G];
-    double red[NG][4];
-};
-
-template <int G>
 struct Shared64 {
     static constexpr int NG = 64 / G;
-    double row[NG][G];
-    double Ut[NG][G * (G + 1)];
     double y1[NG][G];
     double y2[NG][G];
-    int nz1[NG][G];
-    int nz2[NG][G];
-    double piv[NG][2];
-    int fail[NG];
     float term[NG][G];
     double dterm[NG][G];
     int xyz[NG][3][G];
@@ -215,182 +186,37 @@ __device__ inline void epilogue(const SolveArgs &a, bool write, int t, int n, in
 }
 
 // ---------------------------------------------------------------------------
-// Mixed precision kernel.
-// ---------------------------------------------------------------------------
-template <int G>
-__global__ __launch_bounds__(64) void k_solve_mixed(SolveArgs a) {
-    const int EPI = a.epi;
-    constexpr int NG = 64 / G;
-    __shared__ Shared<G> sh;
-    const int lane = threadIdx.x;
-    const int g = lane / G, i = lane % G;
-    const int pos = blockIdx.x * NG + g;
-    const int n_items = a.subset ? a.n_subset : a.T;
-    const bool valid = pos < n_items;
-    const int t = valid ? (a.subset ? a.subset[pos] : pos) : 0;
-    const int n = valid ? min(a.cnt[t], G) : 0;
-
-    int cx, cy, cz, xi, yi, zi;
-    const int nb = stage_coords<G>(a, t, n, g, i, sh.xyz, cx, cy, cz, xi, yi, zi);
-    __syncthreads();
-
-    // Row i of A (FP64, kept for the residual) and b_i; identity padding.
-    double arow[G];
-    const double bi = (i < n) ? cov_eval(a.cov, cx - xi, cy - yi, cz - zi) : 0.0;
-#pragma unroll
-    for (int j = 0; j < G; ++j) {
-        if (i < n && j < n)
-            arow[j] = cov_eval(a.cov, sh.xyz[g][0][j] - xi, sh.xyz[g][1][j] - yi, sh.xyz[g][2][j] - zi);
-        else
-            arow[j] = (i == j) ? 1.0 : 0.0;
-    }
-
-    // n of the largest system in the wave bounds the unrolled loops.
-    int nmax = n;
-#pragma unroll
-    for (int o = 32; o >= G; o >>= 1) nmax = max(nmax, __shfl_xor(nmax, o));
-
-    // FP32 right-looking Cholesky, L overwrites l[] (row i).
-    float l[G];
-#pragma unroll
-    for (int j = 0; j < G; ++j) l[j] = (float)arow[j];
-    const double maxdiag = group_max<G>(i < n ? arow[i] : 0.0);
-    float minpiv = 3.0e38f;
-#pragma unroll
-    for (int k = 0; k < G; ++k) {
-        if (k < nmax) {
-            sh.col[g][i] = l[k];
-            __syncthreads();
-            const float d = sh.col[g][k];
-            if (k < n) minpiv = fminf(minpiv, d);
-            const float s = sqrtf(fmaxf(d, 1.0e-30f));
-            const float inv = 1.0f / s;
-            const float lik = (i > k) ? l[k] * inv : 0.0f;
-            if (i == k) l[k] = s;
-            if (i > k) l[k] = lik;
-#pragma unroll
-            for (int j = k + 1; j < G; ++j) l[j] = fmaf(-lik, sh.col[g][j] * inv, l[j]);
-            __syncthreads();
-        }
-    }
-    // minpiv is group-uniform (each group read the same pivots).
-    bool bad = !(minpiv > 1.0e-4f * (float)maxdiag);
-
-    // Transposed factor in LDS: Lt[k*(G+1)+i] = L[k][i] (padded stride, no bank conflicts).
-#pragma unroll
-    for (int j = 0; j < G; ++j)
-        if (j <= i) sh.Lt[g][i * (G + 1) + j] = l[j];
-    __syncthreads();
-
-    // Two right-hand sides at once: b and (OK only) the vector of ones.
-    const bool two = a.ok_system != 0;
-    auto solve2 = [&](float r1, float r2, float &o1, float &o2) {
-        // forward: L y = r
-#pragma unroll
-        for (int k = 0; k < G; ++k) {
-            if (k < nmax) {
-                if (i == k) {
-                    r1 = r1 / l[k];
-                    r2 = r2 / l[k];
-                    sh.y1[g][k] = r1;
-                    sh.y2[g][k] = r2;
-                }
-                __syncthreads();
-                if (i > k) {
-                    r1 = fmaf(-l[k], sh.y1[g][k], r1);
-                    r2 = fmaf(-l[k], sh.y2[g][k], r2);
-                }
-            }
-        }
-        __syncthreads();
-        // backward: L^T x = y
-#pragma unroll
-        for (int k = G - 1; k >= 0; --k) {
-            if (k < nmax) {
-                if (i == k) {
-                    r1 = r1 / l[k];
-                    r2 = r2 / l[k];
-                    sh.y1[g][k] = r1;
-                    sh.y2[g][k] = r2;
-                }
-                __syncthreads();
-                if (i < k) {
-                    const float lki = sh.Lt[g][k * (G + 1) + i];
-                    r1 = fmaf(-lki, sh.y1[g][k], r1);
-                    r2 = fmaf(-lki, sh.y2[g][k], r2);
-                }
-            }
-        }
-        __syncthreads();
-        o1 = r1;
-        o2 = r2;
-    };
-
-    const double ones_i = (i < n) ? 1.0 : 0.0;
-    float f1, f2;
-    solve2((float)bi, two ? (float)ones_i : 0.0f, f1, f2);
-    double x1 = f1, x2 = f2;
-    double rel = 0.0;
-    for (int step = 0; step < 2; ++step) {
-        // residual in FP64
-        sh.xb[g][i] = x1;
-        __syncthreads();
-        double r1 = bi;
-#pragma unroll
-        for (int j = 0; j < G; ++j) r1 = fma(-arow[j], sh.xb[g][j], r1);
-        __syncthreads();
-        double r2 = 0.0;
-        if (two) {
-            sh.xb[g][i] = x2;
-            __syncthreads();
-            r2 = ones_i;
-#pragma unroll
-            for (int j = 0; j < G; ++j) r2 = fma(-arow[j], sh.xb[g][j], r2);
-            __syncthreads();
-        }
-        float d1, d2;
-        solve2((float)r1, (float)r2, d1, d2);
-        x1 += (double)d1;
-        x2 += (double)d2;
-        if (step == 1) {
-            const double num = group_max<G>(fmax(fabs((double)d1), fabs((double)d2)));
-            const double den = group_max<G>(fmax(fabs(x1), fabs(x2)));
-            rel = num / (den > 0 ? den : 1.0);
-        }
-    }
-    if (!(rel <= 1.0e-9)) bad = true;
-
-    // Weights.  OK: w = x - mu * y with mu = (sum x - 1) / sum y.
-    double w = x1;
-    double variance = 0.0;
-    if (two) {
-        const double sx = group_sum<G>(i < n ? x1 : 0.0);
-        const double sy = group_sum<G>(i < n ? x2 : 0.0);
-        const double mu = (sx - 1.0) / sy;
-        w = x1 - mu * x2;
-    }
-    if (EPI == EPI_SGS) {
-        // variance = C(0) - sum w_i b_i (- w_{n-1} for OK, my_kriging_weights.h:246)
-        variance = a.cov.c0 - group_sum<G>(i < n ? w * bi : 0.0);
-        if (two) {
-            sh.xb[g][i] = w;
-            __syncthreads();
-            variance -= sh.xb[g][n > 0 ? n - 1 : 0];
-            __syncthreads();
-        }
-    }
-
-    const uint8_t st = n == 0 ? ST_NO_NEIGHBOURS : (bad ? ST_FALLBACK : ST_OK);
-    if (valid && i == 0) a.status[t] = st;
-    wave_count(a.counters, 3, valid && i == 0 && st == ST_NO_NEIGHBOURS);
-    wave_count(a.counters, 0, valid && i == 0 && st == ST_OK);
-    epilogue<G>(a, valid && (n == 0 || !bad), t, n, g, i, nb, n > 0, w, variance, sh.term, sh.dterm);
-}
-
-// ---------------------------------------------------------------------------
 // FP64 kernel in the reference's operation order (contraction is off for
 // the whole translation unit; every product and sum rounds separately).
 // ---------------------------------------------------------------------------
+// Value of v held by lane j of this lane's group (j uniform), through
+// v_readlane: no LDS and no barrier.
+template <int G>
+__device__ inline double group_bcast(double v, int j, int g) {
+    const uint64_t b = __double_as_longlong(v);
+    const uint32_t lo0 = __builtin_amdgcn_readlane((uint32_t)b, j);
+    const uint32_t hi0 = __builtin_amdgcn_readlane((uint32_t)(b >> 32), j);
+    if (G == 64) return __longlong_as_double(((uint64_t)hi0 << 32) | lo0);
+    const uint32_t lo1 = __builtin_amdgcn_readlane((uint32_t)b, G + j);
+    const uint32_t hi1 = __builtin_amdgcn_readlane((uint32_t)(b >> 32), G + j);
+    return g == 0 ? __longlong_as_double(((uint64_t)hi0 << 32) | lo0)
+                  : __longlong_as_double(((uint64_t)hi1 << 32) | lo1);
+}
+
+// Covariance from the box table, branch-free (the entry for out-of-box
+// offsets is read from index 0 and replaced by 0).
+__device__ inline double cov_tab(const DevCov &c, int dx, int dy, int dz) {
+    const bool in = dx >= -c.tr[0] && dx <= c.tr[0] && dy >= -c.tr[1] && dy <= c.tr[1] && dz >= -c.tr[2] &&
+                    dz <= c.tr[2];
+    const int idx = ((dz + c.tr[2]) * c.td[1] + (dy + c.tr[1])) * c.td[0] + (dx + c.tr[0]);
+    const double v = c.table[in ? idx : 0];
+    return in ? v : 0.0;
+}
+
+// FP64 in the reference's dpotf2 / dpotrs operation order, one system per
+// G lanes, lane i holding row i of the factor in registers.  Row j and the
+// solution components are broadcast with v_readlane, so the factorisation
+// and both triangular solves use no LDS and no barrier.
 template <int G>
 __global__ __launch_bounds__(64) void k_solve_fp64(SolveArgs a) {
     const int EPI = a.epi;
@@ -410,109 +236,104 @@ __global__ __launch_bounds__(64) void k_solve_fp64(SolveArgs a) {
 
     // A(i,j) = C(x_i, x_j) (my_kriging_weights.h:156-165), identity padding
     double u[G];
-    const double bi = (i < n) ? cov_eval(a.cov, cx - xi, cy - yi, cz - zi) : 0.0;
+    double bi;
+    if (a.cov.table) {
+        bi = (i < n) ? cov_tab(a.cov, cx - xi, cy - yi, cz - zi) : 0.0;
 #pragma unroll
-    for (int j = 0; j < G; ++j) {
-        if (i < n && j < n)
-            u[j] = cov_eval(a.cov, sh.xyz[g][0][j] - xi, sh.xyz[g][1][j] - yi, sh.xyz[g][2][j] - zi);
-        else
-            u[j] = (i == j) ? 1.0 : 0.0;
+        for (int j = 0; j < G; ++j) {
+            const double e = cov_tab(a.cov, sh.xyz[g][0][j] - xi, sh.xyz[g][1][j] - yi, sh.xyz[g][2][j] - zi);
+            u[j] = (i < n && j < n) ? e : ((i == j) ? 1.0 : 0.0);
+        }
+    } else {
+        bi = (i < n) ? cov_eval(a.cov, cx - xi, cy - yi, cz - zi) : 0.0;
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+            if (i < n && j < n)
+                u[j] = cov_eval(a.cov, sh.xyz[g][0][j] - xi, sh.xyz[g][1][j] - yi, sh.xyz[g][2][j] - zi);
+            else
+                u[j] = (i == j) ? 1.0 : 0.0;
+        }
     }
+    // the largest system of the wave bounds the unrolled loops
     int nmax = n;
 #pragma unroll
     for (int o = 32; o >= G; o >>= 1) nmax = max(nmax, __shfl_xor(nmax, o));
+    nmax = __builtin_amdgcn_readfirstlane(nmax);
 
-    // dpotf2 'U' as rows of L = U^T: lane i holds L(i, 0..i).
+    // dpotf2 'U' as rows of L = U^T: lane i holds L(i, 0..i).  Step j:
+    // lane j forms the pivot from its own row; every lane below forms
+    // (A_ij + (-1) * sum_k L_ik L_jk) * (1 / L_jj) with row j broadcast.
     int info = 0;
+    double diag = 1.0;  // L(i, i) of this lane's row
 #pragma unroll
     for (int j = 0; j < G; ++j) {
         if (j < nmax) {
+            double temp = 0.0;
+#pragma unroll
+            for (int k = 0; k < j; ++k) {
+                const double ljk = group_bcast<G>(u[k], j, g);
+                temp = temp + u[k] * ljk;
+            }
+            // pivot on lane j, whose temp is the sequential sum of its squares
+            double ajj = u[j] - temp;
+            const bool fail = i == j && info == 0 && j < n && (ajj <= 0.0 || ajj != ajj);
+            if (!(ajj > 0.0)) ajj = 1.0;  // failed or padding: keep arithmetic finite
+            const double sq = sqrt(ajj);
+            const double rcp_l = 1.0 / sq;
             if (i == j) {
-#pragma unroll
-                for (int k = 0; k < j; ++k) sh.row[g][k] = u[k];
+                u[j] = sq;
+                diag = sq;
             }
-            __syncthreads();
-            if (i == j) {
-                double dot = 0.0;
-#pragma unroll
-                for (int k = 0; k < j; ++k) dot = dot + u[k] * u[k];
-                double ajj = u[j] - dot;
-                if (info == 0 && j < n && (ajj <= 0.0 || ajj != ajj)) {
-                    sh.fail[g] = j + 1;
-                } else {
-                    if (!(ajj > 0.0)) ajj = 1.0;  // failed group or padding: keep arithmetic finite
-                    ajj = sqrt(ajj);
-                    u[j] = ajj;
-                    sh.piv[g][0] = 1.0 / ajj;
-                    sh.fail[g] = 0;
-                }
-            }
-            __syncthreads();
-            if (info == 0 && sh.fail[g] != 0) info = sh.fail[g];
-            const double rcp = sh.piv[g][0];
-            if (i > j) {
-                double temp = 0.0;
-#pragma unroll
-                for (int k = 0; k < j; ++k) temp = temp + u[k] * sh.row[g][k];
-                u[j] = rcp * (u[j] + (-1.0) * temp);
-            }
-            __syncthreads();
+            const double rcp = group_bcast<G>(rcp_l, j, g);
+            const uint64_t fm = __ballot(fail);
+            const bool group_fail = ((fm >> (g * G)) & ((G == 64) ? ~0ull : ((1ull << G) - 1))) != 0;
+            if (info == 0 && group_fail) info = j + 1;
+            if (i > j) u[j] = rcp * (u[j] + (-1.0) * temp);
         }
     }
 
-    // U(i,k) = L(k,i) for the back substitution, through LDS.
-#pragma unroll
-    for (int j = 0; j < G; ++j)
-        if (j <= i) sh.Ut[g][i * (G + 1) + j] = u[j];
-    __syncthreads();
-    double ut[G];
-#pragma unroll
-    for (int k = 0; k < G; ++k) ut[k] = (k >= i) ? sh.Ut[g][k * (G + 1) + i] : 0.0;
-
     const bool two = a.ok_system != 0;
-    // dpotrs: dtrsm('L','U','T','N') then dtrsm('L','U','N','N')
-    auto dpotrs = [&](double r1, double r2, double &o1, double &o2) {
+    // dpotrs: dtrsm('L','U','T','N') (forward with L) then
+    // dtrsm('L','U','N','N') (backward with L^T), both right-hand sides
+    double r1 = bi, r2 = two ? ((i < n) ? 1.0 : 0.0) : 0.0;
 #pragma unroll
-        for (int k = 0; k < G; ++k) {
-            if (k < nmax) {
-                if (i == k) {
-                    r1 = r1 / u[k];
-                    r2 = r2 / u[k];
-                    sh.y1[g][k] = r1;
-                    sh.y2[g][k] = r2;
-                }
-                __syncthreads();
-                if (i > k) {
-                    r1 = r1 - u[k] * sh.y1[g][k];
-                    r2 = r2 - u[k] * sh.y2[g][k];
-                }
+    for (int k = 0; k < G; ++k) {
+        if (k < nmax) {
+            if (i == k) {
+                r1 = r1 / u[k];
+                r2 = r2 / u[k];
+            }
+            const double y1 = group_bcast<G>(r1, k, g);
+            const double y2 = group_bcast<G>(r2, k, g);
+            if (i > k) {
+                r1 = r1 - u[k] * y1;
+                r2 = r2 - u[k] * y2;
             }
         }
-        __syncthreads();
+    }
 #pragma unroll
-        for (int k = G - 1; k >= 0; --k) {
-            if (k < nmax) {
-                if (i == k) {
-                    sh.nz1[g][k] = r1 != 0.0;
-                    sh.nz2[g][k] = r2 != 0.0;
-                    if (r1 != 0.0) r1 = r1 / u[k];
-                    if (r2 != 0.0) r2 = r2 / u[k];
-                    sh.y1[g][k] = r1;
-                    sh.y2[g][k] = r2;
-                }
-                __syncthreads();
-                if (i < k) {
-                    if (sh.nz1[g][k]) r1 = r1 - sh.y1[g][k] * ut[k];
-                    if (sh.nz2[g][k]) r2 = r2 - sh.y2[g][k] * ut[k];
-                }
+    for (int k = G - 1; k >= 0; --k) {
+        if (k < nmax) {
+            if (i == k) {
+                if (r1 != 0.0) r1 = r1 / diag;
+                if (r2 != 0.0) r2 = r2 / diag;
+            }
+            const double y1 = group_bcast<G>(r1, k, g);
+            const double y2 = group_bcast<G>(r2, k, g);
+            // L(k, i): element i of row k, broadcast from lane k
+            double lki = 0.0;
+#pragma unroll
+            for (int c = 0; c < k; ++c) {
+                const double v = group_bcast<G>(u[c], k, g);
+                if (i == c) lki = v;
+            }
+            if (i < k) {
+                if (y1 != 0.0) r1 = r1 - y1 * lki;
+                if (y2 != 0.0) r2 = r2 - y2 * lki;
             }
         }
-        __syncthreads();
-        o1 = r1;
-        o2 = r2;
-    };
-    double x1, x2;
-    dpotrs(bi, two ? ((i < n) ? 1.0 : 0.0) : 0.0, x1, x2);
+    }
+    const double x1 = r1, x2 = r2;
 
     double w = x1;
     double variance = 0.0;
@@ -549,19 +370,144 @@ __global__ __launch_bounds__(64) void k_solve_fp64(SolveArgs a) {
     epilogue<G>(a, valid, t, n, g, i, nb, ok, w, variance, sh.term, sh.dterm);
 }
 
-}  // namespace
+// k_solve_fp64 for orders 33..64: one system per wavefront with the rows of
+// the factor in LDS (row i at stride 65, so a lane's own row reads are
+// conflict-free and row j reads are broadcasts); same operation order.
+struct ShLds64 {
+    double U[64][65];
+    double y1[1][64];
+    double y2[1][64];
+    float term[1][64];
+    double dterm[1][64];
+    int xyz[1][3][64];
+};
 
-void launch_solve_mixed(const SolveArgs &a, int group, hipStream_t s) {
-    const int items = a.subset ? a.n_subset : a.T;
-    if (items <= 0) return;
-    const int g = group <= 32 ? 32 : 64;
-    const int ng = 64 / g;
-    dim3 grid((items + ng - 1) / ng);
-    if (g == 32)
-        hipLaunchKernelGGL(k_solve_mixed<32>, grid, dim3(64), 0, s, a);
-    else
-        hipLaunchKernelGGL(k_solve_mixed<64>, grid, dim3(64), 0, s, a);
+__device__ inline void wave_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+    __builtin_amdgcn_wave_barrier();
 }
+
+__global__ __launch_bounds__(64) void k_solve_fp64_lds(SolveArgs a) {
+    constexpr int G = 64;
+    const int EPI = a.epi;
+    __shared__ ShLds64 sh;
+    const int i = threadIdx.x;
+    const int pos = blockIdx.x;
+    const int n_items = a.subset ? a.n_subset : a.T;
+    const bool valid = pos < n_items;
+    const int t = valid ? (a.subset ? a.subset[pos] : pos) : 0;
+    const int n = valid ? min(a.cnt[t], G) : 0;
+
+    int cx, cy, cz, xi, yi, zi;
+    const int nb = stage_coords<G>(a, t, n, 0, i, sh.xyz, cx, cy, cz, xi, yi, zi);
+    __syncthreads();
+
+    // lower triangle of A (my_kriging_weights.h:156-165), identity padding
+    double bi;
+    if (a.cov.table) {
+        bi = (i < n) ? cov_tab(a.cov, cx - xi, cy - yi, cz - zi) : 0.0;
+        for (int j = 0; j <= i; ++j) {
+            const double e = cov_tab(a.cov, sh.xyz[0][0][j] - xi, sh.xyz[0][1][j] - yi, sh.xyz[0][2][j] - zi);
+            sh.U[i][j] = (i < n && j < n) ? e : ((i == j) ? 1.0 : 0.0);
+        }
+    } else {
+        bi = (i < n) ? cov_eval(a.cov, cx - xi, cy - yi, cz - zi) : 0.0;
+        for (int j = 0; j <= i; ++j)
+            sh.U[i][j] = (i < n && j < n)
+                             ? cov_eval(a.cov, sh.xyz[0][0][j] - xi, sh.xyz[0][1][j] - yi, sh.xyz[0][2][j] - zi)
+                             : ((i == j) ? 1.0 : 0.0);
+    }
+    wave_fence();
+
+    // dpotf2 'U' as rows of L = U^T
+    int info = 0;
+    double diag = 1.0;
+    for (int j = 0; j < n; ++j) {
+        double temp = 0.0;
+        if (i >= j) {
+#pragma unroll 4
+            for (int k = 0; k < j; ++k) temp = temp + sh.U[i][k] * sh.U[j][k];
+        }
+        double ajj = sh.U[i][i] - temp;  // used on lane j only
+        const bool fail = i == j && info == 0 && (ajj <= 0.0 || ajj != ajj);
+        if (!(ajj > 0.0)) ajj = 1.0;
+        const double sq = sqrt(ajj);
+        const double rcp = group_bcast<G>(1.0 / sq, j, 0);
+        if (info == 0 && __ballot(fail) != 0) info = j + 1;
+        if (i == j) {
+            sh.U[j][j] = sq;
+            diag = sq;
+        }
+        if (i > j) sh.U[i][j] = rcp * (sh.U[i][j] + (-1.0) * temp);
+        wave_fence();
+    }
+
+    const bool two = a.ok_system != 0;
+    // dpotrs: forward with L, backward with L^T, both right-hand sides
+    double r1 = bi, r2 = two ? ((i < n) ? 1.0 : 0.0) : 0.0;
+    for (int k = 0; k < n; ++k) {
+        if (i == k) {
+            r1 = r1 / diag;
+            r2 = r2 / diag;
+        }
+        const double y1 = group_bcast<G>(r1, k, 0);
+        const double y2 = group_bcast<G>(r2, k, 0);
+        if (i > k) {
+            const double lik = sh.U[i][k];
+            r1 = r1 - lik * y1;
+            r2 = r2 - lik * y2;
+        }
+    }
+    for (int k = n - 1; k >= 0; --k) {
+        if (i == k) {
+            if (r1 != 0.0) r1 = r1 / diag;
+            if (r2 != 0.0) r2 = r2 / diag;
+        }
+        const double y1 = group_bcast<G>(r1, k, 0);
+        const double y2 = group_bcast<G>(r2, k, 0);
+        if (i < k) {
+            const double lki = sh.U[k][i];
+            if (y1 != 0.0) r1 = r1 - y1 * lki;
+            if (y2 != 0.0) r2 = r2 - y2 * lki;
+        }
+    }
+    const double x1 = r1, x2 = r2;
+
+    double w = x1;
+    double variance = 0.0;
+    sh.y1[0][i] = x1;
+    sh.y2[0][i] = x2;
+    sh.dterm[0][i] = bi;
+    __syncthreads();
+    if (two) {
+        double s_sk = 0, s_one = 0;
+        for (int k = 0; k < n; ++k) {
+            s_sk += sh.y1[0][k];
+            s_one += sh.y2[0][k];
+        }
+        const double mu = (s_sk - 1) / s_one;
+        w = x1 - mu * x2;
+    }
+    __syncthreads();
+    if (EPI == EPI_SGS) {
+        sh.y1[0][i] = w;
+        __syncthreads();
+        double v = a.cov.c0;
+        for (int k = 0; k < n; ++k) v -= sh.y1[0][k] * sh.dterm[0][k];
+        if (two && n > 0) v -= sh.y1[0][n - 1];
+        variance = v;
+        __syncthreads();
+    }
+
+    const bool ok = n > 0 && info == 0;
+    if (valid && i == 0) a.status[t] = n == 0 ? ST_NO_NEIGHBOURS : (ok ? ST_OK : ST_SINGULAR);
+    wave_count(a.counters, 3, valid && i == 0 && n == 0);
+    wave_count(a.counters, 1, valid && i == 0 && ok);
+    wave_count(a.counters, 2, valid && i == 0 && n > 0 && !ok);
+    epilogue<G>(a, valid, t, n, 0, i, nb, ok, w, variance, sh.term, sh.dterm);
+}
+
+}  // namespace
 
 void launch_solve_fp64(const SolveArgs &a, int group, hipStream_t s) {
     const int items = a.subset ? a.n_subset : a.T;
@@ -572,7 +518,7 @@ void launch_solve_fp64(const SolveArgs &a, int group, hipStream_t s) {
     if (g == 32)
         hipLaunchKernelGGL(k_solve_fp64<32>, grid, dim3(64), 0, s, a);
     else
-        hipLaunchKernelGGL(k_solve_fp64<64>, grid, dim3(64), 0, s, a);
+        hipLaunchKernelGGL(k_solve_fp64_lds, grid, dim3(64), 0, s, a);
 }
 
 }  // namespace hpgl_amd

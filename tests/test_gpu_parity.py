@@ -283,13 +283,13 @@ def test_mixed_vs_fp64_error_report(geo):
     assert rel_err(mix.data, ref.data, 1.0) <= 1e-6
 
 
-@pytest.mark.parametrize("max_n", [7, 19, 32, 40])
-def test_uneven_neighbour_counts_buckets(geo, max_n):
-    """Sparse data near the grid edges gives neighbour counts across the
-    solve buckets (4..32 on four lanes per system, wider systems on one
-    lane per row); mixed within 1e-5 of the oracle, FP64 bit-exact."""
+@pytest.mark.parametrize("max_n,frac", [(7, 0.01), (19, 0.01), (32, 0.01), (40, 0.04), (64, 0.08)])
+def test_uneven_neighbour_counts_buckets(geo, max_n, frac):
+    """Neighbour counts across the solve buckets (4..32 on four lanes per
+    system, 33..64 on the FP64 kernel with rows in LDS); mixed within 1e-5
+    of the oracle, FP64 bit-exact."""
     shape = (36, 30, 10)
-    data, mask = scattered(shape, 0.01, 50 + max_n, law="normal")
+    data, mask = scattered(shape, frac, 50 + max_n, law="normal")
     g = Grid(*shape)
     cov = Cov(0, (8, 8, 3), nugget=0.2)
     r = (8, 8, 3)
@@ -300,8 +300,30 @@ def test_uneven_neighbour_counts_buckets(geo, max_n):
         out = geo.simple_kriging((data, mask), g, r, max_n, cov, mean=0.0)
         counts = pcnt[pcnt >= 0]
         assert len(np.unique(counts)) > 3
+        if max_n > 32:
+            assert np.sum(counts > 32) > 100
         assert np.array_equal(out.mask, om)
         if precision == "fp64":
             assert np.array_equal(out.data, od)
         else:
             assert rel_err(out.data, od, 1.0) <= TOL
+
+
+def test_ordinary_kriging_wide_systems_fp64_bit_exact(geo):
+    """OK with 33..64 neighbours on the analytic model: the covariances come
+    from the host-built pair table, so FP64 mode matches the oracle bit for
+    bit and mixed mode within 1e-5."""
+    shape = (40, 36, 12)
+    data, mask = scattered(shape, 0.06, 77)
+    g = Grid(*shape)
+    cov = Cov(1, (10, 9, 4), angles=(20.0, 0.0, 0.0), sill=1.0, nugget=0.1)
+    r, mx = (9, 9, 4), 64
+    od, om = O.ordinary_kriging((data, mask), g, r, mx, cov)
+    for precision in ("fp64", "mixed"):
+        geo.set_precision(precision)
+        out = geo.ordinary_kriging((data, mask), g, r, mx, cov)
+        assert np.array_equal(out.mask, om)
+        if precision == "fp64":
+            assert np.array_equal(out.data, od)
+        else:
+            assert rel_err(out.data[om == 1], od[om == 1], 1.0) <= TOL
