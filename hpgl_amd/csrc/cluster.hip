@@ -34,6 +34,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <stdexcept>
+
 namespace hpgl_amd {
 
 namespace {
@@ -496,10 +498,17 @@ __device__ void finish_(const E *e, int n, int K, int *out, const S &sh, const i
 // One wavefront per workgroup; the wavefronts loop over the target list.
 // PACKED: candidate coordinates come from bin_xyz (x | y << 11 | z << 22),
 // otherwise they are decoded from the cell index.
-template <bool PACKED, int CAP>
+//
+// SPILLED = false: the grid holds about n_list / per_wave wavefronts, far
+// more than are resident, so the hardware balances the uneven per-target
+// work; wavefronts stride over the list, and a list longer than CAP is not
+// sorted but its target appended to spill_list.  SPILLED = true: the
+// wavefronts stride over the spill_count targets of spill_list, sorting
+// lists longer than CAP in their global scratch.
+template <bool PACKED, int CAP, bool SPILLED>
 __global__ __launch_bounds__(64) void k_cluster(ClusterArgs a) {
     __shared__ WaveLds<CAP> sh;
-    const int wave_id = blockIdx.x, n_wave_ids = gridDim.x;
+    const int wave_id = blockIdx.x;
     const int lane = threadIdx.x & 63;
     const uint64_t lt = lanemask_lt();
     const int X = a.X, Y = a.Y;
@@ -509,7 +518,7 @@ __global__ __launch_bounds__(64) void k_cluster(ClusterArgs a) {
     // (8-byte aligned), then elements and counts
     SortBufs gb{nullptr, nullptr, nullptr, nullptr, nullptr};
     uint32_t *ge = nullptr;
-    if (a.scratch) {
+    if (SPILLED) {
         const int nchm = a.scratch_per_wave / 64 + 2;
         uint32_t *base = a.scratch + (size_t)wave_id * cluster_scratch_words(a.scratch_per_wave);
         gb.ml = reinterpret_cast<uint64_t *>(base);
@@ -521,8 +530,9 @@ __global__ __launch_bounds__(64) void k_cluster(ClusterArgs a) {
     }
     const SortBufs lb{sh.e, sh.ml, sh.mr, sh.pl, sh.pr};
     unsigned long long cand_acc = 0, spill_acc = 0;
-    for (int li = wave_id; li < a.n_list; li += n_wave_ids) {
-        const int t = a.list ? a.list[li] : li;
+    const int li_end = SPILLED ? *a.spill_count : a.n_list;
+    for (int li = wave_id; li < li_end; li += gridDim.x) {
+        const int t = SPILLED ? a.spill_list[li] : a.list ? a.list[li] : li;
         const int node = a.targets[t];
         const int x = node % X, y = node % (X * Y) / X, z = node / (X * Y);
         const int bx = x / b.r[0], by = y / b.r[1], bz = z / b.r[2];
@@ -631,7 +641,11 @@ __global__ __launch_bounds__(64) void k_cluster(ClusterArgs a) {
         if (total <= CAP) {
             sane = topk_sort_<true>(lb, sh, total, a.max_n, key);
             if (sane) finish_(sh.e, total, a.max_n, out, sh, a.bin_cells, key);
-        } else if (ge && total <= a.scratch_per_wave) {
+        } else if (!SPILLED && a.spill_list) {
+            if (lane == 0) a.spill_list[atomicAdd(a.spill_count, 1)] = t;
+            wave_sync();
+            continue;  // counted by the spill pass
+        } else if (SPILLED && total <= a.scratch_per_wave) {
             spill_acc += 1;
             for (int p = lane; p < CAP; p += 64) ge[p] = sh.e[p];  // the head went to LDS
             wave_sync();
@@ -661,22 +675,27 @@ int cluster_capacity(long long expected_max) {
 }
 
 template <bool PACKED>
-static void launch_cap(const ClusterArgs &a, int grid, hipStream_t s) {
+static void launch_cap(const ClusterArgs &a, hipStream_t s) {
+    const int grid = (int)(((long long)a.n_list + a.per_wave - 1) / a.per_wave);
     if (a.cap <= kCaps[0])
-        hipLaunchKernelGGL((k_cluster<PACKED, kCaps[0]>), dim3(grid), dim3(64), 0, s, a);
+        hipLaunchKernelGGL((k_cluster<PACKED, kCaps[0], false>), dim3(grid), dim3(64), 0, s, a);
     else if (a.cap <= kCaps[1])
-        hipLaunchKernelGGL((k_cluster<PACKED, kCaps[1]>), dim3(grid), dim3(64), 0, s, a);
+        hipLaunchKernelGGL((k_cluster<PACKED, kCaps[1], false>), dim3(grid), dim3(64), 0, s, a);
     else
-        hipLaunchKernelGGL((k_cluster<PACKED, kCaps[2]>), dim3(grid), dim3(64), 0, s, a);
+        hipLaunchKernelGGL((k_cluster<PACKED, kCaps[2], false>), dim3(grid), dim3(64), 0, s, a);
+    // the spill pass keeps the LDS head small: its lists are sorted in scratch
+    if (a.spill_list)
+        hipLaunchKernelGGL((k_cluster<PACKED, kCaps[0], true>), dim3(a.n_spill_waves), dim3(64), 0, s, a);
 }
 
 void launch_cluster_search(const ClusterArgs &a, hipStream_t s) {
     if (a.n_list <= 0) return;
-    const int grid = a.n_list < a.n_waves ? a.n_list : a.n_waves;
+    if (a.per_wave < 1 || (a.spill_list && (a.n_spill_waves < 1 || !a.spill_count || !a.scratch)))
+        throw std::runtime_error("cluster search: invalid launch arguments");
     if (a.bin_xyz)
-        launch_cap<true>(a, grid, s);
+        launch_cap<true>(a, s);
     else
-        launch_cap<false>(a, grid, s);
+        launch_cap<false>(a, s);
 }
 
 }  // namespace hpgl_amd

@@ -189,6 +189,14 @@ void upload(DBuf<T> &d, const std::vector<T> &h, hipStream_t s) {
 // of doubles); beyond it the solves evaluate the model on the device.
 constexpr long long kMaxPairTable = 1ll << 25;
 
+// Cell-binned search launch: list entries per wavefront of the main pass
+// (a grid many times the resident set lets the hardware balance the uneven
+// per-target work; 8192 resident-sized wavefronts ran 20-25% slower), and
+// wavefronts (each with its own global scratch) of the pass that sorts the
+// lists longer than the LDS capacity.
+constexpr int kClusterPerWave = 32;
+constexpr int kClusterSpillWaves = 2048;
+
 int solve_group(int max_n) {
     int g = 8;
     while (g < max_n) g <<= 1;
@@ -443,17 +451,25 @@ void search(CovAssets &as, const int grid[3], const int radii[3], int max_n, con
         const long long n_pos = ra.rank_positive;
         const double expect = (double)I / ((double)grid[0] * grid[1] * grid[2]) * (double)n_pos;
         ca.cap = cluster_capacity((long long)(expect + 6.0 * std::sqrt(expect) + 64.0));
-        const int waves = 8192;
-        ca.n_waves = waves;
+        ca.per_wave = kClusterPerWave;
         DBuf<uint32_t> scratch;
+        DBuf<int> spill_list, spill_count;
         if (max_cand > ca.cap) {
             ca.scratch_per_wave = (int)max_cand;
-            scratch.alloc((size_t)std::min<long long>(waves, n_clu) * cluster_scratch_words((int)max_cand));
+            ca.n_spill_waves = (int)std::min<long long>(kClusterSpillWaves, n_clu);
+            scratch.alloc((size_t)ca.n_spill_waves * cluster_scratch_words((int)max_cand));
+            spill_list.alloc(n_clu);
+            spill_count.alloc(1);
+            hip_check(hipMemsetAsync(spill_count.p, 0, sizeof(int), s), "memset");
             ca.scratch = scratch.p;
-            ca.n_waves = (int)std::min<long long>(waves, n_clu);
+            ca.spill_list = spill_list.p;
+            ca.spill_count = spill_count.p;
         } else {
             ca.scratch_per_wave = 0;
             ca.scratch = nullptr;
+            ca.spill_list = nullptr;
+            ca.spill_count = nullptr;
+            ca.n_spill_waves = 0;
         }
         DBuf<unsigned int> overflow(1);
         DBuf<unsigned long long> cand(kCounterWords);

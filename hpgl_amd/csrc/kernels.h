@@ -139,9 +139,14 @@ struct ClusterArgs {
     int *cnt;
     int local_bits;           // element = (rank << local_bits) | raw candidate index
     int cap;                  // LDS list capacity, a value returned by cluster_capacity
-    int n_waves;              // grid size (waves loop over the list)
-    // global scratch for candidate lists that do not fit in LDS:
-    // cluster_scratch_words(scratch_per_wave) words per wave
+    int per_wave;             // main pass grid: ceil(n_list / per_wave) wavefronts
+    // lists longer than cap: the main pass appends their targets to
+    // spill_list (spill_count entries; nullptr: an overflow error) and a
+    // second pass of n_spill_waves wavefronts sorts them in global scratch,
+    // cluster_scratch_words(scratch_per_wave) words per wavefront
+    int *spill_list;
+    int *spill_count;
+    int n_spill_waves;
     uint32_t *scratch;
     int scratch_per_wave;
     unsigned int *overflow;   // bit 0: a list exceeded scratch capacity; bit 1: sort replay error
