@@ -34,17 +34,18 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <stdexcept>
 
 namespace hpgl_amd {
 
 namespace {
 
-// Candidates a wavefront holds in LDS: the kernel is built for 1024, 2048
-// and 4096 (about 6, 10 and 19 KB of LDS per wavefront) and the host picks
-// the smallest that the job's expected lists fit; longer lists go to global
-// scratch.
-constexpr int kCaps[3] = {1024, 2048, 4096};
+// Candidates a wavefront holds in LDS (dynamic LDS, 4 bytes each, so the
+// host sizes it to the job's expected lists and occupancy follows): a
+// multiple of 64, at most 4096 (the register-held partition covers 64
+// chunks); longer lists go to global scratch.
+constexpr int kMinCap = 256, kMaxCap = 4096;
 
 // A wavefront orders its own accesses (LDS and the global scratch lists)
 // with a workgroup-scope fence, which waits for them to complete; the
@@ -68,11 +69,9 @@ struct SortBufs {
     int *pl, *pr;
 };
 
-template <int CAP>
+// Fixed LDS of a wavefront; the elements of a list that fits the capacity
+// live in the dynamic LDS.
 struct WaveLds {
-    uint32_t e[CAP];
-    uint64_t ml[CAP / 64 + 1], mr[CAP / 64 + 1];
-    int pl[CAP / 64 + 2], pr[CAP / 64 + 2];
     int lbuf[64], rbuf[64];  // positions of the 64 stop pairs being swapped
     // the 27 bins as 9 rows (k, j) of three consecutive bins i-1, i, i+1,
     // which are adjacent in the bin-sorted cell arrays
@@ -501,20 +500,23 @@ __device__ void finish_(const E *e, int n, int K, int *out, const S &sh, const i
 //
 // SPILLED = false: the grid holds about n_list / per_wave wavefronts, far
 // more than are resident, so the hardware balances the uneven per-target
-// work; wavefronts stride over the list, and a list longer than CAP is not
+// work; wavefronts stride over the list, and a list longer than a.cap is not
 // sorted but its target appended to spill_list.  SPILLED = true: the
 // wavefronts stride over the spill_count targets of spill_list, sorting
-// lists longer than CAP in their global scratch.
-template <bool PACKED, int CAP, bool SPILLED>
+// all their lists in their global scratch.
+template <bool PACKED, bool SPILLED>
 __global__ __launch_bounds__(64) void k_cluster(ClusterArgs a) {
-    __shared__ WaveLds<CAP> sh;
+    __shared__ WaveLds sh;
+    extern __shared__ uint32_t lds_e[];
+    // the spill pass gathers straight into its global scratch
+    const int cap = SPILLED ? 0 : a.cap;
     const int wave_id = blockIdx.x;
     const int lane = threadIdx.x & 63;
     const uint64_t lt = lanemask_lt();
     const int X = a.X, Y = a.Y;
     const BinArgs &b = a.bins;
     const Keyed key{a.local_bits};
-    // global scratch of this wave for lists longer than CAP: masks first
+    // global scratch of this wave for lists longer than a.cap: masks first
     // (8-byte aligned), then elements and counts
     SortBufs gb{nullptr, nullptr, nullptr, nullptr, nullptr};
     uint32_t *ge = nullptr;
@@ -528,7 +530,7 @@ __global__ __launch_bounds__(64) void k_cluster(ClusterArgs a) {
         gb.pr = gb.pl + nchm;
         ge = gb.e;
     }
-    const SortBufs lb{sh.e, sh.ml, sh.mr, sh.pl, sh.pr};
+    const SortBufs lb{lds_e, nullptr, nullptr, nullptr, nullptr};  // masks stay in registers
     unsigned long long cand_acc = 0, spill_acc = 0;
     const int li_end = SPILLED ? *a.spill_count : a.n_list;
     for (int li = wave_id; li < li_end; li += gridDim.x) {
@@ -626,8 +628,8 @@ __global__ __launch_bounds__(64) void k_cluster(ClusterArgs a) {
                 const int pp = pos + __popcll(mk & lt);
                 if (rk[u] != 0) {
                     const uint32_t ev = (rk[u] << key.lb) | (uint32_t)(base + u * 64 + lane);
-                    if (pp < CAP)
-                        sh.e[pp] = ev;
+                    if (pp < cap)
+                        lds_e[pp] = ev;
                     else if (ge && pp < a.scratch_per_wave)
                         ge[pp] = ev;
                 }
@@ -638,16 +640,15 @@ __global__ __launch_bounds__(64) void k_cluster(ClusterArgs a) {
         wave_sync();
         int *out = a.nbr + (size_t)t * a.max_n;
         bool sane = true;
-        if (total <= CAP) {
+        if (total <= cap) {
             sane = topk_sort_<true>(lb, sh, total, a.max_n, key);
-            if (sane) finish_(sh.e, total, a.max_n, out, sh, a.bin_cells, key);
+            if (sane) finish_(lds_e, total, a.max_n, out, sh, a.bin_cells, key);
         } else if (!SPILLED && a.spill_list) {
             if (lane == 0) a.spill_list[atomicAdd(a.spill_count, 1)] = t;
             wave_sync();
             continue;  // counted by the spill pass
         } else if (SPILLED && total <= a.scratch_per_wave) {
             spill_acc += 1;
-            for (int p = lane; p < CAP; p += 64) ge[p] = sh.e[p];  // the head went to LDS
             wave_sync();
             sane = topk_sort_<false>(gb, sh, total, a.max_n, key);
             if (sane) finish_(ge, total, a.max_n, out, sh, a.bin_cells, key);
@@ -669,28 +670,21 @@ __global__ __launch_bounds__(64) void k_cluster(ClusterArgs a) {
 }  // namespace
 
 int cluster_capacity(long long expected_max) {
-    for (int c : kCaps)
-        if (expected_max <= c) return c;
-    return kCaps[2];
+    const long long c = (expected_max + 63) / 64 * 64;
+    return (int)std::min<long long>(kMaxCap, std::max<long long>(kMinCap, c));
 }
 
 template <bool PACKED>
 static void launch_cap(const ClusterArgs &a, hipStream_t s) {
     const int grid = (int)(((long long)a.n_list + a.per_wave - 1) / a.per_wave);
-    if (a.cap <= kCaps[0])
-        hipLaunchKernelGGL((k_cluster<PACKED, kCaps[0], false>), dim3(grid), dim3(64), 0, s, a);
-    else if (a.cap <= kCaps[1])
-        hipLaunchKernelGGL((k_cluster<PACKED, kCaps[1], false>), dim3(grid), dim3(64), 0, s, a);
-    else
-        hipLaunchKernelGGL((k_cluster<PACKED, kCaps[2], false>), dim3(grid), dim3(64), 0, s, a);
-    // the spill pass keeps the LDS head small: its lists are sorted in scratch
-    if (a.spill_list)
-        hipLaunchKernelGGL((k_cluster<PACKED, kCaps[0], true>), dim3(a.n_spill_waves), dim3(64), 0, s, a);
+    hipLaunchKernelGGL((k_cluster<PACKED, false>), dim3(grid), dim3(64), (size_t)a.cap * sizeof(uint32_t), s, a);
+    if (a.spill_list) hipLaunchKernelGGL((k_cluster<PACKED, true>), dim3(a.n_spill_waves), dim3(64), 0, s, a);
 }
 
 void launch_cluster_search(const ClusterArgs &a, hipStream_t s) {
     if (a.n_list <= 0) return;
-    if (a.per_wave < 1 || (a.spill_list && (a.n_spill_waves < 1 || !a.spill_count || !a.scratch)))
+    if (a.per_wave < 1 || a.cap < kMinCap || a.cap > kMaxCap || a.cap % 64 != 0 ||
+        (a.spill_list && (a.n_spill_waves < 1 || !a.spill_count || !a.scratch)))
         throw std::runtime_error("cluster search: invalid launch arguments");
     if (a.bin_xyz)
         launch_cap<true>(a, s);
