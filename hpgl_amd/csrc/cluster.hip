@@ -46,6 +46,7 @@ namespace {
 // multiple of 64, at most 4096 (the register-held partition covers 64
 // chunks); longer lists go to global scratch.
 constexpr int kMinCap = 256, kMaxCap = 4096;
+constexpr size_t kLdsPerCu = 160 * 1024;
 
 // A wavefront orders its own accesses (LDS and the global scratch lists)
 // with a workgroup-scope fence, which waits for them to complete; the
@@ -504,8 +505,12 @@ __device__ void finish_(const E *e, int n, int K, int *out, const S &sh, const i
 // sorted but its target appended to spill_list.  SPILLED = true: the
 // wavefronts stride over the spill_count targets of spill_list, sorting
 // all their lists in their global scratch.
-template <bool PACKED, bool SPILLED>
-__global__ __launch_bounds__(64) void k_cluster(ClusterArgs a) {
+//
+// WAVES8: the register budget of eight wavefronts per SIMD (a few SGPRs
+// move to VGPR lanes), used when the LDS capacity allows 32 wavefronts per
+// CU; otherwise the budget of seven, which LDS already limits to.
+template <bool PACKED, bool SPILLED, bool WAVES8>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES8 ? 8 : 1, 8))) void k_cluster(ClusterArgs a) {
     __shared__ WaveLds sh;
     extern __shared__ uint32_t lds_e[];
     // the spill pass gathers straight into its global scratch
@@ -677,8 +682,12 @@ int cluster_capacity(long long expected_max) {
 template <bool PACKED>
 static void launch_cap(const ClusterArgs &a, hipStream_t s) {
     const int grid = (int)(((long long)a.n_list + a.per_wave - 1) / a.per_wave);
-    hipLaunchKernelGGL((k_cluster<PACKED, false>), dim3(grid), dim3(64), (size_t)a.cap * sizeof(uint32_t), s, a);
-    if (a.spill_list) hipLaunchKernelGGL((k_cluster<PACKED, true>), dim3(a.n_spill_waves), dim3(64), 0, s, a);
+    const size_t lds = (size_t)a.cap * sizeof(uint32_t);
+    if (32 * (lds + sizeof(WaveLds)) <= kLdsPerCu)
+        hipLaunchKernelGGL((k_cluster<PACKED, false, true>), dim3(grid), dim3(64), lds, s, a);
+    else
+        hipLaunchKernelGGL((k_cluster<PACKED, false, false>), dim3(grid), dim3(64), lds, s, a);
+    if (a.spill_list) hipLaunchKernelGGL((k_cluster<PACKED, true, false>), dim3(a.n_spill_waves), dim3(64), 0, s, a);
 }
 
 void launch_cluster_search(const ClusterArgs &a, hipStream_t s) {
