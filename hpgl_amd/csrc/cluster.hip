@@ -18,10 +18,9 @@
 //    k-th left stop (first elements with key <= pivot, scanning up) is
 //    swapped with the k-th right stop (key >= pivot, scanning down) while
 //    the left stop is below the right stop, and the cut is
-//    min(l_{s+1}, r_s) for s swaps.  One ballot pass records, per chunk of
-//    64 positions, the masks of left and right stops and their running
-//    counts; the swap pass expands 64 left stops and the matching 64 right
-//    stops at a time from those masks and performs the (disjoint) swaps;
+//    min(l_{s+1}, r_s) for s swaps.  Two cursors ballot the stop masks of
+//    64-position chunks from both ends, 64 stop pairs are laid out at a
+//    time and their (disjoint) swaps performed together;
 //  * ranges that start at or beyond max_n are never partitioned: they
 //    cannot change the first max_n positions, which after the final
 //    insertion sort are the stable sort of the partition blocks they fall
@@ -56,24 +55,19 @@ __device__ inline void wave_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
+// Mask of the lanes whose predicate holds, straight from the compare.
+__device__ inline uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
 __device__ inline uint64_t lanemask_lt() {
     const int lane = threadIdx.x & 63;
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
-// Sort state of one list: elements, and per chunk of 64 positions of the
-// range being partitioned the masks of left / right stops (ml, mr) and the
-// number of stops before the chunk (pl, pr; entry nch holds the totals).
-struct SortBufs {
-    uint32_t *e;
-    uint64_t *ml, *mr;
-    int *pl, *pr;
-};
-
 // Fixed LDS of a wavefront; the elements of a list that fits the capacity
 // live in the dynamic LDS.
 struct WaveLds {
     int lbuf[64], rbuf[64];  // positions of the 64 stop pairs being swapped
+    int junk[64];            // per-lane target of stores that are not needed
     // the 27 bins as 9 rows (k, j) of three consecutive bins i-1, i, i+1,
     // which are adjacent in the bin-sorted cell arrays
     int seg_start[9];
@@ -137,209 +131,28 @@ __device__ void heap_sort_(E *e, long len, Keyed key) {
     }
 }
 
-// Position (0-63) of the j-th set bit of m (j < popcount(m)).
-__device__ inline int select_bit(uint64_t m, int j) {
-    int pos = 0;
-    int c = __popc((uint32_t)m);
-    if (j >= c) {
-        j -= c;
-        m >>= 32;
-        pos += 32;
-    }
-    c = __popc((uint32_t)m & 0xffffu);
-    if (j >= c) {
-        j -= c;
-        m >>= 16;
-        pos += 16;
-    }
-    c = __popc((uint32_t)m & 0xffu);
-    if (j >= c) {
-        j -= c;
-        m >>= 8;
-        pos += 8;
-    }
-    c = __popc((uint32_t)m & 0xfu);
-    if (j >= c) {
-        j -= c;
-        m >>= 4;
-        pos += 4;
-    }
-    c = __popc((uint32_t)m & 0x3u);
-    if (j >= c) {
-        j -= c;
-        m >>= 2;
-        pos += 2;
-    }
-    if (j >= (int)(m & 1u)) pos += 1;
-    return pos;
+// Number of set bits of the wave-uniform mask m below this lane.
+__device__ inline int bits_below(uint64_t m) {
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-// Offset from the range start of the k-th stop (0-based, k < total) of the
-// masks m / counts p over nch chunks.  Called with the same k by every lane.
-__device__ inline int nth_stop(const uint64_t *m, const int *p, int nch, int k) {
-    int lo = 0, hi = nch;  // largest c < nch with p[c] <= k
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (p[mid] <= k)
-            lo = mid;
-        else
-            hi = mid;
-    }
-    return (lo << 6) + select_bit(m[lo], k - p[lo]);
-}
-
-// Writes buf[idx] = position of the stop with index idx0 + idx, for the
-// stops of index [idx0, idx0 + 64) (masks m / counts p, chunk c0 holds
-// index max(idx0, 0)).
-__device__ inline void expand_stops(const uint64_t *m, const int *p, int nch, int c0, int idx0, int first, int *buf) {
-    const int lane = threadIdx.x & 63;
-    const uint64_t lt = lanemask_lt();
-    for (int c = c0; c < nch && p[c] < idx0 + 64; ++c) {
-        const uint64_t mk = m[c];
-        const int idx = p[c] + __popcll(mk & lt) - idx0;
-        if (((mk >> lane) & 1u) && idx >= 0 && idx < 64) buf[idx] = first + (c << 6) + lane;
-    }
-}
-
-// One __unguarded_partition_pivot on e[lo, hi); returns the cut.
+// std::__unguarded_partition_pivot on e[lo, hi) (e in LDS or in global
+// scratch).  With pivot key kp the left scan stops at elements with
+// key <= kp, the right scan at key >= kp; the k-th left stop L_k is swapped
+// with the k-th right stop R_k (counted from the end) while L_k < R_k, and
+// for s swaps the cut is min(L_s, R_{s-1}).
+//
+// Two cursors produce the stops 64 pairs at a time: the left one walks
+// chunks of 64 positions upwards from the range start, the right one
+// downwards from its end; entering a chunk, a cursor ballots its stop mask
+// and keeps the stops not yet used in a scalar mask.  Swaps only touch
+// positions both scans have passed, so a mask taken before earlier swaps
+// is exact for every position below R_{s-1}, and the stops it reports at
+// or beyond R_{s-1} change neither the swaps nor min(L_s, R_{s-1}).
 template <typename S>
-__device__ int partition_(const SortBufs &b, S &sh, int lo, int hi, Keyed key) {
+__device__ int partition_(uint32_t *e, S &sh, int lo, int hi, Keyed key) {
     const int lane = threadIdx.x & 63;
-    uint32_t *e = b.e;
-    {  // __move_median_to_first(lo, lo+1, mid, hi-1)
-        const int a = lo + 1, bm = lo + (hi - lo) / 2, c = hi - 1;
-        const uint32_t ka = key(e[a]), kb = key(e[bm]), kc = key(e[c]);
-        int sw;
-        if (ka > kb) {
-            if (kb > kc)
-                sw = bm;
-            else if (ka > kc)
-                sw = c;
-            else
-                sw = a;
-        } else if (ka > kc) {
-            sw = a;
-        } else if (kb > kc) {
-            sw = c;
-        } else {
-            sw = bm;
-        }
-        wave_sync();
-        if (lane == 0) {
-            const uint32_t t0 = e[lo];
-            e[lo] = e[sw];
-            e[sw] = t0;
-        }
-        wave_sync();
-    }
-    const uint32_t kp = key(e[lo]);
-    const int first = lo + 1;
-    const int nch = (hi - first + 63) >> 6;
-    // left stops: key <= pivot; right stops: key >= pivot
-    int nL = 0, nR = 0;
-    for (int c = 0; c < nch; ++c) {
-        const int p = first + (c << 6) + lane;
-        const bool v = p < hi;
-        const uint32_t k = v ? key(e[p]) : 0u;
-        const uint64_t mL = __ballot(v && k <= kp), mR = __ballot(v && k >= kp);
-        if (lane == 0) {
-            b.ml[c] = mL;
-            b.mr[c] = mR;
-            b.pl[c] = nL;
-            b.pr[c] = nR;
-        }
-        nL += __popcll(mL);
-        nR += __popcll(mR);
-    }
-    if (lane == 0) {
-        b.pl[nch] = nL;
-        b.pr[nch] = nR;
-    }
-    wave_sync();
-    // swap the k-th left stop with the k-th right stop from the end while
-    // the former is below the latter; s = number of swaps (a prefix of k)
-    const int m = min(nL, nR);
-    int s = 0;
-    int cl = 0, cr = nch - 1;
-    for (int kb = 0; kb < m; kb += 64) {
-        while (cl + 1 < nch && b.pl[cl + 1] <= kb) ++cl;
-        expand_stops(b.ml, b.pl, nch, cl, kb, first, sh.lbuf);
-        // right stops kb .. kb+63 from the end have indices
-        // nR-1-kb .. nR-64-kb counted from the start
-        const int base = nR - 64 - kb;
-        const int from = base > 0 ? base : 0;
-        while (cr > 0 && b.pr[cr] > from) --cr;
-        expand_stops(b.mr, b.pr, nch, cr, base, first, sh.rbuf);
-        wave_sync();
-        const int k = kb + lane;
-        bool c = false;
-        int pl = 0, pr = 0;
-        if (k < m) {
-            pl = sh.lbuf[lane];
-            pr = sh.rbuf[63 - lane];
-            c = pl < pr;
-        }
-        if (c) {
-            const uint32_t tl = e[pl];
-            e[pl] = e[pr];
-            e[pr] = tl;
-        }
-        const int cnt = __popcll(__ballot(c));
-        s += cnt;
-        wave_sync();
-        if (cnt < 64) break;
-    }
-    const int a = s < nL ? first + nth_stop(b.ml, b.pl, nch, s) : 0x7fffffff;
-    const int bq = s >= 1 ? first + nth_stop(b.mr, b.pr, nch, nR - s) : 0x7fffffff;
-    wave_sync();
-    return min(a, bq);
-}
-
-__device__ inline uint64_t readlane64(uint64_t v, int l) {
-    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, l);
-    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), l);
-    return ((uint64_t)hi << 32) | lo;
-}
-
-// Highest lane whose predicate holds (the wavefront must hold at least one).
-__device__ inline int last_lane(bool pred) { return 63 - __clzll(__ballot(pred)); }
-
-// Position (0-63) of the j-th set bit of a wave-uniform mask m.
-__device__ inline int select_bit_wave(uint64_t m, int j) {
-    const int lane = threadIdx.x & 63;
-    const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-    return __ffsll((unsigned long long)__ballot(((m >> lane) & 1u) && below == j)) - 1;
-}
-
-// Offset from the range start of the k-th stop (0-based, k < total) when
-// lane c holds the mask m and the preceding count p of chunk c (c < nch).
-__device__ inline int nth_stop_reg(uint64_t m, int p, int nch, int k) {
-    const int lane = threadIdx.x & 63;
-    const int c = last_lane(lane < nch && p <= k);  // chunk 0 always qualifies
-    return (c << 6) + select_bit_wave(readlane64(m, c), k - __builtin_amdgcn_readlane(p, c));
-}
-
-// buf[idx] = position of the stop with index idx0 + idx for the stops of
-// index [idx0, idx0 + 64), from the chunk masks / counts held by the lanes.
-__device__ inline void expand_stops_reg(uint64_t m, int p, int nch, int idx0, int first, int *buf) {
-    const int lane = threadIdx.x & 63;
-    const uint64_t lt = lanemask_lt();
-    const int from = idx0 > 0 ? idx0 : 0;
-    const int c0 = last_lane(lane < nch && p <= from);
-    const int c1 = last_lane(lane < nch && p < idx0 + 64);
-    for (int c = c0; c <= c1; ++c) {
-        const int pc = __builtin_amdgcn_readlane(p, c);
-        const uint64_t mk = readlane64(m, c);
-        const int idx = pc + __popcll(mk & lt) - idx0;
-        if (((mk >> lane) & 1u) && idx >= 0 && idx < 64) buf[idx] = first + (c << 6) + lane;
-    }
-}
-
-// partition_ for ranges of at most 64 chunks (4097 elements): the chunk
-// masks and counts stay in registers, lane c holding chunk c.
-template <typename S>
-__device__ int partition_reg(uint32_t *e, S &sh, int lo, int hi, Keyed key) {
-    const int lane = threadIdx.x & 63;
+    uint32_t kp;
     {  // __move_median_to_first(lo, lo+1, mid, hi-1)
         const int a = lo + 1, bm = lo + (hi - lo) / 2, c = hi - 1;
         const uint32_t ka = key(e[a]), kb = key(e[bm]), kc = key(e[c]);
@@ -359,6 +172,7 @@ __device__ int partition_reg(uint32_t *e, S &sh, int lo, int hi, Keyed key) {
             sw = bm;
         }
         sw = __builtin_amdgcn_readfirstlane(sw);
+        kp = __builtin_amdgcn_readfirstlane(sw == a ? ka : sw == bm ? kb : kc);
         wave_sync();
         if (lane == 0) {
             const uint32_t t0 = e[lo];
@@ -367,72 +181,76 @@ __device__ int partition_reg(uint32_t *e, S &sh, int lo, int hi, Keyed key) {
         }
         wave_sync();
     }
-    const uint32_t kp = key(e[lo]);
     const int first = lo + 1;
     const int nch = (hi - first + 63) >> 6;
-    uint64_t my_ml = 0, my_mr = 0;
-    int my_pl = 0, my_pr = 0;
-    int nL = 0, nR = 0;
-    for (int c = 0; c < nch; c += 2) {
-        const int p0 = first + (c << 6) + lane, p1 = p0 + 64;
-        const bool v0 = p0 < hi, v1 = p1 < hi;
-        const uint32_t k0 = v0 ? key(e[p0]) : 0u;
-        const uint32_t k1 = v1 ? key(e[p1]) : 0u;
-        const uint64_t mL0 = __ballot(v0 && k0 <= kp), mR0 = __ballot(v0 && k0 >= kp);
-        const uint64_t mL1 = __ballot(v1 && k1 <= kp), mR1 = __ballot(v1 && k1 >= kp);
-        if (lane == c) {
-            my_ml = mL0;
-            my_mr = mR0;
-            my_pl = nL;
-            my_pr = nR;
+    int cl = -1, cr = nch;      // chunks the cursors are in
+    uint64_t lrem = 0, rrem = 0;  // their stops not yet paired
+    int r_prev = 0x7fffffff;    // R_{s-1} of the rounds before
+    for (;;) {
+        // left stops L_{s} .. L_{s+63} into lbuf, in position order
+        int fl = 0;
+        while (fl < 64) {
+            if (lrem == 0) {
+                if (cl + 1 >= nch) break;
+                ++cl;
+                const int p = first + (cl << 6) + lane;
+                const uint32_t k = key(e[min(p, hi - 1)]);
+                lrem = ballot(p < hi && k <= kp);
+                continue;
+            }
+            const int idx = fl + bits_below(lrem);
+            const bool hit = ((lrem >> lane) & 1u) && idx < 64;
+            *(hit ? sh.lbuf + idx : sh.junk + lane) = first + (cl << 6) + lane;
+            const uint64_t took = ballot(hit);
+            lrem &= ~took;
+            fl += __popcll(took);
         }
-        nL += __popcll(mL0);
-        nR += __popcll(mR0);
-        if (lane == c + 1) {
-            my_ml = mL1;
-            my_mr = mR1;
-            my_pl = nL;
-            my_pr = nR;
+        // right stops R_{s} .. R_{s+63} into rbuf, in descending position
+        int fr = 0;
+        while (fr < 64) {
+            if (rrem == 0) {
+                if (cr <= 0) break;
+                --cr;
+                const int p = first + (cr << 6) + lane;
+                const uint32_t k = key(e[min(p, hi - 1)]);
+                rrem = ballot(p < hi && k >= kp);
+                continue;
+            }
+            const int idx = fr + __popcll(rrem) - 1 - bits_below(rrem);
+            const bool hit = ((rrem >> lane) & 1u) && idx < 64;
+            *(hit ? sh.rbuf + idx : sh.junk + lane) = first + (cr << 6) + lane;
+            const uint64_t took = ballot(hit);
+            rrem &= ~took;
+            fr += __popcll(took);
         }
-        nL += __popcll(mL1);
-        nR += __popcll(mR1);
-    }
-    const int m = min(nL, nR);
-    int s = 0;
-    for (int kb = 0; kb < m; kb += 64) {
-        expand_stops_reg(my_ml, my_pl, nch, kb, first, sh.lbuf);
-        // right stops kb .. kb+63 from the end have indices
-        // nR-1-kb .. nR-64-kb counted from the start
-        expand_stops_reg(my_mr, my_pr, nch, nR - 64 - kb, first, sh.rbuf);
         wave_sync();
-        const int k = kb + lane;
-        bool c = false;
-        int pl = 0, pr = 0;
-        if (k < m) {
-            pl = sh.lbuf[lane];
-            pr = sh.rbuf[63 - lane];
-            c = pl < pr;
-        }
+        const int m = min(fl, fr);
+        const int l_at = sh.lbuf[lane], r_at = sh.rbuf[lane];
+        const bool in = lane < m;
+        const int pl = in ? l_at : first, pr = in ? r_at : first;
+        const bool c = in && pl < pr;
+        const uint32_t tl = e[pl], tr = e[pr];
         if (c) {
-            const uint32_t tl = e[pl];
-            e[pl] = e[pr];
+            e[pl] = tr;
             e[pr] = tl;
         }
-        const int cnt = __popcll(__ballot(c));
-        s += cnt;
+        const int cnt = __popcll(ballot(c));
+        if (cnt < 64) {
+            const int a = cnt < fl ? __builtin_amdgcn_readlane(l_at, cnt) : 0x7fffffff;
+            const int bq = cnt >= 1 ? __builtin_amdgcn_readlane(r_at, cnt - 1) : r_prev;
+            wave_sync();
+            return min(a, bq);
+        }
+        r_prev = __builtin_amdgcn_readlane(r_at, 63);
         wave_sync();
-        if (cnt < 64) break;
     }
-    const int a = s < nL ? first + nth_stop_reg(my_ml, my_pl, nch, s) : 0x7fffffff;
-    const int bq = s >= 1 ? first + nth_stop_reg(my_mr, my_pr, nch, nR - s) : 0x7fffffff;
-    return min(a, bq);
 }
 
 // Returns false if the replay reached a state libstdc++'s algorithm cannot
-// reach (a cut outside the range or a deeper pending stack than max_n
+// reach (a cut outside the range or a deeper pending stack than 64
 // ranges); the caller then reports an internal error instead of looping.
-template <bool REG, typename S>
-__device__ bool topk_sort_(const SortBufs &b, S &sh, int n, int K, Keyed key) {
+template <typename S>
+__device__ bool topk_sort_(uint32_t *e, S &sh, int n, int K, Keyed key) {
     const int lane = threadIdx.x & 63;
     if (n <= 1) return true;
     const int lg = 31 - __clz(n);
@@ -446,12 +264,12 @@ __device__ bool topk_sort_(const SortBufs &b, S &sh, int n, int K, Keyed key) {
         while (hi - lo > 16) {
             if (d == 0) {
                 wave_sync();
-                if (lane == 0) heap_sort_(b.e + lo, hi - lo, key);
+                if (lane == 0) heap_sort_(e + lo, hi - lo, key);
                 wave_sync();
                 break;
             }
             --d;
-            const int cut = REG ? partition_reg(b.e, sh, lo, hi, key) : partition_(b, sh, lo, hi, key);
+            const int cut = partition_(e, sh, lo, hi, key);
             if (cut <= lo || cut >= hi || top >= 64) return false;
             if (cut < K) {
                 if (lane == top) {
@@ -521,21 +339,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES8 ? 8 :
     const int X = a.X, Y = a.Y;
     const BinArgs &b = a.bins;
     const Keyed key{a.local_bits};
-    // global scratch of this wave for lists longer than a.cap: masks first
-    // (8-byte aligned), then elements and counts
-    SortBufs gb{nullptr, nullptr, nullptr, nullptr, nullptr};
-    uint32_t *ge = nullptr;
-    if (SPILLED) {
-        const int nchm = a.scratch_per_wave / 64 + 2;
-        uint32_t *base = a.scratch + (size_t)wave_id * cluster_scratch_words(a.scratch_per_wave);
-        gb.ml = reinterpret_cast<uint64_t *>(base);
-        gb.mr = gb.ml + nchm;
-        gb.e = reinterpret_cast<uint32_t *>(gb.mr + nchm);
-        gb.pl = reinterpret_cast<int *>(gb.e + a.scratch_per_wave);
-        gb.pr = gb.pl + nchm;
-        ge = gb.e;
-    }
-    const SortBufs lb{lds_e, nullptr, nullptr, nullptr, nullptr};  // masks stay in registers
+    // global scratch of this wave: the elements of lists longer than a.cap
+    uint32_t *ge = SPILLED ? a.scratch + (size_t)wave_id * cluster_scratch_words(a.scratch_per_wave) : nullptr;
     unsigned long long cand_acc = 0, spill_acc = 0;
     const int li_end = SPILLED ? *a.spill_count : a.n_list;
     for (int li = wave_id; li < li_end; li += gridDim.x) {
@@ -629,7 +434,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES8 ? 8 :
             }
 #pragma unroll
             for (int u = 0; u < kGatherUnroll; ++u) {
-                const uint64_t mk = __ballot(rk[u] != 0);
+                const uint64_t mk = ballot(rk[u] != 0);
                 const int pp = pos + __popcll(mk & lt);
                 if (rk[u] != 0) {
                     const uint32_t ev = (rk[u] << key.lb) | (uint32_t)(base + u * 64 + lane);
@@ -646,7 +451,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES8 ? 8 :
         int *out = a.nbr + (size_t)t * a.max_n;
         bool sane = true;
         if (total <= cap) {
-            sane = topk_sort_<true>(lb, sh, total, a.max_n, key);
+            sane = topk_sort_(lds_e, sh, total, a.max_n, key);
             if (sane) finish_(lds_e, total, a.max_n, out, sh, a.bin_cells, key);
         } else if (!SPILLED && a.spill_list) {
             if (lane == 0) a.spill_list[atomicAdd(a.spill_count, 1)] = t;
@@ -655,7 +460,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES8 ? 8 :
         } else if (SPILLED && total <= a.scratch_per_wave) {
             spill_acc += 1;
             wave_sync();
-            sane = topk_sort_<false>(gb, sh, total, a.max_n, key);
+            sane = topk_sort_(ge, sh, total, a.max_n, key);
             if (sane) finish_(ge, total, a.max_n, out, sh, a.bin_cells, key);
         } else {
             if (lane == 0) atomicOr(a.overflow, 1u);

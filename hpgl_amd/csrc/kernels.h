@@ -153,14 +153,9 @@ struct ClusterArgs {
     unsigned long long *cand_total;  // counter lines: word 0 candidates gathered, word 1 lists sorted in scratch
 };
 void launch_cluster_search(const ClusterArgs &a, hipStream_t s);
-// Scratch words per wave for lists of up to n candidates: stop masks and
-// counts per chunk of 64 and the elements (an even count keeps the masks
-// of the next wave 8-byte aligned).
-__host__ __device__ inline long long cluster_scratch_words(int n) {
-    const long long nchm = n / 64 + 2;
-    const long long w = 4 * nchm + n + 2 * nchm;
-    return w + (w & 1);
-}
+// Scratch words per wave for lists of up to n candidates (the elements,
+// rounded up to a multiple of 64 words).
+__host__ __device__ inline long long cluster_scratch_words(int n) { return ((long long)n + 63) / 64 * 64; }
 // LDS list capacity of the cell-binned search kernel for lists of up to
 // expected_max candidates (the largest built capacity when none suffices).
 int cluster_capacity(long long expected_max);
