@@ -66,8 +66,9 @@ __device__ inline uint64_t lanemask_lt() {
 // Fixed LDS of a wavefront; the elements of a list that fits the capacity
 // live in the dynamic LDS.
 struct WaveLds {
-    int lbuf[64], rbuf[64];  // positions of the 64 stop pairs being swapped
-    int junk[64];            // per-lane target of stores that are not needed
+    // positions of the 64 stop pairs being swapped: left stops in [0, 64),
+    // right stops in [64, 128); [128, 192) takes stores that are not needed
+    int pairs[192];
     // the 27 bins as 9 rows (k, j) of three consecutive bins i-1, i, i+1,
     // which are adjacent in the bin-sorted cell arrays
     int seg_start[9];
@@ -131,6 +132,14 @@ __device__ void heap_sort_(E *e, long len, Keyed key) {
     }
 }
 
+// Per-lane select on a wave-uniform lane mask: t where bit lane of m is
+// set, else f (a single v_cndmask with the mask as its condition).
+__device__ inline int lane_select(uint64_t m, int t, int f) {
+    int r;
+    asm("v_cndmask_b32 %0, %1, %2, %3" : "=v"(r) : "v"(f), "v"(t), "s"(m));
+    return r;
+}
+
 // Number of set bits of the wave-uniform mask m below this lane.
 __device__ inline int bits_below(uint64_t m) {
     return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
@@ -155,7 +164,9 @@ __device__ int partition_(uint32_t *e, S &sh, int lo, int hi, Keyed key) {
     uint32_t kp;
     {  // __move_median_to_first(lo, lo+1, mid, hi-1)
         const int a = lo + 1, bm = lo + (hi - lo) / 2, c = hi - 1;
-        const uint32_t ka = key(e[a]), kb = key(e[bm]), kc = key(e[c]);
+        const uint32_t ka = __builtin_amdgcn_readfirstlane(key(e[a]));
+        const uint32_t kb = __builtin_amdgcn_readfirstlane(key(e[bm]));
+        const uint32_t kc = __builtin_amdgcn_readfirstlane(key(e[c]));
         int sw;
         if (ka > kb) {
             if (kb > kc)
@@ -171,8 +182,7 @@ __device__ int partition_(uint32_t *e, S &sh, int lo, int hi, Keyed key) {
         } else {
             sw = bm;
         }
-        sw = __builtin_amdgcn_readfirstlane(sw);
-        kp = __builtin_amdgcn_readfirstlane(sw == a ? ka : sw == bm ? kb : kc);
+        kp = sw == a ? ka : sw == bm ? kb : kc;
         wave_sync();
         if (lane == 0) {
             const uint32_t t0 = e[lo];
@@ -183,49 +193,44 @@ __device__ int partition_(uint32_t *e, S &sh, int lo, int hi, Keyed key) {
     }
     const int first = lo + 1;
     const int nch = (hi - first + 63) >> 6;
-    int cl = -1, cr = nch;      // chunks the cursors are in
+    const int p0 = first + lane;  // this lane's position in chunk 0
+    int cl = -1, cr = nch;        // chunks the cursors are in
     uint64_t lrem = 0, rrem = 0;  // their stops not yet paired
-    int r_prev = 0x7fffffff;    // R_{s-1} of the rounds before
+    int r_prev = 0x7fffffff;      // R_{s-1} of the rounds before
     for (;;) {
-        // left stops L_{s} .. L_{s+63} into lbuf, in position order
+        // left stops L_{s} .. L_{s+63} into pairs[0, 64), in position order:
+        // use the stops left in the current chunk, then enter the next one
         int fl = 0;
-        while (fl < 64) {
-            if (lrem == 0) {
-                if (cl + 1 >= nch) break;
-                ++cl;
-                const int p = first + (cl << 6) + lane;
-                const uint32_t k = key(e[min(p, hi - 1)]);
-                lrem = ballot(p < hi && k <= kp);
-                continue;
-            }
+        for (;;) {
             const int idx = fl + bits_below(lrem);
-            const bool hit = ((lrem >> lane) & 1u) && idx < 64;
-            *(hit ? sh.lbuf + idx : sh.junk + lane) = first + (cl << 6) + lane;
-            const uint64_t took = ballot(hit);
-            lrem &= ~took;
+            const uint64_t took = lrem & ballot((unsigned)idx < 64u);
+            sh.pairs[lane_select(took, idx, 128 + lane)] = p0 + (cl << 6);
+            lrem ^= took;
             fl += __popcll(took);
+            if (fl == 64 || cl + 1 >= nch) break;
+            ++cl;
+            const int p = p0 + (cl << 6);
+            const uint32_t k = key(e[min(p, hi - 1)]);
+            lrem = ballot(k <= kp) & ballot(p < hi);
         }
-        // right stops R_{s} .. R_{s+63} into rbuf, in descending position
+        // right stops R_{s} .. R_{s+63} into pairs[64, 128), in descending
+        // position order
         int fr = 0;
-        while (fr < 64) {
-            if (rrem == 0) {
-                if (cr <= 0) break;
-                --cr;
-                const int p = first + (cr << 6) + lane;
-                const uint32_t k = key(e[min(p, hi - 1)]);
-                rrem = ballot(p < hi && k >= kp);
-                continue;
-            }
+        for (;;) {
             const int idx = fr + __popcll(rrem) - 1 - bits_below(rrem);
-            const bool hit = ((rrem >> lane) & 1u) && idx < 64;
-            *(hit ? sh.rbuf + idx : sh.junk + lane) = first + (cr << 6) + lane;
-            const uint64_t took = ballot(hit);
-            rrem &= ~took;
+            const uint64_t took = rrem & ballot((unsigned)idx < 64u);
+            sh.pairs[lane_select(took, 64 + idx, 128 + lane)] = p0 + (cr << 6);
+            rrem ^= took;
             fr += __popcll(took);
+            if (fr == 64 || cr <= 0) break;
+            --cr;
+            const int p = p0 + (cr << 6);
+            const uint32_t k = key(e[min(p, hi - 1)]);
+            rrem = ballot(k >= kp) & ballot(p < hi);
         }
         wave_sync();
         const int m = min(fl, fr);
-        const int l_at = sh.lbuf[lane], r_at = sh.rbuf[lane];
+        const int l_at = sh.pairs[lane], r_at = sh.pairs[64 + lane];
         const bool in = lane < m;
         const int pl = in ? l_at : first, pr = in ? r_at : first;
         const bool c = in && pl < pr;
