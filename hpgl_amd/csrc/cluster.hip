@@ -45,6 +45,16 @@ namespace {
 // multiple of 64, at most 4096 (the register-held partition covers 64
 // chunks); longer lists go to global scratch.
 constexpr int kMinCap = 256, kMaxCap = 4096;
+
+// The 27 bins of a target in gather order (k, j, i ascending) as 21
+// segments of the bin-sorted arrays: the 9 bins of the plane below, one
+// segment each, keeping the cells with z >= z_target - span; the 3 rows of
+// the target's plane, each three adjacent bins in one segment; the 9 bins
+// of the plane above, keeping z <= z_target + span.  A bin's cells ascend
+// by cell index, hence by z, so the kept cells are a suffix (below) or a
+// prefix (above) and the order of the candidates is unchanged; the cells
+// dropped have offsets outside the rank table (rank 0).
+constexpr int kSegs = 21, kMaxSegs = 32;
 constexpr size_t kLdsPerCu = 160 * 1024;
 
 // A wavefront orders its own accesses (LDS and the global scratch lists)
@@ -69,11 +79,11 @@ struct WaveLds {
     // positions of the 64 stop pairs being swapped: left stops in [0, 64),
     // right stops in [64, 128); [128, 192) takes stores that are not needed
     int pairs[192];
-    // the 27 bins as 9 rows (k, j) of three consecutive bins i-1, i, i+1,
-    // which are adjacent in the bin-sorted cell arrays
-    int seg_start[9];
-    int seg_cnt[9];
-    int seg_pref[10];
+    // candidate segments (kSegs, then kMaxSegs - kSegs sentinels): first
+    // raw index of each (segment sizes padded to multiples of 64) and the
+    // offset from raw index to position in the bin-sorted arrays
+    int seg_raw[kMaxSegs];
+    int seg_off[kMaxSegs];
 };
 
 constexpr int kGatherUnroll = 4;  // 64-candidate groups whose loads are in flight together
@@ -309,10 +319,14 @@ __device__ void finish_(const E *e, int n, int K, int *out, const S &sh, const i
                 r += (kq > kp) || (kq == kp && q < p);
             }
             if (r < K) {
+                // the segment of raw index j: the last one starting at or
+                // before j (empty segments start where the next one does)
                 const int j = (int)(ep & lmask);
                 int g = 0;
-                while (sh.seg_pref[g + 1] <= j) ++g;
-                out[r] = bin_cells[sh.seg_start[g] + (j - sh.seg_pref[g])];
+#pragma unroll
+                for (int step = kMaxSegs / 2; step >= 1; step >>= 1)
+                    if (sh.seg_raw[g + step] <= j) g += step;
+                out[r] = bin_cells[sh.seg_off[g] + j];
             }
         }
     }
@@ -353,62 +367,71 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES8 ? 8 :
         const int node = a.targets[t];
         const int x = node % X, y = node % (X * Y) / X, z = node / (X * Y);
         const int bx = x / b.r[0], by = y / b.r[1], bz = z / b.r[2];
-        // the 27 bins in gather order (k, j, i ascending) as 9 contiguous
-        // segments of the bin-sorted arrays; bins outside the bin box hold
-        // no cells (bx + 1 <= m0 - 1 always holds, bx - 1 may be -1)
-        if (lane < 9) {
-            const int jj = by - 1 + lane % 3, kk = bz - 1 + lane / 3;
-            int start = 0, cnt = 0;
-            if (jj >= 0 && kk >= 0 && jj < b.m[1] && kk < b.m[2]) {
-                const int row = (kk * b.m[1] + jj) * b.m[0];
-                const int i0 = bx > 0 ? bx - 1 : 0, i1 = bx + 1;
-                start = (int)a.bin_start[row + i0];
-                cnt = (int)(a.bin_start[row + i1] + a.bin_count[row + i1]) - start;
-            }
-            sh.seg_start[lane] = start;
-            sh.seg_cnt[lane] = cnt;
-        }
-        wave_sync();
-        int pref = 0;
-#pragma unroll
-        for (int g = 0; g < 9; ++g) {
-            if (lane == g) sh.seg_pref[g] = pref;
-            pref += sh.seg_cnt[g];
-        }
-        if (lane == 9) sh.seg_pref[9] = pref;
-        const int raw_total = __builtin_amdgcn_readfirstlane(pref);
-        wave_sync();
-        // gather: keep candidates with rank > 0, in raw order.  Each pass
-        // issues the loads of kGatherUnroll groups of 64 before compacting.
-        // A group normally lies in one segment (a scalar cursor gives its
-        // offset); a group that crosses segment ends resolves them per lane.
-        int seg = 0;
-        int seg_end = __builtin_amdgcn_readfirstlane(sh.seg_pref[1]);
-        int seg_off = __builtin_amdgcn_readfirstlane(sh.seg_start[0]);
         const int s0 = a.span[0], s1 = a.span[1], s2 = a.span[2];
         const int rd0 = a.rd[0], rd1 = a.rd[1], rd2 = a.rd[2];
+        // lane g < kSegs: segment g's first position in the bin-sorted
+        // arrays and its length (bins outside the bin box hold no cells;
+        // bx + 1 <= m0 - 1 always holds, bx - 1 may be -1)
+        int start = 0, cnt = 0;
+        if (lane < kSegs) {
+            const int plane = lane < 9 ? 0 : lane < 12 ? 1 : 2;
+            const int q = plane == 0 ? lane : plane == 1 ? lane - 9 : lane - 12;
+            const int kk = bz - 1 + plane;
+            const int jj = by - 1 + (plane == 1 ? q : q / 3);
+            const int i0 = plane == 1 ? (bx > 0 ? bx - 1 : 0) : bx - 1 + q % 3;
+            const int i1 = plane == 1 ? bx + 1 : i0;
+            if (jj >= 0 && kk >= 0 && i0 >= 0 && jj < b.m[1] && kk < b.m[2]) {
+                const int row = (kk * b.m[1] + jj) * b.m[0];
+                start = (int)a.bin_start[row + i0];
+                int end = (int)(a.bin_start[row + i1] + a.bin_count[row + i1]);
+                const int rz = b.r[2];
+                const uint32_t *zo = a.zoff + (size_t)(row + i0) * (rz + 1);
+                if (plane == 0) {
+                    const int d = z - s2 - kk * rz;  // layers of the bin below z - span
+                    if (d > 0) start += (int)zo[min(d, rz)];
+                } else if (plane == 2) {
+                    const int d = z + s2 + 1 - kk * rz;  // layers of the bin up to z + span
+                    if (d < rz) end = start + (int)zo[max(d, 0)];
+                }
+                cnt = max(end - start, 0);
+            }
+        }
+        // raw indices: segments padded to multiples of 64, so that every
+        // group of 64 lies in one segment
+        const int padded = (cnt + 63) & ~63;
+        int incl = padded;
+#pragma unroll
+        for (int d = 1; d < kMaxSegs; d <<= 1) {
+            const int v = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += v;
+        }
+        const int raw0 = incl - padded;
+        const int raw_total = __builtin_amdgcn_readlane(incl, kSegs - 1);
+        if (lane < kMaxSegs) {
+            sh.seg_raw[lane] = lane < kSegs ? raw0 : 0x7fffffff;
+            sh.seg_off[lane] = start - raw0;
+        }
+        const int seg_lim = raw0 + cnt;  // lane g: end of segment g's cells
+        // gather: keep candidates with rank > 0, in raw order.  Each pass
+        // issues the loads of kGatherUnroll groups of 64 before compacting;
+        // a scalar cursor follows the segment of each group.
+        int seg = -1, g_end = 0, g_off = 0, g_lim = 0;
         int pos = 0;
         for (int base = 0; base < raw_total; base += 64 * kGatherUnroll) {
-            // positions in the bin-sorted arrays (no memory reads but LDS)
             int gi[kGatherUnroll];
+            bool valid[kGatherUnroll];
 #pragma unroll
             for (int u = 0; u < kGatherUnroll; ++u) {
                 const int gb = base + u * 64;
-                const int p = gb + lane;
-                while (seg < 8 && gb >= seg_end) {
+                while (gb >= g_end && seg < kSegs - 1) {
                     ++seg;
-                    seg_end = __builtin_amdgcn_readfirstlane(sh.seg_pref[seg + 1]);
-                    seg_off = __builtin_amdgcn_readfirstlane(sh.seg_start[seg] - sh.seg_pref[seg]);
+                    g_end = __builtin_amdgcn_readlane(incl, seg);
+                    g_off = __builtin_amdgcn_readlane(start - raw0, seg);
+                    g_lim = __builtin_amdgcn_readlane(seg_lim, seg);
                 }
-                int g0 = seg_off + p;
-                if (seg < 8 && gb + 63 >= seg_end) {
-                    for (int g = seg + 1; g < 9; ++g) {
-                        const int pg = __builtin_amdgcn_readfirstlane(sh.seg_pref[g]);
-                        if (pg > gb + 63) break;
-                        if (p >= pg) g0 = sh.seg_start[g] - pg + p;
-                    }
-                }
-                gi[u] = p < raw_total ? g0 : 0;
+                const int p = gb + lane;
+                valid[u] = p < g_lim;
+                gi[u] = valid[u] ? g_off + p : 0;
             }
             // coordinates, then rank-table reads, each batch in flight together
             int dx[kGatherUnroll], dy[kGatherUnroll], dz[kGatherUnroll];
@@ -431,8 +454,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES8 ? 8 :
             for (int u = 0; u < kGatherUnroll; ++u) {
                 // offsets within [-span, span] per axis index the rank table
                 const int ox = dx[u] + s0, oy = dy[u] + s1, oz = dz[u] + s2;
-                const bool in = base + u * 64 + lane < raw_total && (unsigned)ox < (unsigned)rd0 &&
-                                (unsigned)oy < (unsigned)rd1 && (unsigned)oz < (unsigned)rd2;
+                const bool in = valid[u] && (unsigned)ox < (unsigned)rd0 && (unsigned)oy < (unsigned)rd1 &&
+                                (unsigned)oz < (unsigned)rd2;
                 const int full = (oz * rd1 + oy) * rd0 + ox;
                 const uint32_t r = a.rank[in ? full : 0];
                 rk[u] = in ? r : 0u;
@@ -483,6 +506,36 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES8 ? 8 :
 }
 
 }  // namespace
+
+// zoff[b (rz + 1) + d] = cells of bin b with z below the bin's first layer
+// + d, d = 0 .. rz (a bin's cells ascend by cell index, hence by z).
+static __global__ void k_bin_zoff(const uint32_t *bin_start, const uint32_t *bin_count, const int *bin_cells, int XY,
+                           BinArgs b, long long n, uint32_t *zoff) {
+    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int rz = b.r[2];
+    const long long bin = i / (rz + 1);
+    const int d = (int)(i - bin * (rz + 1));
+    const int zlim = (int)(bin / ((long long)b.m[0] * b.m[1])) * rz + d;
+    const uint32_t s = bin_start[bin];
+    uint32_t lo = 0, hi = bin_count[bin];
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (bin_cells[s + mid] / XY < zlim)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    zoff[i] = lo;
+}
+
+void launch_bin_zoff(const uint32_t *bin_start, const uint32_t *bin_count, const int *bin_cells, const BinArgs &b,
+                     long long nbins, uint32_t *zoff, hipStream_t s) {
+    const long long n = nbins * (b.r[2] + 1);
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_bin_zoff, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, bin_start, bin_count, bin_cells,
+                       b.X * b.Y, b, n, zoff);
+}
 
 int cluster_capacity(long long expected_max) {
     const long long c = (expected_max + 63) / 64 * 64;

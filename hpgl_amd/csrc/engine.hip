@@ -418,6 +418,11 @@ void search(CovAssets &as, const int grid[3], const int radii[3], int max_n, con
         ca.bin_start = bin_start.p;
         ca.bin_count = bin_count.p;
         ca.bin_cells = cells_out.p;
+        const long long n_zoff = ss.bins.nbins * (ss.bins.r[2] + 1ll);
+        if (n_zoff >= (1ll << 31)) throw std::runtime_error("cluster search: too many bins");
+        DBuf<uint32_t> zoff(n_zoff);
+        launch_bin_zoff(bin_start.p, bin_count.p, cells_out.p, ba, ss.bins.nbins, zoff.p, s);
+        ca.zoff = zoff.p;
         DBuf<uint32_t> xyz;
         ca.bin_xyz = nullptr;
         if (grid[0] <= 2048 && grid[1] <= 2048 && grid[2] <= 1024) {
@@ -439,8 +444,9 @@ void search(CovAssets &as, const int grid[3], const int radii[3], int max_n, con
         // candidates per target are at most 27 bins x limit cells; an element
         // packs (rank << lb) | raw index into 32 bits
         const long long max_cand = 27ll * std::max(ss.bins.limit, 0);
+        const long long max_raw = cluster_raw_bound(std::max(ss.bins.limit, 0));
         int lb = 1;
-        while ((1ll << lb) <= max_cand) ++lb;
+        while ((1ll << lb) < max_raw) ++lb;
         uint32_t max_rank = 0;
         for (uint32_t r : rt.rank) max_rank = std::max(max_rank, r);
         if (lb >= 32 || (unsigned long long)max_rank >= (1ull << (32 - lb)))

@@ -128,6 +128,7 @@ struct ClusterArgs {
     const uint32_t *bin_count;
     const int *bin_cells;     // informed cells sorted by (bin, cell)
     const uint32_t *bin_xyz;  // their coordinates packed x | y << 11 | z << 22 (nullptr: decode the cell)
+    const uint32_t *zoff;     // per bin, cells below each z layer (launch_bin_zoff)
     const uint32_t *rank;     // rank table over [-span, span]^3
     int span[3];
     int rd[3];
@@ -137,7 +138,7 @@ struct ClusterArgs {
     int n_list;
     int *nbr;
     int *cnt;
-    int local_bits;           // element = (rank << local_bits) | raw candidate index
+    int local_bits;           // element = (rank << local_bits) | raw index (< cluster_raw_bound)
     int cap;                  // LDS list capacity, a value returned by cluster_capacity
     int per_wave;             // main pass grid: ceil(n_list / per_wave) wavefronts
     // lists longer than cap: the main pass appends their targets to
@@ -153,6 +154,13 @@ struct ClusterArgs {
     unsigned long long *cand_total;  // counter lines: word 0 candidates gathered, word 1 lists sorted in scratch
 };
 void launch_cluster_search(const ClusterArgs &a, hipStream_t s);
+// zoff[b (r[2] + 1) + d]: cells of bin b whose z lies below the bin's first
+// layer + d, for d = 0 .. r[2].
+void launch_bin_zoff(const uint32_t *bin_start, const uint32_t *bin_count, const int *bin_cells, const BinArgs &b,
+                     long long nbins, uint32_t *zoff, hipStream_t s);
+// Raw candidate indices of a target whose 27 bins hold at most limit cells
+// each lie below this bound (21 segments, each padded to a multiple of 64).
+inline long long cluster_raw_bound(int limit) { return 27ll * limit + 21ll * 64; }
 // Scratch words per wave for lists of up to n candidates (the elements,
 // rounded up to a multiple of 64 words).
 __host__ __device__ inline long long cluster_scratch_words(int n) { return ((long long)n + 63) / 64 * 64; }

@@ -154,6 +154,7 @@ void run_job(Job &job) {
 }
 
 struct Server {
+    double t_created = now_ms();
     int device = 0;
     long long budget = 0;
     long long in_use = 0;
@@ -267,6 +268,8 @@ API int hpgl_amd_server_wait(void *server, long long id, hpgl_amd_job_result_t *
         result->run_ms = j->t_end - j->t_start;
         result->nodes = j->nodes;
         result->hbm_bytes = j->bytes;
+        result->start_ms = j->t_start - s->t_created;
+        result->end_ms = j->t_end - s->t_created;
         std::strncpy(result->message, j->message.c_str(), sizeof(result->message) - 1);
     }
     s->jobs.erase(it);

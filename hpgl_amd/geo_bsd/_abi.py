@@ -147,6 +147,8 @@ class _HPGL_AMD_JOB_RESULT(C.Structure):
         ("run_ms", C.c_double),
         ("nodes", C.c_longlong),
         ("hbm_bytes", C.c_longlong),
+        ("start_ms", C.c_double),
+        ("end_ms", C.c_double),
         ("message", C.c_char * 256),
     ]
 

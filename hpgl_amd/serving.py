@@ -50,6 +50,7 @@ class Job:
                 raise HpglError("unknown job %d" % self.id)
             self.result = {"status": r.status, "queue_ms": r.queue_ms, "latency_ms": r.latency_ms,
                            "run_ms": r.run_ms, "nodes": r.nodes, "hbm_bytes": r.hbm_bytes,
+                           "start_ms": r.start_ms, "end_ms": r.end_ms,
                            "message": r.message.decode(errors="replace")}
             self._keep = None
         return self.result

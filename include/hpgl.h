@@ -365,6 +365,8 @@ struct hpgl_amd_job_result_t {
     double run_ms;                  /* start to completion */
     long long nodes;                /* cells estimated (x categories for IK) */
     long long hbm_bytes;            /* footprint the job was admitted with */
+    double start_ms;                /* start and completion on the server's clock */
+    double end_ms;                  /* (milliseconds since the server was created) */
     char message[256];
 };
 

@@ -8,7 +8,6 @@ same kernels on the same inputs; only the interleaving differs), which is
 inside the stated 1e-5 relative bound for estimates.
 """
 import ctypes as C
-import time
 
 import numpy as np
 import pytest
@@ -78,17 +77,16 @@ def test_results_alone_equal_results_in_the_mix():
             assert r["status"] == 0, (kind, r["message"])
             alone.append((j.out.copy(), j.out_mask.copy()))
     with Server(slots=4) as s:
-        t0 = time.perf_counter()
         handles = [_submit(s, kind, shape, a) for kind, shape, a in jobs]
         results = [h.wait() for h in handles]
-        makespan_ms = (time.perf_counter() - t0) * 1000
     for (kind, _, _), h, r, (o, om) in zip(jobs, handles, results, alone):
         assert r["status"] == 0, (kind, r["message"])
         assert np.array_equal(h.out_mask, om), kind
         assert np.array_equal(h.out, o), kind
-    # the jobs overlapped on the device: their run times add up to more
-    # than the wall time of the mix
-    assert sum(r["run_ms"] for r in results) > 1.2 * makespan_ms
+    # jobs ran at the same time: sorted by start, some job starts before the
+    # previous one has ended
+    spans = sorted((r["start_ms"], r["end_ms"]) for r in results)
+    assert any(b0 < a1 for (_, a1), (b0, _) in zip(spans, spans[1:]))
 
 
 def test_admission_by_footprint_serialises_jobs_over_budget():
@@ -101,6 +99,6 @@ def test_admission_by_footprint_serialises_jobs_over_budget():
         handles = [_submit(s, kind, shape, a) for kind, shape, a in jobs]
         results = [h.wait() for h in handles]
     assert all(r["status"] == 0 for r in results)
-    spans = sorted((r["queue_ms"], r["queue_ms"] + r["run_ms"]) for r in results)
+    spans = sorted((r["start_ms"], r["end_ms"]) for r in results)
     for (a0, a1), (b0, b1) in zip(spans, spans[1:]):
-        assert b0 >= a1 - 1.0  # no two jobs ran at the same time (1 ms clock slack)
+        assert b0 >= a1  # no two jobs ran at the same time
