@@ -56,6 +56,7 @@ constexpr int kMinCap = 256, kMaxCap = 4096;
 // dropped have offsets outside the rank table (rank 0).
 constexpr int kSegs = 21, kMaxSegs = 32;
 constexpr size_t kLdsPerCu = 160 * 1024;
+constexpr int kWavesPerGroup = 2;
 
 // A wavefront orders its own accesses (LDS and the global scratch lists)
 // with a workgroup-scope fence, which waits for them to complete; the
@@ -346,13 +347,21 @@ __device__ void finish_(const E *e, int n, int K, int *out, const S &sh, const i
 // WAVES8: the register budget of eight wavefronts per SIMD (a few SGPRs
 // move to VGPR lanes), used when the LDS capacity allows 32 wavefronts per
 // CU; otherwise the budget of seven, which LDS already limits to.
-template <bool PACKED, bool SPILLED, bool WAVES8>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES8 ? 8 : 1, 8))) void k_cluster(ClusterArgs a) {
-    __shared__ WaveLds sh;
-    extern __shared__ uint32_t lds_e[];
+//
+// W wavefronts per workgroup take consecutive list entries (neighbouring
+// targets along x): their rank-table reads, offsets shifted by one, share
+// the CU's L1 lines.
+template <bool PACKED, bool SPILLED, bool WAVES8, int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WAVES8 ? 8 : 1, 8))) void k_cluster(
+    ClusterArgs a) {
+    __shared__ WaveLds sh_all[W];
+    extern __shared__ uint32_t lds_all[];
     // the spill pass gathers straight into its global scratch
     const int cap = SPILLED ? 0 : a.cap;
-    const int wave_id = blockIdx.x;
+    const int w = W == 1 ? 0 : (int)(threadIdx.x >> 6);
+    WaveLds &sh = sh_all[w];
+    uint32_t *lds_e = lds_all + (size_t)w * cap;
+    const int wave_id = blockIdx.x * W + w;
     const int lane = threadIdx.x & 63;
     const uint64_t lt = lanemask_lt();
     const int X = a.X, Y = a.Y;
@@ -362,7 +371,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES8 ? 8 :
     uint32_t *ge = SPILLED ? a.scratch + (size_t)wave_id * cluster_scratch_words(a.scratch_per_wave) : nullptr;
     unsigned long long cand_acc = 0, spill_acc = 0;
     const int li_end = SPILLED ? *a.spill_count : a.n_list;
-    for (int li = wave_id; li < li_end; li += gridDim.x) {
+    for (int li = wave_id; li < li_end; li += gridDim.x * W) {
         const int t = SPILLED ? a.spill_list[li] : a.list ? a.list[li] : li;
         const int node = a.targets[t];
         const int x = node % X, y = node % (X * Y) / X, z = node / (X * Y);
@@ -544,13 +553,15 @@ int cluster_capacity(long long expected_max) {
 
 template <bool PACKED>
 static void launch_cap(const ClusterArgs &a, hipStream_t s) {
-    const int grid = (int)(((long long)a.n_list + a.per_wave - 1) / a.per_wave);
+    constexpr int W = kWavesPerGroup;
+    const int grid = (int)(((long long)a.n_list + (long long)a.per_wave * W - 1) / ((long long)a.per_wave * W));
     const size_t lds = (size_t)a.cap * sizeof(uint32_t);
     if (32 * (lds + sizeof(WaveLds)) <= kLdsPerCu)
-        hipLaunchKernelGGL((k_cluster<PACKED, false, true>), dim3(grid), dim3(64), lds, s, a);
+        hipLaunchKernelGGL((k_cluster<PACKED, false, true, W>), dim3(grid), dim3(64 * W), W * lds, s, a);
     else
-        hipLaunchKernelGGL((k_cluster<PACKED, false, false>), dim3(grid), dim3(64), lds, s, a);
-    if (a.spill_list) hipLaunchKernelGGL((k_cluster<PACKED, true, false>), dim3(a.n_spill_waves), dim3(64), 0, s, a);
+        hipLaunchKernelGGL((k_cluster<PACKED, false, false, W>), dim3(grid), dim3(64 * W), W * lds, s, a);
+    if (a.spill_list)
+        hipLaunchKernelGGL((k_cluster<PACKED, true, false, 1>), dim3(a.n_spill_waves), dim3(64), 0, s, a);
 }
 
 void launch_cluster_search(const ClusterArgs &a, hipStream_t s) {

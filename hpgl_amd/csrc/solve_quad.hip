@@ -77,6 +77,19 @@ __device__ inline double qmax(double v) {
     v = fmax(v, qswap2(v));
     return v;
 }
+// Reduce-scatter within the quad: lane q receives the sum over the quad's
+// four lanes of their t[q], in the order (t_q + t_{q^1}) + (t_{q^2} +
+// t_{q^3}) (two DPP exchange steps).
+__device__ inline double quad_reduce_scatter(double t0, double t1, double t2, double t3) {
+    const int q = threadIdx.x & 3;
+    const bool odd = q & 1, high = q & 2;
+    // step 1, with lane q^1: keep the columns of my parity, q&1 and q&1 + 2
+    const double ka = (odd ? t1 : t0) + qswap1(odd ? t0 : t1);
+    const double kb = (odd ? t3 : t2) + qswap1(odd ? t2 : t3);
+    // step 2, with lane q^2: keep column q
+    return (high ? kb : ka) + qswap2(high ? ka : kb);
+}
+
 __device__ inline int qmaxi(int v) {
     v = max(v, __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false));
     v = max(v, __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false));
@@ -152,8 +165,8 @@ __device__ inline void q_setup(const SolveArgs &a, QShared<N> &sh, QCtx<N> &c) {
 // coordinates 0 in LDS, so the table read is always in bounds and the
 // result is selected afterwards.
 template <int N>
-__device__ inline double q_aij(const SolveArgs &a, const QShared<N> &sh, int slot, int n, int i, int j) {
-    const double v = cov_at(a.cov, sh.xyz[slot][0][j] - sh.xyz[slot][0][i], sh.xyz[slot][1][j] - sh.xyz[slot][1][i],
+__device__ inline double q_aij(const DevCov &cv, const QShared<N> &sh, int slot, int n, int i, int j) {
+    const double v = cov_at(cv, sh.xyz[slot][0][j] - sh.xyz[slot][0][i], sh.xyz[slot][1][j] - sh.xyz[slot][1][i],
                             sh.xyz[slot][2][j] - sh.xyz[slot][2][i]);
     return (i < n && j < n) ? v : (i == j ? 1.0 : 0.0);
 }
@@ -301,7 +314,7 @@ __global__ __launch_bounds__(64) void k_solve_quad_mixed(SolveArgs a) {
         b[m] = q_bi<N>(a, sh, c, i);
 #pragma unroll
         for (int j = 0; j < 4 * m + 4; ++j) {
-            const double e = q_aij<N>(a, sh, c.slot, c.n, i, j);
+            const double e = q_aij<N>(a.cov, sh, c.slot, c.n, i, j);
             const double aij = (j <= i) ? e : 0.0;
             L[off(m) + j] = (float)aij;
             if (j == i && i < c.n) maxdiag = fmax(maxdiag, aij);
@@ -355,6 +368,10 @@ __global__ __launch_bounds__(64) void k_solve_quad_mixed(SolveArgs a) {
     // has a first correction below 1e-9 of its solution.
     bool conv = false;
     for (int step = 0; step < 2; ++step) {
+        // the table reads of a step are not kept for the next one (that
+        // would hold A in registers): the pointer is opaque per step
+        DevCov cv = a.cov;
+        asm volatile("" : "+s"(cv.table));
         // r = b - A x in FP64 for my rows; x goes through LDS
 #pragma unroll
         for (int m = 0; m < M; ++m) {
@@ -368,15 +385,41 @@ __global__ __launch_bounds__(64) void k_solve_quad_mixed(SolveArgs a) {
             r1[m] = b[m];
             r2[m] = (TWO && 4 * m + q < c.n) ? 1.0 : 0.0;
         }
-#pragma unroll 2
-        for (int j = 0; j < N; ++j) {
-            const double xj1 = sh.x1[c.slot][j];
-            const double xj2 = TWO ? sh.x2[c.slot][j] : 0.0;
+        // A is symmetric: lane q reads the 4x4 blocks (m, mb), mb <= m, of
+        // its rows i = 4m+q only; an off-diagonal block also gives the rows
+        // 4mb+cc their terms A(4mb+cc, i) x_i, summed over the quad.  The
+        // column-block loop stays rolled (an unrolled one keeps too many
+        // values live).
+#pragma unroll 1
+        for (int mb = 0; mb < M; ++mb) {
+            double xc1[4], xc2[4];
+#pragma unroll
+            for (int cc = 0; cc < 4; ++cc) {
+                xc1[cc] = sh.x1[c.slot][4 * mb + cc];
+                xc2[cc] = TWO ? sh.x2[c.slot][4 * mb + cc] : 0.0;
+            }
+            double s1 = 0.0, s2 = 0.0;  // terms of row 4mb+q from the blocks below
 #pragma unroll
             for (int m = 0; m < M; ++m) {
-                const double aij = q_aij<N>(a, sh, c.slot, c.n, 4 * m + q, j);
-                r1[m] = fma(-aij, xj1, r1[m]);
-                if (TWO) r2[m] = fma(-aij, xj2, r2[m]);
+                if (m < mb) continue;
+                const int i = 4 * m + q;
+                double e[4];
+#pragma unroll
+                for (int cc = 0; cc < 4; ++cc) e[cc] = q_aij<N>(cv, sh, c.slot, c.n, i, 4 * mb + cc);
+#pragma unroll
+                for (int cc = 0; cc < 4; ++cc) {
+                    r1[m] = fma(-e[cc], xc1[cc], r1[m]);
+                    if (TWO) r2[m] = fma(-e[cc], xc2[cc], r2[m]);
+                }
+                if (m > mb) {
+                    s1 += quad_reduce_scatter(e[0] * x1[m], e[1] * x1[m], e[2] * x1[m], e[3] * x1[m]);
+                    if (TWO) s2 += quad_reduce_scatter(e[0] * x2[m], e[1] * x2[m], e[2] * x2[m], e[3] * x2[m]);
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                r1[m] -= m == mb ? s1 : 0.0;
+                if (TWO) r2[m] -= m == mb ? s2 : 0.0;
             }
         }
         __syncthreads();
@@ -466,7 +509,7 @@ __global__ __launch_bounds__(64) void k_solve_quad_fp64(SolveArgs a) {
         b[m] = q_bi<N>(a, sh, c, i);
 #pragma unroll
         for (int j = 0; j < 4 * m + 4; ++j) {
-            const double e = q_aij<N>(a, sh, c.slot, c.n, i, j);
+            const double e = q_aij<N>(a.cov, sh, c.slot, c.n, i, j);
             L[off(m) + j] = (j <= i) ? e : 0.0;
         }
     }
