@@ -151,6 +151,12 @@ __device__ inline int lane_select(uint64_t m, int t, int f) {
     return r;
 }
 
+// base[idx] for idx < 2^30: a 32-bit byte offset, so the load takes the
+// scalar-base + vector-offset form (no 64-bit address arithmetic).
+__device__ inline uint32_t load_u32(const uint32_t *base, uint32_t idx) {
+    return *reinterpret_cast<const uint32_t *>(reinterpret_cast<const char *>(base) + (idx << 2));
+}
+
 // Number of set bits of the wave-uniform mask m below this lane.
 __device__ inline int bits_below(uint64_t m) {
     return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
@@ -363,7 +369,6 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WAVES8 ?
     uint32_t *lds_e = lds_all + (size_t)w * cap;
     const int wave_id = blockIdx.x * W + w;
     const int lane = threadIdx.x & 63;
-    const uint64_t lt = lanemask_lt();
     const int X = a.X, Y = a.Y;
     const BinArgs &b = a.bins;
     const Keyed key{a.local_bits};
@@ -416,6 +421,9 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WAVES8 ?
         }
         const int raw0 = incl - padded;
         const int raw_total = __builtin_amdgcn_readlane(incl, kSegs - 1);
+        // segment ends for the cursor; lane kSegs is a sentinel that no group
+        // passes (its cells end at raw_total)
+        const int seg_end = lane < kSegs ? incl : 0x7fffffff;
         if (lane < kMaxSegs) {
             sh.seg_raw[lane] = lane < kSegs ? raw0 : 0x7fffffff;
             sh.seg_off[lane] = start - raw0;
@@ -425,6 +433,8 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WAVES8 ?
         // issues the loads of kGatherUnroll groups of 64 before compacting;
         // a scalar cursor follows the segment of each group.
         int seg = -1, g_end = 0, g_off = 0, g_lim = 0;
+        // offsets from the target as rank-table indices: o = w - (t - span)
+        const int xs = x - s0, ys = y - s1, zs = z - s2;
         int pos = 0;
         for (int base = 0; base < raw_total; base += 64 * kGatherUnroll) {
             int gi[kGatherUnroll];
@@ -432,9 +442,9 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WAVES8 ?
 #pragma unroll
             for (int u = 0; u < kGatherUnroll; ++u) {
                 const int gb = base + u * 64;
-                while (gb >= g_end && seg < kSegs - 1) {
+                while (gb >= g_end) {
                     ++seg;
-                    g_end = __builtin_amdgcn_readlane(incl, seg);
+                    g_end = __builtin_amdgcn_readlane(seg_end, seg);
                     g_off = __builtin_amdgcn_readlane(start - raw0, seg);
                     g_lim = __builtin_amdgcn_readlane(seg_lim, seg);
                 }
@@ -442,37 +452,38 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WAVES8 ?
                 valid[u] = p < g_lim;
                 gi[u] = valid[u] ? g_off + p : 0;
             }
-            // coordinates, then rank-table reads, each batch in flight together
-            int dx[kGatherUnroll], dy[kGatherUnroll], dz[kGatherUnroll];
+            // coordinates as rank-table offsets, then rank-table reads, each
+            // batch in flight together
+            int tx[kGatherUnroll], ty[kGatherUnroll], tz[kGatherUnroll];
 #pragma unroll
             for (int u = 0; u < kGatherUnroll; ++u) {
                 if (PACKED) {
-                    const uint32_t w = a.bin_xyz[gi[u]];
-                    dx[u] = (int)(w & 2047u) - x;
-                    dy[u] = (int)((w >> 11) & 2047u) - y;
-                    dz[u] = (int)(w >> 22) - z;
+                    const uint32_t w = load_u32(a.bin_xyz, (uint32_t)gi[u]);
+                    tx[u] = (int)(w & 2047u) - xs;
+                    ty[u] = (int)((w >> 11) & 2047u) - ys;
+                    tz[u] = (int)(w >> 22) - zs;
                 } else {
                     const int c = a.bin_cells[gi[u]];
-                    dx[u] = c % X - x;
-                    dy[u] = c % (X * Y) / X - y;
-                    dz[u] = c / (X * Y) - z;
+                    tx[u] = c % X - xs;
+                    ty[u] = c % (X * Y) / X - ys;
+                    tz[u] = c / (X * Y) - zs;
                 }
             }
             uint32_t rk[kGatherUnroll];
 #pragma unroll
             for (int u = 0; u < kGatherUnroll; ++u) {
                 // offsets within [-span, span] per axis index the rank table
-                const int ox = dx[u] + s0, oy = dy[u] + s1, oz = dz[u] + s2;
-                const bool in = valid[u] && (unsigned)ox < (unsigned)rd0 && (unsigned)oy < (unsigned)rd1 &&
-                                (unsigned)oz < (unsigned)rd2;
-                const int full = (oz * rd1 + oy) * rd0 + ox;
-                const uint32_t r = a.rank[in ? full : 0];
+                // ((2 span[1] + 1)(2 span[2] + 1) < 2^24, checked on the host)
+                const unsigned ox = (unsigned)tx[u], oy = (unsigned)ty[u], oz = (unsigned)tz[u];
+                const bool in = valid[u] && ox < (unsigned)rd0 && oy < (unsigned)rd1 && oz < (unsigned)rd2;
+                const unsigned full = __umul24(__umul24(oz, (unsigned)rd1) + oy, (unsigned)rd0) + ox;
+                const uint32_t r = load_u32(a.rank, in ? full : 0u);
                 rk[u] = in ? r : 0u;
             }
 #pragma unroll
             for (int u = 0; u < kGatherUnroll; ++u) {
                 const uint64_t mk = ballot(rk[u] != 0);
-                const int pp = pos + __popcll(mk & lt);
+                const int pp = pos + bits_below(mk);
                 if (rk[u] != 0) {
                     const uint32_t ev = (rk[u] << key.lb) | (uint32_t)(base + u * 64 + lane);
                     if (pp < cap)

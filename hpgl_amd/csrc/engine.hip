@@ -316,7 +316,10 @@ const CovAssets &assets_rank(CovAssets &a, hipStream_t s, Stats &st) {
     if (a.have_rank) return a;
     int span[3];
     for (int i = 0; i < 3; ++i) span[i] = std::max(a.use_table ? a.radii[i] : 2 * a.radii[i] - 1, 0);
-    if ((2ll * span[0] + 1) * (2ll * span[1] + 1) * (2ll * span[2] + 1) >= (1ll << 31))
+    // the search kernel indexes the table with 32-bit byte offsets and
+    // 24-bit multiplies: (2 span[1] + 1)(2 span[2] + 1) < 2^24
+    if ((2ll * span[0] + 1) * (2ll * span[1] + 1) * (2ll * span[2] + 1) >= (1ll << 30) ||
+        (2ll * span[1] + 1) * (2ll * span[2] + 1) >= (1ll << 24))
         throw std::runtime_error("search radiuses too large for the cell-binned search");
     const double t0 = now_ms();
     a.rank.build(a.model, a.use_table ? &a.table : nullptr, span);
@@ -425,7 +428,7 @@ void search(CovAssets &as, const int grid[3], const int radii[3], int max_n, con
         ca.zoff = zoff.p;
         DBuf<uint32_t> xyz;
         ca.bin_xyz = nullptr;
-        if (grid[0] <= 2048 && grid[1] <= 2048 && grid[2] <= 1024) {
+        if (grid[0] <= 2048 && grid[1] <= 2048 && grid[2] <= 1024 && I < (1 << 30)) {
             xyz.alloc(std::max(I, 1));
             launch_pack_xyz(cells_out.p, I, grid[0], grid[1], xyz.p, s);
             ca.bin_xyz = xyz.p;
