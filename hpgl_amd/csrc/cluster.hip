@@ -176,7 +176,7 @@ __device__ inline int bits_below(uint64_t m) {
 // is exact for every position below R_{s-1}, and the stops it reports at
 // or beyond R_{s-1} change neither the swaps nor min(L_s, R_{s-1}).
 template <typename S>
-__device__ int partition_(uint32_t *e, S &sh, int lo, int hi, Keyed key) {
+__device__ __forceinline__ int partition_(uint32_t *e, S &sh, int lo, int hi, Keyed key) {
     const int lane = threadIdx.x & 63;
     uint32_t kp;
     {  // __move_median_to_first(lo, lo+1, mid, hi-1)
@@ -268,11 +268,104 @@ __device__ int partition_(uint32_t *e, S &sh, int lo, int hi, Keyed key) {
     }
 }
 
+// Lane i receives v of lane src_i.
+__device__ inline int lane_fetch(int v, int src) { return __builtin_amdgcn_ds_bpermute(src << 2, v); }
+// Lane dst_i receives v of lane i (dst a permutation of the lanes).
+__device__ inline int lane_send(int v, int dst) { return __builtin_amdgcn_ds_permute(dst << 2, v); }
+
+// The introsort steps on e[lo, hi), hi - lo <= 64, and on every range they
+// split off that starts below K, with lane i holding e[lo + i] in a
+// register.  Each __unguarded_partition is two ballots for the stops, one
+// compaction of the left stops and one of the right stops into lane order
+// (lane j then holds L_j and R_j), and one fetch of every swapped lane's
+// partner.  d is the depth budget of [lo, hi); a range that exhausts it is
+// heap-sorted in memory.  Returns false on a state libstdc++'s algorithm
+// cannot reach.
+template <typename S>
+__device__ __forceinline__ bool small_sort_(uint32_t *e, S &sh, int lo, int hi, int d, int K, Keyed key) {
+    const int lane = threadIdx.x & 63;
+    const int n = hi - lo;
+    wave_sync();
+    uint32_t v = lane < n ? e[lo + lane] : 0u;
+    // pending ranges relative to lo: lane i holds stack entry i
+    int s_l = 0, s_h = n, s_d = d;
+    int top = 1;
+    while (top > 0) {
+        --top;
+        int l = __builtin_amdgcn_readlane(s_l, top), h = __builtin_amdgcn_readlane(s_h, top),
+            dd = __builtin_amdgcn_readlane(s_d, top);
+        while (h - l > 16) {
+            if (dd == 0) {
+                if (lane < n) e[lo + lane] = v;
+                wave_sync();
+                if (lane == 0) heap_sort_(e + lo + l, h - l, key);
+                wave_sync();
+                v = lane < n ? e[lo + lane] : 0u;
+                break;
+            }
+            --dd;
+            // __move_median_to_first(l, l+1, mid, h-1)
+            const uint32_t kv = key(v);
+            const int a = l + 1, bm = l + (h - l) / 2, c = h - 1;
+            const uint32_t ka = __builtin_amdgcn_readlane(kv, a), kb = __builtin_amdgcn_readlane(kv, bm),
+                           kc = __builtin_amdgcn_readlane(kv, c);
+            int sw;
+            if (ka > kb) {
+                if (kb > kc)
+                    sw = bm;
+                else if (ka > kc)
+                    sw = c;
+                else
+                    sw = a;
+            } else if (ka > kc) {
+                sw = a;
+            } else if (kb > kc) {
+                sw = c;
+            } else {
+                sw = bm;
+            }
+            const uint32_t kp = sw == a ? ka : sw == bm ? kb : kc;
+            const uint32_t vl = __builtin_amdgcn_readlane(v, l), vs = __builtin_amdgcn_readlane(v, sw);
+            v = lane == l ? vs : lane == sw ? vl : v;
+            const uint32_t k = key(v);
+            // left stops key <= kp, right stops key >= kp, in [l + 1, h)
+            const uint64_t range = (h >= 64 ? ~0ull : (1ull << h) - 1) & ~((1ull << (l + 1)) - 1);
+            const uint64_t lm = ballot(k <= kp) & range, rm = ballot(k >= kp) & range;
+            const int nl = __popcll(lm), nr = __popcll(rm);
+            const int bl = bits_below(lm), br = bits_below(rm);
+            const bool in_l = (lm >> lane) & 1u, in_r = (rm >> lane) & 1u;
+            const int il = bl, jr = nr - 1 - br;  // L index; R index from the top
+            const int l_at = lane_send(lane, in_l ? il : nl + lane - bl);
+            const int r_at = lane_send(lane, in_r ? jr : nr + lane - br);
+            const int sc = __popcll(ballot(lane < min(nl, nr) && l_at < r_at));
+            const int to_r = lane_fetch(r_at, il), to_l = lane_fetch(l_at, jr & 63);
+            const int src = (in_l && il < sc) ? to_r : (in_r && jr < sc) ? to_l : lane;
+            v = (uint32_t)lane_fetch((int)v, src);
+            const int cl = sc < nl ? __builtin_amdgcn_readlane(l_at, sc) : 0x7fffffff;
+            const int cr = sc >= 1 ? __builtin_amdgcn_readlane(r_at, sc - 1) : 0x7fffffff;
+            const int cut = min(cl, cr);
+            if (cut <= l || cut >= h || top >= 64) return false;
+            if (lo + cut < K) {
+                if (lane == top) {
+                    s_l = cut;
+                    s_h = h;
+                    s_d = dd;
+                }
+                ++top;
+            }
+            h = cut;
+        }
+    }
+    if (lane < n) e[lo + lane] = v;
+    wave_sync();
+    return true;
+}
+
 // Returns false if the replay reached a state libstdc++'s algorithm cannot
 // reach (a cut outside the range or a deeper pending stack than 64
 // ranges); the caller then reports an internal error instead of looping.
 template <typename S>
-__device__ bool topk_sort_(uint32_t *e, S &sh, int n, int K, Keyed key) {
+__device__ __forceinline__ bool topk_sort_(uint32_t *e, S &sh, int n, int K, Keyed key) {
     const int lane = threadIdx.x & 63;
     if (n <= 1) return true;
     const int lg = 31 - __clz(n);
@@ -284,6 +377,10 @@ __device__ bool topk_sort_(uint32_t *e, S &sh, int n, int K, Keyed key) {
         int lo = __builtin_amdgcn_readlane(s_lo, top), hi = __builtin_amdgcn_readlane(s_hi, top),
             d = __builtin_amdgcn_readlane(s_d, top);
         while (hi - lo > 16) {
+            if (hi - lo <= 64) {  // this range and all it splits off, in registers
+                if (!small_sort_(e, sh, lo, hi, d, K, key)) return false;
+                break;
+            }
             if (d == 0) {
                 wave_sync();
                 if (lane == 0) heap_sort_(e + lo, hi - lo, key);
