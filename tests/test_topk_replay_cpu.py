@@ -1,8 +1,10 @@
 """The cell-binned search kernel reproduces libstdc++ std::sort's first K
 elements through a parallel restatement of the introsort partition step
-(hpgl_amd/csrc/cluster.hip).  tests/native/topk_replay.cpp replays that
-restatement sequentially and checks it against std::sort on random inputs
-with heavy ties and on inputs that exhaust the depth budget (heap path)."""
+(hpgl_amd/csrc/cluster.hip): the two-cursor partition for ranges longer
+than 64 and the register formulation for shorter ones.
+tests/native/topk_replay.cpp replays both sequentially and checks them
+against std::sort on random inputs with heavy ties, on spatially structured
+keys and on inputs that exhaust the depth budget (heap path)."""
 import os
 import subprocess
 import tempfile
@@ -18,5 +20,7 @@ def test_partition_restatement_matches_std_sort():
         r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.startswith("ok"), r.stdout
-    n_heap = int(r.stdout.split(",")[1].split()[0])
+    fields = r.stdout.split(",")
+    n_heap, n_stream, n_small = (int(f.split()[0]) for f in fields[1:4])
     assert n_heap > 0  # the depth-limit path was exercised
+    assert n_stream > 0 and n_small > 0  # both partition formulations ran
