@@ -144,6 +144,31 @@ def _serving_params(abi, job):
     return ikp
 
 
+def property_io_bench(g, data, mask):
+    """Writes the SK result (every cell) as an INC file and reads it back
+    through the library's host code; values/s and MB/s of each."""
+    import tempfile
+
+    shape = data.shape
+    d = np.asfortranarray(data)
+    m = np.asfortranarray(mask)
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "sk.inc")
+        t0 = time.perf_counter()
+        g.write_property((d, m), path, "SK_ESTIMATE", -99)
+        tw = time.perf_counter() - t0
+        size = os.path.getsize(path)
+        t0 = time.perf_counter()
+        back = g.load_cont_property(path, -99, size=shape)
+        tr = time.perf_counter() - t0
+        same_mask = bool(np.array_equal(back.mask, m))
+    n = d.size
+    return {"format": "INC, one %E value per line", "values": int(n), "file_mb": round(size / 1e6, 1),
+            "write_values_per_s": round(n / tw), "write_mb_per_s": round(size / tw / 1e6, 1),
+            "read_values_per_s": round(n / tr), "read_mb_per_s": round(size / tr / 1e6, 1),
+            "round_trip_mask_equal": same_mask, "threads": 1}
+
+
 def serving_bench(slots=4):
     """Replays the request stream against the library's request loop:
     jobs are submitted at their arrival times and overlap on the GPU."""
@@ -238,6 +263,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-compare", action="store_true")
     ap.add_argument("--no-serving", action="store_true", help="skip the request-stream (p50/p99) measurement")
+    ap.add_argument("--no-io", action="store_true", help="skip the property-file write/read measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (x * 1 -> y, 3 times), the reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
@@ -408,6 +434,12 @@ def main():
                                "sk_rms_rel_err_vs_fp64": float(np.sqrt(np.mean(rel ** 2))),
                                "ik_category_agreement": float(np.mean(mixed_ik == d_iout.cpu().numpy()))}
 
+    if not args.no_io:
+        import hpgl_amd.geo_bsd as geo_api
+
+        shape = (g.x, g.y, g.z)
+        out["property_io"] = property_io_bench(geo_api, d_out.cpu().numpy().reshape(shape, order="F"),
+                                               d_out_mask.cpu().numpy().reshape(shape, order="F"))
     if not args.no_serving:
         trace, out["serving"] = serving_bench()
         if not args.no_cpu_baseline:

@@ -15,6 +15,7 @@
 #include "../../include/hpgl.h"
 #include "engine.h"
 #include "kernels.h"
+#include "propio.h"
 
 using namespace hpgl_amd;
 
@@ -183,22 +184,73 @@ API void hpgl_set_progress_handler(int (*handler)(char *, int, void *), void *pa
 API void hpgl_set_thread_num(int n) { g_threads = n; }
 API int hpgl_get_thread_num(void) { return g_threads; }
 
-#define NOT_ON_PATH(name)                                                                  \
-    {                                                                                      \
-        begin_call();                                                                      \
-        fail(name ": property file I/O is not part of this library");                     \
+// Property files (read_inc_file.cpp, property_writer.cpp, api.cpp:54-193):
+// host code in propio.cpp.  Sizes are the shape volumes, as in the reference.
+namespace {
+long long volume(const hpgl_shape_t &s) { return (long long)s.m_data[0] * s.m_data[1] * s.m_data[2]; }
+template <typename F>
+void io_call(F f) {
+    begin_call();
+    try {
+        f();
+    } catch (const std::exception &e) {
+        fail(e.what());
     }
-API void hpgl_read_inc_file_float(char *, float, int, float *, unsigned char *) NOT_ON_PATH("hpgl_read_inc_file_float")
-API void hpgl_read_inc_file_byte(char *, int, int, unsigned char *, unsigned char *, unsigned char *, int)
-    NOT_ON_PATH("hpgl_read_inc_file_byte")
-API void hpgl_write_inc_file_float(char *, hpgl_cont_masked_array_t *, float, char *)
-    NOT_ON_PATH("hpgl_write_inc_file_float")
-API void hpgl_write_inc_file_byte(char *, hpgl_ind_masked_array_t *, int, char *, unsigned char *, int)
-    NOT_ON_PATH("hpgl_write_inc_file_byte")
-API void hpgl_write_gslib_cont_property(hpgl_cont_masked_array_t *, const char *, const char *, double)
-    NOT_ON_PATH("hpgl_write_gslib_cont_property")
-API void hpgl_write_gslib_byte_property(hpgl_ind_masked_array_t *, const char *, const char *, double,
-                                        unsigned char *, int) NOT_ON_PATH("hpgl_write_gslib_byte_property")
+}
+}  // namespace
+
+API void hpgl_read_inc_file_float(char *filename, float undefined_value, int size, float *data, unsigned char *mask) {
+    io_call([&] { read_inc_float(filename, undefined_value, size, data, mask); });
+}
+
+API void hpgl_read_inc_file_byte(char *filename, int undefined_value, int size, unsigned char *data,
+                                 unsigned char *mask, unsigned char *values, int values_count) {
+    io_call([&] {
+        read_inc_byte(filename, undefined_value, size, data, mask);
+        // informed bytes equal to values[j] become j (api.cpp:85-98)
+        for (int i = 0; i < size; ++i)
+            if (mask[i] != 0)
+                for (int j = 0; j < values_count; ++j)
+                    if (data[i] == values[j]) {
+                        data[i] = (unsigned char)j;
+                        break;
+                    }
+    });
+}
+
+API void hpgl_write_inc_file_float(char *filename, hpgl_cont_masked_array_t *arr, float undefined_value, char *name) {
+    io_call([&] {
+        if (!arr) throw std::runtime_error("hpgl_write_inc_file_float: null array");
+        write_inc_float(filename, name, arr->m_data, arr->m_mask, volume(arr->m_shape), (double)undefined_value);
+    });
+}
+
+API void hpgl_write_inc_file_byte(char *filename, hpgl_ind_masked_array_t *arr, int undefined_value, char *name,
+                                  unsigned char *values, int values_count) {
+    io_call([&] {
+        if (!arr) throw std::runtime_error("hpgl_write_inc_file_byte: null array");
+        write_inc_byte(filename, name, arr->m_data, arr->m_mask, volume(arr->m_shape), (uint8_t)undefined_value,
+                       remap_table(values, values_count, arr->m_indicator_count));
+    });
+}
+
+API void hpgl_write_gslib_cont_property(hpgl_cont_masked_array_t *data, const char *filename, const char *name,
+                                        double undefined_value) {
+    io_call([&] {
+        if (!data) throw std::runtime_error("hpgl_write_gslib_cont_property: null array");
+        write_gslib_float(filename, name, data->m_data, data->m_mask, volume(data->m_shape), undefined_value);
+    });
+}
+
+API void hpgl_write_gslib_byte_property(hpgl_ind_masked_array_t *data, const char *filename, const char *name,
+                                        double undefined_value, unsigned char *values, int values_count) {
+    io_call([&] {
+        if (!data) throw std::runtime_error("hpgl_write_gslib_byte_property: null array");
+        write_gslib_byte(filename, name, data->m_data, data->m_mask, volume(data->m_shape),
+                         (uint8_t)(int)undefined_value,
+                         remap_table(values, values_count, data->m_indicator_count));
+    });
+}
 
 API void hpgl_amd_set_precision(int mode) {
     g_precision = mode == HPGL_AMD_PRECISION_FP64 ? HPGL_AMD_PRECISION_FP64 : HPGL_AMD_PRECISION_MIXED;

@@ -10,10 +10,13 @@ changing what the functions do:
 * ``set_progress_handler(None)`` cleared the *output* handler
   (geo.py:628-632); here it clears the progress handler.
 
-Property file I/O (``load_cont_property``, ``write_property`` ...) is not
-part of the kriging path; those functions raise NotImplementedError.
+* file names and property names are passed to the library as bytes.
+
+Property files (``load_cont_property``, ``write_property`` ...) are read and
+written by the library's host code (INC and GSLIB formats, geo.py:168-362).
 """
 import ctypes as C
+import os
 
 import numpy
 
@@ -259,20 +262,127 @@ def accepts_tuple(arg_name, arg_pos):
     return decorator
 
 
-def _not_on_kriging_path(name):
-    def f(*args, **kargs):
-        raise NotImplementedError("%s: property file I/O is not part of this library" % name)
-
-    f.__name__ = name
-    return f
+def _bytes(s):
+    return s if isinstance(s, bytes) else os.fsencode(s)
 
 
-write_property = _not_on_kriging_path("write_property")
-write_gslib_property = _not_on_kriging_path("write_gslib_property")
-load_cont_property = _not_on_kriging_path("load_cont_property")
-load_ind_property = _not_on_kriging_path("load_ind_property")
-read_inc_file_float = _not_on_kriging_path("read_inc_file_float")
-read_inc_file_byte = _not_on_kriging_path("read_inc_file_byte")
+def _load_prop_cont_slow(filename, undefined_value):
+    """geo.py:168-186: every token that parses as a float, skipping lines
+    that start with "--"."""
+    values = []
+    mask = []
+    with open(filename) as f:
+        for line in f:
+            if line.strip().startswith("--"):
+                continue
+            for part in line.split():
+                try:
+                    val = float(part.strip())
+                except ValueError:
+                    continue
+                values.append(val)
+                mask.append(0 if val == undefined_value else 1)
+    return ContProperty(numpy.array(values, dtype="float32"), numpy.array(mask, dtype="uint8"))
+
+
+def _load_prop_ind_slow(filename, undefined_value, ind_values):
+    """geo.py:188-212: integer tokens mapped to their index in ind_values;
+    the undefined value gives 255 with mask 0."""
+    dict_map = {v: i for i, v in enumerate(ind_values)}
+    values = []
+    mask = []
+    with open(filename) as f:
+        for line in f:
+            if line.strip().startswith("--"):
+                continue
+            for part in line.split():
+                try:
+                    val = int(part.strip())
+                    if val == undefined_value:
+                        values.append(255)
+                        mask.append(0)
+                    else:
+                        values.append(dict_map[val])
+                        mask.append(1)
+                except (ValueError, KeyError):
+                    pass
+    return IndProperty(numpy.array(values, dtype="uint8", order="F"), numpy.array(mask, dtype="uint8", order="F"),
+                       len(ind_values))
+
+
+@accepts_tuple("prop", 0)
+def write_property(prop, filename, prop_name, undefined_value, indicator_values=[]):
+    """geo.py:277-305: INC file, one value per line, "/" at the end."""
+    if prop.data.ndim == 3:
+        sh = _create_hpgl_shape(prop.data.shape)
+    else:
+        sh = _create_hpgl_shape((prop.data.size, 1, 1))
+    if isinstance(prop, ContProperty):
+        marr = _HPGL_CONT_MASKED_ARRAY(data=prop.data.ctypes.data_as(C.POINTER(C.c_float)),
+                                       mask=prop.mask.ctypes.data_as(C.POINTER(C.c_ubyte)), shape=sh)
+        _hpgl_so.hpgl_write_inc_file_float(_bytes(filename), C.byref(marr), undefined_value, _bytes(prop_name))
+    else:
+        marr = _HPGL_IND_MASKED_ARRAY(data=prop.data.ctypes.data_as(C.POINTER(C.c_ubyte)),
+                                      mask=prop.mask.ctypes.data_as(C.POINTER(C.c_ubyte)), shape=sh,
+                                      indicator_count=prop.indicator_count)
+        vals = numpy.array(indicator_values, dtype=numpy.uint8)
+        _hpgl_so.hpgl_write_inc_file_byte(_bytes(filename), C.byref(marr), undefined_value, _bytes(prop_name),
+                                          vals.ctypes.data_as(C.POINTER(C.c_ubyte)), len(indicator_values))
+    check_last_error()
+
+
+@accepts_tuple("prop", 0)
+def write_gslib_property(prop, filename, prop_name, undefined_value, indicator_values=[]):
+    """geo.py:307-322: GSLIB file with one variable."""
+    if isinstance(prop, ContProperty):
+        _hpgl_so.hpgl_write_gslib_cont_property(C.byref(_create_hpgl_cont_masked_array(prop, None)), _bytes(filename),
+                                                _bytes(prop_name), undefined_value)
+    else:
+        _hpgl_so.hpgl_write_gslib_byte_property(
+            C.byref(_create_hpgl_ind_masked_array(prop, None)), _bytes(filename), _bytes(prop_name), undefined_value,
+            _c_array(C.c_ubyte, len(indicator_values), indicator_values), len(indicator_values))
+    check_last_error()
+
+
+def load_cont_property(filename, undefined_value, size=None):
+    """geo.py:324-329: without a size the pure-Python reader is used."""
+    if size is None:
+        return _load_prop_cont_slow(filename, undefined_value)
+    return read_inc_file_float(filename, undefined_value, size)
+
+
+def read_inc_file_float(filename, undefined_value, size):
+    """geo.py:331-342."""
+    data = numpy.zeros(size, dtype="float32", order="F")
+    mask = numpy.zeros(size, dtype="uint8", order="F")
+    _hpgl_so.hpgl_read_inc_file_float(_bytes(filename), undefined_value, size[0] * size[1] * size[2], data, mask)
+    check_last_error()
+    return ContProperty(data, mask)
+
+
+def read_inc_file_byte(filename, undefined_value, size, indicator_values):
+    """geo.py:344-355: values are replaced by their index in
+    indicator_values."""
+    data = numpy.zeros(size, dtype="uint8", order="F")
+    mask = numpy.zeros(size, dtype="uint8", order="F")
+    _hpgl_so.hpgl_read_inc_file_byte(_bytes(filename), undefined_value, size[0] * size[1] * size[2], data, mask,
+                                     numpy.array(indicator_values, dtype="uint8"), len(indicator_values))
+    check_last_error()
+    return IndProperty(data, mask, len(indicator_values))
+
+
+def load_ind_property(filename, undefined_value, indicator_values, size=None):
+    """geo.py:357-362."""
+    if size is None:
+        return _load_prop_ind_slow(filename, undefined_value, indicator_values)
+    return read_inc_file_byte(filename, undefined_value, size, indicator_values)
+
+
+def get_gslib_property(prop_dict, prop_name, undefined_value):
+    """geo.py:608-616: (values, informed) from a dict of arrays."""
+    prop = prop_dict[prop_name]
+    informed_array = (prop != undefined_value).astype(numpy.uint8)
+    return (prop_dict[prop_name], informed_array)
 
 
 def set_thread_num(num):

@@ -49,6 +49,24 @@ _hpgl_so.hpgl_set_thread_num.argtypes = [C.c_int]
 _hpgl_so.hpgl_get_thread_num.restype = C.c_int
 _hpgl_so.hpgl_get_thread_num.argtypes = []
 
+# property files (hpgl_wrap.py:228-277)
+_u8W = numpy.ctypeslib.ndpointer(dtype=numpy.uint8, flags=["F", "W", "A"])
+_f32W = numpy.ctypeslib.ndpointer(dtype=numpy.float32, flags=["F", "W", "A"])
+_hpgl_so.hpgl_read_inc_file_float.restype = None
+_hpgl_so.hpgl_read_inc_file_float.argtypes = [C.c_char_p, C.c_float, C.c_int, _f32W, _u8W]
+_hpgl_so.hpgl_read_inc_file_byte.restype = None
+_hpgl_so.hpgl_read_inc_file_byte.argtypes = [C.c_char_p, C.c_int, C.c_int, _u8W, _u8W, _u8W, C.c_int]
+_hpgl_so.hpgl_write_inc_file_float.restype = None
+_hpgl_so.hpgl_write_inc_file_float.argtypes = [C.c_char_p, _P(_HPGL_CONT_MASKED_ARRAY), C.c_float, C.c_char_p]
+_hpgl_so.hpgl_write_inc_file_byte.restype = None
+_hpgl_so.hpgl_write_inc_file_byte.argtypes = [
+    C.c_char_p, _P(_HPGL_IND_MASKED_ARRAY), C.c_int, C.c_char_p, _P(C.c_ubyte), C.c_int]
+_hpgl_so.hpgl_write_gslib_cont_property.restype = None
+_hpgl_so.hpgl_write_gslib_cont_property.argtypes = [_P(_HPGL_CONT_MASKED_ARRAY), C.c_char_p, C.c_char_p, C.c_double]
+_hpgl_so.hpgl_write_gslib_byte_property.restype = None
+_hpgl_so.hpgl_write_gslib_byte_property.argtypes = [
+    _P(_HPGL_IND_MASKED_ARRAY), C.c_char_p, C.c_char_p, C.c_double, _P(C.c_ubyte), C.c_int]
+
 _hpgl_so.hpgl_ordinary_kriging.restype = None
 _hpgl_so.hpgl_ordinary_kriging.argtypes = [_P(_HPGL_CONT_MASKED_ARRAY), _P(_HPGL_OK_PARAMS), _P(_HPGL_CONT_MASKED_ARRAY)]
 

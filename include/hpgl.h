@@ -142,9 +142,11 @@ void hpgl_set_progress_handler(int (*handler)(char *stage, int percentage, void 
 void hpgl_set_thread_num(int n_threads);
 int hpgl_get_thread_num(void);
 
-/* api.h:171-214: property file I/O is not on the kriging path; these are
- * exported so the reference's ctypes front end loads, and they report
- * "not supported" through hpgl_get_last_exception_message(). */
+/* api.h:171-214, api.cpp:54-193: property files (host code).  INC files are
+ * read as read_inc_file.cpp:8-200 reads them (byte values then mapped to
+ * their index in values[]); INC and GSLIB files are written as
+ * property_writer.cpp writes them.  Errors set the last error message
+ * (the reference throws). */
 void hpgl_read_inc_file_float(char *filename, float undefined_value, int size,
                               float *data, unsigned char *mask);
 void hpgl_read_inc_file_byte(char *filename, int undefined_value, int size,

@@ -11,11 +11,15 @@ Sources:
       problem_7_2.py: SK, SGS, second SK);
   * GsTL-1.3/test/kriging/test_kriging.cpp (8 data points, lines 150-158),
       test_kriging.result (SK and OK weights, combined estimates),
-      gslib/kriging/OK_kt3d.out and SK_kt3d.out (GSLIB estimates).
+      gslib/kriging/OK_kt3d.out and SK_kt3d.out (GSLIB estimates);
+  * dist/misc/test/test_data/NEW_TEST_PROP.INC, NEW_TEST_PROP_01.INC and
+    mp_0_8_52_55_1.inc (55 x 52 x 1 property files the reference's test
+    scripts load), copied unchanged to inc/.
 """
 import json
 import os
 import re
+import shutil
 
 REF = "/root/reference/src"
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -94,11 +98,20 @@ def gstl_parta():
     }
 
 
+def inc_files():
+    src = os.path.join(REF, "dist", "misc", "test", "test_data")
+    os.makedirs(os.path.join(HERE, "inc"), exist_ok=True)
+    for name in ["NEW_TEST_PROP.INC", "NEW_TEST_PROP_01.INC", "mp_0_8_52_55_1.inc"]:
+        shutil.copyfile(os.path.join(src, name), os.path.join(HERE, "inc", name))
+        print("copied", name)
+
+
 def main():
     for name, fn in [("problem_7_2.json", problem_7_2), ("gstl_parta.json", gstl_parta)]:
         with open(os.path.join(HERE, name), "w") as f:
             json.dump(fn(), f, indent=1)
         print("wrote", name)
+    inc_files()
 
 
 if __name__ == "__main__":

@@ -69,14 +69,17 @@ int main(void) {
 
 
 def test_errors_are_reported_not_thrown():
-    """File I/O is off the kriging path: the call returns and sets the error."""
+    """A failing call returns and sets the last error instead of throwing
+    through the C ABI (the reference throws; api.cpp:54-67)."""
     from hpgl_amd.geo_bsd import hpgl_wrap
 
-    lib = hpgl_wrap._hpgl_so
+    lib = C.CDLL(hpgl_wrap.LIBRARY_PATH)  # own handle: argtypes of the shared one stay as declared
     lib.hpgl_read_inc_file_float.argtypes = [C.c_char_p, C.c_float, C.c_int, C.c_void_p, C.c_void_p]
-    lib.hpgl_read_inc_file_float(b"x.inc", -99.0, 0, None, None)
+    lib.hpgl_amd_get_last_error.restype = C.c_int
+    lib.hpgl_get_last_exception_message.restype = C.c_char_p
+    lib.hpgl_read_inc_file_float(b"no-such-dir/x.inc", -99.0, 0, None, None)
     assert lib.hpgl_amd_get_last_error() == 1
-    assert b"not part of this library" in lib.hpgl_get_last_exception_message()
+    assert b"Error opening file" in lib.hpgl_get_last_exception_message()
 
 
 def test_python_api_surface():
