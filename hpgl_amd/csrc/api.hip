@@ -693,6 +693,118 @@ API void hpgl_amd_sgs_simulation_batch_dev(const float *d_in, const unsigned cha
     GUARD_END
 }
 
+// ---- sequential indicator simulation ----
+
+static SisJob sis_job(const hpgl_shape_t *shape, const hpgl_ik_params_t *params, int K) {
+    if (!params) throw std::runtime_error("the indicator parameters are missing");
+    if (K < 2) throw std::runtime_error("sequential indicator simulation needs at least two categories");
+    SisJob j;
+    for (int i = 0; i < 3; ++i) j.grid[i] = shape->m_data[i];
+    // init_sis_params (api_helpers.cpp:70-112)
+    for (int k = 0; k < K; ++k) {
+        const hpgl_ik_params_t &p = params[k];
+        IndicatorParams c;
+        c.cov = cov_of(p.m_covariance_type, p.m_ranges, p.m_angles, p.m_sill, p.m_nugget);
+        for (int i = 0; i < 3; ++i) c.radii[i] = p.m_radiuses[i];
+        c.max_n = p.m_max_neighbours;
+        c.marginal = p.m_marginal_prob;
+        j.cats.push_back(c);
+    }
+    return j;
+}
+
+// Runs the job with conditioning data from host arrays and writes R
+// realisations to host arrays out[R][N], out_mask[R][N].  lvm: K host
+// arrays of N means, or nullptr.
+static void sis_host(SisJob &j, const unsigned char *data, const unsigned char *mask, const float *const *lvm,
+                     const unsigned char *sim_mask, unsigned char *out, unsigned char *out_mask) {
+    const long long N = (long long)j.grid[0] * j.grid[1] * j.grid[2];
+    const int R = (int)j.seeds.size();
+    const int K = (int)j.cats.size();
+    hipStream_t s = thread_stream();
+    DevArray<unsigned char> di(N), dm(N), dsim(sim_mask ? N : 0), dout((size_t)R * N), dom((size_t)R * N);
+    DevArray<float> dlvm(lvm ? (size_t)K * N : 0);
+    di.put(data, s);
+    dm.put(mask, s);
+    if (sim_mask) dsim.put(sim_mask, s);
+    j.d_lvm.clear();
+    if (lvm)
+        for (int k = 0; k < K; ++k) {
+            hip_check(hipMemcpyAsync(dlvm.p + (size_t)k * N, lvm[k], N * sizeof(float), hipMemcpyHostToDevice, s),
+                      "upload mean");
+            j.d_lvm.push_back(dlvm.p + (size_t)k * N);
+        }
+    j.d_in = di.p;
+    j.d_in_mask = dm.p;
+    j.d_sim_mask = sim_mask ? dsim.p : nullptr;
+    j.d_out = dout.p;
+    j.d_out_mask = dom.p;
+    progress("sis", 0);
+    run_sis(j, s, g_precision, t_stats);
+    dout.get(out, s);
+    dom.get(out_mask, s);
+    hip_check(hipStreamSynchronize(s), "sync");
+    progress("sis", 100);
+}
+
+static std::vector<const float *> sis_means(const hpgl_float_array_t *mean_data, int K, long long N) {
+    if (!mean_data) throw std::runtime_error("the local means are missing");
+    std::vector<const float *> m(K);
+    for (int k = 0; k < K; ++k) {
+        if (!mean_data[k].m_data) throw std::runtime_error("a local mean is missing");
+        check_same_size(&mean_data[k].m_shape, N, "a local mean");
+        m[k] = mean_data[k].m_data;
+    }
+    return m;
+}
+
+API void hpgl_sis_simulation(hpgl_ind_masked_array_t *data, hpgl_ik_params_t *params, int indicator_count, int seed,
+                             hpgl_ubyte_array_t *simulation_mask) {
+    GUARD_BEGIN
+    const long long N = volume(&data->m_shape);
+    if (simulation_mask) check_same_size(&simulation_mask->m_shape, N, "the simulation mask");
+    SisJob j = sis_job(&data->m_shape, params, indicator_count);
+    j.seeds = {(long)seed};
+    sis_host(j, data->m_data, data->m_mask, nullptr, simulation_mask ? simulation_mask->m_data : nullptr,
+             data->m_data, data->m_mask);
+    GUARD_END
+}
+
+API void hpgl_sis_simulation_lvm(hpgl_ind_masked_array_t *data, hpgl_ik_params_t *params,
+                                 hpgl_float_array_t *mean_data, int indicator_count, int seed,
+                                 hpgl_ubyte_array_t *simulation_mask, int use_correlograms) {
+    GUARD_BEGIN
+    const long long N = volume(&data->m_shape);
+    if (simulation_mask) check_same_size(&simulation_mask->m_shape, N, "the simulation mask");
+    SisJob j = sis_job(&data->m_shape, params, indicator_count);
+    const std::vector<const float *> m = sis_means(mean_data, indicator_count, N);
+    j.correlogram = use_correlograms != 0;
+    j.seeds = {(long)seed};
+    sis_host(j, data->m_data, data->m_mask, m.data(), simulation_mask ? simulation_mask->m_data : nullptr,
+             data->m_data, data->m_mask);
+    GUARD_END
+}
+
+API void hpgl_amd_sis_simulation_batch(const hpgl_ind_masked_array_t *data, const hpgl_ik_params_t *params,
+                                       int indicator_count, const hpgl_float_array_t *mean_data,
+                                       int use_correlograms, const hpgl_ubyte_array_t *simulation_mask,
+                                       const int *seeds, int R, unsigned char *out, unsigned char *out_mask) {
+    GUARD_BEGIN
+    const long long N = volume(&data->m_shape);
+    if (R < 1 || !seeds) throw std::runtime_error("at least one seed is needed");
+    if (simulation_mask) check_same_size(&simulation_mask->m_shape, N, "the simulation mask");
+    SisJob j = sis_job(&data->m_shape, params, indicator_count);
+    std::vector<const float *> m;
+    if (mean_data) {
+        m = sis_means(mean_data, indicator_count, N);
+        j.correlogram = use_correlograms != 0;
+    }
+    for (int r = 0; r < R; ++r) j.seeds.push_back((long)seeds[r]);
+    sis_host(j, data->m_data, data->m_mask, mean_data ? m.data() : nullptr,
+             simulation_mask ? simulation_mask->m_data : nullptr, out, out_mask);
+    GUARD_END
+}
+
 // Failure of a library component outside this file (the request loop):
 // sets the calling thread's last-error state as a failed entry point does.
 void hpgl_amd::report_error(const char *msg) {

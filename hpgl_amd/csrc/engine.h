@@ -102,9 +102,25 @@ struct SgsJob {
     uint8_t *d_out_mask;        // [R][N]
 };
 
+// R realisations of sequential indicator simulation (K >= 2 categories;
+// K == 2 is median SIS) on device buffers.
+struct SisJob {
+    int grid[3];
+    std::vector<IndicatorParams> cats;  // marginal: the category's marginal probability
+    std::vector<long> seeds;            // one per realisation
+    const uint8_t *d_in;                // conditioning categories
+    const uint8_t *d_in_mask;
+    std::vector<const float *> d_lvm;   // per category, per-cell mean; empty: marginal probabilities
+    bool correlogram = false;           // LVM only: correlogram weights (corellogramed_weights_3)
+    const uint8_t *d_sim_mask = nullptr;  // cells with value 1 are simulated; nullptr = all
+    uint8_t *d_out;                     // [R][N]
+    uint8_t *d_out_mask;                // [R][N]
+};
+
 void run_cont(const ContJob &job, hipStream_t s, int precision, const Dump *dump, Stats &st);
 void run_ind(const IndJob &job, hipStream_t s, int precision, const Dump *dump, Stats &st);
 void run_sgs(const SgsJob &job, hipStream_t s, int precision, const Dump *dump, Stats &st);
+void run_sis(const SisJob &job, hipStream_t s, int precision, Stats &st);
 
 // Device memory held by the library between calls (size-class cache).
 void *pool_alloc(size_t bytes);

@@ -590,7 +590,7 @@ constexpr int kMixedProbe = 16384;
 
 void launch_fp64_subset(const SolveArgs &b, int order, hipStream_t s, Stats &st) {
     KTimer kt(s);
-    if (order <= kQuadMaxFp64 && b.cov.table)
+    if (order <= kQuadMaxFp64 && b.cov.table && !b.correlogram)
         launch_solve_quad(b, order, true, s);
     else
         launch_solve_fp64(b, order <= 32 ? 32 : 64, s);
@@ -603,7 +603,8 @@ void launch_fp64_subset(const SolveArgs &b, int order, hipStream_t s, Stats &st)
 // kernels (orders <= 32 with a covariance table) in FP32 + FP64 refinement,
 // then FP64 for the systems they hand over; wider systems and the
 // analytic-covariance fallback go to the FP64 kernels directly.  FP64 mode:
-// every system through the FP64 kernels.
+// every system through the FP64 kernels.  Correlogram systems always take
+// the FP64 kernels.
 void solve(SolveArgs sa, int epi, int precision, DBuf<uint8_t> &status, hipStream_t s, Stats &st) {
     sa.status = status.p;
     sa.subset = nullptr;
@@ -617,7 +618,7 @@ void solve(SolveArgs sa, int epi, int precision, DBuf<uint8_t> &status, hipStrea
     const int kmax = maxn > kQuadMaxMixed ? 8 : std::max(0, (maxn + 3) / 4 - 1);
     for (int k = 0; k <= kmax; ++k) {
         const int order = k == 8 ? 64 : 4 * (k + 1);
-        const bool mixed = precision == 0 && k < 8 && sa.cov.table;
+        const bool mixed = precision == 0 && k < 8 && sa.cov.table && !sa.correlogram;
         if (!mixed) {
             solve_fp64_bucket(sa, keys, (uint8_t)k, false, status, s, st);
             continue;
@@ -945,6 +946,122 @@ __global__ void k_sgs_out_mask(const int *key, long long n, uint8_t *mask) {
     if (i < n) mask[i] = key[i] != 0x7fffffff ? 1 : 0;
 }
 
+// Path LCGs and mt11213b seeds of all realisations of a call.
+struct SimPaths {
+    Lcg lcg;
+    std::vector<long long> h_R0;
+    DBuf<long long> d_A, d_C, d_R0;
+    DBuf<uint32_t> d_mt_seed;
+};
+
+void sim_paths(long long N, const std::vector<long> &seeds, SimPaths &p, hipStream_t s) {
+    const int R = (int)seeds.size();
+    p.lcg = lcg_for(N);
+    std::vector<long long> h_A(R, p.lcg.A), h_C(R, p.lcg.C);
+    std::vector<uint32_t> h_mt_seed(R);
+    p.h_R0.resize(R);
+    for (int r = 0; r < R; ++r) {
+        // PathGen takes the seed through int (sequential_gaussian_simulation.cpp:45,
+        // sequential_indicator_simulation.cpp:57)
+        const long long r0 = (long long)(int)seeds[r] % N;
+        if (r0 < 0) throw std::runtime_error("the seed, taken as int, must not be negative");
+        p.h_R0[r] = r0;
+        h_mt_seed[r] = (uint32_t)seeds[r];
+    }
+    upload(p.d_A, h_A, s);
+    upload(p.d_C, h_C, s);
+    upload(p.d_R0, p.h_R0, s);
+    upload(p.d_mt_seed, h_mt_seed, s);
+}
+
+// Realisations r0 .. r0 + rc - 1: key of every cell (-1 hard data, path
+// rank of a cell to simulate, INT_MAX never informed) and the simulated
+// cells of each realisation in path order.
+struct SimBatch {
+    DBuf<int> key, targets;
+    std::vector<int> h_T;
+    std::vector<long long> h_off;
+    long long tot = 0;
+    int max_T = 0;
+};
+
+void sim_batch(long long N, const SimPaths &p, int r0, int rc, const uint8_t *hard, const uint8_t *sim_mask,
+               SimBatch &b, hipStream_t s) {
+    DBuf<int> path((size_t)rc * N), rank((size_t)rc * N);
+    b.key.alloc((size_t)rc * N);
+    if (p.lcg.full) {
+        SgsPathArgs pa;
+        pa.N = N;
+        pa.A = p.d_A.p + r0;
+        pa.C = p.d_C.p + r0;
+        pa.R0 = p.d_R0.p + r0;
+        pa.path = path.p;
+        pa.rank = rank.p;
+        launch_lcg_path(pa, rc, s);
+    } else {
+        std::vector<int> hp((size_t)rc * N), hr((size_t)rc * N);
+        for (int r = 0; r < rc; ++r)
+            host_path(N, p.lcg, p.h_R0[r0 + r], hp.data() + (size_t)r * N, hr.data() + (size_t)r * N);
+        hip_check(hipMemcpyAsync(path.p, hp.data(), hp.size() * sizeof(int), hipMemcpyHostToDevice, s), "upload");
+        hip_check(hipMemcpyAsync(rank.p, hr.data(), hr.size() * sizeof(int), hipMemcpyHostToDevice, s), "upload");
+        hip_check(hipStreamSynchronize(s), "sync");
+    }
+    launch_sgs_key(hard, sim_mask, rank.p, N, rc, b.key.p, s);
+    hip_check(hipGetLastError(), "path kernels");
+    rank.release();
+    b.h_T.assign(rc, 0);
+    b.h_off.assign(rc, 0);
+    b.targets.alloc((size_t)rc * N);
+    b.tot = 0;
+    for (int r = 0; r < rc; ++r) {
+        b.h_off[r] = b.tot;
+        b.h_T[r] = select_path_targets(path.p + (size_t)r * N, N, b.key.p + (size_t)r * N, b.targets.p + b.tot, s);
+        b.tot += b.h_T[r];
+    }
+    if (b.tot >= (1ll << 31)) throw std::runtime_error("too many simulated cells in one batch");
+    b.max_T = 0;
+    for (int r = 0; r < rc; ++r) b.max_T = std::max(b.max_T, b.h_T[r]);
+}
+
+// Template scan of every target of the batch among the cells informed
+// before it (the reference's simulations search with the template only,
+// sequential_simulation.h:74, sequential_indicator_simulation.cpp:53).
+void sim_scan(const CovAssets &as, const int grid[3], int max_n, const SimBatch &b, int *nbr, int *cnt,
+              uint64_t *simbits, hipStream_t s, Stats &st) {
+    const long long N = (long long)grid[0] * grid[1] * grid[2];
+    const int stride = std::max(max_n, 1);
+    for (size_t r = 0; r < b.h_T.size(); ++r) {
+        if (b.h_T[r] == 0) continue;
+        ScanArgs sa;
+        sa.mask = nullptr;
+        sa.key = b.key.p + r * N;
+        sa.X = grid[0];
+        sa.Y = grid[1];
+        sa.Z = grid[2];
+        sa.tmpl = (const int4 *)as.d_tmpl.p;
+        sa.tsize = as.tmpl.size();
+        sa.max_n = max_n;
+        sa.targets = b.targets.p + b.h_off[r];
+        sa.list = nullptr;
+        sa.n_list = b.h_T[r];
+        sa.nbr = nbr + (size_t)b.h_off[r] * stride;
+        sa.cnt = cnt + b.h_off[r];
+        sa.simbits = simbits + b.h_off[r];
+        launch_template_scan(sa, 1, s);
+        hip_check(hipGetLastError(), "template scan");
+        st.nodes_template_path += b.h_T[r];
+    }
+}
+
+// Realisations per batch from free device memory, bytes_per_cell per
+// realisation and cell.
+int sim_batch_size(int R, size_t bytes_per_cell, long long N) {
+    size_t free_b = 0, total_b = 0;
+    hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+    const size_t per_real = (size_t)N * bytes_per_cell;
+    return (int)std::max<size_t>(1, std::min<size_t>((size_t)R, (size_t)(free_b * 0.6) / per_real));
+}
+
 }  // namespace
 
 void run_sgs(const SgsJob &job, hipStream_t s, int precision, const Dump *dump, Stats &st) {
@@ -959,17 +1076,8 @@ void run_sgs(const SgsJob &job, hipStream_t s, int precision, const Dump *dump, 
     const int max_n = std::max(job.max_n, 1);
     Events ev;
     hip_check(hipEventRecord(ev.e[0], s), "event");
-
-    const Lcg lcg = lcg_for(N);
-    std::vector<long long> h_A(R, lcg.A), h_C(R, lcg.C), h_R0(R);
-    std::vector<uint32_t> h_mt_seed(R);
-    for (int r = 0; r < R; ++r) {
-        // PathGen takes the seed through int (sequential_gaussian_simulation.cpp:45)
-        const long long r0 = (long long)(int)job.seeds[r] % N;
-        if (r0 < 0) throw std::runtime_error("the seed, taken as int, must not be negative");
-        h_R0[r] = r0;
-        h_mt_seed[r] = (uint32_t)job.seeds[r];
-    }
+    SimPaths paths;
+    sim_paths(N, job.seeds, paths, s);
 
     // normal-score transform of the conditioning data and of the LVM mean
     DBuf<float> cv, cp, data((size_t)N), lvm;
@@ -987,90 +1095,25 @@ void run_sgs(const SgsJob &job, hipStream_t s, int precision, const Dump *dump, 
         hip_check(hipGetLastError(), "cdf forward");
     }
 
-    // covariance table and search template (the reference's SGS searches
-    // with the template only, sequential_simulation.h:74)
     std::shared_ptr<CovAssets> as = get_assets(job.cov, job.radii, true, s, st);
+    const int RC = sim_batch_size(R, 17 + (12 * (size_t)max_n + 32), N);
 
-    // realisations per chunk, from free device memory
-    size_t free_b = 0, total_b = 0;
-    hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
-    const size_t per_real = (size_t)N * 17 + (size_t)N * (12 * (size_t)max_n + 32);
-    int RC = (int)std::max<size_t>(1, std::min<size_t>((size_t)R, (size_t)(free_b * 0.6) / per_real));
-
-    DBuf<long long> d_A, d_C, d_R0;
-    DBuf<uint32_t> d_mt_seed;
-    upload(d_A, h_A, s);
-    upload(d_C, h_C, s);
-    upload(d_R0, h_R0, s);
-    upload(d_mt_seed, h_mt_seed, s);
     DBuf<unsigned long long> counters(kCounterWords);
     hip_check(hipMemsetAsync(counters.p, 0, kCounterWords * sizeof(unsigned long long), s), "memset");
     DBuf<unsigned int> err(1);
 
     for (int r0 = 0; r0 < R; r0 += RC) {
         const int rc = std::min(RC, R - r0);
-        DBuf<int> path((size_t)rc * N), rank((size_t)rc * N), key((size_t)rc * N);
-        if (lcg.full) {
-            SgsPathArgs pa;
-            pa.N = N;
-            pa.A = d_A.p + r0;
-            pa.C = d_C.p + r0;
-            pa.R0 = d_R0.p + r0;
-            pa.path = path.p;
-            pa.rank = rank.p;
-            launch_lcg_path(pa, rc, s);
-        } else {
-            std::vector<int> hp((size_t)rc * N), hr((size_t)rc * N);
-            for (int r = 0; r < rc; ++r) host_path(N, lcg, h_R0[r0 + r], hp.data() + (size_t)r * N, hr.data() + (size_t)r * N);
-            hip_check(hipMemcpyAsync(path.p, hp.data(), hp.size() * sizeof(int), hipMemcpyHostToDevice, s), "upload");
-            hip_check(hipMemcpyAsync(rank.p, hr.data(), hr.size() * sizeof(int), hipMemcpyHostToDevice, s), "upload");
-            hip_check(hipStreamSynchronize(s), "sync");
-        }
-        launch_sgs_key(job.d_in_mask, job.d_sim_mask, rank.p, N, rc, key.p, s);
-        hip_check(hipGetLastError(), "path kernels");
-        rank.release();
-
-        // simulated cells of each realisation in path order
-        std::vector<int> h_T(rc);
-        std::vector<long long> h_off(rc);
-        DBuf<int> targets((size_t)rc * N);
-        long long tot = 0;
-        for (int r = 0; r < rc; ++r) {
-            h_off[r] = tot;
-            h_T[r] = select_path_targets(path.p + (size_t)r * N, N, key.p + (size_t)r * N, targets.p + tot, s);
-            tot += h_T[r];
-        }
-        path.release();
-        if (tot >= (1ll << 31)) throw std::runtime_error("too many simulated cells in one batch");
+        SimBatch b;
+        sim_batch(N, paths, r0, rc, job.d_in_mask, job.d_sim_mask, b, s);
+        const long long tot = b.tot;
         st.nodes_target += tot;
-        int max_T = 0;
-        for (int r = 0; r < rc; ++r) max_T = std::max(max_T, h_T[r]);
 
         // neighbours of every target among the cells informed before it
         DBuf<int> nbr((size_t)std::max<long long>(tot, 1) * max_n), cnt((size_t)std::max<long long>(tot, 1));
         DBuf<uint64_t> simbits((size_t)std::max<long long>(tot, 1));
         hip_check(hipEventRecord(ev.e[1], s), "event");
-        for (int r = 0; r < rc; ++r) {
-            if (h_T[r] == 0) continue;
-            ScanArgs sa;
-            sa.mask = nullptr;
-            sa.key = key.p + (size_t)r * N;
-            sa.X = job.grid[0];
-            sa.Y = job.grid[1];
-            sa.Z = job.grid[2];
-            sa.tmpl = (const int4 *)as->d_tmpl.p;
-            sa.tsize = as->tmpl.size();
-            sa.max_n = job.max_n;
-            sa.targets = targets.p + h_off[r];
-            sa.list = nullptr;
-            sa.n_list = h_T[r];
-            sa.nbr = nbr.p + (size_t)h_off[r] * max_n;
-            sa.cnt = cnt.p + h_off[r];
-            sa.simbits = simbits.p + h_off[r];
-            launch_template_scan(sa, 1, s);
-            hip_check(hipGetLastError(), "template scan");
-            st.nodes_template_path += h_T[r];
-        }
+        sim_scan(*as, job.grid, job.max_n, b, nbr.p, cnt.p, simbits.p, s, st);
         hip_check(hipEventRecord(ev.e[2], s), "event");
         hip_check(hipEventSynchronize(ev.e[2]), "sync");
         st.ms_k_template += ev.ms(1, 2);
@@ -1078,8 +1121,8 @@ void run_sgs(const SgsJob &job, hipStream_t s, int precision, const Dump *dump, 
         st.n_k_template += rc;
         if (dump)
             for (int r = 0; r < rc; ++r)
-                write_dump(dump, (long long)(r0 + r) * N, targets.p + h_off[r], h_T[r],
-                           nbr.p + (size_t)h_off[r] * max_n, cnt.p + h_off[r], max_n, s);
+                write_dump(dump, (long long)(r0 + r) * N, b.targets.p + b.h_off[r], b.h_T[r],
+                           nbr.p + (size_t)b.h_off[r] * max_n, cnt.p + b.h_off[r], max_n, s);
 
         // weights and variances
         DBuf<double> w((size_t)std::max<long long>(tot, 1) * max_n), var((size_t)std::max<long long>(tot, 1));
@@ -1087,7 +1130,7 @@ void run_sgs(const SgsJob &job, hipStream_t s, int precision, const Dump *dump, 
         if (tot > 0) {
             SolveArgs sa;
             std::memset(&sa, 0, sizeof(sa));
-            sa.targets = targets.p;
+            sa.targets = b.targets.p;
             sa.T = (int)tot;
             sa.nbr = nbr.p;
             sa.cnt = cnt.p;
@@ -1109,11 +1152,12 @@ void run_sgs(const SgsJob &job, hipStream_t s, int precision, const Dump *dump, 
         }
 
         // draws: one per simulated cell, in path order
+        const int max_T = b.max_T;
         DBuf<uint32_t> draws((size_t)rc * std::max(max_T, 1));
         DBuf<long long> d_counts;
-        std::vector<long long> h_counts(h_T.begin(), h_T.end());
+        std::vector<long long> h_counts(b.h_T.begin(), b.h_T.end());
         upload(d_counts, h_counts, s);
-        launch_mt11213b(d_mt_seed.p + r0, d_counts.p, std::max(max_T, 1), rc, draws.p, s);
+        launch_mt11213b(paths.d_mt_seed.p + r0, d_counts.p, std::max(max_T, 1), rc, draws.p, s);
 
         // values: conditioning data everywhere, then the ordered sweep
         float *vals = job.d_out + (size_t)r0 * N;
@@ -1127,15 +1171,15 @@ void run_sgs(const SgsJob &job, hipStream_t s, int precision, const Dump *dump, 
         hip_check(hipMemsetAsync(err.p, 0, sizeof(unsigned int), s), "memset");
         DBuf<int> d_T;
         DBuf<long long> d_off;
-        upload(d_T, h_T, s);
-        upload(d_off, h_off, s);
+        upload(d_T, b.h_T, s);
+        upload(d_off, b.h_off, s);
         if (tot > 0) {
             SgsSweepArgs a;
             a.N = N;
             a.max_n = max_n;
             a.T = d_T.p;
             a.t_off = d_off.p;
-            a.targets = targets.p;
+            a.targets = b.targets.p;
             a.nbr = nbr.p;
             a.cnt = cnt.p;
             a.w = w.p;
@@ -1154,11 +1198,188 @@ void run_sgs(const SgsJob &job, hipStream_t s, int precision, const Dump *dump, 
             launch_sgs_sweep(a, rc, waves, s);
             hip_check(hipGetLastError(), "sweep");
         }
-        hipLaunchKernelGGL(k_sgs_out_mask, dim3((unsigned)(((long long)rc * N + 255) / 256)), dim3(256), 0, s, key.p,
-                           (long long)rc * N, job.d_out_mask + (size_t)r0 * N);
+        hipLaunchKernelGGL(k_sgs_out_mask, dim3((unsigned)(((long long)rc * N + 255) / 256)), dim3(256), 0, s,
+                           b.key.p, (long long)rc * N, job.d_out_mask + (size_t)r0 * N);
         if (job.has_cdf)
             launch_cdf_back(vals, job.d_out_mask + (size_t)r0 * N, (long long)rc * N, cv.p, cp.p,
                             (long long)job.cdf_values.size(), s);
+        unsigned int h_err = 0;
+        hip_check(hipMemcpyAsync(&h_err, err.p, sizeof(h_err), hipMemcpyDeviceToHost, s), "copy error");
+        hip_check(hipStreamSynchronize(s), "sync");
+        if (h_err) throw std::runtime_error("sequential simulation: a cell waited on its neighbours past the limit");
+    }
+    hip_check(hipEventRecord(ev.e[5], s), "event");
+    collect_counters(counters.p, s, st);
+    hip_check(hipStreamSynchronize(s), "sync");
+    st.ms_device_total += ev.ms(0, 5);
+    st.ms_wall += now_ms() - w0;
+}
+
+// ---------------------------------------------------------------------------
+// Sequential indicator simulation (sequential_indicator_simulation.cpp:33-187),
+// R realisations per call.  The path, keys, draws and template scans are
+// those of the Gaussian simulation; each kriged category has its own
+// covariance table, template, neighbour lists and simple-kriging weights,
+// and k_sis_sweep samples the categories in path order.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+struct SisCategoryBuffers {
+    DBuf<int> nbr, cnt;
+    DBuf<uint64_t> simbits;
+    DBuf<double> w, var;
+    DBuf<uint8_t> status;
+};
+
+}  // namespace
+
+void run_sis(const SisJob &job, hipStream_t s, int precision, Stats &st) {
+    const double w0 = now_ms();
+    const int K = (int)job.cats.size();
+    if (K < 2) throw std::runtime_error("sequential indicator simulation needs at least two categories");
+    if (K > kSisMaxCategories) throw std::runtime_error("sequential indicator simulation supports at most 256 categories");
+    for (const auto &c : job.cats) {
+        validate_search(job.grid, c.radii, c.max_n);
+        validate_cov(c.cov);
+    }
+    const bool lvm = !job.d_lvm.empty();
+    if (lvm && (int)job.d_lvm.size() != K) throw std::runtime_error("one local mean per category is needed");
+    if (job.correlogram && !lvm) throw std::runtime_error("correlogram weights need local means");
+    const int R = (int)job.seeds.size();
+    if (R < 1) throw std::runtime_error("sequential simulation needs at least one seed");
+    const long long N = (long long)job.grid[0] * job.grid[1] * job.grid[2];
+    // median SIS: only the first category is kriged (sequential_indicator_simulation.cpp:80-100)
+    const int kriged = K == 2 ? 1 : K;
+    Events ev;
+    hip_check(hipEventRecord(ev.e[0], s), "event");
+    SimPaths paths;
+    sim_paths(N, job.seeds, paths, s);
+
+    std::vector<std::shared_ptr<CovAssets>> as(kriged);
+    size_t per_cell = 21;
+    for (int c = 0; c < kriged; ++c) {
+        as[c] = get_assets(job.cats[c].cov, job.cats[c].radii, true, s, st);
+        per_cell += 12 * (size_t)std::max(job.cats[c].max_n, 1) + 32;
+    }
+    const int RC = sim_batch_size(R, per_cell, N);
+
+    DBuf<unsigned long long> counters(kCounterWords);
+    hip_check(hipMemsetAsync(counters.p, 0, kCounterWords * sizeof(unsigned long long), s), "memset");
+    DBuf<unsigned int> err(1);
+
+    for (int r0 = 0; r0 < R; r0 += RC) {
+        const int rc = std::min(RC, R - r0);
+        SimBatch b;
+        sim_batch(N, paths, r0, rc, job.d_in_mask, job.d_sim_mask, b, s);
+        const long long tot = b.tot;
+        const size_t tot1 = (size_t)std::max<long long>(tot, 1);
+        st.nodes_target += tot;
+
+        std::vector<std::unique_ptr<SisCategoryBuffers>> cb(kriged);
+        std::vector<SisCategory> h_cats(kriged);
+        for (int c = 0; c < kriged; ++c) {
+            const IndicatorParams &p = job.cats[c];
+            const int max_n = std::max(p.max_n, 1);
+            cb[c].reset(new SisCategoryBuffers);
+            SisCategoryBuffers &B = *cb[c];
+            B.nbr.alloc(tot1 * max_n);
+            B.cnt.alloc(tot1);
+            B.simbits.alloc(tot1);
+            B.w.alloc(tot1 * max_n);
+            B.var.alloc(tot1);
+            B.status.alloc(tot1);
+            hip_check(hipEventRecord(ev.e[1], s), "event");
+            sim_scan(*as[c], job.grid, p.max_n, b, B.nbr.p, B.cnt.p, B.simbits.p, s, st);
+            hip_check(hipEventRecord(ev.e[2], s), "event");
+            hip_check(hipEventSynchronize(ev.e[2]), "sync");
+            st.ms_k_template += ev.ms(1, 2);
+            st.ms_search += ev.ms(1, 2);
+            st.n_k_template += rc;
+            if (tot > 0) {
+                // simple-kriging weights (sk_weight_calculator_t or
+                // corellogram_weight_calculator_t, kriging_interpolation.h:19-200)
+                SolveArgs sa;
+                std::memset(&sa, 0, sizeof(sa));
+                sa.targets = b.targets.p;
+                sa.T = (int)tot;
+                sa.nbr = B.nbr.p;
+                sa.cnt = B.cnt.p;
+                sa.max_n = max_n;
+                sa.X = job.grid[0];
+                sa.Y = job.grid[1];
+                sa.Z = job.grid[2];
+                sa.cov = make_devcov(as[c]->model, &as[c]->table, as[c]->d_table.p);
+                sa.ok_system = 0;
+                sa.mean_kind = lvm ? 2 : 0;
+                sa.mean_cells = lvm ? job.d_lvm[c] : nullptr;
+                sa.correlogram = job.correlogram ? 1 : 0;
+                sa.out_var = B.var.p;
+                sa.out_w = B.w.p;
+                sa.counters = counters.p;
+                hip_check(hipEventRecord(ev.e[3], s), "event");
+                solve(sa, EPI_SGS, precision, B.status, s, st);
+                hip_check(hipEventRecord(ev.e[4], s), "event");
+                hip_check(hipEventSynchronize(ev.e[4]), "sync");
+                st.ms_solve += ev.ms(3, 4);
+            }
+            SisCategory &h = h_cats[c];
+            h.nbr = B.nbr.p;
+            h.cnt = B.cnt.p;
+            h.w = B.w.p;
+            h.status = B.status.p;
+            h.simbits = B.simbits.p;
+            h.lvm = lvm ? job.d_lvm[c] : nullptr;
+            // marginal probabilities pass through float (ik_params.h:23)
+            h.marginal = (float)p.marginal;
+            h.max_n = max_n;
+        }
+        DBuf<SisCategory> d_cats;
+        upload(d_cats, h_cats, s);
+
+        // draws: one per simulated cell, in path order
+        const int max_T = b.max_T;
+        DBuf<uint32_t> draws((size_t)rc * std::max(max_T, 1));
+        DBuf<long long> d_counts;
+        std::vector<long long> h_counts(b.h_T.begin(), b.h_T.end());
+        upload(d_counts, h_counts, s);
+        launch_mt11213b(paths.d_mt_seed.p + r0, d_counts.p, std::max(max_T, 1), rc, draws.p, s);
+
+        // categories: conditioning data everywhere, then the ordered sweep
+        uint8_t *vals = job.d_out + (size_t)r0 * N;
+        for (int r = 0; r < rc; ++r)
+            hip_check(hipMemcpyAsync(vals + (size_t)r * N, job.d_in, N, hipMemcpyDeviceToDevice, s), "copy data");
+        DBuf<int> done((size_t)rc * N);
+        hip_check(hipMemsetAsync(done.p, 0, (size_t)rc * N * sizeof(int), s), "memset");
+        DBuf<unsigned int> ticket(rc);
+        hip_check(hipMemsetAsync(ticket.p, 0, rc * sizeof(unsigned int), s), "memset");
+        hip_check(hipMemsetAsync(err.p, 0, sizeof(unsigned int), s), "memset");
+        DBuf<int> d_T;
+        DBuf<long long> d_off;
+        upload(d_T, b.h_T, s);
+        upload(d_off, b.h_off, s);
+        if (tot > 0) {
+            SisSweepArgs a;
+            a.N = N;
+            a.kriged = kriged;
+            a.K = K;
+            a.cats = d_cats.p;
+            a.T = d_T.p;
+            a.t_off = d_off.p;
+            a.targets = b.targets.p;
+            a.draws = draws.p;
+            a.draw_stride = std::max(max_T, 1);
+            a.values = vals;
+            a.done = done.p;
+            a.ticket = ticket.p;
+            a.error = err.p;
+            a.spin_limit = 1ll << 26;
+            const int waves = (int)std::max<long long>(1, std::min<long long>((max_T + 63) / 64, 8192 / rc));
+            launch_sis_sweep(a, rc, waves, s);
+            hip_check(hipGetLastError(), "sweep");
+        }
+        hipLaunchKernelGGL(k_sgs_out_mask, dim3((unsigned)(((long long)rc * N + 255) / 256)), dim3(256), 0, s,
+                           b.key.p, (long long)rc * N, job.d_out_mask + (size_t)r0 * N);
         unsigned int h_err = 0;
         hip_check(hipMemcpyAsync(&h_err, err.p, sizeof(h_err), hipMemcpyDeviceToHost, s), "copy error");
         hip_check(hipStreamSynchronize(s), "sync");

@@ -59,6 +59,7 @@ struct SolveArgs {
     float mean_single;
     const float *mean_cells;
     int mean_on_failure;       // EPI_CONT
+    int correlogram;           // scale A and b by indicator sigmas of mean_cells (FP64 kernels only)
     // outputs
     float *out_f;              // EPI_CONT estimate per cell; EPI_IK prob per target; EPI_SGS mean per target
     uint8_t *out_mask;         // EPI_CONT, EPI_MEDIAN
@@ -206,6 +207,37 @@ struct SgsSweepArgs {
     long long spin_limit;
 };
 void launch_sgs_sweep(const SgsSweepArgs &a, int R, int waves_per_realisation, hipStream_t s);
+
+// Sequential indicator simulation (sgs.hip): what the sweep reads of each
+// kriged category, indexed by the target's position in the batch.
+struct SisCategory {
+    const int *nbr;           // [targets][max_n]
+    const int *cnt;
+    const double *w;          // [targets][max_n] simple-kriging weights
+    const uint8_t *status;    // ST_OK when the system was solved
+    const uint64_t *simbits;  // neighbours that are simulated cells
+    const float *lvm;         // per-cell mean of the category, or nullptr
+    float marginal;           // the mean when lvm is nullptr
+    int max_n;
+};
+struct SisSweepArgs {
+    long long N;
+    int kriged;                // categories kriged: K, or 1 for median SIS (K == 2)
+    int K;                     // categories sampled
+    const SisCategory *cats;   // [kriged], device memory
+    const int *T;              // [R] targets per realisation
+    const long long *t_off;    // [R] offset of each realisation's targets
+    const int *targets;        // cells in path order
+    const uint32_t *draws;     // [R][draw_stride]
+    long long draw_stride;
+    uint8_t *values;           // [R][N] categories
+    int *done;                 // [R][N], 1 once the cell's category is written
+    unsigned int *ticket;      // [R]
+    unsigned int *error;
+    long long spin_limit;
+};
+constexpr int kSisMaxCategories = 256;
+void launch_sis_sweep(const SisSweepArgs &a, int R, int waves_per_realisation, hipStream_t s);
 
 // Statistics counters (counters.h): kCounterSlots lines of kCounterStride
 // words; word w of every line accumulates counter w.

@@ -21,6 +21,8 @@
 //     a cell once all its simulated neighbours (all of smaller rank,
 //     claimed earlier by running wavefronts) are marked done.  The wait is
 //     bounded: past the limit the kernel sets an error flag and exits.
+// Sequential indicator simulation uses the same path, keys, draws and
+// template scans (one per category) and its own sweep, k_sis_sweep.
 #include "kernels.h"
 
 #include <hip/hip_runtime.h>
@@ -285,6 +287,129 @@ __global__ __launch_bounds__(64) void k_sgs_sweep(SgsSweepArgs a) {
     }
 }
 
+// Sequential indicator simulation (sequential_indicator_simulation.cpp:67-121)
+// over the same ticketed sweep: a target waits for the simulated neighbours
+// of each kriged category in turn (a flag once set stays set), then forms
+// each category's probability and samples a category.
+//
+// Probability of category c (kriging_interpolation.h:239-272, combine
+// <uchar, double>, combiner.h:10-24): the mean m_c of the target (LVM
+// value or marginal probability), plus w_i ((float)[v_i == c] - m_i) over
+// the neighbours in search order, in double; the mean alone when the
+// system failed or had no neighbours.  Stored as float.  Median SIS
+// (K == 2) krige category 0 only and use {p, 1 - p}.
+//
+// sample (sample.cpp:16-29): s = u * sum of the K probabilities (double,
+// in order), u = draw / (2^32 - 1); the first category whose running sum
+// exceeds s, else the last.  The probabilities stay in LDS, [K][64].
+__device__ inline void sis_value(const SisSweepArgs &a, long long gk, int k_draw, const uint32_t *draws,
+                                 uint8_t *val, float *probs, int lane, int cell) {
+    double sum = 0.0;
+    for (int c = 0; c < a.kriged; ++c) {
+        const SisCategory &C = a.cats[c];
+        const float mc = C.lvm ? C.lvm[cell] : C.marginal;
+        double prob = mc;
+        if (C.status[gk] == ST_OK) {
+            const int n = C.cnt[gk];
+            const int *row = C.nbr + (size_t)gk * C.max_n;
+            const double *w = C.w + (size_t)gk * C.max_n;
+            double r = mc;
+            for (int i = 0; i < n; ++i) {
+                const int nb = row[i];
+                const float v = val[nb] == (uint8_t)c ? 1.0f : 0.0f;
+                const float m = C.lvm ? C.lvm[nb] : mc;
+                r += w[i] * (double)(v - m);
+            }
+            prob = r;
+        }
+        const float p0 = (float)prob;
+        probs[c * 64 + lane] = p0;
+        sum += (double)p0;
+        if (a.kriged == 1) {
+            const float p1 = (float)(1 - prob);
+            probs[64 + lane] = p1;
+            sum += (double)p1;
+        }
+    }
+    const double u = (double)draws[k_draw] / 4294967295.0;
+    const double s = u * sum;
+    double p = 0.0;
+    int pick = a.K - 1;
+    for (int c = 0; c < a.K; ++c) {
+        p += (double)probs[c * 64 + lane];
+        if (s < p) {
+            pick = c;
+            break;
+        }
+    }
+    val[cell] = (uint8_t)pick;
+}
+
+__global__ __launch_bounds__(64) void k_sis_sweep(SisSweepArgs a) {
+    extern __shared__ float sis_probs[];
+    const int r = blockIdx.y;
+    const int lane = threadIdx.x;
+    const int T = a.T[r];
+    const long long off = a.t_off[r];
+    const uint32_t *draws = a.draws + (size_t)r * a.draw_stride;
+    uint8_t *val = a.values + (size_t)r * a.N;
+    int *done = a.done + (size_t)r * a.N;
+    unsigned int *ticket = a.ticket + r;
+    while (true) {
+        if (__hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+        int base = 0;
+        if (lane == 0) base = (int)atomicAdd(ticket, 64u);
+        base = __shfl(base, 0);
+        if (base >= T) break;
+        const int k = base + lane;
+        bool pending = k < T;
+        const long long gk = off + (pending ? k : 0);
+        // category whose simulated neighbours are being waited for
+        int wc = 0;
+        uint64_t waitset = pending ? a.cats[0].simbits[gk] : 0ull;
+        const int *row = a.cats[0].nbr + (size_t)gk * a.cats[0].max_n;
+        long long spins = 0;
+        while (__any(pending)) {
+            if (pending) {
+                while (waitset == 0 && wc + 1 < a.kriged) {
+                    ++wc;
+                    waitset = a.cats[wc].simbits[gk];
+                    row = a.cats[wc].nbr + (size_t)gk * a.cats[wc].max_n;
+                }
+                int idx[8];
+                uint64_t ws = waitset;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    idx[q] = ws ? __ffsll((unsigned long long)ws) - 1 : -1;
+                    if (ws) ws &= ws - 1;
+                }
+                int fl[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    fl[q] = idx[q] >= 0 ? __hip_atomic_load(done + row[idx[q]], __ATOMIC_RELAXED,
+                                                            __HIP_MEMORY_SCOPE_AGENT)
+                                        : 0;
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    if (idx[q] >= 0 && fl[q] != 0) waitset &= ~(1ull << idx[q]);
+                if (waitset == 0 && wc + 1 >= a.kriged) {
+                    // every neighbour of every category is published
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                    const int cell = a.targets[gk];
+                    sis_value(a, gk, k, draws, val, sis_probs, lane, cell);
+                    __hip_atomic_store(done + cell, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                    pending = false;
+                }
+            }
+            if (++spins > a.spin_limit) {
+                if (pending) atomicOr(a.error, 1u);
+                break;
+            }
+            if (__any(pending)) __builtin_amdgcn_s_sleep(1);
+        }
+    }
+}
+
 inline unsigned blocks(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 }  // namespace
@@ -321,6 +446,11 @@ void launch_cdf_back(float *v, const uint8_t *mask, long long n, const float *cv
 
 void launch_sgs_sweep(const SgsSweepArgs &a, int R, int waves_per_realisation, hipStream_t s) {
     hipLaunchKernelGGL(k_sgs_sweep, dim3(waves_per_realisation, R), dim3(64), 0, s, a);
+}
+
+void launch_sis_sweep(const SisSweepArgs &a, int R, int waves_per_realisation, hipStream_t s) {
+    const size_t lds = (size_t)a.K * 64 * sizeof(float);
+    hipLaunchKernelGGL(k_sis_sweep, dim3(waves_per_realisation, R), dim3(64), lds, s, a);
 }
 
 }  // namespace hpgl_amd

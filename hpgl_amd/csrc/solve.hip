@@ -79,6 +79,7 @@ struct Shared64 {
     double y2[NG][G];
     float term[NG][G];
     double dterm[NG][G];
+    double sig[NG][G];
     int xyz[NG][3][G];
 };
 
@@ -121,6 +122,28 @@ __device__ inline float nb_mean(const SolveArgs &a, int nb) {
     if (a.mean_kind == 2) return a.mean_cells[nb];
     if (a.mean_kind == 1) return a.mean_single;
     return 0.0f;
+}
+
+// Sigma of an indicator with mean m, m moved off 0 and 1 by 1e-5
+// (corellogramed_weights_3, my_kriging_weights.h:275-305).
+__device__ inline double indicator_sigma(double m) {
+    const double delta = 0.00001;
+    if (m == 0) m += delta;
+    if (m == 1) m -= delta;
+    return sqrt(m * (1 - m));
+}
+
+// Correlogram systems (corellogramed_weights_3): A(i,j) = C(x_i, x_j)
+// (s_i s_j) and b_i = C(x_i, x_0) (s_i s_0) with the sigmas of the per-cell
+// means; stores this lane's sigma for the group and returns s_i s_0 (1 for
+// plain systems, whose entries are left as they are).
+template <int G>
+__device__ inline double stage_sigma(const SolveArgs &a, int t, int n, int g, int i, int nb, double (*sig)[G]) {
+    if (!a.correlogram) return 1.0;
+    const double s0 = indicator_sigma((double)a.mean_cells[a.targets[t]]);
+    const double si = i < n ? indicator_sigma((double)a.mean_cells[nb]) : 1.0;
+    sig[g][i] = si;
+    return si * s0;
 }
 
 // Writes the result of one system.  `ok` false means failure (singular or
@@ -232,6 +255,7 @@ __global__ __launch_bounds__(64) void k_solve_fp64(SolveArgs a) {
 
     int cx, cy, cz, xi, yi, zi;
     const int nb = stage_coords<G>(a, t, n, g, i, sh.xyz, cx, cy, cz, xi, yi, zi);
+    const double sb = stage_sigma<G>(a, t, n, g, i, nb, sh.sig);
     __syncthreads();
 
     // A(i,j) = C(x_i, x_j) (my_kriging_weights.h:156-165), identity padding
@@ -253,6 +277,13 @@ __global__ __launch_bounds__(64) void k_solve_fp64(SolveArgs a) {
             else
                 u[j] = (i == j) ? 1.0 : 0.0;
         }
+    }
+    if (a.correlogram) {
+        const double si = sh.sig[g][i];
+        bi = bi * sb;
+#pragma unroll
+        for (int j = 0; j < G; ++j)
+            if (i < n && j < n) u[j] = u[j] * (si * sh.sig[g][j]);
     }
     // the largest system of the wave bounds the unrolled loops
     int nmax = n;
@@ -379,6 +410,7 @@ struct ShLds64 {
     double y2[1][64];
     float term[1][64];
     double dterm[1][64];
+    double sig[1][64];
     int xyz[1][3][64];
 };
 
@@ -400,6 +432,7 @@ __global__ __launch_bounds__(64) void k_solve_fp64_lds(SolveArgs a) {
 
     int cx, cy, cz, xi, yi, zi;
     const int nb = stage_coords<G>(a, t, n, 0, i, sh.xyz, cx, cy, cz, xi, yi, zi);
+    const double sb = stage_sigma<G>(a, t, n, 0, i, nb, sh.sig);
     __syncthreads();
 
     // lower triangle of A (my_kriging_weights.h:156-165), identity padding
@@ -416,6 +449,12 @@ __global__ __launch_bounds__(64) void k_solve_fp64_lds(SolveArgs a) {
             sh.U[i][j] = (i < n && j < n)
                              ? cov_eval(a.cov, sh.xyz[0][0][j] - xi, sh.xyz[0][1][j] - yi, sh.xyz[0][2][j] - zi)
                              : ((i == j) ? 1.0 : 0.0);
+    }
+    if (a.correlogram) {
+        const double si = sh.sig[0][i];
+        bi = bi * sb;
+        if (i < n)
+            for (int j = 0; j <= i; ++j) sh.U[i][j] = sh.U[i][j] * (si * sh.sig[0][j]);
     }
     wave_fence();
 

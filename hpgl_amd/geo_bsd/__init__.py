@@ -8,5 +8,6 @@ from .geo import (  # noqa: F401
     _clone_prop, _empty_clone, _create_hpgl_cont_masked_array, _create_hpgl_ind_masked_array,
     _create_hpgl_float_array, _create_hpgl_ubyte_array, _create_hpgl_shape, _c_array)
 from .sgs import sgs_simulation, sgs_simulation_batch, normed_cov_model  # noqa: F401
+from .sis import sis_simulation, sis_simulation_batch  # noqa: F401
 from .cdf import CdfData, calc_cdf  # noqa: F401
 from .hpgl_wrap import last_stats, set_precision, get_precision, HpglError  # noqa: F401

@@ -108,6 +108,18 @@ _hpgl_so.hpgl_amd_sgs_simulation_batch_dev.argtypes = [
     C.c_void_p, C.c_void_p, _P(_HPGL_SHAPE), _P(_HPGL_SGS_PARAMS), _P(hpgl_non_parametric_cdf_t), C.c_void_p,
     C.c_void_p, _P(C.c_long), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
 
+_hpgl_so.hpgl_sis_simulation.restype = None
+_hpgl_so.hpgl_sis_simulation.argtypes = [
+    _P(_HPGL_IND_MASKED_ARRAY), _P(_HPGL_IK_PARAMS), C.c_int, C.c_int, _P(_HPGL_UBYTE_ARRAY)]
+_hpgl_so.hpgl_sis_simulation_lvm.restype = None
+_hpgl_so.hpgl_sis_simulation_lvm.argtypes = [
+    _P(_HPGL_IND_MASKED_ARRAY), _P(_HPGL_IK_PARAMS), _P(_HPGL_FLOAT_ARRAY), C.c_int, C.c_int,
+    _P(_HPGL_UBYTE_ARRAY), C.c_int]
+_hpgl_so.hpgl_amd_sis_simulation_batch.restype = None
+_hpgl_so.hpgl_amd_sis_simulation_batch.argtypes = [
+    _P(_HPGL_IND_MASKED_ARRAY), _P(_HPGL_IK_PARAMS), C.c_int, _P(_HPGL_FLOAT_ARRAY), C.c_int,
+    _P(_HPGL_UBYTE_ARRAY), _P(C.c_int), C.c_int, C.c_void_p, C.c_void_p]
+
 _hpgl_so.hpgl_amd_server_create.restype = C.c_void_p
 _hpgl_so.hpgl_amd_server_create.argtypes = [C.c_int, C.c_longlong]
 _hpgl_so.hpgl_amd_server_submit.restype = C.c_longlong

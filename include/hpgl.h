@@ -224,6 +224,28 @@ void hpgl_median_ik(struct hpgl_ind_masked_array_t *in_data,
                     struct hpgl_median_ik_params_t *params,
                     struct hpgl_ind_masked_array_t *out_data);
 
+/* api.h:279-285, api.cpp:517-545 (sequential_indicator_simulation.cpp:33-157).
+ * In place on data.  params points to indicator_count consecutive structs
+ * (m_marginal_prob of each category); indicator_count == 2 is median SIS.
+ * simulation_mask may be NULL. */
+void hpgl_sis_simulation(struct hpgl_ind_masked_array_t *data,
+                         struct hpgl_ik_params_t *params,
+                         int indicator_count,
+                         int seed,
+                         struct hpgl_ubyte_array_t *simulation_mask);
+
+/* api.h:287-295, api.cpp:547-585 (sequential_indicator_simulation.cpp:159-187).
+ * mean_data: indicator_count per-cell mean arrays.  use_correlograms != 0
+ * selects the correlogram weights (corellogramed_weights_3,
+ * my_kriging_weights.h:258-362); those systems are solved in FP64. */
+void hpgl_sis_simulation_lvm(struct hpgl_ind_masked_array_t *data,
+                             struct hpgl_ik_params_t *params,
+                             struct hpgl_float_array_t *mean_data,
+                             int indicator_count,
+                             int seed,
+                             struct hpgl_ubyte_array_t *simulation_mask,
+                             int use_correlograms);
+
 /* ---- Part 2: hpgl_amd_* extensions (no counterpart in the reference) ---- */
 
 /* Precision of the per-node solves.  MIXED (default): FP32 Cholesky, one or
@@ -327,6 +349,20 @@ void hpgl_amd_sgs_simulation_batch_dev(const float *d_in, const unsigned char *d
                                        const float *d_lvm_mean, const unsigned char *d_sim_mask,
                                        const long *seeds, int R,
                                        float *d_out, unsigned char *d_out_mask, void *stream);
+
+/* R realisations of sequential indicator simulation in one call.
+ * Realisation r is what hpgl_sis_simulation (mean_data == NULL) or
+ * hpgl_sis_simulation_lvm would return with seed = seeds[r]; data is not
+ * modified, realisation r goes to out[r*N ...] and out_mask[r*N ...].  No
+ * reference counterpart: the reference runs one realisation per call. */
+void hpgl_amd_sis_simulation_batch(const struct hpgl_ind_masked_array_t *data,
+                                   const struct hpgl_ik_params_t *params,
+                                   int indicator_count,
+                                   const struct hpgl_float_array_t *mean_data,
+                                   int use_correlograms,
+                                   const struct hpgl_ubyte_array_t *simulation_mask,
+                                   const int *seeds, int R,
+                                   unsigned char *out, unsigned char *out_mask);
 
 /* ---- Request loop with continuous batching (no reference counterpart:
  * the reference runs one blocking call at a time).  Jobs wait in arrival

@@ -29,6 +29,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <memory>
 #include <random>
 #include <stdexcept>
 #include <string>
@@ -803,6 +804,115 @@ static void sgs_core(float *data, unsigned char *mask, const Grid &g, const hpgl
     }
 }
 
+// sigma of an indicator with mean m, m moved off 0 and 1 by 1e-5
+// (my_kriging_weights.h:275-305)
+static double indicator_sigma(double m) {
+    const double delta = 0.00001;
+    if (m == 0) m += delta;
+    if (m == 1) m -= delta;
+    return sqrt(m * (1 - m));
+}
+
+// corellogramed_weights_3 (my_kriging_weights.h:258-362, LAPACK branch):
+// simple kriging on covariances scaled by the indicator sigmas,
+// A(i,j) = C(x_i, x_j) (s_i s_j), b_i = C(x_i, x_0) (s_i s_0).
+static bool corr_weights(const Coord &center, mean_t center_mean, const std::vector<Coord> &c,
+                         const std::vector<mean_t> &means, const JobCov &cov, std::vector<double> &w) {
+    const int n = (int)c.size();
+    w.resize(n);
+    if (n <= 0) return false;
+    const double sc = indicator_sigma(center_mean);
+    std::vector<double> s(n), A((size_t)n * n), b(n);
+    for (int i = 0; i < n; ++i) s[i] = indicator_sigma(means[i]);
+    for (int i = 0; i < n; ++i) {
+        for (int j = 0; j < n; ++j)
+            A[(size_t)i * n + j] = cov(c[i].c[0], c[i].c[1], c[i].c[2], c[j].c[0], c[j].c[1], c[j].c[2]) * (s[i] * s[j]);
+        b[i] = cov(c[i].c[0], c[i].c[1], c[i].c[2], center.c[0], center.c[1], center.c[2]) * (s[i] * sc);
+    }
+    const int info = dpotf2_upper(n, A.data());
+    for (int i = 0; i < n; ++i) w[i] = b[i];
+    dpotrs_upper(n, A.data(), w.data());
+    return info == 0;
+}
+
+// sample (sample.cpp:16-29): the first category whose running sum of
+// probabilities exceeds u * sum, the last one otherwise.
+static int sample_category(mt11213b &gen, const std::vector<float> &probs) {
+    double sum = 0.0;
+    for (float p : probs) sum += p;
+    const double s = double(gen()) / double(mt11213b::max()) * sum;
+    double p = 0;
+    for (size_t i = 0; i < probs.size(); ++i) {
+        p += probs[i];
+        if (s < p) return (int)i;
+    }
+    return (int)probs.size() - 1;
+}
+
+// do_sis (sequential_indicator_simulation.cpp:33-124) for one realisation,
+// in place on data/mask.  lvm: K per-cell mean arrays, or nullptr for the
+// marginal probabilities (single_mean_t, mean_provider.h:17-33).  K == 2 is
+// median SIS: category 0 is kriged and category 1 gets 1 - p.  A category
+// without neighbours or with a singular system takes its mean.
+static void sis_core(unsigned char *data, unsigned char *mask, const Grid &g, const hpgl_ik_params_t *params, int K,
+                     int seed, const float *const *lvm, bool correlogram, const unsigned char *sim_mask) {
+    const int N = g.size();
+    if (K < 2) throw std::runtime_error("sequential indicator simulation needs at least two categories");
+    std::vector<std::unique_ptr<CovModel>> models;
+    std::vector<std::unique_ptr<CovTable>> tables;
+    std::vector<std::unique_ptr<Lookup>> nls;
+    for (int k = 0; k < K; ++k) {
+        const hpgl_ik_params_t &p = params[k];
+        models.emplace_back(new CovModel(cov_from(p.m_covariance_type, p.m_ranges, p.m_angles, p.m_sill, p.m_nugget)));
+        tables.emplace_back(new CovTable(*models[k], p.m_radiuses[0], p.m_radiuses[1], p.m_radiuses[2]));
+        nls.emplace_back(new Lookup(&g, JobCov{models[k].get(), tables[k].get()}, p.m_radiuses, p.m_max_neighbours,
+                                    false));
+    }
+    mt11213b gen((uint32_t)seed);
+    PathGen path((long)N, (long)seed);
+    auto informed = [&](int idx) { return idx < 0 || idx >= N ? false : mask[idx] != 0; };
+    const int kriged = K == 2 ? 1 : K;
+    std::vector<int> idx;
+    std::vector<Coord> coords;
+    std::vector<double> w;
+    std::vector<mean_t> ms;
+    std::vector<unsigned char> vals;
+    std::vector<float> probs;
+    int node = path.next();
+    for (int counter = 0; counter < N; ++counter, node = path.next()) {
+        if (mask[node] != 0) continue;
+        if (sim_mask && sim_mask[node] != 1) continue;
+        probs.clear();
+        for (int k = 0; k < kriged; ++k) {
+            const mean_t mc = lvm ? lvm[k][node] : (mean_t)params[k].m_marginal_prob;
+            nls[k]->find_template(node, informed, idx);
+            double prob = mc;
+            if (!idx.empty()) {
+                Coord center;
+                g.loc(node, center.c[0], center.c[1], center.c[2]);
+                coords.resize(idx.size());
+                ms.resize(idx.size());
+                for (size_t i = 0; i < idx.size(); ++i) {
+                    g.loc(idx[i], coords[i].c[0], coords[i].c[1], coords[i].c[2]);
+                    ms[i] = lvm ? lvm[k][idx[i]] : mc;
+                }
+                const JobCov &cov = nls[k]->cov;
+                const bool ok = correlogram ? corr_weights(center, mc, coords, ms, cov, w)
+                                            : sk_weights(center, coords, cov, w, nullptr);
+                if (ok) {
+                    vals.resize(idx.size());
+                    for (size_t i = 0; i < idx.size(); ++i) vals[i] = data[idx[i]] == k ? 1 : 0;
+                    prob = combine<unsigned char, double>(vals, w, ms, mc);
+                }
+            }
+            probs.push_back((float)prob);
+            if (K == 2) probs.push_back((float)(1 - prob));
+        }
+        data[node] = (unsigned char)sample_category(gen, probs);
+        mask[node] = 1;
+    }
+}
+
 static void transform_forward(float *data, const unsigned char *mask, int N, const NpCdf &cdf) {
     Gauss to;
     for (int i = 0; i < N; ++i)
@@ -1068,6 +1178,31 @@ ORACLE_API void oracle_sgs_lvm_simulation(hpgl_cont_masked_array_t *prop, hpgl_s
     }
     sgs_core(prop->m_data, prop->m_mask, g, p, &mv, sim_mask ? sim_mask->m_data : nullptr);
     if (cdf) transform_back(prop->m_data, prop->m_mask, N, c);
+    GUARD_END
+}
+
+// hpgl_sis_simulation (api.cpp:517-545 -> sequential_indicator_simulation.cpp:137-157),
+// in place on data.
+ORACLE_API void oracle_sis_simulation(hpgl_ind_masked_array_t *data, hpgl_ik_params_t *params, int K, int seed,
+                                      hpgl_ubyte_array_t *sim_mask) {
+    g_last_error.clear();
+    GUARD_BEGIN
+    Grid g{data->m_shape.m_data[0], data->m_shape.m_data[1], data->m_shape.m_data[2]};
+    sis_core(data->m_data, data->m_mask, g, params, K, seed, nullptr, false, sim_mask ? sim_mask->m_data : nullptr);
+    GUARD_END
+}
+
+// hpgl_sis_simulation_lvm (api.cpp:547-585 -> sequential_indicator_simulation.cpp:159-187).
+ORACLE_API void oracle_sis_simulation_lvm(hpgl_ind_masked_array_t *data, hpgl_ik_params_t *params,
+                                          hpgl_float_array_t *mean_data, int K, int seed,
+                                          hpgl_ubyte_array_t *sim_mask, int use_correlograms) {
+    g_last_error.clear();
+    GUARD_BEGIN
+    Grid g{data->m_shape.m_data[0], data->m_shape.m_data[1], data->m_shape.m_data[2]};
+    std::vector<const float *> means(K);
+    for (int k = 0; k < K; ++k) means[k] = mean_data[k].m_data;
+    sis_core(data->m_data, data->m_mask, g, params, K, seed, means.data(), use_correlograms != 0,
+             sim_mask ? sim_mask->m_data : nullptr);
     GUARD_END
 }
 

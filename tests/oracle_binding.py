@@ -54,6 +54,11 @@ lib.oracle_sgs_simulation.argtypes = [_P(abi._HPGL_CONT_MASKED_ARRAY), _P(abi._H
 lib.oracle_sgs_lvm_simulation.argtypes = [_P(abi._HPGL_CONT_MASKED_ARRAY), _P(abi._HPGL_SGS_PARAMS),
                                           _P(abi.hpgl_non_parametric_cdf_t), _P(abi._HPGL_FLOAT_ARRAY),
                                           _P(abi._HPGL_UBYTE_ARRAY)]
+lib.oracle_sis_simulation.argtypes = [_P(abi._HPGL_IND_MASKED_ARRAY), _P(abi._HPGL_IK_PARAMS), C.c_int, C.c_int,
+                                      _P(abi._HPGL_UBYTE_ARRAY)]
+lib.oracle_sis_simulation_lvm.argtypes = [_P(abi._HPGL_IND_MASKED_ARRAY), _P(abi._HPGL_IK_PARAMS),
+                                          _P(abi._HPGL_FLOAT_ARRAY), C.c_int, C.c_int, _P(abi._HPGL_UBYTE_ARRAY),
+                                          C.c_int]
 lib.oracle_simple_kriging_weights.restype = C.c_int
 lib.oracle_simple_kriging_weights.argtypes = [C.c_void_p] * 4 + [C.c_int, _P(abi._HPGL_COV_PARAMS), C.c_void_p,
                                                                   _P(C.c_double)]
@@ -230,6 +235,38 @@ def sgs_simulation(prop, grid, cdf, radiuses, max_neighbours, cov_model, seed, k
         keep.append(md)
         fa = abi._HPGL_FLOAT_ARRAY(data=md.ctypes.data_as(_P(C.c_float)), shape=_shape(grid))
         lib.oracle_sgs_lvm_simulation(C.byref(arr), C.byref(p), c, C.byref(fa), sm)
+    _check()
+    return od, om
+
+
+def sis_simulation(prop, grid, data, seed, marginal_probs, use_correlogram=True, mask=None):
+    """C-level SIS on (data, mask, indicator_count); marginal_probs are K
+    scalars or K per-cell arrays (LVM).  Returns the simulated (data, mask)."""
+    d, m, k = prop
+    od = np.require(d, np.uint8, "F").copy(order="F")
+    om = np.require(m, np.uint8, "F").copy(order="F")
+    K = len(data)
+    lvm = not np.isscalar(marginal_probs[0])
+    ps = (abi._HPGL_IK_PARAMS * K)()
+    for i, dd in enumerate(data):
+        ps[i] = abi._HPGL_IK_PARAMS(radiuses=(C.c_int * 3)(*dd["radiuses"]), max_neighbours=dd["max_neighbours"],
+                                    marginal_prob=0 if lvm else marginal_probs[i], **_cov_kwargs(dd["cov_model"]))
+    keep = []
+    sm = None
+    if mask is not None:
+        mk = np.require(mask, np.uint8, "F")
+        keep.append(mk)
+        sm = C.byref(abi._HPGL_UBYTE_ARRAY(data=mk.ctypes.data_as(_P(C.c_ubyte)), shape=_shape(grid)))
+    arr = _ima(od, om, k, grid)
+    if not lvm:
+        lib.oracle_sis_simulation(C.byref(arr), ps, K, seed, sm)
+    else:
+        means = (abi._HPGL_FLOAT_ARRAY * K)()
+        for i, mp in enumerate(marginal_probs):
+            md = np.require(mp, np.float32, "F")
+            keep.append(md)
+            means[i] = abi._HPGL_FLOAT_ARRAY(data=md.ctypes.data_as(_P(C.c_float)), shape=_shape(grid))
+        lib.oracle_sis_simulation_lvm(C.byref(arr), ps, means, K, seed, sm, 1 if use_correlogram else 0)
     _check()
     return od, om
 
