@@ -94,6 +94,92 @@ def cpu_baseline(g, data, mask, cat, marg, sk, ik, budget_s=15.0, threads=16):
                       % (stride, nodes, a, b)}
 
 
+def make_sis_workload(seed=1006):
+    """The reference's published SIS benchmark shape (doc/hpgl.sourceforge.net/index.html:103-128):
+    166x141x225 grid (5,266,350 cells), 20% informed, two indicators (median
+    SIS), exponential covariance ranges 10/10/10, search radii 20/20/20, at
+    most 12 neighbours.  Categories: 1 above the 0.8 quantile of a standard
+    normal sample, else 0 (marginals 0.8/0.2 as in the reference's SIS test
+    scripts)."""
+    from cases import Cov, Grid, categories, scattered
+
+    g = Grid(166, 141, 225)
+    data, mask = scattered(g.shape, 0.20, seed, law="normal")
+    cat, marg = categories(data, mask, quantiles=(0.8,), k=2)
+    params = [dict(cov=Cov(1, (10, 10, 10)), radii=(20, 20, 20), max_n=12, marginal=marg[k]) for k in range(2)]
+    return g, cat, mask, marg, params
+
+
+# 4,213,080 simulated cells in 1 min 30 s on one Core Duo 1.8 GHz core
+# (doc/hpgl.sourceforge.net/index.html:128)
+SIS_PUBLISHED_CELLS_PER_S = 0.8 * 5266350 / 90.0
+
+
+def sis_bench(torch, lib, abi, hpgl_wrap, steps=2, warmup=1, cpu=True, seed=3241347):
+    """Sequential indicator simulation of one realisation on the published
+    SIS workload, inputs resident in HBM; simulated cells/s.  The CPU leg
+    runs the oracle on the same workload and seed (one thread) and checks
+    the GPU realisation against it."""
+    g, cat, mask, marg, params = make_sis_workload()
+    _, ikp = _params(abi, dict(cov=params[0]["cov"], radii=(1, 1, 1), max_n=1, mean=0.0), params)
+    dev = torch.device("cuda:0")
+    d_cat = torch.from_numpy(cat.ravel(order="F").copy()).to(dev)
+    d_mask = torch.from_numpy(mask.ravel(order="F").copy()).to(dev)
+    d_out = torch.empty_like(d_cat)
+    d_out_mask = torch.empty_like(d_mask)
+    shape = abi._HPGL_SHAPE(data=(C.c_int * 3)(*g.shape), strides=(C.c_int * 3)(1, g.x, g.x * g.y))
+    stream = torch.cuda.current_stream(dev)
+    sp = C.c_void_p(stream.cuda_stream)
+    seeds = (C.c_int * 1)(seed)
+    n_sim = int(np.sum(mask == 0))
+
+    def run():
+        lib.hpgl_amd_sis_simulation_batch_dev(d_cat.data_ptr(), d_mask.data_ptr(), C.byref(shape), ikp, 2, None, 0,
+                                              None, seeds, 1, d_out.data_ptr(), d_out_mask.data_ptr(), sp)
+        hpgl_wrap.check_last_error()
+        return hpgl_wrap.last_stats()
+
+    for _ in range(warmup):
+        run()
+    torch.cuda.synchronize()
+    acc = {}
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = run()
+        for k, v in st.items():
+            acc[k] = acc.get(k, 0) + v
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / steps
+    res = {"workload": "median SIS, 166x141x225 grid, 20% informed, exponential ranges 10/10/10, radii 20/20/20, "
+                       "max_neighbours 12, one realisation",
+           "simulated_cells": n_sim, "value": n_sim / wall, "unit": "simulated cells/s", "ms_per_realisation": wall * 1e3,
+           "ms_template_scan": acc.get("ms_k_template", 0.0) / steps, "ms_solve": acc.get("ms_solve", 0.0) / steps,
+           "ms_device_total": acc.get("ms_device_total", 0.0) / steps,
+           "published_reference": {"value": SIS_PUBLISHED_CELLS_PER_S, "unit": "simulated cells/s",
+                                   "hardware": "1 core, Intel Core Duo 1.8 GHz",
+                                   "source": "doc/hpgl.sourceforge.net/index.html:128"}}
+    res["vs_published"] = res["value"] / SIS_PUBLISHED_CELLS_PER_S
+    if cpu:
+        import oracle_binding as O
+
+        gpu_cat = d_out.cpu().numpy().reshape(g.shape, order="F")
+        hpgl_wrap.set_precision("fp64")
+        run()
+        torch.cuda.synchronize()
+        hpgl_wrap.set_precision("mixed")
+        gpu64 = d_out.cpu().numpy().reshape(g.shape, order="F")
+        data = [{"cov_model": p["cov"], "radiuses": p["radii"], "max_neighbours": p["max_n"]} for p in params]
+        O.set_threads(1)
+        t0 = time.perf_counter()
+        od, om = O.sis_simulation((cat, mask, 2), g, data, seed, marg)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": n_sim / dt, "unit": "simulated cells/s", "cores": 1, "kind": "port",
+                               "sample": "the whole workload (one realisation, same seed)"}
+        res["fp64_equals_oracle"] = bool(np.array_equal(gpu64, od))
+        res["mixed_agreement_vs_oracle"] = float(np.mean(gpu_cat == od))
+    return res
+
+
 def make_serving_trace(n_jobs=60, rate=20.0, seed=1005):
     """Poisson request stream of mixed OK / SK / IK / LVM jobs: grid axes
     uniform in [32, 256] x [32, 256] x [32, 128], informed fraction
@@ -264,6 +350,7 @@ def main():
     ap.add_argument("--no-fp64-compare", action="store_true")
     ap.add_argument("--no-serving", action="store_true", help="skip the request-stream (p50/p99) measurement")
     ap.add_argument("--no-io", action="store_true", help="skip the property-file write/read measurement")
+    ap.add_argument("--no-sis", action="store_true", help="skip the sequential indicator simulation measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (x * 1 -> y, 3 times), the reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
@@ -434,6 +521,8 @@ def main():
                                "sk_rms_rel_err_vs_fp64": float(np.sqrt(np.mean(rel ** 2))),
                                "ik_category_agreement": float(np.mean(mixed_ik == d_iout.cpu().numpy()))}
 
+    if not args.no_sis:
+        out["sis"] = sis_bench(torch, lib, abi, hpgl_wrap, cpu=not args.no_cpu_baseline)
     if not args.no_io:
         import hpgl_amd.geo_bsd as geo_api
 

@@ -805,6 +805,29 @@ API void hpgl_amd_sis_simulation_batch(const hpgl_ind_masked_array_t *data, cons
     GUARD_END
 }
 
+API void hpgl_amd_sis_simulation_batch_dev(const unsigned char *d_in, const unsigned char *d_in_mask,
+                                           const hpgl_shape_t *shape, const hpgl_ik_params_t *params,
+                                           int indicator_count, const float *d_lvm, int use_correlograms,
+                                           const unsigned char *d_sim_mask, const int *seeds, int R,
+                                           unsigned char *d_out, unsigned char *d_out_mask, void *stream) {
+    GUARD_BEGIN
+    if (R < 1 || !seeds) throw std::runtime_error("at least one seed is needed");
+    SisJob j = sis_job(shape, params, indicator_count);
+    const long long N = volume(shape);
+    if (d_lvm) {
+        for (int k = 0; k < indicator_count; ++k) j.d_lvm.push_back(d_lvm + (size_t)k * N);
+        j.correlogram = use_correlograms != 0;
+    }
+    for (int r = 0; r < R; ++r) j.seeds.push_back((long)seeds[r]);
+    j.d_in = d_in;
+    j.d_in_mask = d_in_mask;
+    j.d_sim_mask = d_sim_mask;
+    j.d_out = d_out;
+    j.d_out_mask = d_out_mask;
+    run_sis(j, (hipStream_t)stream, g_precision, t_stats);
+    GUARD_END
+}
+
 // Failure of a library component outside this file (the request loop):
 // sets the calling thread's last-error state as a failed entry point does.
 void hpgl_amd::report_error(const char *msg) {

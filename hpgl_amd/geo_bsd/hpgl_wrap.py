@@ -120,6 +120,11 @@ _hpgl_so.hpgl_amd_sis_simulation_batch.argtypes = [
     _P(_HPGL_IND_MASKED_ARRAY), _P(_HPGL_IK_PARAMS), C.c_int, _P(_HPGL_FLOAT_ARRAY), C.c_int,
     _P(_HPGL_UBYTE_ARRAY), _P(C.c_int), C.c_int, C.c_void_p, C.c_void_p]
 
+_hpgl_so.hpgl_amd_sis_simulation_batch_dev.restype = None
+_hpgl_so.hpgl_amd_sis_simulation_batch_dev.argtypes = [
+    C.c_void_p, C.c_void_p, _P(_HPGL_SHAPE), _P(_HPGL_IK_PARAMS), C.c_int, C.c_void_p, C.c_int, C.c_void_p,
+    _P(C.c_int), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+
 _hpgl_so.hpgl_amd_server_create.restype = C.c_void_p
 _hpgl_so.hpgl_amd_server_create.argtypes = [C.c_int, C.c_longlong]
 _hpgl_so.hpgl_amd_server_submit.restype = C.c_longlong

@@ -364,6 +364,16 @@ void hpgl_amd_sis_simulation_batch(const struct hpgl_ind_masked_array_t *data,
                                    const int *seeds, int R,
                                    unsigned char *out, unsigned char *out_mask);
 
+/* Device-pointer variant of hpgl_amd_sis_simulation_batch: d_lvm holds
+ * indicator_count consecutive per-cell mean arrays (or NULL); seeds stay on
+ * the host.  d_out and d_out_mask hold R*N entries. */
+void hpgl_amd_sis_simulation_batch_dev(const unsigned char *d_in, const unsigned char *d_in_mask,
+                                       const struct hpgl_shape_t *shape,
+                                       const struct hpgl_ik_params_t *params, int indicator_count,
+                                       const float *d_lvm, int use_correlograms,
+                                       const unsigned char *d_sim_mask, const int *seeds, int R,
+                                       unsigned char *d_out, unsigned char *d_out_mask, void *stream);
+
 /* ---- Request loop with continuous batching (no reference counterpart:
  * the reference runs one blocking call at a time).  Jobs wait in arrival
  * order; a free worker slot (own HIP stream) starts the first waiting job
