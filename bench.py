@@ -351,6 +351,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the request-stream (p50/p99) measurement")
     ap.add_argument("--no-io", action="store_true", help="skip the property-file write/read measurement")
     ap.add_argument("--no-sis", action="store_true", help="skip the sequential indicator simulation measurement")
+    ap.add_argument("--sis-only", action="store_true",
+                    help="run only the sequential indicator simulation measurement (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (x * 1 -> y, 3 times), the reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
@@ -382,6 +384,11 @@ def main():
                                                 vp, vp, vp]
     lib.hpgl_amd_indicator_kriging_dev.argtypes = [vp, vp, C.POINTER(abi._HPGL_SHAPE),
                                                    C.POINTER(abi._HPGL_IK_PARAMS), C.c_int, vp, vp, vp]
+
+    if args.sis_only:
+        print(json.dumps({"sis": sis_bench(torch, lib, abi, hpgl_wrap, steps=args.steps, warmup=args.warmup,
+                                           cpu=not args.no_cpu_baseline)}), flush=True)
+        return
 
     g, data, mask, cat, marg, sk, ik = make_cfg2()
     n_unk = int(np.sum(mask == 0))

@@ -985,6 +985,50 @@ struct SimBatch {
     int max_T = 0;
 };
 
+// Wavefronts of an ordered sweep, all realisations together: enough lanes
+// to cover the ready cells of the path (a dependency level of a 4 M-cell
+// path holds ~100 k cells) while keeping the flag polls few (more
+// wavefronts were measured slower: each polls the L2).
+constexpr int kSweepWaves = 1024;
+
+// A second stream of default priority on which the draws are generated
+// while the searches and solves run on the job's stream.
+struct SideStream {
+    hipStream_t s2 = nullptr;
+    hipEvent_t ready = nullptr, done = nullptr;
+    SideStream() {
+        hip_check(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking), "hipStreamCreate");
+        hip_check(hipEventCreateWithFlags(&ready, hipEventDisableTiming), "hipEventCreate");
+        hip_check(hipEventCreateWithFlags(&done, hipEventDisableTiming), "hipEventCreate");
+    }
+    ~SideStream() {
+        (void)hipStreamSynchronize(s2);
+        (void)hipEventDestroy(ready);
+        (void)hipEventDestroy(done);
+        (void)hipStreamDestroy(s2);
+    }
+    SideStream(const SideStream &) = delete;
+    SideStream &operator=(const SideStream &) = delete;
+};
+
+// mt11213b draws of the batch, one per simulated cell in path order, on the
+// side stream after the work queued so far on s; s waits for them at
+// sim_draws_wait.
+void sim_draws(const SimPaths &p, const SimBatch &b, int r0, DBuf<uint32_t> &draws, DBuf<long long> &d_counts,
+               SideStream &side, hipStream_t s) {
+    const int rc = (int)b.h_T.size();
+    draws.alloc((size_t)rc * std::max(b.max_T, 1));
+    std::vector<long long> h_counts(b.h_T.begin(), b.h_T.end());
+    upload(d_counts, h_counts, s);
+    hip_check(hipEventRecord(side.ready, s), "event");
+    hip_check(hipStreamWaitEvent(side.s2, side.ready, 0), "wait event");
+    launch_mt11213b(p.d_mt_seed.p + r0, d_counts.p, std::max(b.max_T, 1), rc, draws.p, side.s2);
+    hip_check(hipGetLastError(), "mt11213b");
+    hip_check(hipEventRecord(side.done, side.s2), "event");
+}
+
+void sim_draws_wait(SideStream &side, hipStream_t s) { hip_check(hipStreamWaitEvent(s, side.done, 0), "wait event"); }
+
 void sim_batch(long long N, const SimPaths &p, int r0, int rc, const uint8_t *hard, const uint8_t *sim_mask,
                SimBatch &b, hipStream_t s) {
     DBuf<int> path((size_t)rc * N), rank((size_t)rc * N);
@@ -1078,6 +1122,7 @@ void run_sgs(const SgsJob &job, hipStream_t s, int precision, const Dump *dump, 
     hip_check(hipEventRecord(ev.e[0], s), "event");
     SimPaths paths;
     sim_paths(N, job.seeds, paths, s);
+    SideStream side;
 
     // normal-score transform of the conditioning data and of the LVM mean
     DBuf<float> cv, cp, data((size_t)N), lvm;
@@ -1108,6 +1153,9 @@ void run_sgs(const SgsJob &job, hipStream_t s, int precision, const Dump *dump, 
         sim_batch(N, paths, r0, rc, job.d_in_mask, job.d_sim_mask, b, s);
         const long long tot = b.tot;
         st.nodes_target += tot;
+        DBuf<uint32_t> draws;
+        DBuf<long long> d_counts;
+        sim_draws(paths, b, r0, draws, d_counts, side, s);
 
         // neighbours of every target among the cells informed before it
         DBuf<int> nbr((size_t)std::max<long long>(tot, 1) * max_n), cnt((size_t)std::max<long long>(tot, 1));
@@ -1151,14 +1199,7 @@ void run_sgs(const SgsJob &job, hipStream_t s, int precision, const Dump *dump, 
             st.ms_solve += ev.ms(3, 4);
         }
 
-        // draws: one per simulated cell, in path order
         const int max_T = b.max_T;
-        DBuf<uint32_t> draws((size_t)rc * std::max(max_T, 1));
-        DBuf<long long> d_counts;
-        std::vector<long long> h_counts(b.h_T.begin(), b.h_T.end());
-        upload(d_counts, h_counts, s);
-        launch_mt11213b(paths.d_mt_seed.p + r0, d_counts.p, std::max(max_T, 1), rc, draws.p, s);
-
         // values: conditioning data everywhere, then the ordered sweep
         float *vals = job.d_out + (size_t)r0 * N;
         for (int r = 0; r < rc; ++r)
@@ -1173,6 +1214,7 @@ void run_sgs(const SgsJob &job, hipStream_t s, int precision, const Dump *dump, 
         DBuf<long long> d_off;
         upload(d_T, b.h_T, s);
         upload(d_off, b.h_off, s);
+        sim_draws_wait(side, s);
         if (tot > 0) {
             SgsSweepArgs a;
             a.N = N;
@@ -1194,7 +1236,7 @@ void run_sgs(const SgsJob &job, hipStream_t s, int precision, const Dump *dump, 
             a.ticket = ticket.p;
             a.error = err.p;
             a.spin_limit = 1ll << 26;
-            const int waves = (int)std::max<long long>(1, std::min<long long>((max_T + 63) / 64, 8192 / rc));
+            const int waves = (int)std::max<long long>(1, std::min<long long>((max_T + 63) / 64, kSweepWaves / rc));
             launch_sgs_sweep(a, rc, waves, s);
             hip_check(hipGetLastError(), "sweep");
         }
@@ -1255,6 +1297,7 @@ void run_sis(const SisJob &job, hipStream_t s, int precision, Stats &st) {
     hip_check(hipEventRecord(ev.e[0], s), "event");
     SimPaths paths;
     sim_paths(N, job.seeds, paths, s);
+    SideStream side;
 
     std::vector<std::shared_ptr<CovAssets>> as(kriged);
     size_t per_cell = 21;
@@ -1275,6 +1318,10 @@ void run_sis(const SisJob &job, hipStream_t s, int precision, Stats &st) {
         const long long tot = b.tot;
         const size_t tot1 = (size_t)std::max<long long>(tot, 1);
         st.nodes_target += tot;
+
+        DBuf<uint32_t> draws;
+        DBuf<long long> d_counts;
+        sim_draws(paths, b, r0, draws, d_counts, side, s);
 
         std::vector<std::unique_ptr<SisCategoryBuffers>> cb(kriged);
         std::vector<SisCategory> h_cats(kriged);
@@ -1337,14 +1384,7 @@ void run_sis(const SisJob &job, hipStream_t s, int precision, Stats &st) {
         DBuf<SisCategory> d_cats;
         upload(d_cats, h_cats, s);
 
-        // draws: one per simulated cell, in path order
         const int max_T = b.max_T;
-        DBuf<uint32_t> draws((size_t)rc * std::max(max_T, 1));
-        DBuf<long long> d_counts;
-        std::vector<long long> h_counts(b.h_T.begin(), b.h_T.end());
-        upload(d_counts, h_counts, s);
-        launch_mt11213b(paths.d_mt_seed.p + r0, d_counts.p, std::max(max_T, 1), rc, draws.p, s);
-
         // categories: conditioning data everywhere, then the ordered sweep
         uint8_t *vals = job.d_out + (size_t)r0 * N;
         for (int r = 0; r < rc; ++r)
@@ -1358,6 +1398,7 @@ void run_sis(const SisJob &job, hipStream_t s, int precision, Stats &st) {
         DBuf<long long> d_off;
         upload(d_T, b.h_T, s);
         upload(d_off, b.h_off, s);
+        sim_draws_wait(side, s);
         if (tot > 0) {
             SisSweepArgs a;
             a.N = N;
@@ -1374,7 +1415,7 @@ void run_sis(const SisJob &job, hipStream_t s, int precision, Stats &st) {
             a.ticket = ticket.p;
             a.error = err.p;
             a.spin_limit = 1ll << 26;
-            const int waves = (int)std::max<long long>(1, std::min<long long>((max_T + 63) / 64, 8192 / rc));
+            const int waves = (int)std::max<long long>(1, std::min<long long>((max_T + 63) / 64, kSweepWaves / rc));
             launch_sis_sweep(a, rc, waves, s);
             hip_check(hipGetLastError(), "sweep");
         }

@@ -205,88 +205,6 @@ __global__ void k_cdf_back(float *v, const uint8_t *mask, long long n, const flo
     v[i] = np_inverse(cv, cp, m, (float)P);
 }
 
-__global__ __launch_bounds__(64) void k_sgs_sweep(SgsSweepArgs a) {
-    const int r = blockIdx.y;
-    const int lane = threadIdx.x;
-    const int T = a.T[r];
-    const long long off = a.t_off[r];
-    const int *targets = a.targets + off;
-    const int *nbr = a.nbr + (size_t)off * a.max_n;
-    const int *cnt = a.cnt + off;
-    const double *wts = a.w + (size_t)off * a.max_n;
-    const uint8_t *status = a.status + off;
-    const uint64_t *simbits = a.simbits + off;
-    const uint32_t *draws = a.draws + (size_t)r * a.draw_stride;
-    float *val = a.values + (size_t)r * a.N;
-    int *done = a.done + (size_t)r * a.N;
-    unsigned int *ticket = a.ticket + r;
-    while (true) {
-        if (__hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
-        int base = 0;
-        if (lane == 0) base = (int)atomicAdd(ticket, 64u);
-        base = __shfl(base, 0);
-        if (base >= T) break;
-        const int k = base + lane;
-        bool pending = k < T;
-        // simulated neighbours not yet seen done (conditioning data never wait)
-        uint64_t waitset = pending ? simbits[k] : 0ull;
-        const int *row = nbr + (size_t)(pending ? k : 0) * a.max_n;
-        long long spins = 0;
-        while (__any(pending)) {
-            if (pending) {
-                // poll up to eight outstanding flags per pass, loads in flight together
-                int idx[8];
-                int m = 0;
-                uint64_t ws = waitset;
-#pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    idx[q] = ws ? __ffsll((unsigned long long)ws) - 1 : -1;
-                    if (ws) ws &= ws - 1;
-                }
-                int fl[8];
-#pragma unroll
-                for (int q = 0; q < 8; ++q)
-                    fl[q] = idx[q] >= 0 ? __hip_atomic_load(done + row[idx[q]], __ATOMIC_RELAXED,
-                                                            __HIP_MEMORY_SCOPE_AGENT)
-                                        : 0;
-#pragma unroll
-                for (int q = 0; q < 8; ++q)
-                    if (idx[q] >= 0 && fl[q] != 0) waitset &= ~(1ull << idx[q]);
-                (void)m;
-                if (waitset == 0) {
-                    // every neighbour's value is published: order the reads after the flags
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-                    const int cell = targets[k];
-                    const int n = cnt[k];
-                    const bool ok = status[k] == ST_OK;
-                    const double v0 = a.var[off + k];
-                    const float m0 = a.lvm ? a.lvm[cell] : 0.0f;
-                    // combine (combiner.h:10-24) in neighbour order
-                    float mean = m0;
-                    if (ok) {
-                        for (int i = 0; i < n; ++i) {
-                            const int nb = row[i];
-                            const float mi = a.lvm ? a.lvm[nb] : 0.0f;
-                            mean += (float)(wts[(size_t)k * a.max_n + i] * (double)(val[nb] - mi));
-                        }
-                    }
-                    // u = x / (2^32 - 1) (bs_random_generator.cpp:134-137)
-                    const double u = (double)draws[k] / 4294967295.0;
-                    const double value = ok ? gauss_inverse(u, (double)mean, v0) : gauss_inverse(u, 0.0, 1.0);
-                    val[cell] = (float)value;
-                    __hip_atomic_store(done + cell, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-                    pending = false;
-                }
-            }
-            if (++spins > a.spin_limit) {
-                if (pending) atomicOr(a.error, 1u);
-                break;
-            }
-            if (__any(pending)) __builtin_amdgcn_s_sleep(1);
-        }
-    }
-}
-
 // Sequential indicator simulation (sequential_indicator_simulation.cpp:67-121)
 // over the same ticketed sweep: a target waits for the simulated neighbours
 // of each kriged category in turn (a flag once set stays set), then forms
@@ -345,69 +263,181 @@ __device__ inline void sis_value(const SisSweepArgs &a, long long gk, int k_draw
     val[cell] = (uint8_t)pick;
 }
 
+__device__ inline int lanes_below(uint64_t m) {
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// The ordered sweep shared by both simulations.  Each lane holds one path
+// position at a time and claims the next free one (one atomic per
+// wavefront for all lanes that need work) as soon as its cell is written,
+// so no lane idles while others of its wavefront wait.  A position waits
+// only for positions claimed before it, all held by running lanes, and the
+// smallest unfinished position has nothing left to wait for, so the sweep
+// always progresses.  The wait is bounded by spin_limit passes without
+// progress in the wavefront; past it the error flag is set and every
+// wavefront leaves.
+//
+// Job supplies, per lane: start(k) (the waitset of simulated neighbours of
+// path position k and its neighbour row), next_set() (moves to the next
+// neighbour list once the current one is done; false when there is none)
+// and write(k) (the value of position k; returns its cell).
+template <typename Job>
+__device__ inline void ordered_sweep(Job &job, int T, unsigned int *ticket, int *done, unsigned int *error,
+                                     long long spin_limit) {
+    const int lane = threadIdx.x;
+    bool pending = false;
+    bool drained = false;  // the ticket has passed T: no more claims
+    int k = 0;
+    long long spins = 0;
+    while (true) {
+        if (!drained) {
+            const uint64_t need = __ballot(!pending);
+            if (need) {
+                const int leader = __ffsll((unsigned long long)need) - 1;
+                int base = 0;
+                if (lane == leader) base = (int)atomicAdd(ticket, (unsigned)__popcll(need));
+                base = __shfl(base, leader);
+                if (base + __popcll(need) >= T) drained = true;
+                if (!pending) {
+                    const int kn = base + lanes_below(need);
+                    if (kn < T) {
+                        k = kn;
+                        pending = true;
+                        job.start(k);
+                    }
+                }
+            }
+        }
+        if (!__any(pending)) break;
+        bool progress = false;
+        if (pending) {
+            while (job.waitset == 0 && job.next_set()) {
+            }
+            // poll up to eight outstanding flags per pass, loads in flight together
+            int idx[8];
+            uint64_t ws = job.waitset;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                idx[q] = ws ? __ffsll((unsigned long long)ws) - 1 : -1;
+                if (ws) ws &= ws - 1;
+            }
+            int fl[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                fl[q] = idx[q] >= 0
+                            ? __hip_atomic_load(done + job.row[idx[q]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                            : 0;
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                if (idx[q] >= 0 && fl[q] != 0) job.waitset &= ~(1ull << idx[q]);
+            if (job.waitset == 0 && !job.next_set()) {
+                // every neighbour value is published: order the reads after the flags
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                const int cell = job.write(k);
+                __hip_atomic_store(done + cell, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                pending = false;
+                progress = true;
+            }
+        }
+        if (__any(progress)) {
+            spins = 0;
+        } else {
+            if (++spins > spin_limit) {
+                atomicOr(error, 1u);
+                break;
+            }
+            if (__hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+}
+
+// Gaussian simulation: the value mean + sqrt(var) Phi^-1(u) of combine
+// (combiner.h:10-24) in neighbour order, N(0, 1) when the system failed.
+struct SgsJobLane {
+    const SgsSweepArgs &a;
+    long long off;
+    const uint32_t *draws;
+    float *val;
+    uint64_t waitset = 0;
+    const int *row = nullptr;
+    long long gk = 0;
+    __device__ SgsJobLane(const SgsSweepArgs &args, long long o, const uint32_t *d, float *v)
+        : a(args), off(o), draws(d), val(v) {}
+    __device__ void start(int k) {
+        gk = off + k;
+        waitset = a.simbits[gk];
+        row = a.nbr + (size_t)gk * a.max_n;
+    }
+    __device__ bool next_set() { return false; }
+    __device__ int write(int k) {
+        const int cell = a.targets[gk];
+        const int n = a.cnt[gk];
+        const bool ok = a.status[gk] == ST_OK;
+        const double v0 = a.var[gk];
+        const float m0 = a.lvm ? a.lvm[cell] : 0.0f;
+        float mean = m0;
+        if (ok) {
+            const double *w = a.w + (size_t)gk * a.max_n;
+            for (int i = 0; i < n; ++i) {
+                const int nb = row[i];
+                const float mi = a.lvm ? a.lvm[nb] : 0.0f;
+                mean += (float)(w[i] * (double)(val[nb] - mi));
+            }
+        }
+        // u = x / (2^32 - 1) (bs_random_generator.cpp:25-28)
+        const double u = (double)draws[k] / 4294967295.0;
+        const double value = ok ? gauss_inverse(u, (double)mean, v0) : gauss_inverse(u, 0.0, 1.0);
+        val[cell] = (float)value;
+        return cell;
+    }
+};
+
+__global__ __launch_bounds__(64) void k_sgs_sweep(SgsSweepArgs a) {
+    const int r = blockIdx.y;
+    SgsJobLane job(a, a.t_off[r], a.draws + (size_t)r * a.draw_stride, a.values + (size_t)r * a.N);
+    ordered_sweep(job, a.T[r], a.ticket + r, a.done + (size_t)r * a.N, a.error, a.spin_limit);
+}
+
+// Indicator simulation: waits for the simulated neighbours of each kriged
+// category in turn (a flag once set stays set), then sis_value.
+struct SisJobLane {
+    const SisSweepArgs &a;
+    long long off;
+    const uint32_t *draws;
+    uint8_t *val;
+    float *probs;
+    uint64_t waitset = 0;
+    const int *row = nullptr;
+    long long gk = 0;
+    int wc = 0;
+    __device__ SisJobLane(const SisSweepArgs &args, long long o, const uint32_t *d, uint8_t *v, float *p)
+        : a(args), off(o), draws(d), val(v), probs(p) {}
+    __device__ void start(int k) {
+        gk = off + k;
+        wc = 0;
+        waitset = a.cats[0].simbits[gk];
+        row = a.cats[0].nbr + (size_t)gk * a.cats[0].max_n;
+    }
+    __device__ bool next_set() {
+        if (wc + 1 >= a.kriged) return false;
+        ++wc;
+        waitset = a.cats[wc].simbits[gk];
+        row = a.cats[wc].nbr + (size_t)gk * a.cats[wc].max_n;
+        return true;
+    }
+    __device__ int write(int k) {
+        const int cell = a.targets[gk];
+        sis_value(a, gk, k, draws, val, probs, threadIdx.x, cell);
+        return cell;
+    }
+};
+
 __global__ __launch_bounds__(64) void k_sis_sweep(SisSweepArgs a) {
     extern __shared__ float sis_probs[];
     const int r = blockIdx.y;
-    const int lane = threadIdx.x;
-    const int T = a.T[r];
-    const long long off = a.t_off[r];
-    const uint32_t *draws = a.draws + (size_t)r * a.draw_stride;
-    uint8_t *val = a.values + (size_t)r * a.N;
-    int *done = a.done + (size_t)r * a.N;
-    unsigned int *ticket = a.ticket + r;
-    while (true) {
-        if (__hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
-        int base = 0;
-        if (lane == 0) base = (int)atomicAdd(ticket, 64u);
-        base = __shfl(base, 0);
-        if (base >= T) break;
-        const int k = base + lane;
-        bool pending = k < T;
-        const long long gk = off + (pending ? k : 0);
-        // category whose simulated neighbours are being waited for
-        int wc = 0;
-        uint64_t waitset = pending ? a.cats[0].simbits[gk] : 0ull;
-        const int *row = a.cats[0].nbr + (size_t)gk * a.cats[0].max_n;
-        long long spins = 0;
-        while (__any(pending)) {
-            if (pending) {
-                while (waitset == 0 && wc + 1 < a.kriged) {
-                    ++wc;
-                    waitset = a.cats[wc].simbits[gk];
-                    row = a.cats[wc].nbr + (size_t)gk * a.cats[wc].max_n;
-                }
-                int idx[8];
-                uint64_t ws = waitset;
-#pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    idx[q] = ws ? __ffsll((unsigned long long)ws) - 1 : -1;
-                    if (ws) ws &= ws - 1;
-                }
-                int fl[8];
-#pragma unroll
-                for (int q = 0; q < 8; ++q)
-                    fl[q] = idx[q] >= 0 ? __hip_atomic_load(done + row[idx[q]], __ATOMIC_RELAXED,
-                                                            __HIP_MEMORY_SCOPE_AGENT)
-                                        : 0;
-#pragma unroll
-                for (int q = 0; q < 8; ++q)
-                    if (idx[q] >= 0 && fl[q] != 0) waitset &= ~(1ull << idx[q]);
-                if (waitset == 0 && wc + 1 >= a.kriged) {
-                    // every neighbour of every category is published
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-                    const int cell = a.targets[gk];
-                    sis_value(a, gk, k, draws, val, sis_probs, lane, cell);
-                    __hip_atomic_store(done + cell, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-                    pending = false;
-                }
-            }
-            if (++spins > a.spin_limit) {
-                if (pending) atomicOr(a.error, 1u);
-                break;
-            }
-            if (__any(pending)) __builtin_amdgcn_s_sleep(1);
-        }
-    }
+    SisJobLane job(a, a.t_off[r], a.draws + (size_t)r * a.draw_stride, a.values + (size_t)r * a.N, sis_probs);
+    ordered_sweep(job, a.T[r], a.ticket + r, a.done + (size_t)r * a.N, a.error, a.spin_limit);
 }
 
 inline unsigned blocks(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
