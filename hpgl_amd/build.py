@@ -14,7 +14,7 @@ SRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "hpgl.so")
 
-SOURCES = ["model.cpp", "propio.cpp", "solve.hip", "solve_quad.hip", "search.hip", "cluster.hip", "engine.hip", "api.hip", "sgs.hip", "server.hip"]
+SOURCES = ["model.cpp", "propio.cpp", "solve.hip", "solve_quad.hip", "search.hip", "cluster.hip", "engine.hip", "api.hip", "sgs.hip", "server.hip", "cokriging.hip"]
 
 # -ffp-contract=off: every product and sum rounds separately, as in the
 # reference's FP64 code; kernels that want fused multiply-adds call fma()

@@ -828,6 +828,132 @@ API void hpgl_amd_sis_simulation_batch_dev(const unsigned char *d_in, const unsi
     GUARD_END
 }
 
+// ---- simple colocated cokriging ----
+
+static CokrigingJob cokriging_m1_job(const hpgl_shape_t *shape, const hpgl_cokriging_m1_params_t *p) {
+    CokrigingJob j;
+    for (int i = 0; i < 3; ++i) {
+        j.grid[i] = shape->m_data[i];
+        j.radii[i] = p->m_radiuses[i];
+    }
+    j.primary = cov_of(p->m_covariance_type, p->m_ranges, p->m_angles, p->m_sill, p->m_nugget);
+    j.mark2 = false;
+    j.secondary_variance = p->m_secondary_variance;
+    j.correlation = p->m_correlation_coef;
+    j.primary_mean = (float)p->m_primary_mean;
+    j.secondary_mean = (float)p->m_secondary_mean;
+    j.max_n = p->m_max_neighbours;
+    return j;
+}
+
+static CokrigingJob cokriging_m2_job(const hpgl_shape_t *shape, const hpgl_cokriging_m2_params_t *p) {
+    CokrigingJob j;
+    for (int i = 0; i < 3; ++i) {
+        j.grid[i] = shape->m_data[i];
+        j.radii[i] = p->m_radiuses[i];
+    }
+    const hpgl_cov_params_t &a = p->m_primary_cov_params, &b = p->m_secondary_cov_params;
+    j.primary = cov_of(a.m_covariance_type, a.m_ranges, a.m_angles, a.m_sill, a.m_nugget);
+    j.secondary = cov_of(b.m_covariance_type, b.m_ranges, b.m_angles, b.m_sill, b.m_nugget);
+    j.mark2 = true;
+    j.correlation = p->m_correlation_coef;
+    j.primary_mean = (float)p->m_primary_mean;
+    j.secondary_mean = (float)p->m_secondary_mean;
+    j.max_n = p->m_max_neighbours;
+    return j;
+}
+
+// Sizes as api.cpp:595-611 checks them.
+static void cokriging_sizes(const hpgl_cont_masked_array_t *in, const hpgl_cont_masked_array_t *sec,
+                            const hpgl_cont_masked_array_t *out) {
+    const long long n = volume(&in->m_shape), n2 = volume(&sec->m_shape), n3 = volume(&out->m_shape);
+    if (n != n2)
+        throw std::runtime_error("hpgl_simple_cokriging: Size of secondary data (" + std::to_string(n2) +
+                                 ") is different from size of primary data (" + std::to_string(n) + ")");
+    if (n != n3)
+        throw std::runtime_error("hpgl_simple_cokriging: Size of output data (" + std::to_string(n3) +
+                                 ") is different from size of primary data (" + std::to_string(n) + ")");
+}
+
+static void cokriging_host(CokrigingJob &j, const hpgl_cont_masked_array_t *in, const hpgl_cont_masked_array_t *sec,
+                           hpgl_cont_masked_array_t *out) {
+    const long long N = volume(&in->m_shape);
+    hipStream_t s = thread_stream();
+    DevArray<float> di(N), ds(N), dout(N);
+    DevArray<unsigned char> dm(N), dsm(N), dom(N);
+    di.put(in->m_data, s);
+    dm.put(in->m_mask, s);
+    ds.put(sec->m_data, s);
+    dsm.put(sec->m_mask, s);
+    dout.put(out->m_data, s);
+    dom.put(out->m_mask, s);
+    j.d_in = di.p;
+    j.d_in_mask = dm.p;
+    j.d_sec = ds.p;
+    j.d_sec_mask = dsm.p;
+    j.d_out = dout.p;
+    j.d_out_mask = dom.p;
+    Dump d;
+    progress("cokriging", 0);
+    run_cokriging(j, s, take_dump(d), t_stats);
+    progress("cokriging", 100);
+    dout.get(out->m_data, s);
+    dom.get(out->m_mask, s);
+    hip_check(hipStreamSynchronize(s), "sync");
+}
+
+API void hpgl_simple_cokriging_mark1(hpgl_cont_masked_array_t *input_data, hpgl_cont_masked_array_t *secondary_data,
+                                     hpgl_cokriging_m1_params_t *params, hpgl_cont_masked_array_t *output_data) {
+    GUARD_BEGIN
+    cokriging_sizes(input_data, secondary_data, output_data);
+    CokrigingJob j = cokriging_m1_job(&input_data->m_shape, params);
+    cokriging_host(j, input_data, secondary_data, output_data);
+    GUARD_END
+}
+
+API void hpgl_simple_cokriging_mark2(hpgl_cont_masked_array_t *primary_data, hpgl_cont_masked_array_t *secondary_data,
+                                     hpgl_cokriging_m2_params_t *params, hpgl_cont_masked_array_t *output_data) {
+    GUARD_BEGIN
+    cokriging_sizes(primary_data, secondary_data, output_data);
+    CokrigingJob j = cokriging_m2_job(&primary_data->m_shape, params);
+    cokriging_host(j, primary_data, secondary_data, output_data);
+    GUARD_END
+}
+
+API void hpgl_amd_simple_cokriging_mark1_dev(const float *d_in, const unsigned char *d_in_mask, const float *d_sec,
+                                             const unsigned char *d_sec_mask, const hpgl_shape_t *shape,
+                                             const hpgl_cokriging_m1_params_t *params, float *d_out,
+                                             unsigned char *d_out_mask, void *stream) {
+    GUARD_BEGIN
+    CokrigingJob j = cokriging_m1_job(shape, params);
+    j.d_in = d_in;
+    j.d_in_mask = d_in_mask;
+    j.d_sec = d_sec;
+    j.d_sec_mask = d_sec_mask;
+    j.d_out = d_out;
+    j.d_out_mask = d_out_mask;
+    Dump d;
+    run_cokriging(j, (hipStream_t)stream, take_dump(d), t_stats);
+    GUARD_END
+}
+
+API void hpgl_amd_simple_cokriging_mark2_dev(const float *d_in, const unsigned char *d_in_mask, const float *d_sec,
+                                             const unsigned char *d_sec_mask, const hpgl_shape_t *shape,
+                                             const hpgl_cokriging_m2_params_t *params, float *d_out,
+                                             unsigned char *d_out_mask, void *stream) {
+    GUARD_BEGIN
+    CokrigingJob j = cokriging_m2_job(shape, params);
+    j.d_in = d_in;
+    j.d_in_mask = d_in_mask;
+    j.d_sec = d_sec;
+    j.d_sec_mask = d_sec_mask;
+    j.d_out = d_out;
+    j.d_out_mask = d_out_mask;
+    Dump d;
+    run_cokriging(j, (hipStream_t)stream, take_dump(d), t_stats);
+    GUARD_END
+}
+
 // Failure of a library component outside this file (the request loop):
 // sets the calling thread's last-error state as a failed entry point does.
 void hpgl_amd::report_error(const char *msg) {

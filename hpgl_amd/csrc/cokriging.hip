@@ -1,0 +1,153 @@
+// cokriging.hip - simple colocated cokriging, Markov models I and II
+// (simple_cokriging_markI.cpp:19-305), one system per wavefront.
+//
+// The system of n neighbours and the colocated secondary value
+// (build_system, :19-55) is (n+1) x (n+1):
+//   A(i,j) = C(x_i, x_j), A(i,n) = A(n,i) = k C_x(x_0, x_i), A(n,n) = var2,
+//   b(i) = C(x_i, x_0), b(n) = k C_x(0),
+// with C the primary covariance, C_x the primary (model I) or secondary
+// (model II) covariance and k the cross-covariance factor.  It is solved by
+// the reference's gauss_solve (gauss_solver.cpp:5-65) in its operation
+// order: for each column the first unused row with a non-zero entry is the
+// pivot, the pivot row is divided by it and every unused row has it
+// subtracted; back substitution starts each unknown X[i] from B[i] (the
+// right-hand side of row i, not of the pivot row of column i: the two agree
+// whenever the pivots come in row order, as for covariance matrices) and
+// subtracts A[pivot row][j] X[j] for j from n down to i+1.  Rows live in LDS
+// (stride 66 doubles); lane i owns rows i and i + 64.  Contraction is off,
+// so with the same covariances the weights are the reference's bit for
+// bit.  combine (:82-104) then rounds to float after each double addition.
+#include "counters.h"
+#include "devcov.h"
+#include "kernels.h"
+
+#include <hip/hip_runtime.h>
+
+namespace hpgl_amd {
+
+namespace {
+
+constexpr int kCoRows = kMaxNeighbours + 1;
+
+struct CoLds {
+    double A[kCoRows][kCoRows + 1];
+    double B[kCoRows];
+    double X[kCoRows];
+    int order[kCoRows];
+    int used[kCoRows];
+    int xyz[3][kMaxNeighbours];
+};
+
+__device__ inline double co_cov(const DevCov &c, int dx, int dy, int dz) {
+    return c.table ? cov_tab(c, dx, dy, dz) : cov_eval(c, dx, dy, dz);
+}
+
+__global__ __launch_bounds__(64) void k_cokriging(CokrigingArgs a) {
+    __shared__ CoLds sh;
+    const int lane = threadIdx.x;
+    const int t = blockIdx.x;
+    const int n = min(a.cnt[t], a.max_n);
+    const int size = n + 1;
+    const int node = a.targets[t];
+    int cx, cy, cz;
+    cell_xyz(node, a.X, a.Y, cx, cy, cz);
+    int xi = 0, yi = 0, zi = 0, nb = -1;
+    if (lane < n) {
+        nb = a.nbr[(size_t)t * a.max_n + lane];
+        cell_xyz(nb, a.X, a.Y, xi, yi, zi);
+        sh.xyz[0][lane] = xi;
+        sh.xyz[1][lane] = yi;
+        sh.xyz[2][lane] = zi;
+    }
+    __syncthreads();
+
+    // build_system (simple_cokriging_markI.cpp:19-55)
+    if (lane < n) {
+        for (int j = 0; j < n; ++j)
+            sh.A[lane][j] = co_cov(a.cov, sh.xyz[0][j] - xi, sh.xyz[1][j] - yi, sh.xyz[2][j] - zi);
+        const double cr = a.coef * co_cov(a.cross, xi - cx, yi - cy, zi - cz);
+        sh.A[lane][n] = cr;
+        sh.A[n][lane] = cr;
+        sh.B[lane] = co_cov(a.cov, cx - xi, cy - yi, cz - zi);
+    }
+    if (lane == 0) {
+        sh.B[n] = a.cross0;
+        sh.A[n][n] = a.sec_var;
+    }
+    for (int r = lane; r < size; r += 64) sh.used[r] = 0;
+    __syncthreads();
+
+    // gauss_solve (gauss_solver.cpp:5-52)
+    bool ok = n > 0;
+    for (int i = 0; ok && i < size; ++i) {
+        const uint64_t lo = __ballot(lane < size && sh.used[lane] == 0 && sh.A[lane][i] != 0.0);
+        int row = lo ? __ffsll((unsigned long long)lo) - 1 : -1;
+        if (row < 0 && size > 64 && sh.used[64] == 0 && sh.A[64][i] != 0.0) row = 64;
+        if (row < 0) {
+            ok = false;  // the matrix is singular
+            break;
+        }
+        const double coef = sh.A[row][i];
+        __syncthreads();
+        if (lane == 0) {
+            sh.used[row] = 1;
+            sh.order[i] = row;
+            sh.B[row] = sh.B[row] / coef;
+        }
+        for (int j = i + lane; j < size; j += 64) sh.A[row][j] = sh.A[row][j] / coef;
+        __syncthreads();
+        for (int r = lane; r < size; r += 64) {
+            if (sh.used[r] != 0) continue;
+            const double c = sh.A[r][i];
+            for (int k = i; k < size; ++k) sh.A[r][k] = sh.A[r][k] - c * sh.A[row][k];
+            sh.B[r] = sh.B[r] - c * sh.B[row];
+        }
+        __syncthreads();
+    }
+
+    // back substitution (gauss_solver.cpp:54-62): X[i] = B[i] - sum over
+    // j = n .. i+1 of A[order[i]][j] X[j], in that order
+    if (ok) {
+        double acc0 = lane < size ? sh.B[lane] : 0.0;
+        double acc1 = lane + 64 < size ? sh.B[lane + 64] : 0.0;
+        const int row0 = lane < size ? sh.order[lane] : 0;
+        const int row1 = lane + 64 < size ? sh.order[lane + 64] : 0;
+        for (int j = size - 1; j >= 0; --j) {
+            if (lane == (j & 63)) sh.X[j] = j < 64 ? acc0 : acc1;
+            __syncthreads();
+            const double xj = sh.X[j];
+            if (lane < j) acc0 = acc0 - sh.A[row0][j] * xj;
+            if (lane + 64 < j) acc1 = acc1 - sh.A[row1][j] * xj;
+        }
+    }
+
+    wave_count(a.counters, 1, lane == 0 && ok);
+    wave_count(a.counters, 2, lane == 0 && n > 0 && !ok);
+    wave_count(a.counters, 3, lane == 0 && n == 0);
+    if (lane != 0) return;
+    // combine (simple_cokriging_markI.cpp:82-104); the failure value :235
+    const float pm = a.primary_mean, sm = a.secondary_mean;
+    const float sv = a.sec_mask[node] != 0 ? a.sec[node] : sm;
+    float result;
+    if (ok) {
+        result = pm;
+        for (int i = 0; i < n; ++i) {
+            const int c = a.nbr[(size_t)t * a.max_n + i];
+            result = (float)((double)result + (double)(a.values[c] - pm) * sh.X[i]);
+        }
+        result = (float)((double)result + sh.X[n] * (double)(sv - sm));
+    } else {
+        result = pm + sv - sm;
+    }
+    a.out[node] = result;
+    a.out_mask[node] = 1;
+}
+
+}  // namespace
+
+void launch_cokriging(const CokrigingArgs &a, hipStream_t s) {
+    if (a.T <= 0) return;
+    hipLaunchKernelGGL(k_cokriging, dim3(a.T), dim3(64), 0, s, a);
+}
+
+}  // namespace hpgl_amd

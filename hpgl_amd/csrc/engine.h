@@ -117,10 +117,34 @@ struct SisJob {
     uint8_t *d_out_mask;                // [R][N]
 };
 
+// Simple colocated cokriging (simple_cokriging_markI.cpp:185-305).  Model
+// I: the cross covariance is the primary model scaled by
+// rho sqrt(var2) / sqrt(C1(0)); model II: the secondary model scaled by
+// rho sqrt(C2(0) / C1(0)), var2 = C2(0).
+struct CokrigingJob {
+    int grid[3];
+    CovSpec primary;
+    bool mark2 = false;
+    CovSpec secondary;          // model II
+    double secondary_variance;  // model I
+    double correlation;
+    float primary_mean;         // mean_t (float) in the reference
+    float secondary_mean;
+    int radii[3];
+    int max_n;
+    const float *d_in;
+    const uint8_t *d_in_mask;
+    const float *d_sec;
+    const uint8_t *d_sec_mask;
+    float *d_out;
+    uint8_t *d_out_mask;
+};
+
 void run_cont(const ContJob &job, hipStream_t s, int precision, const Dump *dump, Stats &st);
 void run_ind(const IndJob &job, hipStream_t s, int precision, const Dump *dump, Stats &st);
 void run_sgs(const SgsJob &job, hipStream_t s, int precision, const Dump *dump, Stats &st);
 void run_sis(const SisJob &job, hipStream_t s, int precision, Stats &st);
+void run_cokriging(const CokrigingJob &job, hipStream_t s, const Dump *dump, Stats &st);
 
 // Device memory held by the library between calls (size-class cache).
 void *pool_alloc(size_t bytes);

@@ -691,6 +691,32 @@ static double device_calc_mean(const float *d_in, const uint8_t *d_mask, long lo
     return count == 0 ? 0.0 : sum / count;
 }
 
+// Analytic covariances (ordinary_kriging.cpp, simple_cokriging_markI.cpp)
+// as the solve kernels read them: from a table of the model over every
+// offset a neighbour pair can take (neighbours lie within 2r-1 of the
+// target per axis on the cell-binned path, r on the template path, so pairs
+// within 4r-2), built on first use; evaluated on the device when that table
+// would be too large.
+DevCov analytic_cov(CovAssets &as, const int radii[3], hipStream_t s, Stats &st) {
+    int pr[3];
+    long long entries = 1;
+    for (int i = 0; i < 3; ++i) {
+        pr[i] = std::max(2 * radii[i], 4 * radii[i] - 2);
+        entries *= 2 * pr[i] + 1;
+    }
+    if (entries > kMaxPairTable) return make_devcov(as.model, nullptr, nullptr);
+    std::lock_guard<std::mutex> g(as.lazy);
+    if (!as.have_pair) {
+        const double t0 = now_ms();
+        as.pair.build(as.model, pr);
+        st.ms_setup_host += now_ms() - t0;
+        upload(as.d_pair, as.pair.v, s);
+        hip_check(hipStreamSynchronize(s), "sync");
+        as.have_pair = true;
+    }
+    return make_devcov(as.model, &as.pair, as.d_pair.p);
+}
+
 void run_cont(const ContJob &job, hipStream_t s, int precision, const Dump *dump, Stats &st) {
     const double w0 = now_ms();
     validate_search(job.grid, job.radii, job.max_n);
@@ -727,31 +753,8 @@ void run_cont(const ContJob &job, hipStream_t s, int precision, const Dump *dump
     sa.X = job.grid[0];
     sa.Y = job.grid[1];
     sa.Z = job.grid[2];
-    // OK covariances are the analytic model (ordinary_kriging.cpp); the solve
-    // kernels read them from a table over every offset a neighbour pair can
-    // take: neighbours lie within 2r-1 of the target per axis (cluster path;
-    // r on the template path), so pairs within 4r-2.
     if (job.ordinary) {
-        int pr[3];
-        long long entries = 1;
-        for (int i = 0; i < 3; ++i) {
-            pr[i] = std::max(2 * job.radii[i], 4 * job.radii[i] - 2);
-            entries *= 2 * pr[i] + 1;
-        }
-        if (entries <= kMaxPairTable) {
-            std::lock_guard<std::mutex> g(as->lazy);
-            if (!as->have_pair) {
-                const double t0 = now_ms();
-                as->pair.build(as->model, pr);
-                st.ms_setup_host += now_ms() - t0;
-                upload(as->d_pair, as->pair.v, s);
-                hip_check(hipStreamSynchronize(s), "sync");
-                as->have_pair = true;
-            }
-            sa.cov = make_devcov(as->model, &as->pair, as->d_pair.p);
-        } else {
-            sa.cov = make_devcov(as->model, nullptr, nullptr);
-        }
+        sa.cov = analytic_cov(*as, job.radii, s, st);
     } else {
         sa.cov = make_devcov(as->model, &as->table, as->d_table.p);
     }
@@ -1429,6 +1432,113 @@ void run_sis(const SisJob &job, hipStream_t s, int precision, Stats &st) {
     hip_check(hipEventRecord(ev.e[5], s), "event");
     collect_counters(counters.p, s, st);
     hip_check(hipStreamSynchronize(s), "sync");
+    st.ms_device_total += ev.ms(0, 5);
+    st.ms_wall += now_ms() - w0;
+}
+
+// ---------------------------------------------------------------------------
+// Simple colocated cokriging, Markov models I and II
+// (simple_cokriging_markI.cpp:185-305): template search with the analytic
+// primary model (sugarbox_neighbour_lookup.h:36-61, no cell-binned path),
+// one (n+1)-system per uninformed cell solved by gauss_solve in FP64.
+// ---------------------------------------------------------------------------
+void run_cokriging(const CokrigingJob &job, hipStream_t s, const Dump *dump, Stats &st) {
+    const double w0 = now_ms();
+    validate_search(job.grid, job.radii, job.max_n);
+    validate_cov(job.primary);
+    if (job.mark2) validate_cov(job.secondary);
+    const long long N = (long long)job.grid[0] * job.grid[1] * job.grid[2];
+    const int max_n = std::max(job.max_n, 1);
+    Events ev;
+    hip_check(hipEventRecord(ev.e[0], s), "event");
+    DBuf<int> targets;
+    const int T = select_indices(IsZero{job.d_in_mask}, (int)N, targets, s);
+    st.nodes_target += T;
+    std::shared_ptr<CovAssets> as = get_assets(job.primary, job.radii, false, s, st);
+    DBuf<int> nbr((size_t)std::max(T, 1) * max_n), cnt((size_t)std::max(T, 1));
+    hip_check(hipEventRecord(ev.e[1], s), "event");
+    if (T > 0) {
+        ScanArgs sa;
+        sa.mask = job.d_in_mask;
+        sa.key = nullptr;
+        sa.X = job.grid[0];
+        sa.Y = job.grid[1];
+        sa.Z = job.grid[2];
+        sa.tmpl = (const int4 *)as->d_tmpl.p;
+        sa.tsize = as->tmpl.size();
+        sa.max_n = job.max_n;
+        sa.targets = targets.p;
+        sa.list = nullptr;
+        sa.n_list = T;
+        sa.nbr = nbr.p;
+        sa.cnt = cnt.p;
+        sa.simbits = nullptr;
+        KTimer kt(s);
+        launch_template_scan(sa, 0, s);
+        hip_check(hipGetLastError(), "template scan");
+        st.ms_k_template += kt.stop();
+        st.n_k_template += 1;
+        st.nodes_template_path += T;
+    }
+    hip_check(hipEventRecord(ev.e[2], s), "event");
+    write_dump(dump, 0, targets.p, T, nbr.p, cnt.p, job.max_n, s);
+
+    DBuf<unsigned long long> counters(kCounterWords);
+    hip_check(hipMemsetAsync(counters.p, 0, kCounterWords * sizeof(unsigned long long), s), "memset");
+    CokrigingArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.targets = targets.p;
+    a.T = T;
+    a.nbr = nbr.p;
+    a.cnt = cnt.p;
+    a.max_n = max_n;
+    a.X = job.grid[0];
+    a.Y = job.grid[1];
+    a.Z = job.grid[2];
+    a.cov = analytic_cov(*as, job.radii, s, st);
+    // C(0) of a model is sill - nugget (cov_model.h:144-181 at h = 0)
+    const double c1 = as->model.c0();
+    std::shared_ptr<CovAssets> as2;
+    if (job.mark2) {
+        // cross_cov_model_mark_ii_t (simple_cokriging_markI.cpp:162-183):
+        // the secondary model's values at target-neighbour offsets, from
+        // its box table over the search radii
+        as2 = get_assets(job.secondary, job.radii, true, s, st);
+        const double c2 = as2->model.c0();
+        a.cross = make_devcov(as2->model, &as2->table, as2->d_table.p);
+        a.sec_var = c2;
+        a.coef = job.correlation * sqrt(c2 / c1);
+        a.cross0 = a.coef * c2;
+    } else {
+        // cross_cov_model_mark_i_t (simple_cokriging_markI.cpp:143-160)
+        a.cross = a.cov;
+        a.sec_var = job.secondary_variance;
+        a.coef = job.correlation * sqrt(job.secondary_variance) / sqrt(c1);
+        a.cross0 = a.coef * c1;
+    }
+    a.primary_mean = job.primary_mean;
+    a.secondary_mean = job.secondary_mean;
+    a.values = job.d_in;
+    a.sec = job.d_sec;
+    a.sec_mask = job.d_sec_mask;
+    a.out = job.d_out;
+    a.out_mask = job.d_out_mask;
+    a.counters = counters.p;
+    hip_check(hipEventRecord(ev.e[3], s), "event");
+    {
+        KTimer kt(s);
+        launch_cokriging(a, s);
+        hip_check(hipGetLastError(), "cokriging");
+        st.ms_k_solve_fp64 += kt.stop();
+        st.n_k_solve_fp64 += T > 0 ? 1 : 0;
+    }
+    hip_check(hipEventRecord(ev.e[4], s), "event");
+    launch_copy_informed(job.d_in, job.d_in_mask, N, job.d_out, job.d_out_mask, s);
+    hip_check(hipEventRecord(ev.e[5], s), "event");
+    collect_counters(counters.p, s, st);
+    hip_check(hipStreamSynchronize(s), "sync");
+    st.ms_search += ev.ms(1, 2);
+    st.ms_solve += ev.ms(3, 4);
     st.ms_device_total += ev.ms(0, 5);
     st.ms_wall += now_ms() - w0;
 }

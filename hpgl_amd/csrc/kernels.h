@@ -239,6 +239,31 @@ struct SisSweepArgs {
 constexpr int kSisMaxCategories = 256;
 void launch_sis_sweep(const SisSweepArgs &a, int R, int waves_per_realisation, hipStream_t s);
 
+// Simple colocated cokriging, Markov models I and II (cokriging.hip): one
+// (n+1)-system per target, solved by gauss_solve in the reference's order.
+struct CokrigingArgs {
+    const int *targets;
+    int T;
+    const int *nbr;
+    const int *cnt;
+    int max_n;
+    int X, Y, Z;
+    DevCov cov;           // primary covariance between neighbours and to the target
+    DevCov cross;         // covariance scaled by coef for the cross terms
+    double coef;          // cross-covariance factor
+    double sec_var;       // A(n, n): the secondary variance
+    double cross0;        // b(n) = coef * cross(0)
+    float primary_mean;
+    float secondary_mean;
+    const float *values;  // primary values
+    const float *sec;     // secondary values and mask
+    const uint8_t *sec_mask;
+    float *out;
+    uint8_t *out_mask;
+    unsigned long long *counters;  // [1] solved, [2] singular, [3] no neighbours
+};
+void launch_cokriging(const CokrigingArgs &a, hipStream_t s);
+
 // Statistics counters (counters.h): kCounterSlots lines of kCounterStride
 // words; word w of every line accumulates counter w.
 constexpr int kCounterSlots = 1024;

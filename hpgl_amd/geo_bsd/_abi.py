@@ -84,6 +84,26 @@ class __hpgl_cov_params_t(C.Structure):
     _fields_ = list(_COV_FIELDS)
 
 
+class __hpgl_cockriging_m1_params_t(C.Structure):
+    _fields_ = _COV_FIELDS + _NB_FIELDS + [
+        ("primary_mean", C.c_double),
+        ("secondary_mean", C.c_double),
+        ("secondary_variance", C.c_double),
+        ("correlation_coef", C.c_double),
+    ]
+
+
+class __hpgl_cockriging_m2_params_t(C.Structure):
+    _fields_ = [
+        ("primary_cov_params", globals()["__hpgl_cov_params_t"]),
+        ("secondary_cov_params", globals()["__hpgl_cov_params_t"]),
+    ] + _NB_FIELDS + [
+        ("primary_mean", C.c_double),
+        ("secondary_mean", C.c_double),
+        ("correlation_coef", C.c_double),
+    ]
+
+
 class hpgl_non_parametric_cdf_t(C.Structure):
     _fields_ = [("values", C.POINTER(C.c_float)), ("probs", C.POINTER(C.c_float)), ("size", C.c_longlong)]
 
@@ -159,4 +179,6 @@ JOB_OK, JOB_SK, JOB_LVM, JOB_IK, JOB_MEDIAN_IK, JOB_SGS = range(6)
 # Module-level alias without name mangling (the reference relies on the
 # double-underscore name only inside this module).
 _HPGL_COV_PARAMS = __hpgl_cov_params_t
+_HPGL_COKRIGING_M1_PARAMS = __hpgl_cockriging_m1_params_t
+_HPGL_COKRIGING_M2_PARAMS = __hpgl_cockriging_m2_params_t
 

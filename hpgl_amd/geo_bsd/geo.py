@@ -31,6 +31,8 @@ from .hpgl_wrap import (
     _HPGL_IK_PARAMS,
     _HPGL_MEDIAN_IK_PARAMS,
     _HPGL_COV_PARAMS,
+    _HPGL_COKRIGING_M1_PARAMS,
+    _HPGL_COKRIGING_M2_PARAMS,
     _hpgl_so,
     check_last_error,
     hpgl_output_handler,
@@ -497,6 +499,54 @@ def indicator_kriging(prop, grid, data, marginal_probs):
         C.byref(_create_hpgl_ind_masked_array(out_prop, grid)),
         _create_hpgl_ik_params(data, len(data), False, marginal_probs),
         len(data))
+    check_last_error()
+    return out_prop
+
+
+@accepts_tuple("prop", 0)
+def simple_cokriging_markI(prop, grid, secondary_data, primary_mean, secondary_mean, secondary_variance,
+                           correlation_coef, radiuses, max_neighbours, cov_model):
+    """Simple colocated cokriging, Markov model I (geo.py:511-534).
+
+    secondary_data: ContProperty (or (data, mask)) on the same grid; cells
+    where it is not informed use secondary_mean.  Every cell of the result
+    is informed.
+    """
+    if isinstance(secondary_data, tuple):
+        secondary_data = ContProperty(*secondary_data)
+    out_prop = _clone_prop(prop)
+    p = _checked_create(
+        _HPGL_COKRIGING_M1_PARAMS, radiuses=_c_array(C.c_int, 3, radiuses), max_neighbours=max_neighbours,
+        primary_mean=primary_mean, secondary_mean=secondary_mean, secondary_variance=secondary_variance,
+        correlation_coef=correlation_coef, **_cov_kwargs(cov_model))
+    _hpgl_so.hpgl_simple_cokriging_mark1(
+        C.byref(_create_hpgl_cont_masked_array(prop, grid)),
+        C.byref(_create_hpgl_cont_masked_array(secondary_data, grid)), C.byref(p),
+        C.byref(_create_hpgl_cont_masked_array(out_prop, grid)))
+    check_last_error()
+    return out_prop
+
+
+def simple_cokriging_markII(grid, primary_data, secondary_data, correlation_coef, radiuses, max_neighbours):
+    """Simple colocated cokriging, Markov model II (geo.py:536-572).
+
+    primary_data, secondary_data: dicts with keys "data" (ContProperty),
+    "mean" and "cov_model".
+    """
+    out_prop = _clone_prop(primary_data["data"])
+    pcp = primary_data["cov_model"]
+    scp = secondary_data["cov_model"]
+    p = _checked_create(
+        _HPGL_COKRIGING_M2_PARAMS,
+        primary_cov_params=_checked_create(_HPGL_COV_PARAMS, **_cov_kwargs(pcp)),
+        secondary_cov_params=_checked_create(_HPGL_COV_PARAMS, **_cov_kwargs(scp)),
+        radiuses=_c_array(C.c_int, 3, radiuses), max_neighbours=max_neighbours,
+        primary_mean=primary_data["mean"], secondary_mean=secondary_data["mean"],
+        correlation_coef=correlation_coef)
+    _hpgl_so.hpgl_simple_cokriging_mark2(
+        C.byref(_create_hpgl_cont_masked_array(primary_data["data"], grid)),
+        C.byref(_create_hpgl_cont_masked_array(secondary_data["data"], grid)), C.byref(p),
+        C.byref(_create_hpgl_cont_masked_array(out_prop, grid)))
     check_last_error()
     return out_prop
 

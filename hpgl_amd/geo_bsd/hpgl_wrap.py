@@ -15,7 +15,8 @@ from ._abi import (  # noqa: F401
     hpgl_output_handler, hpgl_progress_handler, _HPGL_MEAN_KIND, _HPGL_KRIGING_KIND, _HPGL_SHAPE,
     _HPGL_CONT_MASKED_ARRAY, _HPGL_IND_MASKED_ARRAY, _HPGL_UBYTE_ARRAY, _HPGL_FLOAT_ARRAY, _HPGL_OK_PARAMS,
     _HPGL_SK_PARAMS, _HPGL_SGS_PARAMS, _HPGL_MEDIAN_IK_PARAMS, _HPGL_IK_PARAMS, _HPGL_COV_PARAMS,
-    __hpgl_cov_params_t, hpgl_non_parametric_cdf_t, _HPGL_AMD_STATS, _HPGL_AMD_JOB, _HPGL_AMD_JOB_RESULT)
+    __hpgl_cov_params_t, hpgl_non_parametric_cdf_t, _HPGL_AMD_STATS, _HPGL_AMD_JOB, _HPGL_AMD_JOB_RESULT,
+    _HPGL_COKRIGING_M1_PARAMS, _HPGL_COKRIGING_M2_PARAMS)
 
 _PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIBRARY_PATH = os.path.join(_PKG_DIR, "hpgl.so")
@@ -124,6 +125,23 @@ _hpgl_so.hpgl_amd_sis_simulation_batch_dev.restype = None
 _hpgl_so.hpgl_amd_sis_simulation_batch_dev.argtypes = [
     C.c_void_p, C.c_void_p, _P(_HPGL_SHAPE), _P(_HPGL_IK_PARAMS), C.c_int, C.c_void_p, C.c_int, C.c_void_p,
     _P(C.c_int), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+
+_hpgl_so.hpgl_simple_cokriging_mark1.restype = None
+_hpgl_so.hpgl_simple_cokriging_mark1.argtypes = [
+    _P(_HPGL_CONT_MASKED_ARRAY), _P(_HPGL_CONT_MASKED_ARRAY), _P(_HPGL_COKRIGING_M1_PARAMS),
+    _P(_HPGL_CONT_MASKED_ARRAY)]
+_hpgl_so.hpgl_simple_cokriging_mark2.restype = None
+_hpgl_so.hpgl_simple_cokriging_mark2.argtypes = [
+    _P(_HPGL_CONT_MASKED_ARRAY), _P(_HPGL_CONT_MASKED_ARRAY), _P(_HPGL_COKRIGING_M2_PARAMS),
+    _P(_HPGL_CONT_MASKED_ARRAY)]
+_hpgl_so.hpgl_amd_simple_cokriging_mark1_dev.restype = None
+_hpgl_so.hpgl_amd_simple_cokriging_mark1_dev.argtypes = [
+    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, _P(_HPGL_SHAPE), _P(_HPGL_COKRIGING_M1_PARAMS), C.c_void_p,
+    C.c_void_p, C.c_void_p]
+_hpgl_so.hpgl_amd_simple_cokriging_mark2_dev.restype = None
+_hpgl_so.hpgl_amd_simple_cokriging_mark2_dev.argtypes = [
+    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, _P(_HPGL_SHAPE), _P(_HPGL_COKRIGING_M2_PARAMS), C.c_void_p,
+    C.c_void_p, C.c_void_p]
 
 _hpgl_so.hpgl_amd_server_create.restype = C.c_void_p
 _hpgl_so.hpgl_amd_server_create.argtypes = [C.c_int, C.c_longlong]

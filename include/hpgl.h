@@ -123,6 +123,30 @@ struct hpgl_ik_params_t {           /* api.h:110, 96 bytes */
     double m_marginal_prob;
 };
 
+struct hpgl_cokriging_m1_params_t { /* api.h:122, 120 bytes */
+    int m_covariance_type;
+    double m_ranges[3];
+    double m_angles[3];
+    double m_sill;
+    double m_nugget;
+    int m_radiuses[3];
+    int m_max_neighbours;
+    double m_primary_mean;
+    double m_secondary_mean;
+    double m_secondary_variance;
+    double m_correlation_coef;
+};
+
+struct hpgl_cokriging_m2_params_t { /* api.h:137, 184 bytes */
+    struct hpgl_cov_params_t m_primary_cov_params;
+    struct hpgl_cov_params_t m_secondary_cov_params;
+    int m_radiuses[3];
+    int m_max_neighbours;
+    double m_primary_mean;
+    double m_secondary_mean;
+    double m_correlation_coef;
+};
+
 struct hpgl_non_parametric_cdf_t {  /* api.h:148, 24 bytes */
     float *m_values;
     float *m_probs;
@@ -245,6 +269,22 @@ void hpgl_sis_simulation_lvm(struct hpgl_ind_masked_array_t *data,
                              int seed,
                              struct hpgl_ubyte_array_t *simulation_mask,
                              int use_correlograms);
+
+/* api.h:297-302, api.cpp:587-652 (simple_cokriging_markI.cpp:185-242).
+ * Simple colocated cokriging, Markov model I: every cell of output_data is
+ * written (informed primary cells copy the input).  The systems are solved
+ * in FP64 in gauss_solve's operation order whatever the precision setting. */
+void hpgl_simple_cokriging_mark1(struct hpgl_cont_masked_array_t *input_data,
+                                 struct hpgl_cont_masked_array_t *secondary_data,
+                                 struct hpgl_cokriging_m1_params_t *params,
+                                 struct hpgl_cont_masked_array_t *output_data);
+
+/* api.h:304-309, api.cpp:654-712 (simple_cokriging_markI.cpp:244-305).
+ * Markov model II: the cross covariance follows the secondary model. */
+void hpgl_simple_cokriging_mark2(struct hpgl_cont_masked_array_t *primary_data,
+                                 struct hpgl_cont_masked_array_t *secondary_data,
+                                 struct hpgl_cokriging_m2_params_t *params,
+                                 struct hpgl_cont_masked_array_t *output_data);
 
 /* ---- Part 2: hpgl_amd_* extensions (no counterpart in the reference) ---- */
 
@@ -373,6 +413,20 @@ void hpgl_amd_sis_simulation_batch_dev(const unsigned char *d_in, const unsigned
                                        const float *d_lvm, int use_correlograms,
                                        const unsigned char *d_sim_mask, const int *seeds, int R,
                                        unsigned char *d_out, unsigned char *d_out_mask, void *stream);
+
+/* Device-pointer variants of the cokriging entry points (arrays of N
+ * cells already in HBM; the work is queued on `stream` and the call returns
+ * when it has finished).  d_out and d_out_mask are written for every cell. */
+void hpgl_amd_simple_cokriging_mark1_dev(const float *d_in, const unsigned char *d_in_mask,
+                                         const float *d_sec, const unsigned char *d_sec_mask,
+                                         const struct hpgl_shape_t *shape,
+                                         const struct hpgl_cokriging_m1_params_t *params,
+                                         float *d_out, unsigned char *d_out_mask, void *stream);
+void hpgl_amd_simple_cokriging_mark2_dev(const float *d_in, const unsigned char *d_in_mask,
+                                         const float *d_sec, const unsigned char *d_sec_mask,
+                                         const struct hpgl_shape_t *shape,
+                                         const struct hpgl_cokriging_m2_params_t *params,
+                                         float *d_out, unsigned char *d_out_mask, void *stream);
 
 /* ---- Request loop with continuous batching (no reference counterpart:
  * the reference runs one blocking call at a time).  Jobs wait in arrival

@@ -913,6 +913,125 @@ static void sis_core(unsigned char *data, unsigned char *mask, const Grid &g, co
     }
 }
 
+// gauss_solve (gauss_solver.cpp:5-65) on row-major A: for each column the
+// first unused row with a non-zero entry is the pivot (none: singular), the
+// pivot row is divided by it from that column on, and every unused row has
+// the pivot row subtracted; back substitution starts X[i] from B[i] (the
+// right-hand side of row i) and subtracts A[order[i]][j] X[j] for j from
+// size-1 down to i+1.
+static bool gauss_solve(double *A, double *B, double *X, int size) {
+    std::vector<int> used(size, 0), order(size, 0);
+    for (int i = 0; i < size; ++i) {
+        int row = -1;
+        for (int j = 0; j < size; ++j)
+            if (used[j] == 0 && A[(size_t)j * size + i] != 0) {
+                row = j;
+                break;
+            }
+        if (row < 0) return false;
+        used[row] = 1;
+        order[i] = row;
+        const double coef = A[(size_t)row * size + i];
+        for (int j = i; j < size; ++j) A[(size_t)row * size + j] /= coef;
+        B[row] /= coef;
+        for (int j = 0; j < size; ++j) {
+            if (used[j] != 0) continue;
+            const double c = A[(size_t)j * size + i];
+            for (int k = i; k < size; ++k) A[(size_t)j * size + k] -= c * A[(size_t)row * size + k];
+            B[j] -= c * B[row];
+        }
+    }
+    for (int i = size - 1; i >= 0; --i) {
+        const int row = order[i];
+        X[i] = B[i];
+        for (int j = size - 1; j > i; --j) X[i] -= A[(size_t)row * size + j] * X[j];
+    }
+    return true;
+}
+
+// simple_cokriging_markI / markII (simple_cokriging_markI.cpp:19-305) over
+// every cell: informed primary cells copy the input; otherwise the
+// template neighbours (analytic primary model) and the colocated secondary
+// value (its mean where not informed) enter the (n+1)-system of
+// build_system, solved by gauss_solve; combine rounds to float after each
+// double addition; no neighbours or a singular system give
+// primary_mean + secondary_value - secondary_mean.
+static void cokriging(const float *in, const unsigned char *in_mask, const float *sec, const unsigned char *sec_mask,
+                      const Grid &g, const CovModel &primary, const CovModel *secondary, double sec_var_param,
+                      double rho, mean_t pm, mean_t sm, const int radii[3], int max_n, float *out,
+                      unsigned char *out_mask) {
+    const int N = g.size();
+    JobCov cov{&primary, nullptr};
+    Lookup nl(&g, cov, radii, max_n, false);
+    const double zero[3] = {0, 0, 0};
+    const double c1 = primary(zero, zero);
+    double sec_var, coef;
+    const CovModel *cross_model;
+    if (secondary) {  // cross_cov_model_mark_ii_t (:162-183)
+        sec_var = (*secondary)(zero, zero);
+        coef = rho * sqrt(sec_var / c1);
+        cross_model = secondary;
+    } else {  // cross_cov_model_mark_i_t (:143-160)
+        sec_var = sec_var_param;
+        coef = rho * sqrt(sec_var_param) / sqrt(c1);
+        cross_model = &primary;
+    }
+    const double cross0 = coef * (*cross_model)(zero, zero);
+    auto informed = [&](int idx) { return idx < 0 || idx >= N ? false : in_mask[idx] != 0; };
+#pragma omp parallel
+    {
+        std::vector<int> idx;
+        std::vector<Coord> c;
+        std::vector<double> A, B, X;
+#pragma omp for schedule(static)
+        for (int node = 0; node < N; ++node) {
+            float result;
+            if (in_mask[node] != 0) {
+                result = in[node];
+            } else {
+                const float sv = sec_mask[node] != 0 ? sec[node] : sm;
+                nl.find_template(node, informed, idx);
+                dump_node(node, idx);
+                bool ok = !idx.empty();
+                const int n = (int)idx.size();
+                if (ok) {
+                    Coord center;
+                    g.loc(node, center.c[0], center.c[1], center.c[2]);
+                    c.resize(n);
+                    for (int i = 0; i < n; ++i) g.loc(idx[i], c[i].c[0], c[i].c[1], c[i].c[2]);
+                    const int m = n + 1;
+                    A.assign((size_t)m * m, 0.0);
+                    B.assign(m, 0.0);
+                    X.assign(m, 0.0);
+                    for (int i = 0; i < n; ++i)
+                        for (int j = i; j < n; ++j) {
+                            A[(size_t)i * m + j] = cov(c[i].c[0], c[i].c[1], c[i].c[2], c[j].c[0], c[j].c[1], c[j].c[2]);
+                            A[(size_t)j * m + i] = A[(size_t)i * m + j];
+                        }
+                    for (int i = 0; i < n; ++i) {
+                        A[(size_t)n * m + i] = coef * cross_model->at_int(center.c[0], center.c[1], center.c[2],
+                                                                          c[i].c[0], c[i].c[1], c[i].c[2]);
+                        A[(size_t)i * m + n] = A[(size_t)n * m + i];
+                        B[i] = cov(c[i].c[0], c[i].c[1], c[i].c[2], center.c[0], center.c[1], center.c[2]);
+                    }
+                    B[n] = cross0;
+                    A[(size_t)m * m - 1] = sec_var;
+                    ok = gauss_solve(A.data(), B.data(), X.data(), m);
+                }
+                if (ok) {
+                    result = pm;
+                    for (int i = 0; i < n; ++i) result += (in[idx[i]] - pm) * X[i];
+                    result += X[n] * (sv - sm);
+                } else {
+                    result = pm + sv - sm;
+                }
+            }
+            out[node] = result;
+            out_mask[node] = 1;
+        }
+    }
+}
+
 static void transform_forward(float *data, const unsigned char *mask, int N, const NpCdf &cdf) {
     Gauss to;
     for (int i = 0; i < N; ++i)
@@ -1206,6 +1325,34 @@ ORACLE_API void oracle_sis_simulation_lvm(hpgl_ind_masked_array_t *data, hpgl_ik
     GUARD_END
 }
 
+// hpgl_simple_cokriging_mark1 (api.cpp:587-652 -> simple_cokriging_markI.cpp:185-242)
+ORACLE_API void oracle_simple_cokriging_mark1(hpgl_cont_masked_array_t *in, hpgl_cont_masked_array_t *sec,
+                                              hpgl_cokriging_m1_params_t *p, hpgl_cont_masked_array_t *out) {
+    g_last_error.clear();
+    GUARD_BEGIN
+    Grid g{in->m_shape.m_data[0], in->m_shape.m_data[1], in->m_shape.m_data[2]};
+    CovModel model(cov_from(p->m_covariance_type, p->m_ranges, p->m_angles, p->m_sill, p->m_nugget));
+    cokriging(in->m_data, in->m_mask, sec->m_data, sec->m_mask, g, model, nullptr, p->m_secondary_variance,
+              p->m_correlation_coef, (mean_t)p->m_primary_mean, (mean_t)p->m_secondary_mean, p->m_radiuses,
+              p->m_max_neighbours, out->m_data, out->m_mask);
+    GUARD_END
+}
+
+// hpgl_simple_cokriging_mark2 (api.cpp:654-712 -> simple_cokriging_markI.cpp:244-305)
+ORACLE_API void oracle_simple_cokriging_mark2(hpgl_cont_masked_array_t *in, hpgl_cont_masked_array_t *sec,
+                                              hpgl_cokriging_m2_params_t *p, hpgl_cont_masked_array_t *out) {
+    g_last_error.clear();
+    GUARD_BEGIN
+    Grid g{in->m_shape.m_data[0], in->m_shape.m_data[1], in->m_shape.m_data[2]};
+    const hpgl_cov_params_t &a = p->m_primary_cov_params, &b = p->m_secondary_cov_params;
+    CovModel m1(cov_from(a.m_covariance_type, a.m_ranges, a.m_angles, a.m_sill, a.m_nugget));
+    CovModel m2(cov_from(b.m_covariance_type, b.m_ranges, b.m_angles, b.m_sill, b.m_nugget));
+    cokriging(in->m_data, in->m_mask, sec->m_data, sec->m_mask, g, m1, &m2, 0.0, p->m_correlation_coef,
+              (mean_t)p->m_primary_mean, (mean_t)p->m_secondary_mean, p->m_radiuses, p->m_max_neighbours,
+              out->m_data, out->m_mask);
+    GUARD_END
+}
+
 // hpgl_simple_kriging_weights (api.cpp:292-342 -> simple_kriging.cpp:65-85):
 // analytic covariance on real coordinates.
 ORACLE_API int oracle_simple_kriging_weights(float *center, float *xs, float *ys, float *zs, int n,
@@ -1266,6 +1413,12 @@ ORACLE_API int oracle_cholesky_solve(int n, const double *a, const double *b, do
     dpotrs_upper(n, A.data(), x);
     if (factor) std::copy(A.begin(), A.end(), factor);
     return info;
+}
+
+// gauss_solve on a row-major n x n matrix; returns 1 on success.
+ORACLE_API int oracle_gauss_solve(int n, const double *a, const double *b, double *x) {
+    std::vector<double> A(a, a + (size_t)n * n), B(b, b + n);
+    return gauss_solve(A.data(), B.data(), x, n) ? 1 : 0;
 }
 
 // Covariance C(offset) of the analytic model.

@@ -59,6 +59,12 @@ lib.oracle_sis_simulation.argtypes = [_P(abi._HPGL_IND_MASKED_ARRAY), _P(abi._HP
 lib.oracle_sis_simulation_lvm.argtypes = [_P(abi._HPGL_IND_MASKED_ARRAY), _P(abi._HPGL_IK_PARAMS),
                                           _P(abi._HPGL_FLOAT_ARRAY), C.c_int, C.c_int, _P(abi._HPGL_UBYTE_ARRAY),
                                           C.c_int]
+lib.oracle_simple_cokriging_mark1.argtypes = [_P(abi._HPGL_CONT_MASKED_ARRAY), _P(abi._HPGL_CONT_MASKED_ARRAY),
+                                              _P(abi._HPGL_COKRIGING_M1_PARAMS), _P(abi._HPGL_CONT_MASKED_ARRAY)]
+lib.oracle_simple_cokriging_mark2.argtypes = [_P(abi._HPGL_CONT_MASKED_ARRAY), _P(abi._HPGL_CONT_MASKED_ARRAY),
+                                              _P(abi._HPGL_COKRIGING_M2_PARAMS), _P(abi._HPGL_CONT_MASKED_ARRAY)]
+lib.oracle_gauss_solve.restype = C.c_int
+lib.oracle_gauss_solve.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
 lib.oracle_simple_kriging_weights.restype = C.c_int
 lib.oracle_simple_kriging_weights.argtypes = [C.c_void_p] * 4 + [C.c_int, _P(abi._HPGL_COV_PARAMS), C.c_void_p,
                                                                   _P(C.c_double)]
@@ -269,6 +275,45 @@ def sis_simulation(prop, grid, data, seed, marginal_probs, use_correlogram=True,
         lib.oracle_sis_simulation_lvm(C.byref(arr), ps, means, K, seed, sm, 1 if use_correlogram else 0)
     _check()
     return od, om
+
+
+def simple_cokriging_markI(prop, grid, secondary, primary_mean, secondary_mean, secondary_variance,
+                           correlation_coef, radiuses, max_neighbours, cov_model):
+    ind, inm = _cont(*prop)
+    sd, sm = _cont(*secondary)
+    od, om = ind.copy(order="F"), inm.copy(order="F")
+    p = abi._HPGL_COKRIGING_M1_PARAMS(radiuses=(C.c_int * 3)(*radiuses), max_neighbours=max_neighbours,
+                                      primary_mean=primary_mean, secondary_mean=secondary_mean,
+                                      secondary_variance=secondary_variance, correlation_coef=correlation_coef,
+                                      **_cov_kwargs(cov_model))
+    lib.oracle_simple_cokriging_mark1(C.byref(_cma(ind, inm, grid)), C.byref(_cma(sd, sm, grid)), C.byref(p),
+                                      C.byref(_cma(od, om, grid)))
+    _check()
+    return od, om
+
+
+def simple_cokriging_markII(grid, primary, secondary, correlation_coef, radiuses, max_neighbours):
+    """primary/secondary: dicts with "data" ((values, mask)), "mean", "cov_model"."""
+    ind, inm = _cont(*primary["data"])
+    sd, sm = _cont(*secondary["data"])
+    od, om = ind.copy(order="F"), inm.copy(order="F")
+    p = abi._HPGL_COKRIGING_M2_PARAMS(primary_cov_params=cov_params(primary["cov_model"]),
+                                      secondary_cov_params=cov_params(secondary["cov_model"]),
+                                      radiuses=(C.c_int * 3)(*radiuses), max_neighbours=max_neighbours,
+                                      primary_mean=primary["mean"], secondary_mean=secondary["mean"],
+                                      correlation_coef=correlation_coef)
+    lib.oracle_simple_cokriging_mark2(C.byref(_cma(ind, inm, grid)), C.byref(_cma(sd, sm, grid)), C.byref(p),
+                                      C.byref(_cma(od, om, grid)))
+    _check()
+    return od, om
+
+
+def gauss_solve(a, b):
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.zeros_like(b)
+    ok = lib.oracle_gauss_solve(len(b), _ptr(a), _ptr(b), _ptr(x))
+    return bool(ok), x
 
 
 def simple_kriging_weights(center, xs, ys, zs, cm):
