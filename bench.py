@@ -180,6 +180,93 @@ def sis_bench(torch, lib, abi, hpgl_wrap, steps=2, warmup=1, cpu=True, seed=3241
     return res
 
 
+def make_cokriging_workload(seed=1007):
+    """Colocated cokriging (Markov model I) on the grid of the reference's
+    published benchmarks (166x141x225, 20% informed, exponential ranges
+    10/10/10, radii 20/20/20, max_neighbours 12); the secondary variable is
+    a smooth random field informed everywhere; primary mean 0, secondary
+    mean 0, secondary variance 0.07 and correlation 0.25 as in the
+    reference's regression script (geo_testing/test_compare.py:165-166)."""
+    from cases import Cov, Grid, scattered, smooth_field
+
+    g = Grid(166, 141, 225)
+    data, mask = scattered(g.shape, 0.20, seed, law="normal")
+    sec = smooth_field(g.shape, 10.0, seed + 1)
+    smask = np.ones(g.shape, np.uint8, order="F")
+    params = dict(cov=Cov(1, (10, 10, 10)), radii=(20, 20, 20), max_n=12, primary_mean=0.0, secondary_mean=0.0,
+                  secondary_variance=0.07, correlation=0.25)
+    return g, data, mask, sec, smask, params
+
+
+def cokriging_bench(torch, lib, abi, hpgl_wrap, steps=2, warmup=1, cpu=True):
+    """Estimated cells/s of simple colocated cokriging, inputs resident in
+    HBM.  The CPU leg runs the oracle on every 64th cell (16 threads) and the
+    GPU result is compared with it on those cells."""
+    g, data, mask, sec, smask, prm = make_cokriging_workload()
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a.ravel(order="F"))).to(dev)
+    d_in, d_m, d_s, d_sm = t(data), t(mask), t(sec), t(smask)
+    d_out, d_om = torch.empty_like(d_in), torch.empty_like(d_m)
+    c = prm["cov"]
+    p = abi._HPGL_COKRIGING_M1_PARAMS(covariance_type=c.type, ranges=(C.c_double * 3)(*c.ranges),
+                                      angles=(C.c_double * 3)(*c.angles), sill=c.sill, nugget=c.nugget,
+                                      radiuses=(C.c_int * 3)(*prm["radii"]), max_neighbours=prm["max_n"],
+                                      primary_mean=prm["primary_mean"], secondary_mean=prm["secondary_mean"],
+                                      secondary_variance=prm["secondary_variance"],
+                                      correlation_coef=prm["correlation"])
+    shape = abi._HPGL_SHAPE(data=(C.c_int * 3)(*g.shape), strides=(C.c_int * 3)(1, g.x, g.x * g.y))
+    stream = torch.cuda.current_stream(dev)
+    sp = C.c_void_p(stream.cuda_stream)
+    n_est = int(np.sum(mask == 0))
+
+    def run():
+        d_out.copy_(d_in)
+        d_om.copy_(d_m)
+        lib.hpgl_amd_simple_cokriging_mark1_dev(d_in.data_ptr(), d_m.data_ptr(), d_s.data_ptr(), d_sm.data_ptr(),
+                                                C.byref(shape), C.byref(p), d_out.data_ptr(), d_om.data_ptr(), sp)
+        hpgl_wrap.check_last_error()
+        return hpgl_wrap.last_stats()
+
+    for _ in range(warmup):
+        run()
+    torch.cuda.synchronize()
+    acc = {}
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = run()
+        for k, v in st.items():
+            acc[k] = acc.get(k, 0) + v
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / steps
+    res = {"workload": "simple colocated cokriging (Markov model I), 166x141x225 grid, 20% informed, exponential "
+                       "ranges 10/10/10, radii 20/20/20, max_neighbours 12, secondary informed everywhere",
+           "estimated_cells": n_est, "value": n_est / wall, "unit": "estimated cells/s", "ms_per_job": wall * 1e3,
+           "ms_template_scan": acc.get("ms_k_template", 0.0) / steps,
+           "ms_solve": acc.get("ms_k_solve_fp64", 0.0) / steps,
+           "singular": int(acc.get("nodes_singular", 0) / steps), "dtype": "f64"}
+    if cpu:
+        import oracle_binding as O
+
+        stride = 64
+        out = d_out.cpu().numpy().reshape(g.shape, order="F")
+        O.set_threads(16)
+        O.set_sample_stride(stride)
+        t0 = time.perf_counter()
+        od, om = O.simple_cokriging_markI((data, mask), g, (sec, smask), prm["primary_mean"], prm["secondary_mean"],
+                                          prm["secondary_variance"], prm["correlation"], prm["radii"], prm["max_n"],
+                                          c)
+        dt = time.perf_counter() - t0
+        O.set_sample_stride(1)
+        cells = np.arange(g.size) % stride == 0
+        flat_m = mask.ravel(order="F") == 0
+        sampled = int(np.sum(cells & flat_m))
+        res["cpu_baseline"] = {"value": sampled / dt, "unit": "estimated cells/s", "cores": 16, "kind": "port",
+                               "sample": "every 64th cell of the same job (%d estimated cells)" % sampled}
+        sel = cells
+        res["equals_oracle_on_sample"] = bool(np.array_equal(out.ravel(order="F")[sel], od.ravel(order="F")[sel]))
+    return res
+
+
 def make_serving_trace(n_jobs=60, rate=20.0, seed=1005):
     """Poisson request stream of mixed OK / SK / IK / LVM jobs: grid axes
     uniform in [32, 256] x [32, 256] x [32, 128], informed fraction
@@ -351,6 +438,9 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the request-stream (p50/p99) measurement")
     ap.add_argument("--no-io", action="store_true", help="skip the property-file write/read measurement")
     ap.add_argument("--no-sis", action="store_true", help="skip the sequential indicator simulation measurement")
+    ap.add_argument("--no-cokriging", action="store_true", help="skip the cokriging measurement")
+    ap.add_argument("--cokriging-only", action="store_true",
+                    help="run only the cokriging measurement (for profiling)")
     ap.add_argument("--sis-only", action="store_true",
                     help="run only the sequential indicator simulation measurement (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -385,6 +475,11 @@ def main():
     lib.hpgl_amd_indicator_kriging_dev.argtypes = [vp, vp, C.POINTER(abi._HPGL_SHAPE),
                                                    C.POINTER(abi._HPGL_IK_PARAMS), C.c_int, vp, vp, vp]
 
+    if args.cokriging_only:
+        print(json.dumps({"cokriging": cokriging_bench(torch, lib, abi, hpgl_wrap, steps=args.steps,
+                                                       warmup=args.warmup, cpu=not args.no_cpu_baseline)}),
+              flush=True)
+        return
     if args.sis_only:
         print(json.dumps({"sis": sis_bench(torch, lib, abi, hpgl_wrap, steps=args.steps, warmup=args.warmup,
                                            cpu=not args.no_cpu_baseline)}), flush=True)
@@ -530,6 +625,8 @@ def main():
 
     if not args.no_sis:
         out["sis"] = sis_bench(torch, lib, abi, hpgl_wrap, cpu=not args.no_cpu_baseline)
+    if not args.no_cokriging:
+        out["cokriging"] = cokriging_bench(torch, lib, abi, hpgl_wrap, cpu=not args.no_cpu_baseline)
     if not args.no_io:
         import hpgl_amd.geo_bsd as geo_api
 

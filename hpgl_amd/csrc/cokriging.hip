@@ -14,9 +14,11 @@
 // right-hand side of row i, not of the pivot row of column i: the two agree
 // whenever the pivots come in row order, as for covariance matrices) and
 // subtracts A[pivot row][j] X[j] for j from n down to i+1.  Rows live in LDS
-// (stride 66 doubles); lane i owns rows i and i + 64.  Contraction is off,
-// so with the same covariances the weights are the reference's bit for
-// bit.  combine (:82-104) then rounds to float after each double addition.
+// sized for max_n + 1 unknowns (stride max_n + 2 doubles), so small systems
+// leave room for many wavefronts per CU; lane i owns rows i and i + 64.
+// Contraction is off, so with the same covariances the weights are the
+// reference's bit for bit.  combine (:82-104) then rounds to float after
+// each double addition.
 #include "counters.h"
 #include "devcov.h"
 #include "kernels.h"
@@ -27,15 +29,26 @@ namespace hpgl_amd {
 
 namespace {
 
-constexpr int kCoRows = kMaxNeighbours + 1;
-
+// The wavefront's LDS for systems of up to S = max_n + 1 unknowns.
 struct CoLds {
-    double A[kCoRows][kCoRows + 1];
-    double B[kCoRows];
-    double X[kCoRows];
-    int order[kCoRows];
-    int used[kCoRows];
-    int xyz[3][kMaxNeighbours];
+    double *A;  // [S][S + 1]
+    double *B;  // [S]
+    double *X;  // [S]
+    int *order;  // [S]
+    int *used;   // [S]
+    int *xyz;    // [3][max_n]
+    int stride;
+    __device__ CoLds(double *base, int max_n) {
+        const int S = max_n + 1;
+        stride = S + 1;
+        A = base;
+        B = A + (size_t)S * stride;
+        X = B + S;
+        order = reinterpret_cast<int *>(X + S);
+        used = order + S;
+        xyz = used + S;
+    }
+    __device__ double &a(int r, int c) { return A[r * stride + c]; }
 };
 
 __device__ inline double co_cov(const DevCov &c, int dx, int dy, int dz) {
@@ -43,7 +56,9 @@ __device__ inline double co_cov(const DevCov &c, int dx, int dy, int dz) {
 }
 
 __global__ __launch_bounds__(64) void k_cokriging(CokrigingArgs a) {
-    __shared__ CoLds sh;
+    extern __shared__ double co_lds[];
+    CoLds sh(co_lds, a.max_n);
+    const int mn = a.max_n;
     const int lane = threadIdx.x;
     const int t = blockIdx.x;
     const int n = min(a.cnt[t], a.max_n);
@@ -55,24 +70,24 @@ __global__ __launch_bounds__(64) void k_cokriging(CokrigingArgs a) {
     if (lane < n) {
         nb = a.nbr[(size_t)t * a.max_n + lane];
         cell_xyz(nb, a.X, a.Y, xi, yi, zi);
-        sh.xyz[0][lane] = xi;
-        sh.xyz[1][lane] = yi;
-        sh.xyz[2][lane] = zi;
+        sh.xyz[lane] = xi;
+        sh.xyz[mn + lane] = yi;
+        sh.xyz[2 * mn + lane] = zi;
     }
     __syncthreads();
 
     // build_system (simple_cokriging_markI.cpp:19-55)
     if (lane < n) {
         for (int j = 0; j < n; ++j)
-            sh.A[lane][j] = co_cov(a.cov, sh.xyz[0][j] - xi, sh.xyz[1][j] - yi, sh.xyz[2][j] - zi);
+            sh.a(lane, j) = co_cov(a.cov, sh.xyz[j] - xi, sh.xyz[mn + j] - yi, sh.xyz[2 * mn + j] - zi);
         const double cr = a.coef * co_cov(a.cross, xi - cx, yi - cy, zi - cz);
-        sh.A[lane][n] = cr;
-        sh.A[n][lane] = cr;
+        sh.a(lane, n) = cr;
+        sh.a(n, lane) = cr;
         sh.B[lane] = co_cov(a.cov, cx - xi, cy - yi, cz - zi);
     }
     if (lane == 0) {
         sh.B[n] = a.cross0;
-        sh.A[n][n] = a.sec_var;
+        sh.a(n, n) = a.sec_var;
     }
     for (int r = lane; r < size; r += 64) sh.used[r] = 0;
     __syncthreads();
@@ -80,26 +95,26 @@ __global__ __launch_bounds__(64) void k_cokriging(CokrigingArgs a) {
     // gauss_solve (gauss_solver.cpp:5-52)
     bool ok = n > 0;
     for (int i = 0; ok && i < size; ++i) {
-        const uint64_t lo = __ballot(lane < size && sh.used[lane] == 0 && sh.A[lane][i] != 0.0);
+        const uint64_t lo = __ballot(lane < size && sh.used[lane] == 0 && sh.a(lane, i) != 0.0);
         int row = lo ? __ffsll((unsigned long long)lo) - 1 : -1;
-        if (row < 0 && size > 64 && sh.used[64] == 0 && sh.A[64][i] != 0.0) row = 64;
+        if (row < 0 && size > 64 && sh.used[64] == 0 && sh.a(64, i) != 0.0) row = 64;
         if (row < 0) {
             ok = false;  // the matrix is singular
             break;
         }
-        const double coef = sh.A[row][i];
+        const double coef = sh.a(row, i);
         __syncthreads();
         if (lane == 0) {
             sh.used[row] = 1;
             sh.order[i] = row;
             sh.B[row] = sh.B[row] / coef;
         }
-        for (int j = i + lane; j < size; j += 64) sh.A[row][j] = sh.A[row][j] / coef;
+        for (int j = i + lane; j < size; j += 64) sh.a(row, j) = sh.a(row, j) / coef;
         __syncthreads();
         for (int r = lane; r < size; r += 64) {
             if (sh.used[r] != 0) continue;
-            const double c = sh.A[r][i];
-            for (int k = i; k < size; ++k) sh.A[r][k] = sh.A[r][k] - c * sh.A[row][k];
+            const double c = sh.a(r, i);
+            for (int k = i; k < size; ++k) sh.a(r, k) = sh.a(r, k) - c * sh.a(row, k);
             sh.B[r] = sh.B[r] - c * sh.B[row];
         }
         __syncthreads();
@@ -116,8 +131,8 @@ __global__ __launch_bounds__(64) void k_cokriging(CokrigingArgs a) {
             if (lane == (j & 63)) sh.X[j] = j < 64 ? acc0 : acc1;
             __syncthreads();
             const double xj = sh.X[j];
-            if (lane < j) acc0 = acc0 - sh.A[row0][j] * xj;
-            if (lane + 64 < j) acc1 = acc1 - sh.A[row1][j] * xj;
+            if (lane < j) acc0 = acc0 - sh.a(row0, j) * xj;
+            if (lane + 64 < j) acc1 = acc1 - sh.a(row1, j) * xj;
         }
     }
 
@@ -147,7 +162,9 @@ __global__ __launch_bounds__(64) void k_cokriging(CokrigingArgs a) {
 
 void launch_cokriging(const CokrigingArgs &a, hipStream_t s) {
     if (a.T <= 0) return;
-    hipLaunchKernelGGL(k_cokriging, dim3(a.T), dim3(64), 0, s, a);
+    const size_t S = (size_t)a.max_n + 1;
+    const size_t lds = sizeof(double) * (S * (S + 1) + 2 * S) + sizeof(int) * (2 * S + 3 * (size_t)a.max_n);
+    hipLaunchKernelGGL(k_cokriging, dim3(a.T), dim3(64), lds, s, a);
 }
 
 }  // namespace hpgl_amd

@@ -985,6 +985,7 @@ static void cokriging(const float *in, const unsigned char *in_mask, const float
         std::vector<double> A, B, X;
 #pragma omp for schedule(static)
         for (int node = 0; node < N; ++node) {
+            if (g_sample_stride > 1 && node % g_sample_stride != 0) continue;
             float result;
             if (in_mask[node] != 0) {
                 result = in[node];
