@@ -1,7 +1,8 @@
-"""Builds hpgl_amd/hpgl.so for gfx950 with hipcc (no CMake, no JIT).
+"""Builds hpgl_amd/hpgl.so and hpgl_amd/_cvariogram.so for gfx950 with
+hipcc (no CMake, no JIT).
 
 Usage: python hpgl_amd/build.py [--jobs N]
-Objects go to hpgl_amd/_build/; the library to hpgl_amd/hpgl.so.
+Objects go to hpgl_amd/_build/.
 """
 import argparse
 import concurrent.futures
@@ -13,6 +14,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "hpgl.so")
+# the experimental-variogram module, a library of its own as in the reference
+VARIOGRAM_OUT = os.path.join(HERE, "_cvariogram.so")
+VARIOGRAM_SOURCES = ["variogram.hip"]
 
 SOURCES = ["model.cpp", "propio.cpp", "solve.hip", "solve_quad.hip", "search.hip", "cluster.hip", "engine.hip", "api.hip", "sgs.hip", "server.hip", "cokriging.hip"]
 
@@ -43,13 +47,15 @@ def build(jobs=None):
     os.makedirs(BUILD, exist_ok=True)
     jobs = jobs or min(8, os.cpu_count() or 4)
     with concurrent.futures.ThreadPoolExecutor(jobs) as ex:
-        objs = list(ex.map(_compile, SOURCES))
-    tmp = OUT + ".tmp"
-    cmd = ["hipcc", "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError("link failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
-    os.replace(tmp, OUT)
+        objs = list(ex.map(_compile, SOURCES + VARIOGRAM_SOURCES))
+    n = len(SOURCES)
+    for out, o in ((OUT, objs[:n]), (VARIOGRAM_OUT, objs[n:])):
+        tmp = out + ".tmp"
+        cmd = ["hipcc", "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + o
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("link failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
+        os.replace(tmp, out)
     return OUT
 
 

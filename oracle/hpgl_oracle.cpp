@@ -36,6 +36,7 @@
 #include <vector>
 #include <omp.h>
 
+#include "../include/cvariogram.h"
 #include "../include/hpgl.h"
 
 namespace orc {
@@ -1387,6 +1388,220 @@ ORACLE_API int oracle_simple_kriging_weights(float *center, float *xs, float *ys
         g_last_error = e.what();
         return -1;
     }
+}
+
+// ---- experimental variograms (src/geo_bsd/_cvariogram) ----
+// Pinned by: the solved-problem 5.2 variograms (point sets), brute force
+// over all cell pairs on small grids and a linear-trend closed form (grid),
+// the rotation closed forms (ellipsoid), a hand-worked column (layers).
+
+namespace orc {
+
+static double vdot(const vector_t &a, const vector_t &b) {  // variograms.cpp:9-17
+    double r = 0.0;
+    for (int i = 0; i < 3; ++i) r += a.m_data[i] * b.m_data[i];
+    return r;
+}
+
+static bool vtunnel(const variogram_search_template_t &t, const vector_t &v) {  // :26-40
+    const double ss1 = vdot(v, t.m_ellipsoid.m_direction1);
+    const double ss2 = vdot(v, t.m_ellipsoid.m_direction2);
+    const double ss3 = vdot(v, t.m_ellipsoid.m_direction3);
+    const double s2 = ss2 / t.m_ellipsoid.m_R2;
+    const double s3 = ss3 / t.m_ellipsoid.m_R3;
+    const double dist = sqrt(s2 * s2 + s3 * s3);
+    return (dist <= 1.0) && (t.m_tol_distance * dist <= ss1);
+}
+
+static void vwindow(const variogram_search_template_t &t, double mn[3], double mx[3]) {  // :84-128
+    for (int c = 0; c < 3; ++c) {
+        mn[c] = 1e10;
+        mx[c] = -1e10;
+    }
+    for (int i = 0; i < 2; ++i)
+        for (int j = -1; j < 3; j += 2)
+            for (int k = -1; k < 3; k += 2) {
+                double DI[3], DJ[3], DK[3], V[3];
+                for (int c = 0; c < 3; ++c) {
+                    DI[c] = t.m_ellipsoid.m_direction1.m_data[c] * t.m_lag_separation;
+                    DI[c] = DI[c] * t.m_num_lags;
+                    DI[c] = DI[c] * i;
+                    DJ[c] = t.m_ellipsoid.m_direction2.m_data[c] * t.m_ellipsoid.m_R2;
+                    DJ[c] = DJ[c] * j;
+                    DK[c] = t.m_ellipsoid.m_direction3.m_data[c] * t.m_ellipsoid.m_R3;
+                    DK[c] = DK[c] * k;
+                    V[c] = DI[c] + DJ[c];
+                    V[c] = V[c] + DK[c];
+                }
+                for (int c = 0; c < 3; ++c) {
+                    if (V[c] < mn[c]) mn[c] = V[c];
+                    if (V[c] > mx[c]) mx[c] = V[c];
+                }
+            }
+}
+
+}  // namespace orc
+
+// calc_variograms (variograms.cpp:362-446) with percentToUse = 100 (the
+// reference's sampling below 100 draws from the process-global rand()).
+ORACLE_API void oracle_calc_variograms(variogram_search_template_t *templ, hard_data_t *data, float *result,
+                                       int result_length) {
+    g_last_error.clear();
+    GUARD_BEGIN
+    const int lag_count = templ->m_num_lags <= result_length ? templ->m_num_lags : result_length;
+    std::vector<double> sum(lag_count, 0.0);
+    std::vector<int> cnt(lag_count, 0);
+    double mn[3], mx[3];
+    vwindow(*templ, mn, mx);
+    for (int i2 = (int)mn[0]; i2 <= mx[0]; ++i2)
+        for (int j2 = (int)mn[1]; j2 <= mx[1]; ++j2)
+            for (int k2 = (int)mn[2]; k2 <= mx[2]; ++k2) {
+                vector_t vec = {{(double)i2, (double)j2, (double)k2}};
+                if (!vtunnel(*templ, vec)) continue;
+                int doffset = 0, moffset = 0;
+                {
+                    double a = 0, b = 0;
+                    for (int c = 0; c < 3; ++c) {
+                        a += vec.m_data[c] * data->m_data_strides[c];
+                        b += vec.m_data[c] * data->m_mask_strides[c];
+                    }
+                    doffset = (int)a;
+                    moffset = (int)b;
+                }
+                for (int z1 = 0; z1 < data->m_data_shape[2]; ++z1)
+                    for (int y1 = 0; y1 < data->m_data_shape[1]; ++y1)
+                        for (int x1 = 0; x1 < data->m_data_shape[0]; ++x1) {
+                            const long long mi = (long long)data->m_mask_strides[0] * x1 +
+                                                 (long long)data->m_mask_strides[1] * y1 +
+                                                 (long long)data->m_mask_strides[2] * z1;
+                            const long long di = (long long)data->m_data_strides[0] * x1 +
+                                                 (long long)data->m_data_strides[1] * y1 +
+                                                 (long long)data->m_data_strides[2] * z1;
+                            if (data->m_mask[mi] == 0) continue;
+                            const double v1 = data->m_data[di];
+                            const int x = x1 + i2, y = y1 + j2, z = z1 + k2;
+                            if (x < 0 || y < 0 || z < 0 || x >= data->m_mask_shape[0] ||
+                                y >= data->m_mask_shape[1] || z >= data->m_mask_shape[2])
+                                continue;
+                            if (data->m_mask[mi + moffset] == 0) continue;
+                            double r = 0;
+                            for (int c = 0; c < 3; ++c) r += vec.m_data[c] * vec.m_data[c];
+                            const double dist = sqrt(r);
+                            const double v2 = data->m_data[di + doffset];
+                            double var = v1 - v2;
+                            var = var * var;
+                            // update_lags (:331-360)
+                            int lag_min = (int)ceil((dist - (templ->m_lag_width / 2) - templ->m_first_lag_distance) /
+                                                    templ->m_lag_separation);
+                            if (lag_min >= lag_count) continue;
+                            int lag_max = (int)floor((dist + (templ->m_lag_width / 2) - templ->m_first_lag_distance) /
+                                                     templ->m_lag_separation);
+                            if (lag_max < 0) continue;
+                            if (lag_max >= lag_count) lag_max = lag_count - 1;
+                            if (lag_min < 0) lag_min = 0;
+                            for (int l = lag_min; l <= lag_max; ++l) {
+                                sum[l] += var;
+                                cnt[l] += 1;
+                            }
+                        }
+            }
+    for (int i = 0; i < lag_count; ++i) result[i] = (float)(sum[i] / cnt[i] / 2);
+    GUARD_END
+}
+
+// calc_variograms_from_point_set (variograms.cpp:448-502)
+ORACLE_API void oracle_calc_variograms_from_point_set(variogram_search_template_t *templ, cont_point_set_t *ps,
+                                                      float *result, int result_length) {
+    g_last_error.clear();
+    GUARD_BEGIN
+    const int lag_count = templ->m_num_lags <= result_length ? templ->m_num_lags : result_length;
+    std::vector<double> sum(lag_count, 0.0), start(lag_count), end(lag_count);
+    std::vector<int> cnt(lag_count, 0);
+    for (int i = 0; i < lag_count; ++i) {  // init_lag_list (:138-149)
+        const double d = i * templ->m_lag_separation + templ->m_first_lag_distance;
+        start[i] = d - templ->m_lag_width / 2;
+        end[i] = d + templ->m_lag_width / 2;
+    }
+    for (int a = 0; a < ps->size; ++a)
+        for (int b = 0; b < ps->size; ++b) {
+            vector_t vec;
+            vec.m_data[0] = ps->xs[a] - ps->xs[b];
+            vec.m_data[1] = ps->ys[a] - ps->ys[b];
+            vec.m_data[2] = ps->zs[a] - ps->zs[b];
+            if (!vtunnel(*templ, vec)) continue;
+            double r = 0;
+            for (int c = 0; c < 3; ++c) r += vec.m_data[c] * vec.m_data[c];
+            const double dist = sqrt(r);
+            const double var = pow(ps->values[a] - ps->values[b], 2);
+            for (int l = 0; l < lag_count; ++l)
+                if (start[l] <= dist && dist < end[l]) {
+                    sum[l] += var;
+                    cnt[l] += 1;
+                }
+        }
+    for (int i = 0; i < lag_count; ++i) {
+        result[i] = (float)(sum[i] / cnt[i] / 2);
+        if (cnt[i] == 0) result[i] = 0;
+    }
+    GUARD_END
+}
+
+// fill_ellipsoid_directions (ellipsoid.cpp:39-71)
+ORACLE_API void oracle_fill_ellipsoid_directions(ellipsoid_t *ell, double azimuth, double dip, double rotation) {
+    const double coef = 3.14159265358 / 180.0;
+    const double ra = azimuth * coef, rd = dip * coef, rr = rotation * coef;
+    double A[9] = {cos(ra), -sin(ra), 0, sin(ra), cos(ra), 0, 0, 0, 1};
+    double B[9] = {cos(rd), 0, -sin(rd), 0, 1, 0, sin(rd), 0, cos(rd)};
+    double C[9] = {1, 0, 0, 0, cos(rr), -sin(rr), 0, sin(rr), cos(rr)};
+    auto mul = [](const double *m1, const double *m2, double *m3) {
+        double r[9];
+        for (int row = 0; row < 3; ++row)
+            for (int col = 0; col < 3; ++col) {
+                double acc = 0.0;
+                for (int i = 0; i < 3; ++i) acc += m1[row * 3 + i] * m2[i * 3 + col];
+                r[row * 3 + col] = acc;
+            }
+        std::memcpy(m3, r, sizeof(r));
+    };
+    double M[9] = {0};
+    mul(A, B, M);
+    mul(M, C, M);
+    for (int i = 0; i < 3; ++i) {
+        ell->m_direction1.m_data[i] = M[i * 3 + 0];
+        ell->m_direction2.m_data[i] = M[i * 3 + 1];
+        ell->m_direction3.m_data[i] = M[i * 3 + 2];
+    }
+}
+
+// stack_layers (stack_layers.h:6-63); cells below the cube (k < 0), which
+// the reference writes out of bounds when the running depth is negative,
+// are skipped.
+ORACLE_API void oracle_stack_layers(float_data_t *layers, int *markers, int count, int nz, float scalez, int blank,
+                                    float_data_t *result) {
+    const int nx = layers[0].m_data_shape[0], ny = layers[0].m_data_shape[1];
+    std::vector<double> cum((size_t)nx * ny, 0.0);
+    for (int layer = 0; layer < count; ++layer)
+        for (int i = 0; i < nx; ++i)
+            for (int j = 0; j < ny; ++j) {
+                const int m = layers[layer].m_data_strides[0] * i + layers[layer].m_data_strides[1] * j;
+                const float t = layers[layer].m_data[m];
+                if (t > 0) {
+                    const float old_k = cum[m];
+                    cum[m] += t / scalez;
+                    const float new_k = cum[m];
+                    int k_next = std::floor(new_k) + 1;
+                    if (k_next > nz) k_next = nz;
+                    for (int k = std::max((int)std::ceil(old_k), 0); k < k_next; ++k)
+                        result->m_data[result->m_data_strides[0] * i + result->m_data_strides[1] * j +
+                                       result->m_data_strides[2] * k] = markers[layer];
+                } else {
+                    const int k_start = std::ceil(cum[m] + (t / scalez));
+                    for (int k = std::max(k_start, 0); k < nz; ++k)
+                        result->m_data[result->m_data_strides[0] * i + result->m_data_strides[1] * j +
+                                       result->m_data_strides[2] * k] = blank;
+                    cum[m] += (t / scalez);
+                }
+            }
 }
 
 // ---- introspection hooks used by the tests ----

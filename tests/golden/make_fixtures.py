@@ -14,7 +14,11 @@ Sources:
       gslib/kriging/OK_kt3d.out and SK_kt3d.out (GSLIB estimates);
   * dist/misc/test/test_data/NEW_TEST_PROP.INC, NEW_TEST_PROP_01.INC and
     mp_0_8_52_55_1.inc (55 x 52 x 1 property files the reference's test
-    scripts load), copied unchanged to inc/.
+    scripts load), copied unchanged to inc/;
+  * sample-scripts/solved_problems/5.2. Variogram Calculation/
+      3welldata.txt and allwelldata.txt (inputs, copied to variogram/) and
+      Result/2d/1_2d_variogram.txt, Result/3d/1_3d_variogram.txt (printed
+      semivariograms of 1_2d_variogram.py and 1_3d_variogram.py).
 """
 import json
 import os
@@ -106,8 +110,36 @@ def inc_files():
         print("copied", name)
 
 
+def problem_5_2():
+    """The point-set semivariograms of solved problem 5.2 (printed results of
+    its scripts, Result/2d/1_2d_variogram.txt and Result/3d/1_3d_variogram.txt)
+    with the scripts' search templates (1_2d_variogram.py:36-79,
+    1_3d_variogram.py:104-116); the two well-data files are copied beside."""
+    d = os.path.join(REF, "sample-scripts", "solved_problems", "5.2. Variogram Calculation")
+    out = os.path.join(HERE, "variogram")
+    os.makedirs(out, exist_ok=True)
+    for name in ["3welldata.txt", "allwelldata.txt"]:
+        shutil.copyfile(os.path.join(d, name), os.path.join(out, name))
+    r2 = open(os.path.join(d, "Result", "2d", "1_2d_variogram.txt")).read()
+    v2, h2 = r2.split("Horizontal")
+    r3 = open(os.path.join(d, "Result", "3d", "1_3d_variogram.txt")).read()
+    ell = lambda R2, R3, az, dip: {"R1": 1, "R2": R2, "R3": R3, "azimuth": az, "dip": dip, "rotation": 0}
+    return {
+        "2d_vertical": {"file": "3welldata.txt", "x": "X(Easting)(m)", "y": None, "z": "Z(m)", "value": "Por",
+                        "lag_width": 4, "lag_separation": 4, "tol_distance": 4, "num_lags": 8,
+                        "ellipsoid": ell(0.1, 0.1, 0, 90), "expected": _floats(v2.split(":", 1)[1])},
+        "2d_horizontal": {"file": "3welldata.txt", "x": "X(Easting)(m)", "y": None, "z": "Z(m)", "value": "Por",
+                          "lag_width": 1000, "lag_separation": 1000, "tol_distance": 1000, "num_lags": 3,
+                          "ellipsoid": ell(0.1, 4.5, 0, 0), "expected": _floats(h2.split(":", 1)[1])},
+        "3d_vertical": {"file": "allwelldata.txt", "x": "X", "y": "Y", "z": "Z", "value": "Por",
+                        "lag_width": 4, "lag_separation": 4, "tol_distance": 1.5, "num_lags": 11,
+                        "ellipsoid": ell(0.1, 10, 0, 90), "expected": _floats(r3.split(":", 1)[1])},
+    }
+
+
 def main():
-    for name, fn in [("problem_7_2.json", problem_7_2), ("gstl_parta.json", gstl_parta)]:
+    for name, fn in [("problem_7_2.json", problem_7_2), ("gstl_parta.json", gstl_parta),
+                     (os.path.join("variogram", "problem_5_2.json"), problem_5_2)]:
         with open(os.path.join(HERE, name), "w") as f:
             json.dump(fn(), f, indent=1)
         print("wrote", name)

@@ -29,6 +29,17 @@ def _load_abi():
 abi = _load_abi()
 
 
+def _load_cvar_abi():
+    spec = importlib.util.spec_from_file_location("_cvar_abi", os.path.join(ROOT, "hpgl_amd", "geo_bsd",
+                                                                             "_cvar_abi.py"))
+    m = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(m)
+    return m
+
+
+cvabi = _load_cvar_abi()
+
+
 def _load():
     src = os.path.join(ORACLE_DIR, "hpgl_oracle.cpp")
     if not os.path.exists(ORACLE_SO) or os.path.getmtime(ORACLE_SO) < os.path.getmtime(src):
@@ -65,6 +76,13 @@ lib.oracle_simple_cokriging_mark2.argtypes = [_P(abi._HPGL_CONT_MASKED_ARRAY), _
                                               _P(abi._HPGL_COKRIGING_M2_PARAMS), _P(abi._HPGL_CONT_MASKED_ARRAY)]
 lib.oracle_gauss_solve.restype = C.c_int
 lib.oracle_gauss_solve.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+lib.oracle_calc_variograms.argtypes = [_P(cvabi.variogram_search_template_t), _P(cvabi.hard_data_t), C.c_void_p,
+                                       C.c_int]
+lib.oracle_calc_variograms_from_point_set.argtypes = [_P(cvabi.variogram_search_template_t),
+                                                      _P(cvabi.cont_point_set_t), C.c_void_p, C.c_int]
+lib.oracle_fill_ellipsoid_directions.argtypes = [_P(cvabi.ellipsoid_t), C.c_double, C.c_double, C.c_double]
+lib.oracle_stack_layers.argtypes = [_P(cvabi.float_data_t), _P(C.c_int), C.c_int, C.c_int, C.c_float, C.c_int,
+                                    _P(cvabi.float_data_t)]
 lib.oracle_simple_kriging_weights.restype = C.c_int
 lib.oracle_simple_kriging_weights.argtypes = [C.c_void_p] * 4 + [C.c_int, _P(abi._HPGL_COV_PARAMS), C.c_void_p,
                                                                   _P(C.c_double)]
@@ -314,6 +332,51 @@ def gauss_solve(a, b):
     x = np.zeros_like(b)
     ok = lib.oracle_gauss_solve(len(b), _ptr(a), _ptr(b), _ptr(x))
     return bool(ok), x
+
+
+def ellipsoid(R1, R2, R3, azimuth, dip, rotation):
+    e = cvabi.ellipsoid_t(R1=R1, R2=R2, R3=R3)
+    lib.oracle_fill_ellipsoid_directions(C.byref(e), azimuth, dip, rotation)
+    return e
+
+
+def variogram_template(lag_width, lag_separation, tol_distance, num_lags, first_lag_distance, ell):
+    return cvabi.variogram_search_template_t(lag_width=lag_width, lag_separation=lag_separation,
+                                             tol_distance=tol_distance, num_lags=num_lags,
+                                             first_lag_distance=first_lag_distance, ellipsoid=ell)
+
+
+def calc_variograms(templ, values, mask):
+    """Gridded semivariogram of (values, mask) arrays (any strides)."""
+    v = np.asarray(values, np.float32)
+    m = np.asarray(mask, np.uint8)
+    st = lambda a: (C.c_int * 3)(*[s // a.itemsize for s in a.strides])
+    hd = cvabi.hard_data_t(data=v.ctypes.data_as(_P(C.c_float)), mask=m.ctypes.data_as(_P(C.c_ubyte)),
+                           data_shape=(C.c_int * 3)(*v.shape), data_strides=st(v),
+                           mask_shape=(C.c_int * 3)(*m.shape), mask_strides=st(m))
+    out = np.zeros(templ.num_lags, np.float32)
+    lib.oracle_calc_variograms(C.byref(templ), C.byref(hd), _ptr(out), out.size)
+    _check()
+    return out
+
+
+def calc_variograms_from_point_set(templ, xs, ys, zs, values):
+    cols = [np.ascontiguousarray(a, np.float32) for a in (xs, ys, zs, values)]
+    ps = cvabi.cont_point_set_t(xs=cols[0].ctypes.data_as(_P(C.c_float)), ys=cols[1].ctypes.data_as(_P(C.c_float)),
+                                zs=cols[2].ctypes.data_as(_P(C.c_float)),
+                                values=cols[3].ctypes.data_as(_P(C.c_float)), values_count=cols[3].size)
+    out = np.zeros(templ.num_lags, np.float32)
+    lib.oracle_calc_variograms_from_point_set(C.byref(templ), C.byref(ps), _ptr(out), out.size)
+    _check()
+    return out
+
+
+def stack_layers(layers, markers, nz, scalez, blank_value, result):
+    fd = lambda a: cvabi.float_data_t(data=a.ctypes.data_as(_P(C.c_float)), data_shape=(C.c_int * 3)(*a.shape),
+                                      data_strides=(C.c_int * 3)(*[s // a.itemsize for s in a.strides]))
+    ls = (cvabi.float_data_t * len(layers))(*[fd(a) for a in layers])
+    lib.oracle_stack_layers(ls, (C.c_int * len(markers))(*markers), len(layers), nz, scalez, blank_value,
+                            C.byref(fd(result)))
 
 
 def simple_kriging_weights(center, xs, ys, zs, cm):

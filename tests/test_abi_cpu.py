@@ -25,13 +25,53 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_variogram_library_exports_every_declared_symbol():
+    src = open(os.path.join(ROOT, "include", "cvariogram.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = sorted(set(re.findall(r"\b([a-z_]+)\s*\(", src)) - {"if", "for"})
+    lib = C.CDLL(os.path.join(ROOT, "hpgl_amd", "_cvariogram.so"))
+    assert {"calc_variograms", "calc_variograms_from_point_set", "fill_ellipsoid_directions",
+            "cvar_stack_layers"} <= set(names)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_variogram_struct_layouts():
+    """_cvariogram/api.h:23-90 as the C compiler lays them out from include/cvariogram.h."""
+    import subprocess
+    import tempfile
+
+    v = O.cvabi
+    prog = r"""
+#include <stdio.h>
+#include <stddef.h>
+#include "cvariogram.h"
+int main(void) {
+    printf("%zu %zu %zu %zu %zu %zu\n", sizeof(struct ellipsoid_t), sizeof(struct variogram_search_template_t),
+           offsetof(struct variogram_search_template_t, m_ellipsoid), sizeof(struct cont_point_set_t),
+           sizeof(struct hard_data_t), sizeof(struct float_data_t));
+    return 0;
+}
+"""
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "l.c")
+        open(c, "w").write(prog)
+        exe = os.path.join(d, "l")
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        got = [int(x) for x in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
+    want = [C.sizeof(v.ellipsoid_t), C.sizeof(v.variogram_search_template_t),
+            v.variogram_search_template_t.ellipsoid.offset, C.sizeof(v.cont_point_set_t), C.sizeof(v.hard_data_t),
+            C.sizeof(v.float_data_t)]
+    assert got == want == [96, 136, 40, 40, 64, 32]
+
+
 def test_struct_sizes_match_reference_layouts():
     # LP64 layouts of geo_bsd/hpgl/api.h measured for the reference
     a = O.abi
     sizes = {"_HPGL_SHAPE": 24, "_HPGL_CONT_MASKED_ARRAY": 40, "_HPGL_IND_MASKED_ARRAY": 48, "_HPGL_FLOAT_ARRAY": 32,
              "_HPGL_UBYTE_ARRAY": 32, "_HPGL_OK_PARAMS": 88, "_HPGL_SK_PARAMS": 104, "_HPGL_COV_PARAMS": 72,
              "_HPGL_SGS_PARAMS": 112, "_HPGL_MEDIAN_IK_PARAMS": 104, "_HPGL_IK_PARAMS": 96,
-             "hpgl_non_parametric_cdf_t": 24}
+             "hpgl_non_parametric_cdf_t": 24, "_HPGL_COKRIGING_M1_PARAMS": 120, "_HPGL_COKRIGING_M2_PARAMS": 184}
     for name, size in sizes.items():
         assert C.sizeof(getattr(a, name)) == size, name
     assert a._HPGL_SK_PARAMS.automatic_mean.offset == 88 and a._HPGL_SK_PARAMS.mean.offset == 96
