@@ -267,6 +267,47 @@ def cokriging_bench(torch, lib, abi, hpgl_wrap, steps=2, warmup=1, cpu=True):
     return res
 
 
+def variogram_bench(cpu=True):
+    """Experimental semivariogram of gridded data on the 256x256x128, 1%-informed
+    grid: a horizontal tunnel (azimuth 0, dip 0, half-bandwidths 10 and 2
+    cells, tolerance 2) of 20 lags of 4 cells.  Throughput = candidate pairs
+    (informed cell x tunnel offset) per second of the whole call (host arrays
+    in, PCIe copies included).  The CPU leg runs the oracle on the first 8 z
+    layers of the same data."""
+    from hpgl_amd.geo_bsd import cvariogram as cv
+
+    g, data, mask, _, _, _, _ = make_cfg2()
+    args = (4.0, 4.0, 2.0, 20, 0.0)
+    ell = (1, 10.0, 2.0, 0, 0, 0)
+    t = cv.VariogramSearchTemplate(*args, cv.Ellipsoid(*ell))
+    cv.CalcVariograms(t, (data, mask))  # warm-up
+    t0 = time.perf_counter()
+    _, vario = cv.CalcVariograms(t, (data, mask))
+    wall = time.perf_counter() - t0
+    st = cv.last_stats()
+    res = {"workload": "gridded semivariogram, 256x256x128 grid, 1% informed, horizontal tunnel (half-bandwidths "
+                       "10/2 cells, tolerance 2), 20 lags of 4 cells",
+           "offsets": st["offsets"], "candidate_pairs": st["pairs_tested"],
+           "value": st["pairs_tested"] / wall, "unit": "candidate pairs/s", "ms_per_call": wall * 1e3,
+           "device_ms": st["device_ms"], "variogram": [round(float(v), 6) for v in vario]}
+    if cpu:
+        import oracle_binding as O
+
+        sub = (np.asfortranarray(data[:, :, :8]), np.asfortranarray(mask[:, :, :8]))
+        ot = O.variogram_template(*args[:4], args[4], O.ellipsoid(*ell))
+        t0 = time.perf_counter()
+        want = O.calc_variograms(ot, *sub)
+        dt = time.perf_counter() - t0
+        pairs = int(np.sum(sub[1] != 0)) * st["offsets"]
+        _, got = cv.CalcVariograms(t, sub)
+        fin = np.isfinite(want)
+        res["cpu_baseline"] = {"value": pairs / dt, "unit": "candidate pairs/s", "cores": 1, "kind": "port",
+                               "sample": "the first 8 z layers (256x256x8) of the same data"}
+        res["max_rel_diff_vs_oracle_on_sample"] = float(
+            np.max(np.abs(got[fin] - want[fin]) / np.maximum(np.abs(want[fin]), 1e-6), initial=0.0))
+    return res
+
+
 def make_serving_trace(n_jobs=60, rate=20.0, seed=1005):
     """Poisson request stream of mixed OK / SK / IK / LVM jobs: grid axes
     uniform in [32, 256] x [32, 256] x [32, 128], informed fraction
@@ -439,6 +480,9 @@ def main():
     ap.add_argument("--no-io", action="store_true", help="skip the property-file write/read measurement")
     ap.add_argument("--no-sis", action="store_true", help="skip the sequential indicator simulation measurement")
     ap.add_argument("--no-cokriging", action="store_true", help="skip the cokriging measurement")
+    ap.add_argument("--no-variogram", action="store_true", help="skip the experimental variogram measurement")
+    ap.add_argument("--variogram-only", action="store_true",
+                    help="run only the experimental variogram measurement (for profiling)")
     ap.add_argument("--cokriging-only", action="store_true",
                     help="run only the cokriging measurement (for profiling)")
     ap.add_argument("--sis-only", action="store_true",
@@ -475,6 +519,9 @@ def main():
     lib.hpgl_amd_indicator_kriging_dev.argtypes = [vp, vp, C.POINTER(abi._HPGL_SHAPE),
                                                    C.POINTER(abi._HPGL_IK_PARAMS), C.c_int, vp, vp, vp]
 
+    if args.variogram_only:
+        print(json.dumps({"variogram": variogram_bench(cpu=not args.no_cpu_baseline)}), flush=True)
+        return
     if args.cokriging_only:
         print(json.dumps({"cokriging": cokriging_bench(torch, lib, abi, hpgl_wrap, steps=args.steps,
                                                        warmup=args.warmup, cpu=not args.no_cpu_baseline)}),
@@ -627,6 +674,8 @@ def main():
         out["sis"] = sis_bench(torch, lib, abi, hpgl_wrap, cpu=not args.no_cpu_baseline)
     if not args.no_cokriging:
         out["cokriging"] = cokriging_bench(torch, lib, abi, hpgl_wrap, cpu=not args.no_cpu_baseline)
+    if not args.no_variogram:
+        out["variogram"] = variogram_bench(cpu=not args.no_cpu_baseline)
     if not args.no_io:
         import hpgl_amd.geo_bsd as geo_api
 
