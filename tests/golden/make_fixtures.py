@@ -18,7 +18,10 @@ Sources:
   * sample-scripts/solved_problems/5.2. Variogram Calculation/
       3welldata.txt and allwelldata.txt (inputs, copied to variogram/) and
       Result/2d/1_2d_variogram.txt, Result/3d/1_3d_variogram.txt (printed
-      semivariograms of 1_2d_variogram.py and 1_3d_variogram.py).
+      semivariograms of 1_2d_variogram.py and 1_3d_variogram.py);
+  * andrey/test_data/BIG_SOFT_DATA_160_141_20.INC and
+    BIG_SOFT_DATA_CON_160_141_20.INC (the 166 x 141 x 20 inputs of
+    geo_testing/test_compare.py), parsed into big_soft_data.npz.
 """
 import json
 import os
@@ -137,6 +140,44 @@ def problem_5_2():
     }
 
 
+def _read_inc(path):
+    """Values of a property file: after the name line (the first line that
+    starts with a letter), white-space separated, "--" comments to the end of
+    the line, "/" terminator."""
+    vals = []
+    with open(path) as f:
+        for line in f:
+            if line[:1].isalpha():
+                break
+        for line in f:
+            t = line.split()
+            if not t or t[0].startswith("--"):
+                continue
+            if t[0].startswith("/"):
+                break
+            vals.extend(float(v) for v in t)
+    return vals
+
+
+def big_soft_data():
+    """The 166 x 141 x 20 property files of the reference's regression script
+    (geo_testing/test_compare.py:30-34, andrey/test_data/): the indicator
+    property (values 0/1, undefined -99) and the continuous one, stored as
+    arrays in Fortran order (cell = x + 166 y + 166*141 z)."""
+    import numpy as np
+
+    d = os.path.join(REF, "andrey", "test_data")
+    out = {}
+    for key, name in (("ind", "BIG_SOFT_DATA_160_141_20.INC"), ("con", "BIG_SOFT_DATA_CON_160_141_20.INC")):
+        v = np.array(_read_inc(os.path.join(d, name)), np.float64)
+        assert v.size == 166 * 141 * 20, v.size
+        mask = (v != -99).astype(np.uint8)
+        out[key + "_mask"] = mask
+        out[key] = np.where(mask == 1, v, -99).astype(np.float32)
+    np.savez_compressed(os.path.join(HERE, "big_soft_data.npz"), **out)
+    print("wrote big_soft_data.npz")
+
+
 def main():
     for name, fn in [("problem_7_2.json", problem_7_2), ("gstl_parta.json", gstl_parta),
                      (os.path.join("variogram", "problem_5_2.json"), problem_5_2)]:
@@ -144,6 +185,7 @@ def main():
             json.dump(fn(), f, indent=1)
         print("wrote", name)
     inc_files()
+    big_soft_data()
 
 
 if __name__ == "__main__":
