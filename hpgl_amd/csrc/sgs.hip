@@ -86,38 +86,49 @@ __device__ inline uint32_t mt_mix(uint32_t hi_word, uint32_t lo_word) {
     return (y >> 1) ^ ((y & 1u) ? kMtA : 0u);
 }
 
+// One workgroup per realisation.  Each twist writes the new state into the
+// other of two LDS buffers: the first n - m words read only the old state,
+// the rest (and the last word, which needs new words 0 and m - 1) read new
+// words of the first part, so a twist takes two barriers; every thread
+// tempers and stores its own new word as soon as it has it.
 __global__ __launch_bounds__(384) void k_mt11213b(const uint32_t *seeds, const long long *counts, long long stride,
                                                  uint32_t *out) {
-    __shared__ uint32_t x[kMtN];
+    __shared__ uint32_t buf[2][kMtN];
     const int r = blockIdx.x;
     const int k = threadIdx.x;
     const long long count = counts[r];
     uint32_t *o = out + (size_t)r * stride;
     if (k == 0) {
         uint32_t v = seeds[r];
-        x[0] = v;
+        buf[0][0] = v;
         for (int i = 1; i < kMtN; ++i) {
             v = 1812433253u * (v ^ (v >> 30)) + (uint32_t)i;
-            x[i] = v;
+            buf[0][i] = v;
         }
     }
     __syncthreads();
+    int cur = 0;
     for (long long base = 0; base < count; base += kMtN) {
-        // phase A: k < n-m, old values only
+        const uint32_t *x = buf[cur];
+        uint32_t *y = buf[cur ^ 1];
         uint32_t nv = 0;
-        if (k < kMtN - kMtM) nv = x[k + kMtM] ^ mt_mix(x[k], x[k + 1]);
+        // words k < n - m: old state only
+        if (k < kMtN - kMtM) {
+            nv = x[k + kMtM] ^ mt_mix(x[k], x[k + 1]);
+            y[k] = nv;
+        }
         __syncthreads();
-        if (k < kMtN - kMtM) x[k] = nv;
-        __syncthreads();
-        // phase B: n-m <= k < n-1, x[k+m-n] from phase A
-        if (k >= kMtN - kMtM && k < kMtN - 1) nv = x[k + kMtM - kMtN] ^ mt_mix(x[k], x[k + 1]);
-        __syncthreads();
-        if (k >= kMtN - kMtM && k < kMtN - 1) x[k] = nv;
-        __syncthreads();
-        if (k == 0) x[kMtN - 1] = x[kMtM - 1] ^ mt_mix(x[kMtN - 1], x[0]);
-        __syncthreads();
+        // words n - m <= k < n - 1 read new word k + m - n; the last word
+        // reads new words m - 1 and 0
+        if (k >= kMtN - kMtM && k < kMtN - 1) {
+            nv = y[k + kMtM - kMtN] ^ mt_mix(x[k], x[k + 1]);
+            y[k] = nv;
+        } else if (k == kMtN - 1) {
+            nv = y[kMtM - 1] ^ mt_mix(x[kMtN - 1], y[0]);
+            y[k] = nv;
+        }
         if (k < kMtN && base + k < count) {
-            uint32_t z = x[k];
+            uint32_t z = nv;
             z ^= (z >> 11) & 0xffffffffu;
             z ^= (z << 7) & 0x31b6ab00u;
             z ^= (z << 15) & 0xffe50000u;
@@ -125,6 +136,7 @@ __global__ __launch_bounds__(384) void k_mt11213b(const uint32_t *seeds, const l
             o[base + k] = z;
         }
         __syncthreads();
+        cur ^= 1;
     }
 }
 
