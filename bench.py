@@ -308,6 +308,67 @@ def variogram_bench(cpu=True):
     return res
 
 
+def sgs_bench(torch, lib, abi, hpgl_wrap, R=64, cpu=True):
+    """Sequential Gaussian simulation, R realisations batched in one call
+    (200x200x100 grid, 1% conditioning data as normal scores, spherical
+    ranges 30/30/6, sill 1, nugget 0, radii 30/30/6, max_neighbours 32, SK,
+    no transform, seeds 1000..1000+R-1), inputs resident in HBM; simulated
+    cells/s over all realisations.  The CPU leg runs the oracle on one
+    realisation of a 100x100x50 grid with the same density and model (the
+    reference's simulation is serial)."""
+    from cases import Cov, Grid, scattered
+
+    g = Grid(200, 200, 100)
+    data, mask = scattered(g.shape, 0.01, 1008, law="normal")
+    cov = Cov(0, (30, 30, 6), sill=1.0, nugget=0.0)
+    p = abi._HPGL_SGS_PARAMS(covariance_type=cov.type, ranges=(C.c_double * 3)(*cov.ranges),
+                             angles=(C.c_double * 3)(*cov.angles), sill=cov.sill, nugget=cov.nugget,
+                             radiuses=(C.c_int * 3)(30, 30, 6), max_neighbours=32, kriging_kind=1, seed=1000,
+                             min_neighbours=0)
+    dev = torch.device("cuda:0")
+    d_in = torch.from_numpy(data.ravel(order="F").copy()).to(dev)
+    d_m = torch.from_numpy(mask.ravel(order="F").copy()).to(dev)
+    d_out = torch.empty(R * g.size, dtype=torch.float32, device=dev)
+    d_om = torch.empty(R * g.size, dtype=torch.uint8, device=dev)
+    shape = abi._HPGL_SHAPE(data=(C.c_int * 3)(*g.shape), strides=(C.c_int * 3)(1, g.x, g.x * g.y))
+    seeds = (C.c_long * R)(*[1000 + r for r in range(R)])
+    stream = torch.cuda.current_stream(dev)
+    sp = C.c_void_p(stream.cuda_stream)
+
+    def run(n):
+        lib.hpgl_amd_sgs_simulation_batch_dev(d_in.data_ptr(), d_m.data_ptr(), C.byref(shape), C.byref(p), None,
+                                              None, None, seeds, n, d_out.data_ptr(), d_om.data_ptr(), sp)
+        hpgl_wrap.check_last_error()
+        return hpgl_wrap.last_stats()
+
+    run(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = run(R)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    n_sim = int(np.sum(mask == 0)) * R
+    res = {"workload": "SGS, %d realisations in one call, 200x200x100 grid, 1%% conditioning (normal scores), "
+                       "spherical ranges 30/30/6, radii 30/30/6, max_neighbours 32, SK" % R,
+           "simulated_cells": n_sim, "value": n_sim / wall, "unit": "simulated cells/s",
+           "ms_per_call": wall * 1e3, "ms_per_realisation": wall * 1e3 / R,
+           "ms_template_scan": st["ms_k_template"], "ms_solve": st["ms_solve"],
+           "solves": {"fp32_accepted": int(st["solves_fp32"]), "fp64": int(st["solves_fp64"])}}
+    if cpu:
+        import oracle_binding as O
+
+        gs = Grid(100, 100, 50)
+        sd, sm = scattered(gs.shape, 0.01, 1009, law="normal")
+        O.set_threads(1)
+        t0 = time.perf_counter()
+        O.sgs_simulation((sd, sm), gs, None, (30, 30, 6), 32, cov, 1000, kriging_type="sk")
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": int(np.sum(sm == 0)) / dt, "unit": "simulated cells/s", "cores": 1,
+                               "kind": "port", "sample": "one realisation of a 100x100x50 grid, same density "
+                                                         "and model (%.1f s)" % dt}
+    return res
+
+
 def make_serving_trace(n_jobs=60, rate=20.0, seed=1005):
     """Poisson request stream of mixed OK / SK / IK / LVM jobs: grid axes
     uniform in [32, 256] x [32, 256] x [32, 128], informed fraction
@@ -479,6 +540,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the request-stream (p50/p99) measurement")
     ap.add_argument("--no-io", action="store_true", help="skip the property-file write/read measurement")
     ap.add_argument("--no-sis", action="store_true", help="skip the sequential indicator simulation measurement")
+    ap.add_argument("--no-sgs", action="store_true", help="skip the batched SGS measurement")
+    ap.add_argument("--sgs-only", action="store_true", help="run only the batched SGS measurement (for profiling)")
     ap.add_argument("--no-cokriging", action="store_true", help="skip the cokriging measurement")
     ap.add_argument("--no-variogram", action="store_true", help="skip the experimental variogram measurement")
     ap.add_argument("--variogram-only", action="store_true",
@@ -519,6 +582,9 @@ def main():
     lib.hpgl_amd_indicator_kriging_dev.argtypes = [vp, vp, C.POINTER(abi._HPGL_SHAPE),
                                                    C.POINTER(abi._HPGL_IK_PARAMS), C.c_int, vp, vp, vp]
 
+    if args.sgs_only:
+        print(json.dumps({"sgs": sgs_bench(torch, lib, abi, hpgl_wrap, cpu=not args.no_cpu_baseline)}), flush=True)
+        return
     if args.variogram_only:
         print(json.dumps({"variogram": variogram_bench(cpu=not args.no_cpu_baseline)}), flush=True)
         return
@@ -670,6 +736,8 @@ def main():
                                "sk_rms_rel_err_vs_fp64": float(np.sqrt(np.mean(rel ** 2))),
                                "ik_category_agreement": float(np.mean(mixed_ik == d_iout.cpu().numpy()))}
 
+    if not args.no_sgs:
+        out["sgs"] = sgs_bench(torch, lib, abi, hpgl_wrap, cpu=not args.no_cpu_baseline)
     if not args.no_sis:
         out["sis"] = sis_bench(torch, lib, abi, hpgl_wrap, cpu=not args.no_cpu_baseline)
     if not args.no_cokriging:
