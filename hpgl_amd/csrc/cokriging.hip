@@ -158,10 +158,125 @@ __global__ __launch_bounds__(64) void k_cokriging(CokrigingArgs a) {
     a.out_mask[node] = 1;
 }
 
+// Systems of at most 16 unknowns (max_neighbours <= 15): four per
+// wavefront, 16 lanes each, lane i of a group owning row i.  Same
+// operation order as k_cokriging; a group whose system is singular (or has
+// no neighbours) keeps taking part in the wavefront's barriers.
+constexpr int kGroupRows = 16;
+
+struct CoGroupLds {
+    double A[4][kGroupRows][kGroupRows + 1];
+    double B[4][kGroupRows];
+    double X[4][kGroupRows];
+    int order[4][kGroupRows];
+    int used[4][kGroupRows];
+    int xyz[4][3][kGroupRows];
+};
+
+__global__ __launch_bounds__(64) void k_cokriging_g16(CokrigingArgs a) {
+    __shared__ CoGroupLds sh;
+    const int lane = threadIdx.x;
+    const int g = lane >> 4, i = lane & 15;
+    const int t = blockIdx.x * 4 + g;
+    const bool valid = t < a.T;
+    const int n = valid ? min(a.cnt[t], a.max_n) : 0;
+    const int size = n + 1;
+    const int node = valid ? a.targets[t] : 0;
+    int cx, cy, cz;
+    cell_xyz(node, a.X, a.Y, cx, cy, cz);
+    int xi = 0, yi = 0, zi = 0;
+    if (i < n) {
+        const int nb = a.nbr[(size_t)t * a.max_n + i];
+        cell_xyz(nb, a.X, a.Y, xi, yi, zi);
+        sh.xyz[g][0][i] = xi;
+        sh.xyz[g][1][i] = yi;
+        sh.xyz[g][2][i] = zi;
+    }
+    __syncthreads();
+    // build_system (simple_cokriging_markI.cpp:19-55)
+    if (i < n) {
+        for (int j = 0; j < n; ++j)
+            sh.A[g][i][j] = co_cov(a.cov, sh.xyz[g][0][j] - xi, sh.xyz[g][1][j] - yi, sh.xyz[g][2][j] - zi);
+        const double cr = a.coef * co_cov(a.cross, xi - cx, yi - cy, zi - cz);
+        sh.A[g][i][n] = cr;
+        sh.A[g][n][i] = cr;
+        sh.B[g][i] = co_cov(a.cov, cx - xi, cy - yi, cz - zi);
+    }
+    if (i == 0 && valid) {
+        sh.B[g][n] = a.cross0;
+        sh.A[g][n][n] = a.sec_var;
+    }
+    sh.used[g][i] = 0;
+    __syncthreads();
+    // the largest system of the wavefront bounds the column loop
+    int smax = valid ? size : 0;
+    for (int o = 16; o <= 32; o <<= 1) smax = max(smax, __shfl_xor(smax, o));
+    // gauss_solve (gauss_solver.cpp:5-52)
+    bool ok = valid && n > 0;
+    for (int c = 0; c < smax; ++c) {
+        const bool act = ok && c < size;
+        const uint64_t m = __ballot(act && i < size && sh.used[g][i] == 0 && sh.A[g][i][c] != 0.0);
+        const uint32_t gm = (uint32_t)(m >> (16 * g)) & 0xffffu;
+        const int row = gm ? __ffs(gm) - 1 : -1;
+        if (act && row < 0) ok = false;  // the matrix is singular
+        const bool step = act && row >= 0;
+        const double coef = step ? sh.A[g][row][c] : 1.0;
+        __syncthreads();
+        if (step) {
+            if (i == 0) {
+                sh.used[g][row] = 1;
+                sh.order[g][c] = row;
+                sh.B[g][row] = sh.B[g][row] / coef;
+            }
+            if (i >= c && i < size) sh.A[g][row][i] = sh.A[g][row][i] / coef;
+        }
+        __syncthreads();
+        if (step && i < size && i != row && sh.used[g][i] == 0) {
+            const double f = sh.A[g][i][c];
+            for (int k = c; k < size; ++k) sh.A[g][i][k] = sh.A[g][i][k] - f * sh.A[g][row][k];
+            sh.B[g][i] = sh.B[g][i] - f * sh.B[g][row];
+        }
+        __syncthreads();
+    }
+    // back substitution (gauss_solver.cpp:54-62)
+    double acc = ok && i < size ? sh.B[g][i] : 0.0;
+    const int rowi = ok && i < size ? sh.order[g][i] : 0;
+    for (int j = smax - 1; j >= 0; --j) {
+        if (ok && i == j && j < size) sh.X[g][j] = acc;
+        __syncthreads();
+        if (ok && j < size && i < j) acc = acc - sh.A[g][rowi][j] * sh.X[g][j];
+    }
+    wave_count(a.counters, 1, i == 0 && ok);
+    wave_count(a.counters, 2, i == 0 && valid && n > 0 && !ok);
+    wave_count(a.counters, 3, i == 0 && valid && n == 0);
+    __syncthreads();
+    if (i != 0 || !valid) return;
+    // combine (simple_cokriging_markI.cpp:82-104); the failure value :235
+    const float pm = a.primary_mean, sm = a.secondary_mean;
+    const float sv = a.sec_mask[node] != 0 ? a.sec[node] : sm;
+    float result;
+    if (ok) {
+        result = pm;
+        for (int k = 0; k < n; ++k) {
+            const int cc = a.nbr[(size_t)t * a.max_n + k];
+            result = (float)((double)result + (double)(a.values[cc] - pm) * sh.X[g][k]);
+        }
+        result = (float)((double)result + sh.X[g][n] * (double)(sv - sm));
+    } else {
+        result = pm + sv - sm;
+    }
+    a.out[node] = result;
+    a.out_mask[node] = 1;
+}
+
 }  // namespace
 
 void launch_cokriging(const CokrigingArgs &a, hipStream_t s) {
     if (a.T <= 0) return;
+    if (a.max_n < kGroupRows) {
+        hipLaunchKernelGGL(k_cokriging_g16, dim3((a.T + 3) / 4), dim3(64), 0, s, a);
+        return;
+    }
     const size_t S = (size_t)a.max_n + 1;
     const size_t lds = sizeof(double) * (S * (S + 1) + 2 * S) + sizeof(int) * (2 * S + 3 * (size_t)a.max_n);
     hipLaunchKernelGGL(k_cokriging, dim3(a.T), dim3(64), lds, s, a);
