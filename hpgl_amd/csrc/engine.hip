@@ -982,6 +982,10 @@ void sim_paths(long long N, const std::vector<long> &seeds, SimPaths &p, hipStre
 // cells of each realisation in path order.
 struct SimBatch {
     DBuf<int> key, targets;
+    // per realisation, the positions of its targets ordered by cell: the
+    // template scans walk the targets in that order so that the probes of a
+    // wavefront's 64 lanes fall on neighbouring cells
+    DBuf<int> by_cell;
     std::vector<int> h_T;
     std::vector<long long> h_off;
     long long tot = 0;
@@ -1068,6 +1072,25 @@ void sim_batch(long long N, const SimPaths &p, int r0, int rc, const uint8_t *ha
     if (b.tot >= (1ll << 31)) throw std::runtime_error("too many simulated cells in one batch");
     b.max_T = 0;
     for (int r = 0; r < rc; ++r) b.max_T = std::max(b.max_T, b.h_T[r]);
+    b.by_cell.alloc((size_t)std::max<long long>(b.tot, 1));
+    if (b.max_T > 0) {
+        DBuf<int> pos(b.max_T), keys_out(b.max_T);
+        hipLaunchKernelGGL(k_iota, dim3((unsigned)((b.max_T + 255) / 256)), dim3(256), 0, s, pos.p, b.max_T);
+        int end_bit = 1;
+        while ((1ll << end_bit) < N) ++end_bit;
+        size_t bytes = 0;
+        hip_check(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, b.targets.p, keys_out.p, pos.p, b.by_cell.p,
+                                                     b.max_T, 0, end_bit, s),
+                  "radix sort");
+        DBuf<uint8_t> tmp(bytes);
+        for (int r = 0; r < rc; ++r) {
+            if (b.h_T[r] == 0) continue;
+            size_t nb = bytes;
+            hip_check(hipcub::DeviceRadixSort::SortPairs(tmp.p, nb, b.targets.p + b.h_off[r], keys_out.p, pos.p,
+                                                         b.by_cell.p + b.h_off[r], b.h_T[r], 0, end_bit, s),
+                      "radix sort");
+        }
+    }
 }
 
 // Template scan of every target of the batch among the cells informed
@@ -1089,7 +1112,7 @@ void sim_scan(const CovAssets &as, const int grid[3], int max_n, const SimBatch 
         sa.tsize = as.tmpl.size();
         sa.max_n = max_n;
         sa.targets = b.targets.p + b.h_off[r];
-        sa.list = nullptr;
+        sa.list = b.by_cell.p + b.h_off[r];
         sa.n_list = b.h_T[r];
         sa.nbr = nbr + (size_t)b.h_off[r] * stride;
         sa.cnt = cnt + b.h_off[r];
