@@ -244,11 +244,24 @@ __device__ inline void sis_value(const SisSweepArgs &a, long long gk, int k_draw
             const int *row = C.nbr + (size_t)gk * C.max_n;
             const double *w = C.w + (size_t)gk * C.max_n;
             double r = mc;
-            for (int i = 0; i < n; ++i) {
-                const int nb = row[i];
-                const float v = val[nb] == (uint8_t)c ? 1.0f : 0.0f;
-                const float m = C.lvm ? C.lvm[nb] : mc;
-                r += w[i] * (double)(v - m);
+            // eight neighbours' loads in flight together, then the terms in
+            // neighbour order
+            for (int i0 = 0; i0 < n; i0 += 8) {
+                int nb[8];
+                double wv[8];
+                float vv[8], mv[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) nb[j] = i0 + j < n ? row[i0 + j] : 0;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const bool in = i0 + j < n;
+                    wv[j] = in ? w[i0 + j] : 0.0;
+                    vv[j] = (in && val[nb[j]] == (uint8_t)c) ? 1.0f : 0.0f;
+                    mv[j] = (in && C.lvm) ? C.lvm[nb[j]] : mc;
+                }
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (i0 + j < n) r += wv[j] * (double)(vv[j] - mv[j]);
             }
             prob = r;
         }
@@ -391,10 +404,24 @@ struct SgsJobLane {
         float mean = m0;
         if (ok) {
             const double *w = a.w + (size_t)gk * a.max_n;
-            for (int i = 0; i < n; ++i) {
-                const int nb = row[i];
-                const float mi = a.lvm ? a.lvm[nb] : 0.0f;
-                mean += (float)(w[i] * (double)(val[nb] - mi));
+            // eight neighbours' loads in flight together, then the terms in
+            // neighbour order
+            for (int i0 = 0; i0 < n; i0 += 8) {
+                int nb[8];
+                double wv[8];
+                float vv[8], mv[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) nb[j] = i0 + j < n ? row[i0 + j] : 0;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const bool in = i0 + j < n;
+                    wv[j] = in ? w[i0 + j] : 0.0;
+                    vv[j] = in ? val[nb[j]] : 0.0f;
+                    mv[j] = (in && a.lvm) ? a.lvm[nb[j]] : 0.0f;
+                }
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (i0 + j < n) mean += (float)(wv[j] * (double)(vv[j] - mv[j]));
             }
         }
         // u = x / (2^32 - 1) (bs_random_generator.cpp:25-28)
