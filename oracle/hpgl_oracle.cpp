@@ -619,7 +619,8 @@ static CovParams cov_from(int type, const double r[3], const double a[3], double
 
 // ---------------------------------------------------------------------------
 // Indicator kriging (indicator_kriging.h:58-124, cdf_utils.cpp:9-24) and
-// median IK (median_ik.cpp:24-93).  Pinned by: closed form + brute force.
+// median IK (median_ik.cpp:24-93).  Pinned by: a brute-force restatement
+// on the oracle's own neighbour lists (tests/test_oracle_ik_bruteforce.py).
 // ---------------------------------------------------------------------------
 static unsigned char most_probable_category(const std::vector<float> &p) {
     double max_prob = p[0];
@@ -636,7 +637,9 @@ static unsigned char most_probable_category(const std::vector<float> &p) {
 
 // ---------------------------------------------------------------------------
 // SGS chain.  Pinned by: problem 7.2 SGS vector (20/20), Boost's mt11213b
-// validation value, the Hull-Dobell full-period permutation property.
+// validation value, the Hull-Dobell full-period permutation property; the
+// OK kind, LVM and the simulation mask by a brute-force restatement
+// (tests/test_oracle_sgs_bruteforce.py).
 // ---------------------------------------------------------------------------
 typedef std::mersenne_twister_engine<uint32_t, 32, 351, 175, 19, 0xccab8ee7u, 11, 0xffffffffu, 7, 0x31b6ab00u, 15,
                                      0xffe50000u, 17, 1812433253u>
