@@ -24,6 +24,8 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <map>
+#include <thread>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -50,6 +52,44 @@ struct DevBuf {
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
 };
+
+// Device buffers, stream and events of the gridded variogram for the
+// calling thread on one device, kept between calls: a buffer grows when a
+// call needs more and is otherwise reused; all of it is held until the
+// process ends.
+struct Grow {
+    void *p = nullptr;
+    size_t n = 0;
+    template <typename T>
+    T *get(size_t count) {
+        const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+        if (bytes > n) {
+            if (p) (void)hipFree(p);
+            p = nullptr;
+            n = 0;
+            hip_ok(hipMalloc(&p, bytes), "hipMalloc");
+            n = bytes;
+        }
+        return static_cast<T *>(p);
+    }
+};
+struct Workspace {
+    Grow data, mask, vals, cells, offs, psum, pcnt, osum, ocnt;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+};
+Workspace &workspace() {
+    static thread_local std::map<int, Workspace> per_device;
+    int dev = 0;
+    hip_ok(hipGetDevice(&dev), "hipGetDevice");
+    Workspace &w = per_device[dev];
+    if (!w.s) {
+        hip_ok(hipStreamCreateWithFlags(&w.s, hipStreamNonBlocking), "hipStreamCreate");
+        hip_ok(hipEventCreate(&w.e0), "hipEventCreate");
+        hip_ok(hipEventCreate(&w.e1), "hipEventCreate");
+    }
+    return w;
+}
 
 // ---- host restatements of the geometry (variograms.cpp:9-149) ----
 
@@ -218,6 +258,17 @@ __global__ __launch_bounds__(kCellsPerBlock) void k_grid_pairs(GridArgs a) {
     }
 }
 
+// The dense device grid from the informed cells: mask 1 and the value at
+// each listed cell (the mask was cleared before).
+__global__ void k_scatter_cells(const int *cells, const float *vals, int n, int X, int Y, float *data,
+                                unsigned char *mask) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long long c = cells[3 * i] + (long long)X * (cells[3 * i + 1] + (long long)Y * cells[3 * i + 2]);
+    data[c] = vals[i];
+    mask[c] = 1;
+}
+
 // Partials of each offset summed over chunks in chunk order.
 __global__ void k_sum_chunks(const double *psum, const unsigned long long *pcnt, int n_chunks, int n_offs,
                              double *osum, unsigned long long *ocnt) {
@@ -322,6 +373,77 @@ CVAR_API void fill_ellipsoid_directions(ellipsoid_t *ell, double azimuth, double
     GUARD_END
 }
 
+// Informed cells of `data` with z in [z0, z1) as (x, y, z) triples in the
+// reference's z, y, x order (variograms.cpp:403-413) and their values.
+// Rows with unit mask stride skip 32 empty mask bytes at a time.
+static void informed_cells_z(const hard_data_t &d, int z0, int z1, std::vector<int> &cells,
+                             std::vector<float> &vals) {
+    const int *ds = d.m_data_shape;
+    const long long ms0 = d.m_mask_strides[0], ms1 = d.m_mask_strides[1], ms2 = d.m_mask_strides[2];
+    const long long ds0 = d.m_data_strides[0], ds1 = d.m_data_strides[1], ds2 = d.m_data_strides[2];
+    for (int z = z0; z < z1; ++z)
+        for (int y = 0; y < ds[1]; ++y) {
+            const unsigned char *row = d.m_mask + ms1 * y + ms2 * z;
+            const float *vrow = d.m_data + ds1 * y + ds2 * z;
+            int x = 0;
+            while (x < ds[0]) {
+                if (ms0 == 1 && x + 32 <= ds[0]) {
+                    uint64_t w[4];
+                    std::memcpy(w, row + x, 32);
+                    if ((w[0] | w[1] | w[2] | w[3]) == 0) {
+                        x += 32;
+                        continue;
+                    }
+                }
+                const int xe = ms0 == 1 ? std::min(x + 32, ds[0]) : x + 1;
+                for (; x < xe; ++x)
+                    if (row[ms0 * x] != 0) {
+                        cells.push_back(x);
+                        cells.push_back(y);
+                        cells.push_back(z);
+                        vals.push_back(vrow[ds0 * x]);
+                    }
+            }
+        }
+}
+
+// All informed cells, the z layers split among host threads for large
+// grids; the parts are joined in z order, so the order is that of a single
+// scan.
+static void informed_cells(const hard_data_t &d, std::vector<int> &cells, std::vector<float> &vals) {
+    const int *ds = d.m_data_shape;
+    const long long n = (long long)ds[0] * ds[1] * ds[2];
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int T = (int)std::min<long long>({(long long)std::min(hw, 16u), (long long)ds[2], n / (1ll << 20) + 1});
+    if (T <= 1) {
+        informed_cells_z(d, 0, ds[2], cells, vals);
+        return;
+    }
+    std::vector<std::vector<int>> pc(T);
+    std::vector<std::vector<float>> pv(T);
+    std::vector<std::thread> th;
+    th.reserve(T - 1);
+    auto range = [&](int t) { return std::make_pair((int)((long long)ds[2] * t / T), (int)((long long)ds[2] * (t + 1) / T)); };
+    for (int t = 1; t < T; ++t) {
+        try {
+            th.emplace_back([&, t] { informed_cells_z(d, range(t).first, range(t).second, pc[t], pv[t]); });
+        } catch (...) {
+            break;  // no more threads: the remaining parts are scanned here
+        }
+    }
+    for (int t = 0; t < T; ++t)
+        if (t == 0 || t > (int)th.size()) informed_cells_z(d, range(t).first, range(t).second, pc[t], pv[t]);
+    for (auto &x : th) x.join();
+    size_t nc = 0;
+    for (int t = 0; t < T; ++t) nc += pv[t].size();
+    cells.reserve(3 * nc);
+    vals.reserve(nc);
+    for (int t = 0; t < T; ++t) {
+        cells.insert(cells.end(), pc[t].begin(), pc[t].end());
+        vals.insert(vals.end(), pv[t].begin(), pv[t].end());
+    }
+}
+
 CVAR_API void calc_variograms(variogram_search_template_t *templ, hard_data_t *data, float *result, int result_length,
                               int percentToUse) {
     GUARD_BEGIN
@@ -348,15 +470,8 @@ CVAR_API void calc_variograms(variogram_search_template_t *templ, hard_data_t *d
         for (int l = 0; l < lag_count; ++l) result[l] = NAN;
         return;
     }
-    for (int z = 0; z < ds[2]; ++z)
-        for (int y = 0; y < ds[1]; ++y)
-            for (int x = 0; x < ds[0]; ++x)
-                if (data->m_mask[(long long)data->m_mask_strides[0] * x + (long long)data->m_mask_strides[1] * y +
-                                 (long long)data->m_mask_strides[2] * z] != 0) {
-                    cells.push_back(x);
-                    cells.push_back(y);
-                    cells.push_back(z);
-                }
+    std::vector<float> cell_vals;
+    informed_cells(*data, cells, cell_vals);
     const int n_cells = (int)(cells.size() / 3);
     const int n_offs = (int)offs.size();
     std::vector<double> lag_sum(lag_count, 0.0);
@@ -365,49 +480,58 @@ CVAR_API void calc_variograms(variogram_search_template_t *templ, hard_data_t *d
     t_pairs = (long long)n_offs * n_cells;
     t_device_ms = 0;
     if (n_cells > 0 && n_offs > 0) {
-        // the arrays as the strides reach them (elements from the base pointer)
-        const size_t n_data = (size_t)extent_d + 1, n_mask = (size_t)extent_m + 1;
-        for (int c = 0; c < 3; ++c)
-            if (data->m_data_strides[c] < 0 || data->m_mask_strides[c] < 0)
-                throw std::runtime_error("calc_variograms: negative strides are not supported");
-        DevBuf<float> d_data(n_data);
-        DevBuf<unsigned char> d_mask(n_mask);
-        DevBuf<int> d_cells(cells.size()), d_offs(3 * (size_t)n_offs);
-        hipStream_t s;
-        hip_ok(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+        // the device holds a dense copy of the grid rebuilt from the informed
+        // cells (mask 1 and the value there, mask 0 elsewhere; values of
+        // uninformed cells are never read), so only the cell list crosses
+        // PCIe, not the caller's whole data and mask arrays
+        (void)extent_d;
+        (void)extent_m;
+        Workspace &ws = workspace();
+        const int n_chunks = (n_cells + kCellsPerBlock - 1) / kCellsPerBlock;
+        float *const d_data = ws.data.get<float>((size_t)n_all);
+        float *const d_vals = ws.vals.get<float>((size_t)n_cells);
+        unsigned char *const d_mask = ws.mask.get<unsigned char>((size_t)n_all);
+        int *const d_cells = ws.cells.get<int>(cells.size());
+        int *const d_offs = ws.offs.get<int>(3 * (size_t)n_offs);
+        double *const psum = ws.psum.get<double>((size_t)n_chunks * n_offs);
+        double *const osum = ws.osum.get<double>((size_t)n_offs);
+        unsigned long long *const pcnt = ws.pcnt.get<unsigned long long>((size_t)n_chunks * n_offs);
+        unsigned long long *const ocnt = ws.ocnt.get<unsigned long long>((size_t)n_offs);
+        const hipStream_t s = ws.s;
+        const hipEvent_t e0 = ws.e0, e1 = ws.e1;
         std::vector<int> h_offs(3 * (size_t)n_offs);
         for (int o = 0; o < n_offs; ++o)
             for (int c = 0; c < 3; ++c) h_offs[3 * o + c] = offs[o].d[c];
-        const int n_chunks = (n_cells + kCellsPerBlock - 1) / kCellsPerBlock;
-        DevBuf<double> psum((size_t)n_chunks * n_offs), osum(n_offs);
-        DevBuf<unsigned long long> pcnt((size_t)n_chunks * n_offs), ocnt(n_offs);
-        hipEvent_t e0, e1;
-        hip_ok(hipEventCreate(&e0), "hipEventCreate");
-        hip_ok(hipEventCreate(&e1), "hipEventCreate");
         try {
-            hip_ok(hipMemcpyAsync(d_data.p, data->m_data, n_data * sizeof(float), hipMemcpyHostToDevice, s), "copy");
-            hip_ok(hipMemcpyAsync(d_mask.p, data->m_mask, n_mask, hipMemcpyHostToDevice, s), "copy");
-            hip_ok(hipMemcpyAsync(d_cells.p, cells.data(), cells.size() * sizeof(int), hipMemcpyHostToDevice, s),
+            hip_ok(hipMemcpyAsync(d_cells, cells.data(), cells.size() * sizeof(int), hipMemcpyHostToDevice, s),
                    "copy");
-            hip_ok(hipMemcpyAsync(d_offs.p, h_offs.data(), h_offs.size() * sizeof(int), hipMemcpyHostToDevice, s),
+            hip_ok(hipMemcpyAsync(d_vals, cell_vals.data(), cell_vals.size() * sizeof(float), hipMemcpyHostToDevice,
+                                  s),
+                   "copy");
+            hip_ok(hipMemsetAsync(d_mask, 0, (size_t)n_all, s), "memset");
+            hipLaunchKernelGGL(k_scatter_cells, dim3((n_cells + 255) / 256), dim3(256), 0, s, d_cells, d_vals,
+                               n_cells, ds[0], ds[1], d_data, d_mask);
+            hip_ok(hipGetLastError(), "k_scatter_cells");
+            hip_ok(hipMemcpyAsync(d_offs, h_offs.data(), h_offs.size() * sizeof(int), hipMemcpyHostToDevice, s),
                    "copy");
             GridArgs a;
-            a.data = d_data.p;
-            a.mask = d_mask.p;
+            a.data = d_data;
+            a.mask = d_mask;
+            const int dense[3] = {1, ds[0], ds[0] * ds[1]};
             for (int c = 0; c < 3; ++c) {
                 a.dshape[c] = ds[c];
-                a.dstride[c] = data->m_data_strides[c];
-                a.mshape[c] = data->m_mask_shape[c];
-                a.mstride[c] = data->m_mask_strides[c];
+                a.dstride[c] = dense[c];
+                a.mshape[c] = ds[c];
+                a.mstride[c] = dense[c];
             }
-            a.cells = d_cells.p;
+            a.cells = d_cells;
             a.n_cells = n_cells;
-            a.offs = d_offs.p;
+            a.offs = d_offs;
             a.n_offs = n_offs;
             a.percent = percentToUse;
             a.o_base = 0;
-            a.psum = psum.p;
-            a.pcnt = pcnt.p;
+            a.psum = psum;
+            a.pcnt = pcnt;
             hip_ok(hipEventRecord(e0, s), "event");
             const int batches = (n_offs + kOffsetBatch - 1) / kOffsetBatch;
             for (int b0 = 0; b0 < batches; b0 += 65535) {
@@ -417,20 +541,20 @@ CVAR_API void calc_variograms(variogram_search_template_t *templ, hard_data_t *d
                 ab.offs = a.offs + 3 * (size_t)b0 * kOffsetBatch;
                 ab.n_offs = std::min(n_offs - b0 * kOffsetBatch, nb * kOffsetBatch);
                 ab.o_base = b0 * kOffsetBatch;
-                ab.psum = psum.p;  // chunk-major with the launch's own offset count
-                ab.pcnt = pcnt.p;
+                ab.psum = psum;  // chunk-major with the launch's own offset count
+                ab.pcnt = pcnt;
                 hipLaunchKernelGGL(k_grid_pairs, dim3(n_chunks, nb), dim3(kCellsPerBlock), 0, s, ab);
                 hip_ok(hipGetLastError(), "k_grid_pairs");
-                hipLaunchKernelGGL(k_sum_chunks, dim3((ab.n_offs + 255) / 256), dim3(256), 0, s, psum.p, pcnt.p,
-                                   n_chunks, ab.n_offs, osum.p + (size_t)b0 * kOffsetBatch,
-                                   ocnt.p + (size_t)b0 * kOffsetBatch);
+                hipLaunchKernelGGL(k_sum_chunks, dim3((ab.n_offs + 255) / 256), dim3(256), 0, s, psum, pcnt,
+                                   n_chunks, ab.n_offs, osum + (size_t)b0 * kOffsetBatch,
+                                   ocnt + (size_t)b0 * kOffsetBatch);
                 hip_ok(hipGetLastError(), "k_sum_chunks");
             }
             hip_ok(hipEventRecord(e1, s), "event");
             std::vector<double> h_sum(n_offs);
             std::vector<unsigned long long> h_cnt(n_offs);
-            hip_ok(hipMemcpyAsync(h_sum.data(), osum.p, n_offs * sizeof(double), hipMemcpyDeviceToHost, s), "copy");
-            hip_ok(hipMemcpyAsync(h_cnt.data(), ocnt.p, n_offs * sizeof(unsigned long long), hipMemcpyDeviceToHost, s),
+            hip_ok(hipMemcpyAsync(h_sum.data(), osum, n_offs * sizeof(double), hipMemcpyDeviceToHost, s), "copy");
+            hip_ok(hipMemcpyAsync(h_cnt.data(), ocnt, n_offs * sizeof(unsigned long long), hipMemcpyDeviceToHost, s),
                    "copy");
             hip_ok(hipStreamSynchronize(s), "sync");
             float ms = 0;
@@ -444,14 +568,8 @@ CVAR_API void calc_variograms(variogram_search_template_t *templ, hard_data_t *d
                 }
         } catch (...) {
             (void)hipStreamSynchronize(s);
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
-            (void)hipStreamDestroy(s);
             throw;
         }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        (void)hipStreamDestroy(s);
     }
     // variograms.cpp:439-442 (the count is an int there; 0 pairs give NaN)
     for (int l = 0; l < lag_count; ++l) result[l] = (float)(lag_sum[l] / (double)lag_cnt[l] / 2);

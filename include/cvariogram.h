@@ -65,7 +65,10 @@ void fill_ellipsoid_directions(struct ellipsoid_t *ell, double azimuth, double d
 /* api.h:117-122, variograms.cpp:362-446: semivariogram of gridded hard data
  * for the first min(num_lags, result_length) lags (NaN where a lag has no
  * pair).  percentToUse < 100 keeps each pair with that probability by a
- * deterministic hash of (offset, cell) instead of the C library's rand(). */
+ * deterministic hash of (offset, cell) instead of the C library's rand().
+ * Only the informed cells cross PCIe; the device buffers (a dense copy of
+ * the grid, about 5 bytes per cell) are kept per calling thread and device
+ * between calls and held until the process ends. */
 void calc_variograms(struct variogram_search_template_t *templ, struct hard_data_t *data,
                      float *result_covariations, int result_length, int percentToUse);
 

@@ -63,6 +63,12 @@ def test_grid_variogram_c_order_strides_and_determinism(cv):
     assert np.array_equal(a, b, equal_nan=True)
     ot = O.variogram_template(1.0, 1.0, 0.8, 12, 0.5, O.ellipsoid(1, 3.0, 3.0, 60, 0, 0))
     _close(a, O.calc_variograms(ot, vals, mask))
+    # reversed views (negative strides from the first element, as the
+    # reference addresses them) give the variogram of their contiguous copies
+    rv, rm = vals[::-1, :, ::-1], mask[::-1, :, ::-1]
+    _, c = cv.CalcVariograms(t, (rv, rm))
+    _, d = cv.CalcVariograms(t, (np.asfortranarray(rv), np.asfortranarray(rm)))
+    assert np.array_equal(c, d, equal_nan=True)
 
 
 def test_grid_variogram_sampling_is_deterministic_and_close(cv):
