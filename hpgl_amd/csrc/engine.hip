@@ -179,6 +179,21 @@ __global__ void k_iota(int *p, int n) {
     if (i < n) p[i] = i;
 }
 
+// Sort key of path position i (target cell targets[i]) for the template
+// scans: the cell, below the position's density bucket b = floor(log2((H +
+// i) / H)) (at most 15), H the cells informed before the path starts.  A
+// target at position i scans about max_n N / (H + i) template offsets, so
+// the lanes of a wavefront in one bucket need similar scan lengths (within
+// a factor of two), while cell order inside the bucket keeps their probes
+// close together.
+__global__ void k_scan_order_keys(const int *targets, int T, long long H, int cell_bits, uint32_t *keys) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= T) return;
+    const unsigned long long q = (unsigned long long)(H + i) / (unsigned long long)H;
+    const int b = min(63 - __clzll(q), 15);
+    keys[i] = ((uint32_t)b << cell_bits) | (uint32_t)targets[i];
+}
+
 template <typename T>
 void upload(DBuf<T> &d, const std::vector<T> &h, hipStream_t s) {
     d.alloc(h.size());
@@ -1074,20 +1089,30 @@ void sim_batch(long long N, const SimPaths &p, int r0, int rc, const uint8_t *ha
     for (int r = 0; r < rc; ++r) b.max_T = std::max(b.max_T, b.h_T[r]);
     b.by_cell.alloc((size_t)std::max<long long>(b.tot, 1));
     if (b.max_T > 0) {
-        DBuf<int> pos(b.max_T), keys_out(b.max_T);
+        // positions ordered by (density bucket, cell); grids of more than
+        // 2^28 cells sort by cell alone
+        DBuf<int> pos(b.max_T);
+        DBuf<uint32_t> keys_in(b.max_T), keys_out(b.max_T);
         hipLaunchKernelGGL(k_iota, dim3((unsigned)((b.max_T + 255) / 256)), dim3(256), 0, s, pos.p, b.max_T);
-        int end_bit = 1;
-        while ((1ll << end_bit) < N) ++end_bit;
+        int cell_bits = 1;
+        while ((1ll << cell_bits) < N) ++cell_bits;
+        const bool buckets = cell_bits <= 28;
+        const int end_bit = buckets ? cell_bits + 4 : cell_bits;
         size_t bytes = 0;
-        hip_check(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, b.targets.p, keys_out.p, pos.p, b.by_cell.p,
+        hip_check(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, keys_in.p, keys_out.p, pos.p, b.by_cell.p,
                                                      b.max_T, 0, end_bit, s),
                   "radix sort");
         DBuf<uint8_t> tmp(bytes);
         for (int r = 0; r < rc; ++r) {
-            if (b.h_T[r] == 0) continue;
+            const int T = b.h_T[r];
+            if (T == 0) continue;
+            const long long H = std::max<long long>(1, N - T);
+            hipLaunchKernelGGL(k_scan_order_keys, dim3((unsigned)((T + 255) / 256)), dim3(256), 0, s,
+                               b.targets.p + b.h_off[r], T, buckets ? H : (1ll << 62), cell_bits, keys_in.p);
+            hip_check(hipGetLastError(), "scan order keys");
             size_t nb = bytes;
-            hip_check(hipcub::DeviceRadixSort::SortPairs(tmp.p, nb, b.targets.p + b.h_off[r], keys_out.p, pos.p,
-                                                         b.by_cell.p + b.h_off[r], b.h_T[r], 0, end_bit, s),
+            hip_check(hipcub::DeviceRadixSort::SortPairs(tmp.p, nb, keys_in.p, keys_out.p, pos.p,
+                                                         b.by_cell.p + b.h_off[r], T, 0, end_bit, s),
                       "radix sort");
         }
     }
