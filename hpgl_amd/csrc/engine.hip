@@ -180,17 +180,19 @@ __global__ void k_iota(int *p, int n) {
 }
 
 // Sort key of path position i (target cell targets[i]) for the template
-// scans: the cell, below the position's density bucket b = floor(log2((H +
-// i) / H)) (at most 15), H the cells informed before the path starts.  A
-// target at position i scans about max_n N / (H + i) template offsets, so
-// the lanes of a wavefront in one bucket need similar scan lengths (within
-// a factor of two), while cell order inside the bucket keeps their probes
-// close together.
+// scans: the cell, below the position's density bucket
+// b = floor(log2((H + i) / H) / 4) (at most 15), H the cells informed before
+// the path starts.  A target at position i scans about max_n N / (H + i)
+// template offsets, so the lanes of a wavefront in one bucket need scan
+// lengths within a factor of 16, while cell order inside the bucket keeps
+// their probes close together.  Measured on 64 SGS realisations of a
+// 200x200x100 grid with 1% hard data: no buckets 702 ms of scans, factor-2
+// buckets 484, factor 4 416, factor 8 387, factor 16 386.
 __global__ void k_scan_order_keys(const int *targets, int T, long long H, int cell_bits, uint32_t *keys) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= T) return;
     const unsigned long long q = (unsigned long long)(H + i) / (unsigned long long)H;
-    const int b = min(63 - __clzll(q), 15);
+    const int b = min((63 - __clzll(q)) >> 2, 15);
     keys[i] = ((uint32_t)b << cell_bits) | (uint32_t)targets[i];
 }
 
