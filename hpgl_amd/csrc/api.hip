@@ -293,6 +293,14 @@ API void hpgl_amd_set_neighbour_dump(int *indices, int *counts, long long n_cell
     t_dump.max_n = max_n;
 }
 
+API int hpgl_amd_pool_selftest(void) {
+    int rc = -100;
+    GUARD_BEGIN
+    rc = pool_selftest();
+    GUARD_END
+    return rc;
+}
+
 API int hpgl_amd_device_count(void) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) return 0;
@@ -435,6 +443,7 @@ API void hpgl_ordinary_kriging(hpgl_cont_masked_array_t *in, hpgl_ok_params_t *p
     const long long N = volume(&in->m_shape);
     if (volume(&out->m_shape) != N) throw std::runtime_error("input and output sizes differ");
     hipStream_t s = thread_stream();
+    PoolStream pool_stream(s);
     DevArray<float> di(N), dout(N);
     DevArray<unsigned char> dm(N), dom(N);
     di.put(in->m_data, s);
@@ -455,6 +464,7 @@ API void hpgl_simple_kriging(float *input_data, unsigned char *input_mask, hpgl_
     const long long N = volume(in_shape);
     if (volume(out_shape) != N) throw std::runtime_error("input and output sizes differ");
     hipStream_t s = thread_stream();
+    PoolStream pool_stream(s);
     DevArray<float> di(N), dout(N);
     DevArray<unsigned char> dm(N), dom(N);
     di.put(input_data, s);
@@ -475,6 +485,7 @@ API void hpgl_lvm_kriging(float *input_data, unsigned char *input_mask, hpgl_sha
     const long long N = volume(in_shape);
     if (volume(out_shape) != N || volume(mean_shape) != N) throw std::runtime_error("array sizes differ");
     hipStream_t s = thread_stream();
+    PoolStream pool_stream(s);
     DevArray<float> di(N), dmean(N), dout(N);
     DevArray<unsigned char> dm(N), dom(N);
     di.put(input_data, s);
@@ -495,6 +506,7 @@ API void hpgl_indicator_kriging(hpgl_ind_masked_array_t *in, hpgl_ind_masked_arr
     const long long N = volume(&in->m_shape);
     if (volume(&out->m_shape) != N) throw std::runtime_error("input and output sizes differ");
     hipStream_t s = thread_stream();
+    PoolStream pool_stream(s);
     DevArray<unsigned char> di(N), dm(N), dout(N), dom(N);
     di.put(in->m_data, s);
     dm.put(in->m_mask, s);
@@ -513,6 +525,7 @@ API void hpgl_median_ik(hpgl_ind_masked_array_t *in, hpgl_median_ik_params_t *p,
     if (volume(&out->m_shape) != N)
         throw std::runtime_error("median_ik_for_two_indicators: Input property size is not equal output property size");
     hipStream_t s = thread_stream();
+    PoolStream pool_stream(s);
     DevArray<unsigned char> di(N), dm(N), dout(N), dom(N);
     di.put(in->m_data, s);
     dm.put(in->m_mask, s);
@@ -553,6 +566,7 @@ API int hpgl_simple_kriging_weights(float *center, float *xs, float *ys, float *
     c.c0 = m.c0();
     c.type = m.s.type;
     hipStream_t s = thread_stream();
+    PoolStream pool_stream(s);
     DevArray<float> dc(3), dx(n), dy(n), dz(n);
     DevArray<double> A((size_t)n * n), w(n);
     DevArray<int> info(1);
@@ -604,6 +618,7 @@ static void sgs_host(SgsJob &j, const float *data, const unsigned char *mask, co
     const long long N = (long long)j.grid[0] * j.grid[1] * j.grid[2];
     const int R = (int)j.seeds.size();
     hipStream_t s = thread_stream();
+    PoolStream pool_stream(s);
     DevArray<float> di(N), dlvm(lvm_mean ? N : 0), dout((size_t)R * N);
     DevArray<unsigned char> dm(N), dsim(sim_mask ? N : 0), dom((size_t)R * N);
     di.put(data, s);
@@ -722,6 +737,7 @@ static void sis_host(SisJob &j, const unsigned char *data, const unsigned char *
     const int R = (int)j.seeds.size();
     const int K = (int)j.cats.size();
     hipStream_t s = thread_stream();
+    PoolStream pool_stream(s);
     DevArray<unsigned char> di(N), dm(N), dsim(sim_mask ? N : 0), dout((size_t)R * N), dom((size_t)R * N);
     DevArray<float> dlvm(lvm ? (size_t)K * N : 0);
     di.put(data, s);
@@ -879,6 +895,7 @@ static void cokriging_host(CokrigingJob &j, const hpgl_cont_masked_array_t *in, 
                            hpgl_cont_masked_array_t *out) {
     const long long N = volume(&in->m_shape);
     hipStream_t s = thread_stream();
+    PoolStream pool_stream(s);
     DevArray<float> di(N), ds(N), dout(N);
     DevArray<unsigned char> dm(N), dsm(N), dom(N);
     di.put(in->m_data, s);

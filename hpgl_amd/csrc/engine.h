@@ -147,8 +147,30 @@ void run_sis(const SisJob &job, hipStream_t s, int precision, Stats &st);
 void run_cokriging(const CokrigingJob &job, hipStream_t s, const Dump *dump, Stats &st);
 
 // Device memory held by the library between calls (size-class cache).
+// Frees are ordered on the stream named by the innermost PoolStream of the
+// calling thread: a freed block is reused only after the work queued on that
+// stream before the free has completed, or by a stream made to wait for it.
 void *pool_alloc(size_t bytes);
 void pool_free(void *p);
+// Blocks on the free list whose freeing stream has not yet reached the free.
+size_t pool_pending_blocks();
+// Checks the stream ordering of pool frees on the current device (0 holds,
+// 1 inconclusive, negative broken).
+int pool_selftest();
+
+// Names the stream that the calling thread's pool frees are ordered on
+// until the scope ends.
+class PoolStream {
+  public:
+    explicit PoolStream(hipStream_t s);
+    ~PoolStream();
+    PoolStream(const PoolStream &) = delete;
+    PoolStream &operator=(const PoolStream &) = delete;
+
+  private:
+    hipStream_t prev_;
+    bool prev_set_;
+};
 
 void hip_check(hipError_t e, const char *what);
 

@@ -28,46 +28,125 @@ void hip_check(hipError_t e, const char *what) {
 }
 
 // ---------------------------------------------------------------------------
-// Device memory cache: freed blocks are kept per size class (powers of two
-// from 256 B) and reused, so repeated jobs do not call hipMalloc.
+// Device memory cache: freed blocks are kept per (device, size class), the
+// classes powers of two from 256 B, and reused, so repeated jobs do not call
+// hipMalloc.  Frees are stream-ordered: a block freed while kernels that use
+// it may still be queued on the freeing thread's stream (PoolStream) carries
+// an event recorded on that stream at the free.  It is handed out again only
+// once the event has completed, or to a caller with a stream that is then
+// made to wait for the event (the same stream, or any stream when memory is
+// short); a block freed with no stream set waits for the whole device first.
 // ---------------------------------------------------------------------------
 namespace {
+struct FreeBlock {
+    void *p;
+    hipStream_t s;    // stream the free was ordered on (nullptr: none)
+    hipEvent_t done;  // completes when the freeing stream's earlier work has (nullptr: free now)
+};
 std::mutex g_pool_mu;
-std::multimap<size_t, void *> g_free;
-std::map<void *, size_t> g_live;
+std::multimap<std::pair<int, size_t>, FreeBlock> g_free;  // (device, size class) -> block
+std::map<void *, std::pair<int, size_t>> g_live;
+std::vector<hipEvent_t> g_spare_events;
+thread_local hipStream_t t_pool_stream = nullptr;
+thread_local bool t_pool_stream_set = false;
 
 size_t size_class(size_t bytes) {
     size_t c = 256;
     while (c < bytes) c <<= 1;
     return c;
 }
+
+int current_device() {
+    int d = 0;
+    hip_check(hipGetDevice(&d), "hipGetDevice");
+    return d;
+}
+
+// hipEventQuery reports a pending event as hipErrorNotReady, which HIP also
+// latches as the thread's last error; clear it unless an earlier error is
+// waiting to be read.
+bool event_done(hipEvent_t e) {
+    const hipError_t before = hipPeekAtLastError();
+    const hipError_t r = hipEventQuery(e);
+    if (r == hipErrorNotReady && before == hipSuccess) (void)hipGetLastError();
+    return r == hipSuccess;
+}
+
+// Under g_pool_mu.
+void recycle_event(hipEvent_t e) {
+    if (e) g_spare_events.push_back(e);
+}
+
+// Under g_pool_mu: a free block of the class usable on the calling thread's
+// stream, or nullptr.  wait_any: take a block whose event is still pending
+// and order the calling stream after it.
+void *take_block(int dev, size_t c, bool wait_any) {
+    auto range = g_free.equal_range({dev, c});
+    auto pick = g_free.end();
+    bool need_wait = false;
+    for (auto it = range.first; it != range.second; ++it) {
+        const FreeBlock &b = it->second;
+        if (!b.done || event_done(b.done)) {
+            pick = it;
+            need_wait = false;
+            break;
+        }
+        if (t_pool_stream_set && (wait_any || b.s == t_pool_stream) && pick == g_free.end()) {
+            pick = it;
+            need_wait = true;
+        }
+    }
+    if (pick == g_free.end()) return nullptr;
+    FreeBlock b = pick->second;
+    if (need_wait) hip_check(hipStreamWaitEvent(t_pool_stream, b.done, 0), "hipStreamWaitEvent");
+    recycle_event(b.done);
+    g_free.erase(pick);
+    g_live[b.p] = {dev, c};
+    return b.p;
+}
 }  // namespace
+
+PoolStream::PoolStream(hipStream_t s) : prev_(t_pool_stream), prev_set_(t_pool_stream_set) {
+    t_pool_stream = s;
+    t_pool_stream_set = true;
+}
+
+PoolStream::~PoolStream() {
+    t_pool_stream = prev_;
+    t_pool_stream_set = prev_set_;
+}
 
 void *pool_alloc(size_t bytes) {
     if (bytes == 0) bytes = 1;
     const size_t c = size_class(bytes);
+    const int dev = current_device();
     {
         std::lock_guard<std::mutex> lk(g_pool_mu);
-        auto it = g_free.find(c);
-        if (it != g_free.end()) {
-            void *p = it->second;
-            g_free.erase(it);
-            g_live[p] = c;
-            return p;
-        }
+        if (void *p = take_block(dev, c, false)) return p;
     }
     void *p = nullptr;
     hipError_t e = hipMalloc(&p, c);
     if (e != hipSuccess) {
-        // release cached blocks and retry once
+        (void)hipGetLastError();
+        // memory is short: reuse a pending block of the class behind its
+        // event, else return the cached blocks of this device and retry
         std::lock_guard<std::mutex> lk(g_pool_mu);
-        for (auto &kv : g_free) (void)hipFree(kv.second);
-        g_free.clear();
+        if (void *q = take_block(dev, c, true)) return q;
+        for (auto it = g_free.begin(); it != g_free.end();) {
+            if (it->first.first != dev) {
+                ++it;
+                continue;
+            }
+            if (it->second.done) (void)hipEventSynchronize(it->second.done);
+            (void)hipFree(it->second.p);
+            recycle_event(it->second.done);
+            it = g_free.erase(it);
+        }
         e = hipMalloc(&p, c);
     }
     hip_check(e, "hipMalloc");
     std::lock_guard<std::mutex> lk(g_pool_mu);
-    g_live[p] = c;
+    g_live[p] = {dev, c};
     return p;
 }
 
@@ -76,8 +155,94 @@ void pool_free(void *p) {
     std::lock_guard<std::mutex> lk(g_pool_mu);
     auto it = g_live.find(p);
     if (it == g_live.end()) return;
-    g_free.emplace(it->second, p);
+    FreeBlock b{p, nullptr, nullptr};
+    if (t_pool_stream_set) {
+        hipEvent_t ev = nullptr;
+        if (!g_spare_events.empty()) {
+            ev = g_spare_events.back();
+            g_spare_events.pop_back();
+        } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+            ev = nullptr;
+        }
+        if (ev && hipEventRecord(ev, t_pool_stream) == hipSuccess) {
+            b.s = t_pool_stream;
+            b.done = ev;
+        } else {
+            recycle_event(ev);
+            (void)hipDeviceSynchronize();
+        }
+    } else {
+        (void)hipDeviceSynchronize();
+    }
+    g_free.emplace(it->second, b);
     g_live.erase(it);
+}
+
+namespace {
+// Writes v to p[0] after spinning for about `ticks` ticks of the 100 MHz
+// constant clock (bounded: at most 2^26 polls).
+__global__ void k_pool_probe_spin(int *p, int v, long long ticks) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const long long t0 = wall_clock64();
+    for (int i = 0; i < (1 << 26) && wall_clock64() - t0 < ticks; ++i) __builtin_amdgcn_s_sleep(8);
+    p[0] = v;
+}
+}  // namespace
+
+// A block freed on stream A while a kernel that writes it is still queued
+// on A must not be handed to stream B before that kernel has completed, and
+// may be handed back to A at once.  Returns 0 when that holds, 1 when the
+// spin ended before the check could be made (inconclusive), and a negative
+// code when the pool broke the ordering.
+int pool_selftest() {
+    hipStream_t a = nullptr, b = nullptr;
+    hip_check(hipStreamCreateWithFlags(&a, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipStreamCreateWithFlags(&b, hipStreamNonBlocking), "hipStreamCreate");
+    int rc = 0;
+    const size_t bytes = (size_t)3 << 20;  // a size class the library's jobs rarely use
+    void *x = nullptr, *y = nullptr, *z = nullptr;
+    {
+        PoolStream ps(a);
+        x = pool_alloc(bytes);
+        hipLaunchKernelGGL(k_pool_probe_spin, dim3(1), dim3(64), 0, a, (int *)x, 7, 5000000ll);  // ~50 ms
+        hip_check(hipGetLastError(), "spin kernel");
+        pool_free(x);
+    }
+    {
+        PoolStream ps(b);
+        const bool pending = pool_pending_blocks() > 0;
+        y = pool_alloc(bytes);
+        const hipError_t q = hipStreamQuery(a);
+        if (q == hipErrorNotReady) (void)hipGetLastError();
+        const bool still_running = q == hipErrorNotReady;
+        if (!pending || !still_running) rc = 1;
+        else if (y == x) rc = -1;  // handed to another stream while pending
+        pool_free(y);
+    }
+    hip_check(hipStreamSynchronize(b), "sync");
+    if (rc == 0) {
+        {
+            PoolStream ps(a);
+            z = pool_alloc(bytes);  // x again (its stream), or y (already free)
+            pool_free(z);
+        }
+        if (z != x && z != y) rc = -2;
+    }
+    hip_check(hipStreamSynchronize(a), "sync");
+    int v = 0;
+    hip_check(hipMemcpy(&v, x, sizeof(int), hipMemcpyDeviceToHost), "copy");
+    if (v != 7 && rc >= 0) rc = -3;
+    (void)hipStreamDestroy(a);
+    (void)hipStreamDestroy(b);
+    return rc;
+}
+
+size_t pool_pending_blocks() {
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    size_t n = 0;
+    for (auto &kv : g_free)
+        if (kv.second.done && !event_done(kv.second.done)) ++n;
+    return n;
 }
 
 namespace {
@@ -179,10 +344,18 @@ __global__ void k_iota(int *p, int n) {
     if (i < n) p[i] = i;
 }
 
+// Adds the number of non-zero bytes of m[0..n) to *out (one atomic per
+// wavefront).
+__global__ void k_count_nonzero(const uint8_t *m, long long n, unsigned long long *out) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned long long b = __ballot(i < n && m[i] != 0);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(out, (unsigned long long)__popcll(b));
+}
+
 // Sort key of path position i (target cell targets[i]) for the template
 // scans: the cell, below the position's density bucket
-// b = floor(log2((H + i) / H) / 4) (at most 15), H the cells informed before
-// the path starts.  A target at position i scans about max_n N / (H + i)
+// b = floor(log2((H + i) / H) / 4) (at most 15), H the hard-data cells,
+// informed before the path starts.  A target at position i scans about max_n N / (H + i)
 // template offsets, so the lanes of a wavefront in one bucket need scan
 // lengths within a factor of 16, while cell order inside the bucket keeps
 // their probes close together.  Measured on 64 SGS realisations of a
@@ -272,6 +445,9 @@ struct CovAssets {
     BoxTable pair;
     DBuf<double> d_pair;
     explicit CovAssets(const CovSpec &c) : model(c) {}
+    // Jobs on other streams may have read these buffers after the last
+    // holder's stream did: wait for the device before they are freed.
+    ~CovAssets() { (void)hipDeviceSynchronize(); }
 };
 
 constexpr size_t kAssetCacheEntries = 32;
@@ -281,8 +457,10 @@ struct AssetCache {
     std::list<std::pair<std::string, std::shared_ptr<CovAssets>>> lru;  // most recent first
 };
 AssetCache &asset_cache() {
-    static AssetCache c;
-    return c;
+    // never destroyed: its buffers must not be freed after the HIP runtime
+    // has shut down at process exit
+    static AssetCache *c = new AssetCache;
+    return *c;
 }
 
 // Assets of (cov, radii, use_table) on the current device; builds them on a
@@ -735,6 +913,7 @@ DevCov analytic_cov(CovAssets &as, const int radii[3], hipStream_t s, Stats &st)
 }
 
 void run_cont(const ContJob &job, hipStream_t s, int precision, const Dump *dump, Stats &st) {
+    PoolStream pool_stream(s);
     const double w0 = now_ms();
     validate_search(job.grid, job.radii, job.max_n);
     validate_cov(job.cov);
@@ -798,6 +977,7 @@ void run_cont(const ContJob &job, hipStream_t s, int precision, const Dump *dump
 }
 
 void run_ind(const IndJob &job, hipStream_t s, int precision, const Dump *dump, Stats &st) {
+    PoolStream pool_stream(s);
     const double w0 = now_ms();
     const long long N = (long long)job.grid[0] * job.grid[1] * job.grid[2];
     const int K = (int)job.cats.size();
@@ -1105,10 +1285,18 @@ void sim_batch(long long N, const SimPaths &p, int r0, int rc, const uint8_t *ha
                                                      b.max_T, 0, end_bit, s),
                   "radix sort");
         DBuf<uint8_t> tmp(bytes);
+        // hard-data cells (a simulation mask leaves other cells uninformed)
+        DBuf<unsigned long long> d_hard(1);
+        hip_check(hipMemsetAsync(d_hard.p, 0, sizeof(unsigned long long), s), "memset");
+        hipLaunchKernelGGL(k_count_nonzero, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, hard, N, d_hard.p);
+        hip_check(hipGetLastError(), "count hard data");
+        unsigned long long h_hard = 0;
+        hip_check(hipMemcpyAsync(&h_hard, d_hard.p, sizeof(h_hard), hipMemcpyDeviceToHost, s), "copy count");
+        hip_check(hipStreamSynchronize(s), "sync");
+        const long long H = std::max<long long>(1, (long long)h_hard);
         for (int r = 0; r < rc; ++r) {
             const int T = b.h_T[r];
             if (T == 0) continue;
-            const long long H = std::max<long long>(1, N - T);
             hipLaunchKernelGGL(k_scan_order_keys, dim3((unsigned)((T + 255) / 256)), dim3(256), 0, s,
                                b.targets.p + b.h_off[r], T, buckets ? H : (1ll << 62), cell_bits, keys_in.p);
             hip_check(hipGetLastError(), "scan order keys");
@@ -1162,6 +1350,7 @@ int sim_batch_size(int R, size_t bytes_per_cell, long long N) {
 }  // namespace
 
 void run_sgs(const SgsJob &job, hipStream_t s, int precision, const Dump *dump, Stats &st) {
+    PoolStream pool_stream(s);
     const double w0 = now_ms();
     validate_search(job.grid, job.radii, job.max_n);
     validate_cov(job.cov);
@@ -1330,6 +1519,7 @@ struct SisCategoryBuffers {
 }  // namespace
 
 void run_sis(const SisJob &job, hipStream_t s, int precision, Stats &st) {
+    PoolStream pool_stream(s);
     const double w0 = now_ms();
     const int K = (int)job.cats.size();
     if (K < 2) throw std::runtime_error("sequential indicator simulation needs at least two categories");
@@ -1493,6 +1683,7 @@ void run_sis(const SisJob &job, hipStream_t s, int precision, Stats &st) {
 // one (n+1)-system per uninformed cell solved by gauss_solve in FP64.
 // ---------------------------------------------------------------------------
 void run_cokriging(const CokrigingJob &job, hipStream_t s, const Dump *dump, Stats &st) {
+    PoolStream pool_stream(s);
     const double w0 = now_ms();
     validate_search(job.grid, job.radii, job.max_n);
     validate_cov(job.primary);

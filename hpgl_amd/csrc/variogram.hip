@@ -55,8 +55,8 @@ struct DevBuf {
 
 // Device buffers, stream and events of the gridded variogram for the
 // calling thread on one device, kept between calls: a buffer grows when a
-// call needs more and is otherwise reused; all of it is held until the
-// process ends.
+// call needs more and is otherwise reused; all of it is released when the
+// thread exits.
 struct Grow {
     void *p = nullptr;
     size_t n = 0;
@@ -77,6 +77,23 @@ struct Workspace {
     Grow data, mask, vals, cells, offs, psum, pcnt, osum, ocnt;
     hipStream_t s = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
+    int dev = 0;
+    Workspace() = default;
+    Workspace(const Workspace &) = delete;
+    Workspace &operator=(const Workspace &) = delete;
+    ~Workspace() {
+        if (!s) return;
+        int cur = 0;
+        const bool switch_dev = hipGetDevice(&cur) == hipSuccess && cur != dev;
+        if (switch_dev) (void)hipSetDevice(dev);
+        (void)hipStreamSynchronize(s);
+        for (Grow *g : {&data, &mask, &vals, &cells, &offs, &psum, &pcnt, &osum, &ocnt})
+            if (g->p) (void)hipFree(g->p);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipStreamDestroy(s);
+        if (switch_dev) (void)hipSetDevice(cur);
+    }
 };
 Workspace &workspace() {
     static thread_local std::map<int, Workspace> per_device;
@@ -84,6 +101,7 @@ Workspace &workspace() {
     hip_ok(hipGetDevice(&dev), "hipGetDevice");
     Workspace &w = per_device[dev];
     if (!w.s) {
+        w.dev = dev;
         hip_ok(hipStreamCreateWithFlags(&w.s, hipStreamNonBlocking), "hipStreamCreate");
         hip_ok(hipEventCreate(&w.e0), "hipEventCreate");
         hip_ok(hipEventCreate(&w.e1), "hipEventCreate");
@@ -470,6 +488,8 @@ CVAR_API void calc_variograms(variogram_search_template_t *templ, hard_data_t *d
         for (int l = 0; l < lag_count; ++l) result[l] = NAN;
         return;
     }
+    // the device grid is indexed with 32-bit cell offsets (k_grid_pairs)
+    if (n_all >= (1ll << 31)) throw std::runtime_error("calc_variograms: grid has more than 2^31-1 cells");
     std::vector<float> cell_vals;
     informed_cells(*data, cells, cell_vals);
     const int n_cells = (int)(cells.size() / 3);

@@ -300,6 +300,13 @@ int hpgl_amd_get_precision(void);
 /* Number of HIP devices visible to the library (0 without a GPU). */
 int hpgl_amd_device_count(void);
 
+/* Diagnostic with no counterpart in the reference: checks that a device
+ * block the library frees while kernels that use it are still queued on one
+ * stream is not handed to another stream before they complete.  Returns 0
+ * when that holds, 1 when the check was inconclusive (the probe kernel ended
+ * first), negative when the ordering is broken, -100 on an error. */
+int hpgl_amd_pool_selftest(void);
+
 /* Non-zero when the last call on this thread failed; the message is
  * available from hpgl_get_last_exception_message(). */
 int hpgl_amd_get_last_error(void);
