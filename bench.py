@@ -36,6 +36,72 @@ FP32_PEAK_TFLOPS = 157.3
 FP64_PEAK_TFLOPS = 78.6
 
 
+def cpu_setup(build_native=True):
+    """Host set-up of the CPU baseline, before the oracle library is loaded:
+    one OpenMP thread per physical core this process may run on (capped by
+    the OMP_NUM_THREADS share when one is set) and the oracle rebuilt with
+    -march=native on this machine (the reference's release flags,
+    SConstruct:24) when the compiler is there.  The legs run in child
+    processes with OMP_PROC_BIND=close and OMP_PLACES=cores (cpu_leg).
+    Returns what was used, with lscpu's totals."""
+    import subprocess
+
+    try:
+        out = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True, text=True, check=True).stdout
+        rows = [tuple(int(v) for v in line.split(",")[:3]) for line in out.splitlines()
+                if line and not line.startswith("#")]
+    except Exception:
+        rows = [(c, c, 0) for c in range(os.cpu_count() or 1)]
+    allowed = os.sched_getaffinity(0)
+    phys_all = {(core, sock) for _, core, sock in rows}
+    phys_allowed = {(core, sock) for cpu, core, sock in rows if cpu in allowed} or phys_all
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = min(len(phys_allowed), share) if share > 0 else len(phys_allowed)
+    flags = "-O2 -fopenmp -ffp-contract=off"
+    so = None
+    if build_native:
+        r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "native"], capture_output=True,
+                           text=True)
+        if r.returncode == 0:
+            so = os.path.join(ROOT, "oracle", "_build", "liboracle_native.so")
+            os.environ["ORACLE_SO"] = so  # the legs that run the oracle in this process load it too
+            flags += " -march=native"
+    return {"threads": int(threads), "oracle_so": so, "lscpu_physical_cores": len(phys_all),
+            "lscpu_sockets": len({s for _, _, s in rows}), "physical_cores_allowed": len(phys_allowed),
+            "omp_num_threads_share": share or None, "omp_proc_bind": "close", "omp_places": "cores",
+            "oracle_flags": flags}
+
+
+CPU = {"threads": 16}  # filled by cpu_setup() when a CPU baseline runs
+
+
+def progress(msg):
+    """A progress line on stderr (stdout carries only the JSON line)."""
+    print("[bench %s] %s" % (time.strftime("%H:%M:%S"), msg), file=sys.stderr, flush=True)
+
+
+def median_time(fn, runs=3):
+    """Median wall time of `runs` calls of fn (and the last call's value)."""
+    ts = []
+    out = None
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        out = fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), [round(t, 3) for t in ts], out
+
+
+def err_stats(got, want, floor):
+    """max and RMS of |got - want| / max(|want|, floor) over the cells given."""
+    got = np.asarray(got, np.float64).ravel()
+    want = np.asarray(want, np.float64).ravel()
+    if got.size == 0:
+        return {"max_rel": 0.0, "rms_rel": 0.0, "cells": 0, "floor": floor}
+    rel = np.abs(got - want) / np.maximum(np.abs(want), floor)
+    return {"max_rel": float(rel.max()), "rms_rel": float(np.sqrt(np.mean(rel ** 2))), "cells": int(rel.size),
+            "floor": floor}
+
+
 def make_cfg2(seed=1002):
     from cases import Cov, Grid, categories, scattered
 
@@ -62,36 +128,266 @@ def _params(abi, sk, ik):
     return skp, ikp
 
 
-def cpu_baseline(g, data, mask, cat, marg, sk, ik, budget_s=15.0, threads=16):
-    """The CPU oracle (a C++/OpenMP restatement of the reference's CPU path)
-    on a bounded sample of the same workload: every s-th cell of the grid is
-    estimated, s chosen from a short pilot so the sample takes ~budget_s."""
-    import oracle_binding as O
+def cpu_leg(spec, timeout=900):
+    """Runs one CPU-baseline leg of this script in a child process, so that
+    the oracle's OpenMP runtime starts with this leg's environment
+    (OMP_PROC_BIND=close, OMP_PLACES=cores, one thread per physical core)
+    and binds none of this process's threads; returns the child's summary
+    and the arrays it wrote."""
+    import subprocess
+    import tempfile
 
-    O.set_threads(threads)
-    n_unk = int(np.sum(mask == 0))
+    progress("CPU leg %s (oracle, %d threads)" % (spec["leg"], CPU["threads"]))
+    env = dict(os.environ)
+    env.update({"OMP_PROC_BIND": "close", "OMP_PLACES": "cores", "OMP_NUM_THREADS": str(CPU["threads"])})
+    if CPU.get("oracle_so"):
+        env["ORACLE_SO"] = CPU["oracle_so"]
+    with tempfile.TemporaryDirectory() as tmp:
+        spec = dict(spec, out=os.path.join(tmp, "leg.npz"), threads=CPU["threads"])
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-worker", json.dumps(spec)], env=env,
+                           capture_output=True, text=True, timeout=timeout)
+        if r.returncode != 0:
+            raise RuntimeError("CPU leg %s failed: %s" % (spec["leg"], r.stderr[-2000:]))
+        with np.load(spec["out"]) as z:
+            arrays = {k: z[k] for k in z.files}
+    return json.loads(r.stdout.strip().splitlines()[-1]), arrays
+
+
+def _sample_stride(run, cells, budget_s, pilot):
+    """Stride s such that run(s) (which estimates every s-th cell) takes about
+    budget_s: a pilot run at stride `pilot`, then a second one at the stride
+    it suggests (capped at ten times the pilot's samples), whose time per
+    sample has the run's fixed set-up cost spread over more cells."""
+    stride = pilot
+    for _ in range(2):
+        t0 = time.perf_counter()
+        run(stride)
+        per_sample = (time.perf_counter() - t0) / (cells / stride)
+        nxt = max(1, int(cells * per_sample / budget_s))
+        if nxt >= stride:
+            return nxt
+        stride = max(nxt, stride // 10)
+    return stride
+
+
+def _worker_headline(O, spec, arrays):
+    g, data, mask, cat, marg, sk, ik = make_cfg2()
     ikd = [{"cov_model": p["cov"], "radiuses": p["radii"], "max_neighbours": p["max_n"]} for p in ik]
+    res = {}
 
     def run(stride):
         O.set_sample_stride(stride)
         t0 = time.perf_counter()
-        O.simple_kriging((data, mask), g, sk["radii"], sk["max_n"], sk["cov"], mean=sk["mean"])
-        t1 = time.perf_counter()
-        O.indicator_kriging((cat, mask, 5), g, ikd, marg)
-        t2 = time.perf_counter()
+        res["sk"] = O.simple_kriging((data, mask), g, sk["radii"], sk["max_n"], sk["cov"], mean=sk["mean"])
+        res["t_sk"] = time.perf_counter() - t0
+        res["ik"] = O.indicator_kriging((cat, mask, 5), g, ikd, marg)
         O.set_sample_stride(1)
-        return t1 - t0, t2 - t1
 
-    pilot = 4096
-    a, b = run(pilot)
-    per_sample = (a + b) / (g.size / pilot)
-    stride = max(1, int(g.size * per_sample / budget_s))
-    a, b = run(stride)
-    sampled_unk = n_unk / stride  # uniform sampling of cells
-    nodes = sampled_unk * (1 + 5)
-    return {"value": nodes / (a + b), "unit": "kriged nodes/s", "cores": threads, "kind": "port",
-            "sample": "every %d-th cell of the same SK+IK workload (%.0f kriged nodes, SK %.1f s + IK %.1f s)"
-                      % (stride, nodes, a, b)}
+    stride = _sample_stride(run, g.size, spec["budget_s"], 4096)
+    med, all_t, _ = median_time(lambda: run(stride), spec["runs"])
+    arrays.update(sk_d=res["sk"][0].ravel(order="F"), sk_m=res["sk"][1].ravel(order="F"),
+                  ik_c=res["ik"][0].ravel(order="F"))
+    n_unk = int(np.sum(mask == 0))
+    return {"stride": stride, "seconds": med, "runs_s": all_t, "kriged_nodes": n_unk / stride * 6}
+
+
+def _worker_cfg3(O, spec, arrays):
+    g, data, mask, twins, cov, radii, max_n = make_cfg3()
+    res = {}
+
+    def run(stride):
+        O.set_sample_stride(stride)
+        res["ok"] = O.ordinary_kriging((data, mask), g, radii, max_n, cov)
+        O.set_sample_stride(1)
+
+    stride = _sample_stride(run, g.size, spec["budget_s"], 997)
+    med, all_t, _ = median_time(lambda: run(stride), spec["runs"])
+    arrays.update(d=res["ok"][0].ravel(order="F"), m=res["ok"][1].ravel(order="F"))
+    return {"stride": stride, "seconds": med, "runs_s": all_t}
+
+
+def _worker_sgs(O, spec, arrays):
+    g, data, mask, cov, radii, max_n = make_sgs_workload()
+    P = spec["threads"]
+    t0 = time.perf_counter()
+    bd, _ = O.sgs_simulation_batch((data, mask), g, radii, max_n, cov, [1000 + r for r in range(P)])
+    t_batch = time.perf_counter() - t0
+    arrays.update(batch=bd)
+    return {"batch_s": t_batch, "batch_realisations": P}
+
+
+def _worker_serving(O, spec, arrays):
+    from cases import Cov, Grid
+
+    jobs = make_serving_trace()
+    service = []
+    nodes_all = 0
+    for i, job in enumerate(jobs):
+        g = Grid(*job["shape"])
+        n_unk = int(np.sum(job["mask"] == 0))
+        nodes = g.size * 3 if job["kind"] == "ik" else n_unk
+        stride = max(1, g.size // spec["nodes_per_sample"])
+        cov = Cov(job["ctype"], job["ranges"], sill=1.0, nugget=0.1)
+        O.set_sample_stride(stride)
+        t0 = time.perf_counter()
+        if job["kind"] == "ok":
+            od, om = O.ordinary_kriging((job["data"], job["mask"]), g, job["radii"], job["max_n"], cov)
+        elif job["kind"] == "sk":
+            od, om = O.simple_kriging((job["data"], job["mask"]), g, job["radii"], job["max_n"], cov)
+        elif job["kind"] == "lvm":
+            od, om = O.lvm_kriging((job["data"], job["mask"]), g, job["mean"], job["radii"], job["max_n"], cov)
+        else:
+            ikd = [{"cov_model": cov, "radiuses": job["radii"], "max_neighbours": job["max_n"]} for _ in range(3)]
+            od, om = O.indicator_kriging((job["cat"], job["mask"], 3), g, ikd, job["marg"])
+        dt = time.perf_counter() - t0
+        O.set_sample_stride(1)
+        sel = np.arange(g.size) % stride == 0
+        arrays["d%d" % i] = od.ravel(order="F")[sel]
+        arrays["m%d" % i] = om.ravel(order="F")[sel]
+        service.append(dt * stride)
+        nodes_all += nodes
+    end = 0.0
+    lat = []
+    for job, sv in zip(jobs, service):
+        start = max(job["t"], end)
+        end = start + sv
+        lat.append((end - job["t"]) * 1000.0)
+    return {"p50_latency_ms": round(float(np.percentile(lat, 50)), 1),
+            "p99_latency_ms": round(float(np.percentile(lat, 99)), 1), "nodes_per_s": nodes_all / end}
+
+
+def cpu_worker(spec):
+    """Child-process side of cpu_leg: loads the oracle, runs the leg and
+    writes its arrays; prints the leg's summary as the last line."""
+    import oracle_binding as O
+
+    O.set_threads(spec["threads"])
+    arrays = {}
+    fn = {"headline": _worker_headline, "cfg3": _worker_cfg3, "sgs": _worker_sgs,
+          "serving": _worker_serving}[spec["leg"]]
+    out = fn(O, spec, arrays)
+    out["omp_max_threads"] = O.lib.oracle_get_threads()
+    np.savez(spec["out"], **arrays)
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(g, mask, gpu=None, budget_s=5.0, runs=3):
+    """The CPU oracle (a C++/OpenMP restatement of the reference's CPU path)
+    on a bounded sample of the same SK + IK workload, in a child process:
+    every s-th cell of the grid is estimated, s chosen from a short pilot
+    so that one pass takes about budget_s; the median of `runs` passes is
+    reported.  The GPU outputs in `gpu` (SK values in mixed and FP64 mode,
+    IK categories) are compared with the oracle's on the sampled cells."""
+    info, arr = cpu_leg({"leg": "headline", "budget_s": budget_s, "runs": runs})
+    stride = info["stride"]
+    nodes = info["kriged_nodes"]
+    out = {"value": nodes / info["seconds"], "unit": "kriged nodes/s", "cores": CPU["threads"], "kind": "port",
+           "sample": "every %d-th cell of the same SK+IK workload (%.0f kriged nodes), median of %d runs %s s"
+                     % (stride, nodes, runs, info["runs_s"]), "host": dict(CPU, omp_max_threads=info["omp_max_threads"])}
+    if gpu is not None:
+        samp = np.arange(g.size) % stride == 0
+        sel = samp & (mask.ravel(order="F") == 0)
+        od, om, ic = arr["sk_d"], arr["sk_m"], arr["ik_c"]
+        cmp = {"sampled_cells": int(sel.sum())}
+        for key in ("sk_mixed", "sk_fp64"):
+            if key in gpu:
+                cmp[key + "_vs_oracle"] = err_stats(gpu[key][sel], od[sel], 1e-3)
+                cmp[key + "_masks_equal"] = bool(np.array_equal(gpu[key + "_mask"][sel], om[sel]))
+        for key in ("ik_mixed", "ik_fp64"):
+            if key in gpu:
+                cmp[key + "_category_agreement_vs_oracle"] = float(np.mean(gpu[key][samp] == ic[samp]))
+        out["parity"] = cmp
+    return out
+
+
+def make_cfg3(shape=(300, 300, 100), seed=1003):
+    """Precision-stress workload: informed density varying smoothly over the
+    grid from 0.1% to 5% (logistic of a Gaussian-filtered noise field,
+    sigma 15 cells), values N(0, 1), 0.5% of the samples with a twin in an
+    x-adjacent cell (value + N(0, 1e-3)); ordinary kriging with a Gaussian
+    covariance of range 25, sill 1, nugget 1e-4, radii 25/25/25 (radii are
+    not given by the workload; chosen equal to the range), max_neighbours 64."""
+    from cases import Cov, Grid, uneven_twins
+
+    g = Grid(*shape)
+    data, mask, twins = uneven_twins(shape, seed)
+    return g, data, mask, twins, Cov(2, (25, 25, 25), sill=1.0, nugget=1e-4), (25, 25, 25), 64
+
+
+def cfg3_bench(torch, lib, abi, hpgl_wrap, steps=1, cpu=True, budget_s=5.0, runs=3):
+    """Batched uneven OK on the precision-stress workload, inputs resident in
+    HBM: kriged nodes/s in mixed mode and in FP64 mode, the FP32-accepted /
+    FP64 / singular counts of each, the neighbour-count histogram, and the
+    mixed result's error against FP64.  The CPU leg runs the oracle on every
+    s-th cell (median of `runs` passes) and compares both GPU results with
+    it on those cells (masks = the dpotrf decisions, values)."""
+    g, data, mask, twins, cov, radii, max_n = make_cfg3()
+    p = abi._HPGL_OK_PARAMS(radiuses=(C.c_int * 3)(*radii), max_neighbours=max_n, covariance_type=cov.type,
+                            ranges=(C.c_double * 3)(*cov.ranges), angles=(C.c_double * 3)(*cov.angles),
+                            sill=cov.sill, nugget=cov.nugget)
+    dev = torch.device("cuda:0")
+    d_in = torch.from_numpy(data.ravel(order="F").copy()).to(dev)
+    d_m = torch.from_numpy(mask.ravel(order="F").copy()).to(dev)
+    d_out = torch.empty_like(d_in)
+    d_om = torch.empty_like(d_m)
+    shape = abi._HPGL_SHAPE(data=(C.c_int * 3)(*g.shape), strides=(C.c_int * 3)(1, g.x, g.x * g.y))
+    stream = torch.cuda.current_stream(dev)
+    sp = C.c_void_p(stream.cuda_stream)
+    n_unk = int(np.sum(mask == 0))
+
+    def run():
+        d_out.copy_(d_in)
+        d_om.copy_(d_m)
+        lib.hpgl_amd_ordinary_kriging_dev(d_in.data_ptr(), d_m.data_ptr(), C.byref(shape), C.byref(p),
+                                          d_out.data_ptr(), d_om.data_ptr(), sp)
+        hpgl_wrap.check_last_error()
+        return hpgl_wrap.last_stats()
+
+    res = {"workload": "OK, 300x300x100 grid, informed density 0.1-5%% (logistic field), %d near-duplicate "
+                       "twins, Gaussian range 25, nugget 1e-4, radii 25/25/25, max_neighbours 64" % twins,
+           "cells": g.size, "informed": int(mask.sum()), "kriged_nodes_per_job": n_unk}
+    outs = {}
+    for prec in ("mixed", "fp64"):
+        hpgl_wrap.set_precision(prec)
+        run()  # warm: assets, code objects
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sts = [run() for _ in range(steps)]
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / steps
+        st = sts[-1]
+        res[prec] = {"value": n_unk / wall, "unit": "kriged nodes/s", "ms_per_job": wall * 1e3,
+                     "solves": {"fp32_accepted": int(st["solves_fp32"]), "fp64": int(st["solves_fp64"]),
+                                "singular": int(st["nodes_singular"]),
+                                "no_neighbours": int(st["nodes_no_neighbours"])},
+                     "kernel_ms": {k: round(st[k], 3) for k in ("ms_k_template", "ms_k_cluster", "ms_k_solve_mixed",
+                                                                "ms_k_solve_fp64")},
+                     "paths": {"template": int(st["nodes_template_path"]), "cluster": int(st["nodes_cluster_path"])},
+                     "mean_neighbours": st["neighbours_total"] / max(1, st["nodes_target"])}
+        outs[prec] = (d_out.cpu().numpy(), d_om.cpu().numpy())
+    hpgl_wrap.set_precision("mixed")
+    unk = mask.ravel(order="F") == 0
+    ok = unk & (outs["fp64"][1] == 1)
+    res["mixed_vs_fp64"] = err_stats(outs["mixed"][0][ok], outs["fp64"][0][ok], 1e-3)
+    res["mixed_vs_fp64"]["masks_equal"] = bool(np.array_equal(outs["mixed"][1], outs["fp64"][1]))
+    if cpu:
+        info, arr = cpu_leg({"leg": "cfg3", "budget_s": budget_s, "runs": runs})
+        stride = info["stride"]
+        od, om = arr["d"], arr["m"]
+        sel = unk & (np.arange(g.size) % stride == 0)
+        n_s = int(sel.sum())
+        res["cpu_baseline"] = {"value": n_s / info["seconds"], "unit": "kriged nodes/s", "cores": CPU["threads"],
+                               "kind": "port", "sample": "every %d-th cell (%d kriged nodes), median of %d runs %s s"
+                                                         % (stride, n_s, runs, info["runs_s"])}
+        par = {"sampled_cells": n_s, "oracle_failures": int(np.sum(om[sel] == 0))}
+        for prec in ("mixed", "fp64"):
+            d, m = outs[prec]
+            par[prec + "_masks_equal"] = bool(np.array_equal(m[sel], om[sel]))
+            both = sel & (om == 1)
+            par[prec + "_vs_oracle"] = err_stats(d[both], od[both], 1e-3)
+            par[prec + "_bit_exact_cells"] = float(np.mean(d[both] == od[both])) if both.any() else 1.0
+        res["parity"] = par
+    return res
 
 
 def make_sis_workload(seed=1006):
@@ -249,7 +545,7 @@ def cokriging_bench(torch, lib, abi, hpgl_wrap, steps=2, warmup=1, cpu=True):
 
         stride = 64
         out = d_out.cpu().numpy().reshape(g.shape, order="F")
-        O.set_threads(16)
+        O.set_threads(CPU["threads"])
         O.set_sample_stride(stride)
         t0 = time.perf_counter()
         od, om = O.simple_cokriging_markI((data, mask), g, (sec, smask), prm["primary_mean"], prm["secondary_mean"],
@@ -260,7 +556,7 @@ def cokriging_bench(torch, lib, abi, hpgl_wrap, steps=2, warmup=1, cpu=True):
         cells = np.arange(g.size) % stride == 0
         flat_m = mask.ravel(order="F") == 0
         sampled = int(np.sum(cells & flat_m))
-        res["cpu_baseline"] = {"value": sampled / dt, "unit": "estimated cells/s", "cores": 16, "kind": "port",
+        res["cpu_baseline"] = {"value": sampled / dt, "unit": "estimated cells/s", "cores": CPU["threads"], "kind": "port",
                                "sample": "every 64th cell of the same job (%d estimated cells)" % sampled}
         sel = cells
         res["equals_oracle_on_sample"] = bool(np.array_equal(out.ravel(order="F")[sel], od.ravel(order="F")[sel]))
@@ -308,22 +604,29 @@ def variogram_bench(cpu=True):
     return res
 
 
-def sgs_bench(torch, lib, abi, hpgl_wrap, R=64, cpu=True):
-    """Sequential Gaussian simulation, R realisations batched in one call
-    (200x200x100 grid, 1% conditioning data as normal scores, spherical
-    ranges 30/30/6, sill 1, nugget 0, radii 30/30/6, max_neighbours 32, SK,
-    no transform, seeds 1000..1000+R-1), inputs resident in HBM; simulated
-    cells/s over all realisations.  The CPU leg runs the oracle on one
-    realisation of a 100x100x50 grid with the same density and model (the
-    reference's simulation is serial)."""
+def make_sgs_workload():
+    """200x200x100 grid, 1% conditioning data (normal scores, seed 1008),
+    spherical ranges 30/30/6, sill 1, nugget 0, radii 30/30/6 (radii are not
+    given by the workload; chosen equal to the ranges), max_neighbours 32."""
     from cases import Cov, Grid, scattered
 
     g = Grid(200, 200, 100)
     data, mask = scattered(g.shape, 0.01, 1008, law="normal")
-    cov = Cov(0, (30, 30, 6), sill=1.0, nugget=0.0)
+    return g, data, mask, Cov(0, (30, 30, 6), sill=1.0, nugget=0.0), (30, 30, 6), 32
+
+
+def sgs_bench(torch, lib, abi, hpgl_wrap, R=64, cpu=True):
+    """Sequential Gaussian simulation, R realisations batched in one call on
+    the SGS workload (SK, no transform, seeds 1000..1000+R-1), inputs
+    resident in HBM; simulated cells/s over all realisations.  The CPU leg
+    runs the oracle on the same workload: one realisation on one thread
+    (the reference's simulation is serial), then one realisation per core
+    side by side (seeds 1000..1000+P-1); the GPU realisations of the same
+    seeds are compared with them cell for cell."""
+    g, data, mask, cov, radii, max_n = make_sgs_workload()
     p = abi._HPGL_SGS_PARAMS(covariance_type=cov.type, ranges=(C.c_double * 3)(*cov.ranges),
                              angles=(C.c_double * 3)(*cov.angles), sill=cov.sill, nugget=cov.nugget,
-                             radiuses=(C.c_int * 3)(30, 30, 6), max_neighbours=32, kriging_kind=1, seed=1000,
+                             radiuses=(C.c_int * 3)(*radii), max_neighbours=max_n, kriging_kind=1, seed=1000,
                              min_neighbours=0)
     dev = torch.device("cuda:0")
     d_in = torch.from_numpy(data.ravel(order="F").copy()).to(dev)
@@ -347,7 +650,8 @@ def sgs_bench(torch, lib, abi, hpgl_wrap, R=64, cpu=True):
     st = run(R)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    n_sim = int(np.sum(mask == 0)) * R
+    n_sim1 = int(np.sum(mask == 0))
+    n_sim = n_sim1 * R
     res = {"workload": "SGS, %d realisations in one call, 200x200x100 grid, 1%% conditioning (normal scores), "
                        "spherical ranges 30/30/6, radii 30/30/6, max_neighbours 32, SK" % R,
            "simulated_cells": n_sim, "value": n_sim / wall, "unit": "simulated cells/s",
@@ -355,17 +659,25 @@ def sgs_bench(torch, lib, abi, hpgl_wrap, R=64, cpu=True):
            "ms_template_scan": st["ms_k_template"], "ms_solve": st["ms_solve"],
            "solves": {"fp32_accepted": int(st["solves_fp32"]), "fp64": int(st["solves_fp64"])}}
     if cpu:
-        import oracle_binding as O
-
-        gs = Grid(100, 100, 50)
-        sd, sm = scattered(gs.shape, 0.01, 1009, law="normal")
-        O.set_threads(1)
-        t0 = time.perf_counter()
-        O.sgs_simulation((sd, sm), gs, None, (30, 30, 6), 32, cov, 1000, kriging_type="sk")
-        dt = time.perf_counter() - t0
-        res["cpu_baseline"] = {"value": int(np.sum(sm == 0)) / dt, "unit": "simulated cells/s", "cores": 1,
-                               "kind": "port", "sample": "one realisation of a 100x100x50 grid, same density "
-                                                         "and model (%.1f s)" % dt}
+        info, arr = cpu_leg({"leg": "sgs"})
+        P = info["batch_realisations"]
+        res["cpu_baseline"] = {
+            "value": P * n_sim1 / info["batch_s"], "unit": "simulated cells/s", "cores": CPU["threads"],
+            "kind": "port",
+            "sample": "%d realisations of the same workload side by side, one per core (seeds 1000..%d), %.1f s; "
+                      "64 realisations take %d such rounds" % (P, 999 + P, info["batch_s"], -(-64 // P)),
+            "serial_one_realisation": {"value": n_sim1 / info["batch_s"], "unit": "simulated cells/s",
+                                       "cores": 1, "seconds": round(info["batch_s"], 2),
+                                       "note": "each realisation runs serially on one core while the other "
+                                               "cores run theirs"}}
+        out = d_out.view(R, g.size)[:P].cpu().numpy()
+        om = d_om.view(R, g.size)[:P].cpu().numpy()
+        simc = mask.ravel(order="F") == 0
+        want = arr["batch"][:, simc]
+        got = out[:, simc]
+        res["parity"] = {"realisations": P, "vs_oracle": err_stats(got, want, 1.0),
+                         "bit_exact_cells": float(np.mean(got == want)),
+                         "masks_all_set": bool(np.all(om[:, simc] == 1))}
     return res
 
 
@@ -476,7 +788,8 @@ def serving_bench(slots=4):
         raise RuntimeError("serving job failed: %s" % failed[0])
     lat = np.array([r["latency_ms"] for r in results])
     nodes = sum(r["nodes"] for r in results)
-    return jobs, {"jobs": len(jobs), "arrival_rate_per_s": 20.0, "slots": slots,
+    outputs = [(h.out, h.out_mask) for h in handles]
+    return jobs, outputs, {"jobs": len(jobs), "arrival_rate_per_s": 20.0, "slots": slots,
                   "p50_latency_ms": round(float(np.percentile(lat, 50)), 2),
                   "p99_latency_ms": round(float(np.percentile(lat, 99)), 2),
                   "mean_queue_ms": round(float(np.mean([r["queue_ms"] for r in results])), 2),
@@ -484,49 +797,43 @@ def serving_bench(slots=4):
                   "trace": "Poisson 20 jobs/s, OK/SK/IK/LVM, grids 32..256 x 32..256 x 32..128, seed 1005"}
 
 
-def serving_cpu_baseline(jobs, threads=16, nodes_per_sample=20000):
-    """The same trace processed first-in first-out by the CPU oracle: each
-    job's service time is measured on a sample of its cells (every s-th
-    cell, about nodes_per_sample of them) and scaled to the job; latencies
-    follow from the arrival times of a single FIFO server."""
-    import oracle_binding as O
-    from cases import Cov, Grid
-
-    O.set_threads(threads)
-    service = []
-    nodes_all = 0
-    for job in jobs:
-        g = Grid(*job["shape"])
-        n_unk = int(np.sum(job["mask"] == 0))
-        nodes = g.size * 3 if job["kind"] == "ik" else n_unk
-        stride = max(1, g.size // nodes_per_sample)
-        cov = Cov(job["ctype"], job["ranges"], sill=1.0, nugget=0.1)
-        O.set_sample_stride(stride)
-        t0 = time.perf_counter()
-        if job["kind"] == "ok":
-            O.ordinary_kriging((job["data"], job["mask"]), g, job["radii"], job["max_n"], cov)
-        elif job["kind"] == "sk":
-            O.simple_kriging((job["data"], job["mask"]), g, job["radii"], job["max_n"], cov)
-        elif job["kind"] == "lvm":
-            O.lvm_kriging((job["data"], job["mask"]), g, job["mean"], job["radii"], job["max_n"], cov)
-        else:
-            ikd = [{"cov_model": cov, "radiuses": job["radii"], "max_neighbours": job["max_n"]} for _ in range(3)]
-            O.indicator_kriging((job["cat"], job["mask"], 3), g, ikd, job["marg"])
-        dt = time.perf_counter() - t0
-        O.set_sample_stride(1)
-        service.append(dt * stride)
-        nodes_all += nodes
-    end = 0.0
-    lat = []
-    for job, sv in zip(jobs, service):
-        start = max(job["t"], end)
-        end = start + sv
-        lat.append((end - job["t"]) * 1000.0)
-    return {"p50_latency_ms": round(float(np.percentile(lat, 50)), 1),
-            "p99_latency_ms": round(float(np.percentile(lat, 99)), 1),
-            "nodes_per_s": nodes_all / end, "cores": threads, "kind": "port",
+def serving_cpu_baseline(jobs, outputs, nodes_per_sample=20000):
+    """The same trace processed first-in first-out by the CPU oracle (in a
+    child process): each job's service time is measured on a sample of its
+    cells (every s-th cell, about nodes_per_sample of them) and scaled to
+    the job; latencies follow from the arrival times of a single FIFO
+    server.  Every job's GPU result from the request loop is compared with
+    the oracle's on the sampled cells."""
+    info, arr = cpu_leg({"leg": "serving", "nodes_per_sample": nodes_per_sample})
+    worst_cont = {"max_rel": 0.0, "rms_sq": 0.0, "cells": 0}
+    mask_mismatch = 0
+    ik_agree = []
+    for i, (job, (out, out_mask)) in enumerate(zip(jobs, outputs)):
+        n = int(np.prod(job["shape"]))
+        stride = max(1, n // nodes_per_sample)
+        sel = np.arange(n) % stride == 0
+        got = out.ravel(order="F")[sel]
+        gm = out_mask.ravel(order="F")[sel]
+        want, wm = arr["d%d" % i], arr["m%d" % i]
+        mask_mismatch += int(np.sum(gm != wm))
+        if job["kind"] == "ik":
+            ik_agree.append(float(np.mean(got == want)))
+            continue
+        unk = job["mask"].ravel(order="F")[sel] == 0
+        both = unk & (wm == 1)
+        e = err_stats(got[both], want[both], 1e-3)
+        worst_cont["max_rel"] = max(worst_cont["max_rel"], e["max_rel"])
+        worst_cont["rms_sq"] += e["rms_rel"] ** 2 * e["cells"]
+        worst_cont["cells"] += e["cells"]
+    parity = {"ok_sk_lvm_vs_oracle": {"max_rel": worst_cont["max_rel"],
+                                      "rms_rel": float(np.sqrt(worst_cont["rms_sq"] / max(1, worst_cont["cells"]))),
+                                      "cells": worst_cont["cells"], "floor": 1e-3},
+              "mask_mismatches": mask_mismatch,
+              "ik_category_agreement_min": min(ik_agree) if ik_agree else None}
+    return {"p50_latency_ms": info["p50_latency_ms"], "p99_latency_ms": info["p99_latency_ms"],
+            "nodes_per_s": info["nodes_per_s"], "cores": CPU["threads"], "kind": "port",
             "sample": "each job's cells sampled every s-th (about %d per job), service time scaled by s, "
-                      "FIFO single server over the arrival trace" % nodes_per_sample}
+                      "FIFO single server over the arrival trace" % nodes_per_sample, "parity": parity}
 
 
 def main():
@@ -543,6 +850,10 @@ def main():
     ap.add_argument("--no-sgs", action="store_true", help="skip the batched SGS measurement")
     ap.add_argument("--sgs-only", action="store_true", help="run only the batched SGS measurement (for profiling)")
     ap.add_argument("--no-cokriging", action="store_true", help="skip the cokriging measurement")
+    ap.add_argument("--cpu-worker", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--no-host-path", action="store_true", help="skip the host-pointer geo_bsd timing")
+    ap.add_argument("--no-cfg3", action="store_true", help="skip the precision-stress (uneven OK) measurement")
+    ap.add_argument("--cfg3-only", action="store_true", help="run only the precision-stress OK measurement")
     ap.add_argument("--no-variogram", action="store_true", help="skip the experimental variogram measurement")
     ap.add_argument("--variogram-only", action="store_true",
                     help="run only the experimental variogram measurement (for profiling)")
@@ -558,6 +869,10 @@ def main():
         print("bench.py measures one GPU: the kriging path has no multi-GPU mode in this build "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+
+    if args.cpu_worker:
+        cpu_worker(json.loads(args.cpu_worker))
+        return
 
     import torch
 
@@ -581,6 +896,15 @@ def main():
                                                 vp, vp, vp]
     lib.hpgl_amd_indicator_kriging_dev.argtypes = [vp, vp, C.POINTER(abi._HPGL_SHAPE),
                                                    C.POINTER(abi._HPGL_IK_PARAMS), C.c_int, vp, vp, vp]
+    lib.hpgl_amd_ordinary_kriging_dev.argtypes = [vp, vp, C.POINTER(abi._HPGL_SHAPE), C.POINTER(abi._HPGL_OK_PARAMS),
+                                                  vp, vp, vp]
+    if not args.no_cpu_baseline:
+        CPU.update(cpu_setup())
+
+    if args.cfg3_only:
+        print(json.dumps({"cfg3": cfg3_bench(torch, lib, abi, hpgl_wrap, steps=max(1, args.steps),
+                                             cpu=not args.no_cpu_baseline)}), flush=True)
+        return
 
     if args.sgs_only:
         print(json.dumps({"sgs": sgs_bench(torch, lib, abi, hpgl_wrap, cpu=not args.no_cpu_baseline)}), flush=True)
@@ -598,6 +922,7 @@ def main():
                                            cpu=not args.no_cpu_baseline)}), flush=True)
         return
 
+    progress("headline SK + IK workload (256x256x128)")
     g, data, mask, cat, marg, sk, ik = make_cfg2()
     n_unk = int(np.sum(mask == 0))
     nodes_per_step = n_unk * (1 + len(ik))
@@ -638,7 +963,16 @@ def main():
                     acc[k] = acc.get(k, 0) + v
 
     hpgl_wrap.set_precision(args.precision)
-    for _ in range(args.warmup):
+    # the first step builds the covariance tables, templates and rank tables
+    # (then resident) and loads the code objects: the cold-call cost
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    step(True)
+    torch.cuda.synchronize()
+    cold = {"first_step_ms": round((time.perf_counter() - t0) * 1e3, 2),
+            "host_setup_ms": round(acc.get("ms_setup_host", 0.0), 2)}
+    acc.clear()
+    for _ in range(max(0, args.warmup - 1)):
         step(False)
     torch.cuda.synchronize()
     ev0 = torch.cuda.Event(enable_timing=True)
@@ -715,12 +1049,15 @@ def main():
                   "cluster_candidates": int(acc.get("cluster_candidates", 0)),
                   "cluster_lists_spilled": int(acc.get("cluster_lists_spilled", 0))},
         "host_setup_ms_per_step": round(acc.get("ms_setup_host", 0.0) / args.steps, 2),
+        "cold_call": cold,
         "device_ms_per_step": round(float(ev0.elapsed_time(ev1)) / args.steps, 3),
     }
 
+    gpu_out = {}
+    if args.precision == "mixed":
+        gpu_out.update(sk_mixed=d_out.cpu().numpy(), sk_mixed_mask=d_out_mask.cpu().numpy(),
+                       ik_mixed=d_iout.cpu().numpy())
     if not args.no_fp64_compare and args.precision == "mixed":
-        mixed_sk = d_out.cpu().numpy()
-        mixed_ik = d_iout.cpu().numpy()
         hpgl_wrap.set_precision("fp64")
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -728,21 +1065,45 @@ def main():
         torch.cuda.synchronize()
         t64 = time.perf_counter() - t0
         hpgl_wrap.set_precision("mixed")
-        ref = d_out.cpu().numpy().astype(np.float64)
+        gpu_out.update(sk_fp64=d_out.cpu().numpy(), sk_fp64_mask=d_out_mask.cpu().numpy(),
+                       ik_fp64=d_iout.cpu().numpy())
         m = mask.ravel(order="F") == 0
-        rel = np.abs(mixed_sk[m].astype(np.float64) - ref[m]) / np.maximum(np.abs(ref[m]), 1e-3)
+        e = err_stats(gpu_out["sk_mixed"][m], gpu_out["sk_fp64"][m], 1e-3)
         out["fp64_compare"] = {"fp64_value": nodes_per_step / t64, "fp64_ms_per_step": t64 * 1000,
-                               "sk_max_rel_err_vs_fp64": float(rel.max()),
-                               "sk_rms_rel_err_vs_fp64": float(np.sqrt(np.mean(rel ** 2))),
-                               "ik_category_agreement": float(np.mean(mixed_ik == d_iout.cpu().numpy()))}
+                               "sk_max_rel_err_vs_fp64": e["max_rel"], "sk_rms_rel_err_vs_fp64": e["rms_rel"],
+                               "rel_err_floor": 1e-3,
+                               "sk_masks_equal": bool(np.array_equal(gpu_out["sk_mixed_mask"],
+                                                                     gpu_out["sk_fp64_mask"])),
+                               "ik_category_agreement": float(np.mean(gpu_out["ik_mixed"] == gpu_out["ik_fp64"]))}
+    if not args.no_host_path:
+        # the drop-in path a geo_bsd user calls: host arrays, PCIe copies,
+        # the front end's clone of the input
+        import hpgl_amd.geo_bsd as geo_api
 
+        ikd = [{"cov_model": p["cov"], "radiuses": p["radii"], "max_neighbours": p["max_n"]} for p in ik]
+        hpgl_wrap.set_precision(args.precision)
+        t0 = time.perf_counter()
+        geo_api.simple_kriging((data, mask), g, sk["radii"], sk["max_n"], sk["cov"], mean=sk["mean"])
+        geo_api.indicator_kriging((cat, mask, 5), g, ikd, list(marg))
+        th = time.perf_counter() - t0
+        out["host_pointer_path"] = {"value": nodes_per_step / th, "unit": "kriged nodes/s", "ms_per_step": th * 1e3,
+                                    "note": "geo_bsd.simple_kriging + indicator_kriging on numpy arrays: the same "
+                                            "workload with host-device copies and the input clone"}
+
+    if not args.no_cfg3:
+        progress("precision-stress OK (300x300x100)")
+        out["cfg3"] = cfg3_bench(torch, lib, abi, hpgl_wrap, cpu=not args.no_cpu_baseline)
     if not args.no_sgs:
+        progress("SGS, 64 realisations")
         out["sgs"] = sgs_bench(torch, lib, abi, hpgl_wrap, cpu=not args.no_cpu_baseline)
     if not args.no_sis:
+        progress("SIS")
         out["sis"] = sis_bench(torch, lib, abi, hpgl_wrap, cpu=not args.no_cpu_baseline)
     if not args.no_cokriging:
+        progress("cokriging")
         out["cokriging"] = cokriging_bench(torch, lib, abi, hpgl_wrap, cpu=not args.no_cpu_baseline)
     if not args.no_variogram:
+        progress("variogram")
         out["variogram"] = variogram_bench(cpu=not args.no_cpu_baseline)
     if not args.no_io:
         import hpgl_amd.geo_bsd as geo_api
@@ -751,12 +1112,15 @@ def main():
         out["property_io"] = property_io_bench(geo_api, d_out.cpu().numpy().reshape(shape, order="F"),
                                                d_out_mask.cpu().numpy().reshape(shape, order="F"))
     if not args.no_serving:
-        trace, out["serving"] = serving_bench()
+        progress("serving stream")
+        trace, outputs, out["serving"] = serving_bench()
         if not args.no_cpu_baseline:
-            out["serving"]["cpu_baseline"] = serving_cpu_baseline(trace)
-        del trace
+            progress("CPU baseline of the serving stream")
+            out["serving"]["cpu_baseline"] = serving_cpu_baseline(trace, outputs)
+        del trace, outputs
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(g, data, mask, cat, marg, sk, ik)
+        progress("CPU baseline of the headline workload")
+        out["cpu_baseline"] = cpu_baseline(g, mask, gpu=gpu_out)
     print(json.dumps(out), flush=True)
 
 

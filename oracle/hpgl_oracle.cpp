@@ -542,14 +542,20 @@ struct Means {
     }
 };
 
+// Outcomes of the last cont_kriging call: nodes without neighbours and
+// nodes whose system failed (dpotrf/dpotrs info != 0), for the tests.
+static long long g_last_no_neighbours = 0;
+static long long g_last_singular = 0;
+
 static void cont_kriging(const float *in, const unsigned char *in_mask, const Grid &g, const Lookup &nl,
                          const Means &means, bool ok_system, bool mean_on_failure, float *out,
                          unsigned char *out_mask) {
     const int N = g.size();
+    long long n_none = 0, n_sing = 0;
     for (int node = 0; node < N; ++node)
         if (in_mask[node] != 0) nl.cl->add_node(node);
     auto informed = [&](int idx) { return idx < 0 || idx >= N ? false : in_mask[idx] != 0; };
-#pragma omp parallel
+#pragma omp parallel reduction(+ : n_none, n_sing)
     {
         std::vector<int> idx;
         std::vector<Coord> coords;
@@ -586,6 +592,8 @@ static void cont_kriging(const float *in, const unsigned char *in_mask, const Gr
                         result = 2;
                     }
                 }
+                n_none += result == 1;
+                n_sing += result == 2;
                 if (result == 0) {
                     out[node] = value;
                     out_mask[node] = 1;
@@ -603,6 +611,8 @@ static void cont_kriging(const float *in, const unsigned char *in_mask, const Gr
             }
         }
     }
+    g_last_no_neighbours = n_none;
+    g_last_singular = n_sing;
 }
 
 static CovParams cov_from(int type, const double r[3], const double a[3], double sill, double nugget) {
@@ -1074,6 +1084,11 @@ ORACLE_API void oracle_set_threads(int n) {
 }
 ORACLE_API int oracle_get_threads() { return omp_get_max_threads(); }
 ORACLE_API void oracle_set_progress_tick(int on) { g_progress_tick = on; }
+// Nodes without neighbours and singular systems of the last OK/SK/LVM call.
+ORACLE_API void oracle_last_counts(long long *out) {
+    out[0] = g_last_no_neighbours;
+    out[1] = g_last_singular;
+}
 ORACLE_API void oracle_set_sample_stride(int stride) { g_sample_stride = stride > 1 ? stride : 1; }
 ORACLE_API void oracle_set_neighbour_dump(int *indices, int *counts, long long n_cells, int max_n) {
     g_dump_idx = indices;
@@ -1283,6 +1298,31 @@ ORACLE_API void oracle_sgs_simulation(hpgl_cont_masked_array_t *prop, hpgl_sgs_p
     sgs_core(prop->m_data, prop->m_mask, g, p, nullptr, sim_mask ? sim_mask->m_data : nullptr);
     if (cdf) transform_back(prop->m_data, prop->m_mask, N, c);
     GUARD_END
+}
+
+// R realisations of hpgl_sgs_simulation (no cdf, no simulation mask) side
+// by side, one realisation per OpenMP thread at a time: the CPU baseline of
+// a batch of realisations (the reference's simulation is serial per call,
+// so a batch spreads over cores only across realisations).  data and mask
+// hold R consecutive copies of the property; seeds[r] replaces m_seed.
+ORACLE_API void oracle_sgs_simulation_batch(float *data, unsigned char *mask, const hpgl_shape_t *shape,
+                                            const hpgl_sgs_params_t *p, const long *seeds, int R) {
+    g_last_error.clear();
+    Grid g{shape->m_data[0], shape->m_data[1], shape->m_data[2]};
+    const long long N = g.size();
+    std::string err;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int r = 0; r < R; ++r) {
+        try {
+            hpgl_sgs_params_t q = *p;
+            q.m_seed = seeds[r];
+            sgs_core(data + r * N, mask + r * N, g, &q, nullptr, nullptr);
+        } catch (const std::exception &e) {
+#pragma omp critical(oracle_batch_error)
+            err = e.what();
+        }
+    }
+    if (!err.empty()) g_last_error = err;
 }
 
 // hpgl_sgs_lvm_simulation (api.cpp:440-467 -> sequential_gaussian_simulation.cpp:77-128)

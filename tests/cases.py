@@ -76,3 +76,37 @@ def rel_err(a, b, scale):
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
     return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), scale))) if a.size else 0.0
+
+
+def uneven_twins(shape, seed, sigma=15.0, twin_frac=0.005, twin_sd=1e-3):
+    """Precision-stress property: informed density varying smoothly from
+    0.1% to 5%, p = 0.001 + 0.049 / (1 + exp(-2 f)) with f white noise
+    Gaussian-filtered (sigma cells, wrapped edges) and standardised; each
+    cell is informed with probability p, values N(0, 1).  Then a fraction
+    twin_frac of the samples get a twin in an x-adjacent uninformed cell
+    (x + 1, else x - 1) with the value plus N(0, twin_sd): near-duplicate
+    samples one cell apart (grid coordinates are integers, so sub-cell
+    jitter cannot be expressed).  Returns data, mask and the count of twins."""
+    from scipy.ndimage import gaussian_filter
+
+    rng = np.random.default_rng(seed)
+    noise = rng.standard_normal(shape)
+    f = gaussian_filter(noise, sigma, mode="wrap")
+    f = (f - f.mean()) / (f.std() + 1e-12)
+    p = 0.001 + 0.049 / (1.0 + np.exp(-2.0 * f))
+    mask = (rng.random(shape) < p).astype(np.uint8)
+    data = np.full(shape, -99.0, np.float32)
+    n_inf = int(mask.sum())
+    data[mask == 1] = rng.standard_normal(n_inf).astype(np.float32)
+    src = np.argwhere(mask == 1)
+    pick = src[rng.choice(len(src), int(round(twin_frac * len(src))), replace=False)]
+    X = shape[0]
+    twins = 0
+    for x, y, z in pick:
+        for nx in (x + 1, x - 1):
+            if 0 <= nx < X and mask[nx, y, z] == 0:
+                mask[nx, y, z] = 1
+                data[nx, y, z] = np.float32(data[x, y, z] + rng.normal(0.0, twin_sd))
+                twins += 1
+                break
+    return np.asfortranarray(data), np.asfortranarray(mask), twins

@@ -41,10 +41,13 @@ cvabi = _load_cvar_abi()
 
 
 def _load():
+    # ORACLE_SO may name another build of the same source (the CPU baseline
+    # loads the -march=native one)
+    path = os.environ.get("ORACLE_SO", ORACLE_SO)
     src = os.path.join(ORACLE_DIR, "hpgl_oracle.cpp")
-    if not os.path.exists(ORACLE_SO) or os.path.getmtime(ORACLE_SO) < os.path.getmtime(src):
+    if path == ORACLE_SO and (not os.path.exists(ORACLE_SO) or os.path.getmtime(ORACLE_SO) < os.path.getmtime(src)):
         subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
-    return C.CDLL(ORACLE_SO)
+    return C.CDLL(path)
 
 
 lib = _load()
@@ -60,6 +63,9 @@ lib.oracle_indicator_kriging.argtypes = [_P(abi._HPGL_IND_MASKED_ARRAY), _P(abi.
                                          _P(abi._HPGL_IK_PARAMS), C.c_int]
 lib.oracle_median_ik.argtypes = [_P(abi._HPGL_IND_MASKED_ARRAY), _P(abi._HPGL_MEDIAN_IK_PARAMS),
                                  _P(abi._HPGL_IND_MASKED_ARRAY)]
+lib.oracle_last_counts.argtypes = [C.c_void_p]
+lib.oracle_sgs_simulation_batch.argtypes = [C.c_void_p, C.c_void_p, _P(abi._HPGL_SHAPE), _P(abi._HPGL_SGS_PARAMS),
+                                            C.c_void_p, C.c_int]
 lib.oracle_sgs_simulation.argtypes = [_P(abi._HPGL_CONT_MASKED_ARRAY), _P(abi._HPGL_SGS_PARAMS),
                                       _P(abi.hpgl_non_parametric_cdf_t), _P(C.c_double), _P(abi._HPGL_UBYTE_ARRAY)]
 lib.oracle_sgs_lvm_simulation.argtypes = [_P(abi._HPGL_CONT_MASKED_ARRAY), _P(abi._HPGL_SGS_PARAMS),
@@ -263,6 +269,24 @@ def sgs_simulation(prop, grid, cdf, radiuses, max_neighbours, cov_model, seed, k
     return od, om
 
 
+def sgs_simulation_batch(prop, grid, radiuses, max_neighbours, cov_model, seeds, kriging_type="sk"):
+    """len(seeds) realisations of the C-level SGS (no cdf), one per OpenMP
+    thread at a time; returns (values, masks) of shape (R, N), each row a
+    Fortran-order property."""
+    data, m = prop
+    d, mk = _cont(data, m)
+    R = len(seeds)
+    od = np.tile(d.ravel(order="F"), (R, 1))
+    om = np.tile(mk.ravel(order="F"), (R, 1))
+    p = abi._HPGL_SGS_PARAMS(radiuses=(C.c_int * 3)(*radiuses), max_neighbours=max_neighbours,
+                             kriging_kind={"sk": 1, "ok": 0}[kriging_type], seed=int(seeds[0]), min_neighbours=0,
+                             **_cov_kwargs(cov_model))
+    sd = np.ascontiguousarray(seeds, np.int64)
+    lib.oracle_sgs_simulation_batch(_ptr(od), _ptr(om), C.byref(_shape(grid)), C.byref(p), _ptr(sd), R)
+    _check()
+    return od, om
+
+
 def sis_simulation(prop, grid, data, seed, marginal_probs, use_correlogram=True, mask=None):
     """C-level SIS on (data, mask, indicator_count); marginal_probs are K
     scalars or K per-cell arrays (LVM).  Returns the simulated (data, mask)."""
@@ -437,6 +461,13 @@ def gauss_inverse(p):
 
 def gauss_prob(x):
     return lib.oracle_gauss_prob(x)
+
+
+def last_counts():
+    """(nodes without neighbours, singular systems) of the last OK/SK/LVM call."""
+    out = np.zeros(2, np.int64)
+    lib.oracle_last_counts(_ptr(out))
+    return int(out[0]), int(out[1])
 
 
 def set_threads(n):

@@ -331,3 +331,18 @@ def test_failure_policies():
     assert om[39] == 0 and out[39] == -99 and om[1] == 1
     out, om = O.simple_kriging((data, mask), G(40, 1, 1), (2, 1, 1), 4, CM(0, (2, 1, 1)), mean=1.5)
     assert om[39] == 1 and out[39] == np.float32(1.5)
+
+
+def test_sgs_batch_equals_serial_calls():
+    """The CPU baseline's batch of realisations (one per OpenMP thread) gives
+    each realisation exactly as a serial call with that seed."""
+    from cases import Cov, Grid, scattered
+
+    g = Grid(30, 20, 10)
+    d, m = scattered(g.shape, 0.02, 5, law="normal")
+    cov = Cov(0, (8, 8, 3))
+    seeds = [1000, 1001, 1002, 1003]
+    bd, bm = O.sgs_simulation_batch((d, m), g, (8, 8, 3), 12, cov, seeds)
+    for r, s in enumerate(seeds):
+        sd, sm = O.sgs_simulation((d, m), g, None, (8, 8, 3), 12, cov, s)
+        assert np.array_equal(bd[r], sd.ravel(order="F")) and np.array_equal(bm[r], sm.ravel(order="F"))
