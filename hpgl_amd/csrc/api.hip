@@ -8,6 +8,8 @@
 
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -583,6 +585,78 @@ API int hpgl_simple_kriging_weights(float *center, float *xs, float *ys, float *
     hip_check(hipStreamSynchronize(s), "sync");
     if (hinfo != 0) throw std::runtime_error("simple_kriging_weights: singular system");
     for (int i = 0; i < n; ++i) weights[i] = (float)hw[i];
+    GUARD_END
+    return 0;
+}
+
+// ---- simple_kriging_weights without allocation ----
+//
+// Device-pointer variant of hpgl_simple_kriging_weights for callers that
+// cannot allocate (code under graph capture, serving loops): the system
+// matrix, the double weights and the factorisation status live in the
+// file-scope array `scratch` (about 2 MiB per device, held until the
+// process ends).  Its device address is looked up once per device with
+// hipGetSymbolAddress; host code never passes the array's name itself.
+// Every call on a device shares that device's array, so calls that overlap
+// on one GPU race.
+namespace {
+constexpr int kNoAllocMaxPoints = 512;
+constexpr size_t kScratchMatrix = (size_t)kNoAllocMaxPoints * kNoAllocMaxPoints * sizeof(double);
+constexpr size_t kScratchWeights = (size_t)kNoAllocMaxPoints * sizeof(double);
+constexpr size_t kScratchBytes = kScratchMatrix + kScratchWeights + 256;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scratch_mu;
+std::map<int, unsigned char *> g_scratch_ptr;  // device -> address of its `scratch`
+
+unsigned char *scratch_address() {
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    auto it = g_scratch_ptr.find(dev);
+    if (it != g_scratch_ptr.end()) return it->second;
+    void *p = nullptr;
+    hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (!p) throw std::runtime_error("hipGetSymbolAddress(scratch) returned a null address");
+    g_scratch_ptr[dev] = static_cast<unsigned char *>(p);
+    return static_cast<unsigned char *>(p);
+}
+
+// out[i] = (float)w[i] when the factorisation succeeded; untouched otherwise.
+__global__ void k_weights_out(const double *w, const int *info, int n, float *out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && *info == 0) out[i] = (float)w[i];
+}
+}  // namespace
+
+API int hpgl_amd_simple_kriging_weights_noalloc(const float *center, const float *xs, const float *ys,
+                                                const float *zs, int n, const hpgl_cov_params_t *p, float *weights,
+                                                void *stream) {
+    GUARD_BEGIN
+    if (n <= 0) return 0;
+    if (n > kNoAllocMaxPoints)
+        throw std::runtime_error("simple_kriging_weights_noalloc: at most " + std::to_string(kNoAllocMaxPoints) +
+                                 " points");
+    CovModel m(cov_of(p->m_covariance_type, p->m_ranges, p->m_angles, p->m_sill, p->m_nugget));
+    DevCov c;
+    std::memset(&c, 0, sizeof(c));
+    for (int i = 0; i < 9; ++i) c.t[i] = m.t[i];
+    c.range0 = m.s.ranges[0];
+    c.c0 = m.c0();
+    c.type = m.s.type;
+    const hipStream_t s = (hipStream_t)stream;
+    unsigned char *base = scratch_address();
+    double *A = reinterpret_cast<double *>(base);
+    double *w = reinterpret_cast<double *>(base + kScratchMatrix);
+    int *info = reinterpret_cast<int *>(base + kScratchMatrix + kScratchWeights);
+    hipLaunchKernelGGL(k_sk_weights_real, dim3(1), dim3(64), 0, s, center, xs, ys, zs, n, c, A, w, info);
+    hip_check(hipGetLastError(), "sk weights");
+    hipLaunchKernelGGL(k_weights_out, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, w, info, n, weights);
+    hip_check(hipGetLastError(), "sk weights out");
+    int hinfo = 0;
+    hip_check(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, s), "copy status");
+    hip_check(hipStreamSynchronize(s), "sync");
+    if (hinfo != 0) throw std::runtime_error("simple_kriging_weights: singular system");
     GUARD_END
     return 0;
 }

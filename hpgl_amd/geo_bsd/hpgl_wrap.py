@@ -160,6 +160,9 @@ _hpgl_so.hpgl_amd_get_last_error.restype = C.c_int
 _hpgl_so.hpgl_amd_get_last_error.argtypes = []
 _hpgl_so.hpgl_amd_get_last_stats.restype = None
 _hpgl_so.hpgl_amd_get_last_stats.argtypes = [_P(_HPGL_AMD_STATS)]
+_hpgl_so.hpgl_amd_simple_kriging_weights_noalloc.restype = C.c_int
+_hpgl_so.hpgl_amd_simple_kriging_weights_noalloc.argtypes = [C.c_void_p] * 4 + [C.c_int, _P(_HPGL_COV_PARAMS),
+                                                                               C.c_void_p, C.c_void_p]
 _hpgl_so.hpgl_amd_pool_selftest.restype = C.c_int
 _hpgl_so.hpgl_amd_pool_selftest.argtypes = []
 _hpgl_so.hpgl_amd_set_neighbour_dump.restype = None

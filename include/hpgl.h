@@ -213,6 +213,20 @@ int hpgl_simple_kriging_weights(float *center_coords, float *neighbours_x,
                                 int neighbours_count, struct hpgl_cov_params_t *params,
                                 float *weights);
 
+/* No counterpart in the reference: a variant of hpgl_simple_kriging_weights
+ * that allocates nothing, for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops).  center (3 floats),
+ * xs, ys, zs (n floats) and weights (n floats) are device pointers; the
+ * work is queued on `stream` and the call returns after it completes.  The
+ * system and its factorisation use a static device array of about 2 MiB per
+ * device: n is limited to 512 (larger n sets the last error), and calls
+ * that overlap on one GPU race.  Returns 0; a singular system leaves the
+ * weights unwritten and sets the last error, as hpgl_simple_kriging_weights
+ * does. */
+int hpgl_amd_simple_kriging_weights_noalloc(const float *center, const float *xs, const float *ys,
+                                            const float *zs, int n, const struct hpgl_cov_params_t *params,
+                                            float *weights, void *stream);
+
 /* api.h:242-251, api.cpp:344-383 (lvm_kriging.cpp:16-37). */
 void hpgl_lvm_kriging(float *input_data, unsigned char *input_mask,
                       struct hpgl_shape_t *input_data_shape,
