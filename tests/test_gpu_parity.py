@@ -344,3 +344,32 @@ def test_ordinary_kriging_wide_systems_fp64_bit_exact(geo):
             assert np.array_equal(out.data, od)
         else:
             assert rel_err(out.data[om == 1], od[om == 1], 1.0) <= TOL
+
+
+@pytest.mark.parametrize("case_idx", [0, 1, 2])
+def test_cell_binned_ties_and_partial_overflow(geo, case_idx):
+    """Lattice data with heavy covariance ties and a few overfull bins, so a
+    job mixes both search paths: every neighbour list (order of tied
+    candidates included) and every mask equal the oracle's, whose tie order
+    is pinned by tests/test_oracle_cluster_ties.py."""
+    import test_oracle_cluster_ties as T
+
+    kind, shape, spacing, extra, cov, radii, mx = T.CASES[case_idx]
+    g = Grid(*shape)
+    data, mask = T.lattice_case(shape, spacing, extra, 7)
+    pidx, pcnt = _dump_product(g.size, mx)
+    if kind == "sk":
+        out = geo.simple_kriging((data, mask), g, radii, mx, cov, mean=0.0)
+    else:
+        out = geo.ordinary_kriging((data, mask), g, radii, mx, cov)
+    st = geo.last_stats()
+    d = T._run(kind, g, data, mask, cov, radii, mx)
+    assert st["nodes_cluster_path"] > 0 and st["nodes_template_path"] > 0
+    _same_lists(pidx, pcnt, d.idx, d.cnt)
+    if kind == "sk":
+        od, om = O.simple_kriging((data, mask), g, radii, mx, cov, mean=0.0)
+    else:
+        od, om = O.ordinary_kriging((data, mask), g, radii, mx, cov)
+    assert np.array_equal(out.mask, om)
+    m = om == 1
+    assert rel_err(out.data[m], od[m], 1.0) <= TOL
