@@ -88,6 +88,8 @@ struct WaveLds {
 };
 
 constexpr int kGatherUnroll = 4;  // 64-candidate groups whose loads are in flight together
+// Targets gathered together when consecutive list entries share their bins.
+constexpr int kTargetsPerRun = 4;
 
 struct Keyed {
     int lb;
@@ -436,187 +438,280 @@ __device__ void finish_(const E *e, int n, int K, int *out, const S &sh, const i
     }
 }
 
+constexpr int kRankPad = kClusterRankPad;
+
+// G consecutive rank-table entries from entry idx (idx >= 0 counts from the
+// start of the padded allocation), one vector load: G = 2 and 4 need only
+// dword alignment for global loads.
+template <int G>
+struct RankVec {
+    uint32_t v[G];
+};
+typedef uint32_t u32x2_a4 __attribute__((ext_vector_type(2), aligned(4)));
+typedef uint32_t u32x4_a4 __attribute__((ext_vector_type(4), aligned(4)));
+template <int G>
+__device__ inline RankVec<G> load_ranks(const uint32_t *base, uint32_t idx) {
+    RankVec<G> r;
+    const char *p = reinterpret_cast<const char *>(base) + (idx << 2);
+    if constexpr (G == 1) {
+        r.v[0] = *reinterpret_cast<const uint32_t *>(p);
+    } else if constexpr (G == 2) {
+        const u32x2_a4 w = *reinterpret_cast<const u32x2_a4 *>(p);
+        r.v[0] = w.x;
+        r.v[1] = w.y;
+    } else {
+        static_assert(G == 4, "groups of 1, 2 or 4 targets");
+        const u32x4_a4 w = *reinterpret_cast<const u32x4_a4 *>(p);
+        r.v[0] = w.x;
+        r.v[1] = w.y;
+        r.v[2] = w.z;
+        r.v[3] = w.w;
+    }
+    return r;
+}
+
+// Gather of one run of nt <= G targets that share their 27 bins: the same
+// y, z and x bin, at x = x0 .. x0 + nt - 1.  Their candidates are the same
+// cells, and a candidate's rank-table offsets for the nt targets are nt
+// consecutive entries (the x offset drops by one per target), so one
+// coordinate load and one G-entry rank read per lane serve all of them.
+// Target g keeps the candidates with rank > 0 in raw order at e0 + g cap
+// (LDS up to cap, then the spill scratch ge, if any); returns the list
+// lengths in pos[0 .. nt).
+template <bool PACKED, int G>
+__device__ __forceinline__ void gather_run_(const ClusterArgs &a, WaveLds &sh, int x, int y, int z, int nt,
+                                            uint32_t *e0, int cap, uint32_t *ge, int *pos) {
+    const int lane = threadIdx.x & 63;
+    const int X = a.X, Y = a.Y;
+    const BinArgs &b = a.bins;
+    const int lb = a.local_bits;
+    const int bx = x / b.r[0], by = y / b.r[1], bz = z / b.r[2];
+    const int s0 = a.span[0], s1 = a.span[1], s2 = a.span[2];
+    const int rd0 = a.rd[0], rd1 = a.rd[1], rd2 = a.rd[2];
+    // lane g < kSegs: segment g's first position in the bin-sorted arrays
+    // and its length (bins outside the bin box hold no cells; bx + 1 <=
+    // m0 - 1 always holds, bx - 1 may be -1)
+    int start = 0, cnt = 0;
+    if (lane < kSegs) {
+        const int plane = lane < 9 ? 0 : lane < 12 ? 1 : 2;
+        const int q = plane == 0 ? lane : plane == 1 ? lane - 9 : lane - 12;
+        const int kk = bz - 1 + plane;
+        const int jj = by - 1 + (plane == 1 ? q : q / 3);
+        const int i0 = plane == 1 ? (bx > 0 ? bx - 1 : 0) : bx - 1 + q % 3;
+        const int i1 = plane == 1 ? bx + 1 : i0;
+        if (jj >= 0 && kk >= 0 && i0 >= 0 && jj < b.m[1] && kk < b.m[2]) {
+            const int row = (kk * b.m[1] + jj) * b.m[0];
+            start = (int)a.bin_start[row + i0];
+            int end = (int)(a.bin_start[row + i1] + a.bin_count[row + i1]);
+            const int rz = b.r[2];
+            const uint32_t *zo = a.zoff + (size_t)(row + i0) * (rz + 1);
+            if (plane == 0) {
+                const int d = z - s2 - kk * rz;  // layers of the bin below z - span
+                if (d > 0) start += (int)zo[min(d, rz)];
+            } else if (plane == 2) {
+                const int d = z + s2 + 1 - kk * rz;  // layers of the bin up to z + span
+                if (d < rz) end = start + (int)zo[max(d, 0)];
+            }
+            cnt = max(end - start, 0);
+        }
+    }
+    // raw indices: segments padded to multiples of 64, so that every group
+    // of 64 lies in one segment
+    const int padded = (cnt + 63) & ~63;
+    int incl = padded;
+#pragma unroll
+    for (int d = 1; d < kMaxSegs; d <<= 1) {
+        const int v = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += v;
+    }
+    const int raw0 = incl - padded;
+    const int raw_total = __builtin_amdgcn_readlane(incl, kSegs - 1);
+    // segment ends for the cursor; lane kSegs is a sentinel that no group
+    // passes (its cells end at raw_total)
+    const int seg_end = lane < kSegs ? incl : 0x7fffffff;
+    if (lane < kMaxSegs) {
+        sh.seg_raw[lane] = lane < kSegs ? raw0 : 0x7fffffff;
+        sh.seg_off[lane] = start - raw0;
+    }
+    const int seg_lim = raw0 + cnt;  // lane g: end of segment g's cells
+    // gather: each pass issues the loads of kGatherUnroll groups of 64
+    // before compacting; a scalar cursor follows the segment of each group
+    int seg = -1, g_end = 0, g_off = 0, g_lim = 0;
+    // offsets from target 0 as rank-table indices: o = w - (t0 - span)
+    const int xs = x - s0, ys = y - s1, zs = z - s2;
+#pragma unroll
+    for (int g = 0; g < G; ++g) pos[g] = 0;
+    for (int base = 0; base < raw_total; base += 64 * kGatherUnroll) {
+        int gi[kGatherUnroll];
+        bool valid[kGatherUnroll];
+#pragma unroll
+        for (int u = 0; u < kGatherUnroll; ++u) {
+            const int gb = base + u * 64;
+            while (gb >= g_end) {
+                ++seg;
+                g_end = __builtin_amdgcn_readlane(seg_end, seg);
+                g_off = __builtin_amdgcn_readlane(start - raw0, seg);
+                g_lim = __builtin_amdgcn_readlane(seg_lim, seg);
+            }
+            const int p = gb + lane;
+            valid[u] = p < g_lim;
+            gi[u] = valid[u] ? g_off + p : 0;
+        }
+        int tx[kGatherUnroll], ty[kGatherUnroll], tz[kGatherUnroll];
+#pragma unroll
+        for (int u = 0; u < kGatherUnroll; ++u) {
+            if (PACKED) {
+                const uint32_t w = load_u32(a.bin_xyz, (uint32_t)gi[u]);
+                tx[u] = (int)(w & 2047u) - xs;
+                ty[u] = (int)((w >> 11) & 2047u) - ys;
+                tz[u] = (int)(w >> 22) - zs;
+            } else {
+                const int c = a.bin_cells[gi[u]];
+                tx[u] = c % X - xs;
+                ty[u] = c % (X * Y) / X - ys;
+                tz[u] = c / (X * Y) - zs;
+            }
+        }
+        uint32_t rk[kGatherUnroll][G];
+#pragma unroll
+        for (int u = 0; u < kGatherUnroll; ++u) {
+            // offsets within [-span, span] per axis index the rank table
+            // ((2 span[1] + 1)(2 span[2] + 1) < 2^24, checked on the host);
+            // target g sees the x offset tx - g
+            const int ox = tx[u];
+            const unsigned oy = (unsigned)ty[u], oz = (unsigned)tz[u];
+            const bool yz = valid[u] && oy < (unsigned)rd1 && oz < (unsigned)rd2;
+            const bool any = yz && ox >= 0 && ox - (nt - 1) < rd0;
+            const unsigned full = __umul24(__umul24(oz, (unsigned)rd1) + oy, (unsigned)rd0) + (unsigned)ox;
+            const RankVec<G> r = load_ranks<G>(a.rank, any ? full + kRankPad - (G - 1) : 0u);
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const bool in = yz && g < nt && (unsigned)(ox - g) < (unsigned)rd0;
+                rk[u][g] = in ? r.v[G - 1 - g] : 0u;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kGatherUnroll; ++u) {
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                if (g >= nt) continue;
+                const uint64_t mk = ballot(rk[u][g] != 0);
+                const int pp = pos[g] + bits_below(mk);
+                if (rk[u][g] != 0) {
+                    const uint32_t ev = (rk[u][g] << lb) | (uint32_t)(base + u * 64 + lane);
+                    if (pp < cap)
+                        e0[g * cap + pp] = ev;
+                    else if (ge && pp < a.scratch_per_wave)
+                        ge[pp] = ev;
+                }
+                pos[g] += __popcll(mk);
+            }
+        }
+    }
+}
+
 // One wavefront per workgroup; the wavefronts loop over the target list.
 // PACKED: candidate coordinates come from bin_xyz (x | y << 11 | z << 22),
 // otherwise they are decoded from the cell index.
 //
 // SPILLED = false: the grid holds about n_list / per_wave wavefronts, far
 // more than are resident, so the hardware balances the uneven per-target
-// work; wavefronts stride over the list, and a list longer than a.cap is not
-// sorted but its target appended to spill_list.  SPILLED = true: the
-// wavefronts stride over the spill_count targets of spill_list, sorting
-// all their lists in their global scratch.
+// work; wavefronts stride over the list in chunks of G consecutive
+// entries, gather each run of targets that share their 27 bins (same y, z
+// and x bin, consecutive x) together, then sort the runs' lists one by one;
+// a list longer than a.cap is not sorted but its target appended to
+// spill_list.  SPILLED = true (G = 1): the wavefronts stride over the
+// spill_count targets of spill_list, sorting all their lists in their
+// global scratch.
 //
 // WAVES8: the register budget of eight wavefronts per SIMD (a few SGPRs
 // move to VGPR lanes), used when the LDS capacity allows 32 wavefronts per
 // CU; otherwise the budget of seven, which LDS already limits to.
 //
-// W wavefronts per workgroup take consecutive list entries (neighbouring
-// targets along x): their rank-table reads, offsets shifted by one, share
-// the CU's L1 lines.
-template <bool PACKED, bool SPILLED, bool WAVES8, int W>
+// W wavefronts per workgroup take consecutive chunks (neighbouring targets
+// along x): their rank-table reads, offsets shifted by G, share the CU's L1
+// lines.
+template <bool PACKED, bool SPILLED, bool WAVES8, int W, int G>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WAVES8 ? 8 : 1, 8))) void k_cluster(
     ClusterArgs a) {
+    static_assert(!SPILLED || G == 1, "the spill pass sorts one target per wavefront");
     __shared__ WaveLds sh_all[W];
     extern __shared__ uint32_t lds_all[];
     // the spill pass gathers straight into its global scratch
     const int cap = SPILLED ? 0 : a.cap;
     const int w = W == 1 ? 0 : (int)(threadIdx.x >> 6);
     WaveLds &sh = sh_all[w];
-    uint32_t *lds_e = lds_all + (size_t)w * cap;
+    uint32_t *const e0 = lds_all + (size_t)w * G * cap;
     const int wave_id = blockIdx.x * W + w;
     const int lane = threadIdx.x & 63;
     const int X = a.X, Y = a.Y;
-    const BinArgs &b = a.bins;
     const Keyed key{a.local_bits};
     // global scratch of this wave: the elements of lists longer than a.cap
     uint32_t *ge = SPILLED ? a.scratch + (size_t)wave_id * cluster_scratch_words(a.scratch_per_wave) : nullptr;
     unsigned long long cand_acc = 0, spill_acc = 0;
     const int li_end = SPILLED ? *a.spill_count : a.n_list;
-    for (int li = wave_id; li < li_end; li += gridDim.x * W) {
-        const int t = SPILLED ? a.spill_list[li] : a.list ? a.list[li] : li;
-        const int node = a.targets[t];
-        const int x = node % X, y = node % (X * Y) / X, z = node / (X * Y);
-        const int bx = x / b.r[0], by = y / b.r[1], bz = z / b.r[2];
-        const int s0 = a.span[0], s1 = a.span[1], s2 = a.span[2];
-        const int rd0 = a.rd[0], rd1 = a.rd[1], rd2 = a.rd[2];
-        // lane g < kSegs: segment g's first position in the bin-sorted
-        // arrays and its length (bins outside the bin box hold no cells;
-        // bx + 1 <= m0 - 1 always holds, bx - 1 may be -1)
-        int start = 0, cnt = 0;
-        if (lane < kSegs) {
-            const int plane = lane < 9 ? 0 : lane < 12 ? 1 : 2;
-            const int q = plane == 0 ? lane : plane == 1 ? lane - 9 : lane - 12;
-            const int kk = bz - 1 + plane;
-            const int jj = by - 1 + (plane == 1 ? q : q / 3);
-            const int i0 = plane == 1 ? (bx > 0 ? bx - 1 : 0) : bx - 1 + q % 3;
-            const int i1 = plane == 1 ? bx + 1 : i0;
-            if (jj >= 0 && kk >= 0 && i0 >= 0 && jj < b.m[1] && kk < b.m[2]) {
-                const int row = (kk * b.m[1] + jj) * b.m[0];
-                start = (int)a.bin_start[row + i0];
-                int end = (int)(a.bin_start[row + i1] + a.bin_count[row + i1]);
-                const int rz = b.r[2];
-                const uint32_t *zo = a.zoff + (size_t)(row + i0) * (rz + 1);
-                if (plane == 0) {
-                    const int d = z - s2 - kk * rz;  // layers of the bin below z - span
-                    if (d > 0) start += (int)zo[min(d, rz)];
-                } else if (plane == 2) {
-                    const int d = z + s2 + 1 - kk * rz;  // layers of the bin up to z + span
-                    if (d < rz) end = start + (int)zo[max(d, 0)];
-                }
-                cnt = max(end - start, 0);
+    const int n_chunks = (li_end + G - 1) / G;
+    for (int ch = wave_id; ch < n_chunks; ch += gridDim.x * W) {
+        const int ent0 = ch * G;
+        const int ne = min(G, li_end - ent0);
+        // lane g < ne: entry ent0 + g's target and cell
+        int my_t = 0, my_node = 0;
+        if (lane < ne) {
+            const int li = ent0 + lane;
+            my_t = SPILLED ? a.spill_list[li] : a.list ? a.list[li] : li;
+            my_node = a.targets[my_t];
+        }
+        for (int r0 = 0; r0 < ne;) {
+            const int node0 = __builtin_amdgcn_readlane(my_node, r0);
+            const int x = node0 % X, y = node0 % (X * Y) / X, z = node0 / (X * Y);
+            const int bx = x / a.bins.r[0];
+            int nt = 1;
+            while (r0 + nt < ne) {
+                const int xn = x + nt;
+                if (__builtin_amdgcn_readlane(my_node, r0 + nt) != node0 + nt || xn >= X || xn / a.bins.r[0] != bx)
+                    break;
+                ++nt;
             }
-        }
-        // raw indices: segments padded to multiples of 64, so that every
-        // group of 64 lies in one segment
-        const int padded = (cnt + 63) & ~63;
-        int incl = padded;
+            int pos[G];
+            if (G > 1 && nt > 1)
+                gather_run_<PACKED, G>(a, sh, x, y, z, nt, e0, cap, ge, pos);
+            else
+                gather_run_<PACKED, 1>(a, sh, x, y, z, 1, e0, cap, ge, pos);
+            wave_sync();
+            for (int g = 0; g < nt; ++g) {
+                const int t = __builtin_amdgcn_readlane(my_t, r0 + g);
+                int total = pos[0];
 #pragma unroll
-        for (int d = 1; d < kMaxSegs; d <<= 1) {
-            const int v = __shfl_up(incl, d, 64);
-            if (lane >= d) incl += v;
-        }
-        const int raw0 = incl - padded;
-        const int raw_total = __builtin_amdgcn_readlane(incl, kSegs - 1);
-        // segment ends for the cursor; lane kSegs is a sentinel that no group
-        // passes (its cells end at raw_total)
-        const int seg_end = lane < kSegs ? incl : 0x7fffffff;
-        if (lane < kMaxSegs) {
-            sh.seg_raw[lane] = lane < kSegs ? raw0 : 0x7fffffff;
-            sh.seg_off[lane] = start - raw0;
-        }
-        const int seg_lim = raw0 + cnt;  // lane g: end of segment g's cells
-        // gather: keep candidates with rank > 0, in raw order.  Each pass
-        // issues the loads of kGatherUnroll groups of 64 before compacting;
-        // a scalar cursor follows the segment of each group.
-        int seg = -1, g_end = 0, g_off = 0, g_lim = 0;
-        // offsets from the target as rank-table indices: o = w - (t - span)
-        const int xs = x - s0, ys = y - s1, zs = z - s2;
-        int pos = 0;
-        for (int base = 0; base < raw_total; base += 64 * kGatherUnroll) {
-            int gi[kGatherUnroll];
-            bool valid[kGatherUnroll];
-#pragma unroll
-            for (int u = 0; u < kGatherUnroll; ++u) {
-                const int gb = base + u * 64;
-                while (gb >= g_end) {
-                    ++seg;
-                    g_end = __builtin_amdgcn_readlane(seg_end, seg);
-                    g_off = __builtin_amdgcn_readlane(start - raw0, seg);
-                    g_lim = __builtin_amdgcn_readlane(seg_lim, seg);
-                }
-                const int p = gb + lane;
-                valid[u] = p < g_lim;
-                gi[u] = valid[u] ? g_off + p : 0;
-            }
-            // coordinates as rank-table offsets, then rank-table reads, each
-            // batch in flight together
-            int tx[kGatherUnroll], ty[kGatherUnroll], tz[kGatherUnroll];
-#pragma unroll
-            for (int u = 0; u < kGatherUnroll; ++u) {
-                if (PACKED) {
-                    const uint32_t w = load_u32(a.bin_xyz, (uint32_t)gi[u]);
-                    tx[u] = (int)(w & 2047u) - xs;
-                    ty[u] = (int)((w >> 11) & 2047u) - ys;
-                    tz[u] = (int)(w >> 22) - zs;
+                for (int k = 1; k < G; ++k)
+                    if (k == g) total = pos[k];
+                uint32_t *const eg = e0 + g * cap;
+                int *out = a.nbr + (size_t)t * a.max_n;
+                bool sane = true;
+                bool spilled = false;
+                if (total <= cap) {
+                    sane = topk_sort_(eg, sh, total, a.max_n, key);
+                    if (sane) finish_(eg, total, a.max_n, out, sh, a.bin_cells, key);
+                } else if (!SPILLED && a.spill_list) {
+                    if (lane == 0) a.spill_list[atomicAdd(a.spill_count, 1)] = t;
+                    spilled = true;  // counted by the spill pass
+                } else if (SPILLED && total <= a.scratch_per_wave) {
+                    spill_acc += 1;
+                    wave_sync();
+                    sane = topk_sort_(ge, sh, total, a.max_n, key);
+                    if (sane) finish_(ge, total, a.max_n, out, sh, a.bin_cells, key);
                 } else {
-                    const int c = a.bin_cells[gi[u]];
-                    tx[u] = c % X - xs;
-                    ty[u] = c % (X * Y) / X - ys;
-                    tz[u] = c / (X * Y) - zs;
+                    if (lane == 0) atomicOr(a.overflow, 1u);
+                    sane = false;
                 }
-            }
-            uint32_t rk[kGatherUnroll];
-#pragma unroll
-            for (int u = 0; u < kGatherUnroll; ++u) {
-                // offsets within [-span, span] per axis index the rank table
-                // ((2 span[1] + 1)(2 span[2] + 1) < 2^24, checked on the host)
-                const unsigned ox = (unsigned)tx[u], oy = (unsigned)ty[u], oz = (unsigned)tz[u];
-                const bool in = valid[u] && ox < (unsigned)rd0 && oy < (unsigned)rd1 && oz < (unsigned)rd2;
-                const unsigned full = __umul24(__umul24(oz, (unsigned)rd1) + oy, (unsigned)rd0) + ox;
-                const uint32_t r = load_u32(a.rank, in ? full : 0u);
-                rk[u] = in ? r : 0u;
-            }
-#pragma unroll
-            for (int u = 0; u < kGatherUnroll; ++u) {
-                const uint64_t mk = ballot(rk[u] != 0);
-                const int pp = pos + bits_below(mk);
-                if (rk[u] != 0) {
-                    const uint32_t ev = (rk[u] << key.lb) | (uint32_t)(base + u * 64 + lane);
-                    if (pp < cap)
-                        lds_e[pp] = ev;
-                    else if (ge && pp < a.scratch_per_wave)
-                        ge[pp] = ev;
+                if (lane == 0 && !spilled) {
+                    cand_acc += (unsigned long long)total;
+                    a.cnt[t] = sane ? min(total, a.max_n) : 0;
+                    if (!sane) atomicOr(a.overflow, 2u);
                 }
-                pos += __popcll(mk);
+                wave_sync();
             }
+            r0 += nt;
         }
-        const int total = pos;
-        wave_sync();
-        int *out = a.nbr + (size_t)t * a.max_n;
-        bool sane = true;
-        if (total <= cap) {
-            sane = topk_sort_(lds_e, sh, total, a.max_n, key);
-            if (sane) finish_(lds_e, total, a.max_n, out, sh, a.bin_cells, key);
-        } else if (!SPILLED && a.spill_list) {
-            if (lane == 0) a.spill_list[atomicAdd(a.spill_count, 1)] = t;
-            wave_sync();
-            continue;  // counted by the spill pass
-        } else if (SPILLED && total <= a.scratch_per_wave) {
-            spill_acc += 1;
-            wave_sync();
-            sane = topk_sort_(ge, sh, total, a.max_n, key);
-            if (sane) finish_(ge, total, a.max_n, out, sh, a.bin_cells, key);
-        } else {
-            if (lane == 0) atomicOr(a.overflow, 1u);
-            sane = false;
-        }
-        if (lane == 0) {
-            cand_acc += (unsigned long long)total;
-            a.cnt[t] = sane ? min(total, a.max_n) : 0;
-            if (!sane) atomicOr(a.overflow, 2u);
-        }
-        wave_sync();
     }
     if (lane == 0 && a.cand_total && cand_acc != 0) atomicAdd(counter_line(a.cand_total), cand_acc);
     if (lane == 0 && a.cand_total && spill_acc != 0) atomicAdd(counter_line(a.cand_total) + 1, spill_acc);
@@ -659,17 +754,32 @@ int cluster_capacity(long long expected_max) {
     return (int)std::min<long long>(kMaxCap, std::max<long long>(kMinCap, c));
 }
 
+// Lists of G targets fit in LDS with 32 wavefronts per CU.
+static bool fits_waves8(int G, int cap) {
+    return 32 * ((size_t)G * cap * sizeof(uint32_t) + sizeof(WaveLds)) <= kLdsPerCu;
+}
+
+// Main pass: runs of up to G = 4 (or 2) targets when their lists still fit
+// the LDS of 32 wavefronts per CU, single targets otherwise.  Occupancy
+// matters more than the shared gathers: on the 256x256x128 SK + IK
+// workload, four-target runs for every call (LDS for 14 wavefronts per CU)
+// took 66 ms per call against 39 ms for single targets, while two-target
+// runs where 32 wavefronts still fit took 21.5 ms against 23.3 ms.
 template <bool PACKED>
 static void launch_cap(const ClusterArgs &a, hipStream_t s) {
     constexpr int W = kWavesPerGroup;
     const int grid = (int)(((long long)a.n_list + (long long)a.per_wave * W - 1) / ((long long)a.per_wave * W));
-    const size_t lds = (size_t)a.cap * sizeof(uint32_t);
-    if (32 * (lds + sizeof(WaveLds)) <= kLdsPerCu)
-        hipLaunchKernelGGL((k_cluster<PACKED, false, true, W>), dim3(grid), dim3(64 * W), W * lds, s, a);
+    const size_t lds1 = (size_t)a.cap * sizeof(uint32_t);
+    if (kTargetsPerRun >= 4 && fits_waves8(4, a.cap))
+        hipLaunchKernelGGL((k_cluster<PACKED, false, true, W, 4>), dim3(grid), dim3(64 * W), W * 4 * lds1, s, a);
+    else if (kTargetsPerRun >= 2 && fits_waves8(2, a.cap))
+        hipLaunchKernelGGL((k_cluster<PACKED, false, true, W, 2>), dim3(grid), dim3(64 * W), W * 2 * lds1, s, a);
+    else if (fits_waves8(1, a.cap))
+        hipLaunchKernelGGL((k_cluster<PACKED, false, true, W, 1>), dim3(grid), dim3(64 * W), W * lds1, s, a);
     else
-        hipLaunchKernelGGL((k_cluster<PACKED, false, false, W>), dim3(grid), dim3(64 * W), W * lds, s, a);
+        hipLaunchKernelGGL((k_cluster<PACKED, false, false, W, 1>), dim3(grid), dim3(64 * W), W * lds1, s, a);
     if (a.spill_list)
-        hipLaunchKernelGGL((k_cluster<PACKED, true, false, 1>), dim3(a.n_spill_waves), dim3(64), 0, s, a);
+        hipLaunchKernelGGL((k_cluster<PACKED, true, false, 1, 1>), dim3(a.n_spill_waves), dim3(64), 0, s, a);
 }
 
 void launch_cluster_search(const ClusterArgs &a, hipStream_t s) {

@@ -130,7 +130,7 @@ struct ClusterArgs {
     const int *bin_cells;     // informed cells sorted by (bin, cell)
     const uint32_t *bin_xyz;  // their coordinates packed x | y << 11 | z << 22 (nullptr: decode the cell)
     const uint32_t *zoff;     // per bin, cells below each z layer (launch_bin_zoff)
-    const uint32_t *rank;     // rank table over [-span, span]^3
+    const uint32_t *rank;     // rank table over [-span, span]^3, after kClusterRankPad zero entries
     int span[3];
     int rd[3];
     int max_n;
@@ -155,6 +155,10 @@ struct ClusterArgs {
     unsigned long long *cand_total;  // counter lines: word 0 candidates gathered, word 1 lists sorted in scratch
 };
 void launch_cluster_search(const ClusterArgs &a, hipStream_t s);
+// Zero entries before and after the rank table in its device allocation: a
+// search that reads the entries of several neighbouring targets with one
+// vector load may start up to three entries before the first one.
+constexpr int kClusterRankPad = 4;
 // zoff[b (r[2] + 1) + d]: cells of bin b whose z lies below the bin's first
 // layer + d, for d = 0 .. r[2].
 void launch_bin_zoff(const uint32_t *bin_start, const uint32_t *bin_count, const int *bin_cells, const BinArgs &b,
