@@ -88,8 +88,10 @@ struct WaveLds {
 };
 
 constexpr int kGatherUnroll = 4;  // 64-candidate groups whose loads are in flight together
-// Targets gathered together when consecutive list entries share their bins.
+// Targets gathered together when consecutive list entries share their bins,
+// and the fewest wavefronts per CU whose LDS lists that may cost.
 constexpr int kTargetsPerRun = 4;
+constexpr int kMinWavesForRuns = 32;
 
 struct Keyed {
     int lb;
@@ -754,30 +756,42 @@ int cluster_capacity(long long expected_max) {
     return (int)std::min<long long>(kMaxCap, std::max<long long>(kMinCap, c));
 }
 
-// Lists of G targets fit in LDS with 32 wavefronts per CU.
-static bool fits_waves8(int G, int cap) {
-    return 32 * ((size_t)G * cap * sizeof(uint32_t) + sizeof(WaveLds)) <= kLdsPerCu;
+// Wavefronts per CU that the LDS holds with lists of G targets of cap
+// elements each.
+static int waves_per_cu(int G, int cap) {
+    return (int)(kLdsPerCu / ((size_t)G * cap * sizeof(uint32_t) + sizeof(WaveLds)));
 }
 
-// Main pass: runs of up to G = 4 (or 2) targets when their lists still fit
-// the LDS of 32 wavefronts per CU, single targets otherwise.  Occupancy
-// matters more than the shared gathers: on the 256x256x128 SK + IK
-// workload, four-target runs for every call (LDS for 14 wavefronts per CU)
-// took 66 ms per call against 39 ms for single targets, while two-target
-// runs where 32 wavefronts still fit took 21.5 ms against 23.3 ms.
+// Main pass: runs of up to G = 4 (or 2) targets when their lists leave
+// room in LDS for at least kMinWavesForRuns wavefronts per CU, single
+// targets otherwise; the eight-wavefront register budget when 32
+// wavefronts fit.
 template <bool PACKED>
 static void launch_cap(const ClusterArgs &a, hipStream_t s) {
     constexpr int W = kWavesPerGroup;
     const int grid = (int)(((long long)a.n_list + (long long)a.per_wave * W - 1) / ((long long)a.per_wave * W));
     const size_t lds1 = (size_t)a.cap * sizeof(uint32_t);
-    if (kTargetsPerRun >= 4 && fits_waves8(4, a.cap))
-        hipLaunchKernelGGL((k_cluster<PACKED, false, true, W, 4>), dim3(grid), dim3(64 * W), W * 4 * lds1, s, a);
-    else if (kTargetsPerRun >= 2 && fits_waves8(2, a.cap))
-        hipLaunchKernelGGL((k_cluster<PACKED, false, true, W, 2>), dim3(grid), dim3(64 * W), W * 2 * lds1, s, a);
-    else if (fits_waves8(1, a.cap))
-        hipLaunchKernelGGL((k_cluster<PACKED, false, true, W, 1>), dim3(grid), dim3(64 * W), W * lds1, s, a);
-    else
-        hipLaunchKernelGGL((k_cluster<PACKED, false, false, W, 1>), dim3(grid), dim3(64 * W), W * lds1, s, a);
+    const int G = (kTargetsPerRun >= 4 && waves_per_cu(4, a.cap) >= kMinWavesForRuns)   ? 4
+                  : (kTargetsPerRun >= 2 && waves_per_cu(2, a.cap) >= kMinWavesForRuns) ? 2
+                                                                                         : 1;
+    const bool w8 = waves_per_cu(G, a.cap) >= 32;
+    const size_t lds = W * G * lds1;
+    if (G == 4) {
+        if (w8)
+            hipLaunchKernelGGL((k_cluster<PACKED, false, true, W, 4>), dim3(grid), dim3(64 * W), lds, s, a);
+        else
+            hipLaunchKernelGGL((k_cluster<PACKED, false, false, W, 4>), dim3(grid), dim3(64 * W), lds, s, a);
+    } else if (G == 2) {
+        if (w8)
+            hipLaunchKernelGGL((k_cluster<PACKED, false, true, W, 2>), dim3(grid), dim3(64 * W), lds, s, a);
+        else
+            hipLaunchKernelGGL((k_cluster<PACKED, false, false, W, 2>), dim3(grid), dim3(64 * W), lds, s, a);
+    } else {
+        if (w8)
+            hipLaunchKernelGGL((k_cluster<PACKED, false, true, W, 1>), dim3(grid), dim3(64 * W), lds, s, a);
+        else
+            hipLaunchKernelGGL((k_cluster<PACKED, false, false, W, 1>), dim3(grid), dim3(64 * W), lds, s, a);
+    }
     if (a.spill_list)
         hipLaunchKernelGGL((k_cluster<PACKED, true, false, 1, 1>), dim3(a.n_spill_waves), dim3(64), 0, s, a);
 }

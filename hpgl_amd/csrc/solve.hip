@@ -18,6 +18,7 @@
 // summed in float in neighbour order by the group's first lane.
 #include "counters.h"
 #include "devcov.h"
+#include "divrn.h"
 #include "kernels.h"
 
 #include <hip/hip_runtime.h>
@@ -269,14 +270,16 @@ __global__ __launch_bounds__(64) void k_solve_fp64(SolveArgs a) {
 
     const bool two = a.ok_system != 0;
     // dpotrs: dtrsm('L','U','T','N') (forward with L) then
-    // dtrsm('L','U','N','N') (backward with L^T), both right-hand sides
+    // dtrsm('L','U','N','N') (backward with L^T), both right-hand sides;
+    // the divisions by L(i, i) go through its reciprocal (div_rn, exact)
+    const double dinv = 1.0 / diag;
     double r1 = bi, r2 = two ? ((i < n) ? 1.0 : 0.0) : 0.0;
 #pragma unroll
     for (int k = 0; k < G; ++k) {
         if (k < nmax) {
             if (i == k) {
-                r1 = r1 / u[k];
-                r2 = r2 / u[k];
+                r1 = div_rn(r1, diag, dinv);
+                if (two) r2 = div_rn(r2, diag, dinv);
             }
             const double y1 = group_bcast<G>(r1, k, g);
             const double y2 = group_bcast<G>(r2, k, g);
@@ -290,8 +293,8 @@ __global__ __launch_bounds__(64) void k_solve_fp64(SolveArgs a) {
     for (int k = G - 1; k >= 0; --k) {
         if (k < nmax) {
             if (i == k) {
-                if (r1 != 0.0) r1 = r1 / diag;
-                if (r2 != 0.0) r2 = r2 / diag;
+                if (r1 != 0.0) r1 = div_rn(r1, diag, dinv);
+                if (r2 != 0.0) r2 = div_rn(r2, diag, dinv);
             }
             const double y1 = group_bcast<G>(r1, k, g);
             const double y2 = group_bcast<G>(r2, k, g);
@@ -426,12 +429,14 @@ __global__ __launch_bounds__(64) void k_solve_fp64_lds(SolveArgs a) {
     }
 
     const bool two = a.ok_system != 0;
-    // dpotrs: forward with L, backward with L^T, both right-hand sides
+    // dpotrs: forward with L, backward with L^T, both right-hand sides; the
+    // divisions by L(i, i) go through its reciprocal (div_rn, exact)
+    const double dinv = 1.0 / diag;
     double r1 = bi, r2 = two ? ((i < n) ? 1.0 : 0.0) : 0.0;
     for (int k = 0; k < n; ++k) {
         if (i == k) {
-            r1 = r1 / diag;
-            r2 = r2 / diag;
+            r1 = div_rn(r1, diag, dinv);
+            if (two) r2 = div_rn(r2, diag, dinv);
         }
         const double y1 = group_bcast<G>(r1, k, 0);
         const double y2 = group_bcast<G>(r2, k, 0);
@@ -443,8 +448,8 @@ __global__ __launch_bounds__(64) void k_solve_fp64_lds(SolveArgs a) {
     }
     for (int k = n - 1; k >= 0; --k) {
         if (i == k) {
-            if (r1 != 0.0) r1 = r1 / diag;
-            if (r2 != 0.0) r2 = r2 / diag;
+            if (r1 != 0.0) r1 = div_rn(r1, diag, dinv);
+            if (r2 != 0.0) r2 = div_rn(r2, diag, dinv);
         }
         const double y1 = group_bcast<G>(r1, k, 0);
         const double y2 = group_bcast<G>(r2, k, 0);

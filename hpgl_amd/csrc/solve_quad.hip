@@ -22,6 +22,7 @@
 //   compiled with contraction off), so with the same A and b the weights
 //   and the success/failure decision are the reference's bit for bit.
 #include "counters.h"
+#include "divrn.h"
 #include "kernels.h"
 
 #include <hip/hip_runtime.h>
@@ -264,14 +265,17 @@ __device__ inline void q_epilogue(const SolveArgs &a, QShared<N> *shp, const QCt
 // ---------------------------------------------------------------------------
 // Mixed precision.
 // ---------------------------------------------------------------------------
+// Dinv[m]: reciprocal of the diagonal entry of my row 4m+q (the FP32 factor
+// only seeds the FP64 refinement, so products by reciprocals replace the
+// divisions).
 template <int N>
-__device__ inline void q_solve32(const float *L, int q, float *v) {
+__device__ inline void q_solve32(const float *L, const float *Dinv, int q, float *v) {
     constexpr int M = N / 4;
     // forward: L y = v (column sweep, y_k broadcast from its row owner)
 #pragma unroll
     for (int k = 0; k < N; ++k) {
         const int mk = k >> 2, ko = k & 3;
-        if (q == ko) v[mk] = v[mk] / L[off(mk) + k];
+        if (q == ko) v[mk] = v[mk] * Dinv[mk];
         const float yk = qb(v[mk], ko);
 #pragma unroll
         for (int m = 0; m < M; ++m)
@@ -291,7 +295,7 @@ __device__ inline void q_solve32(const float *L, int q, float *v) {
             }
         p += qswap1(p);
         p += qswap2(p);
-        if (q == io) v[mi] = (v[mi] - p) / L[off(mi) + i];
+        if (q == io) v[mi] = (v[mi] - p) * Dinv[mi];
     }
 }
 
@@ -322,15 +326,18 @@ __global__ __launch_bounds__(64) void k_solve_quad_mixed(SolveArgs a) {
     }
     maxdiag = qmax(maxdiag);
 
-    // right-looking Cholesky in place
+    // right-looking Cholesky in place (hardware square root and reciprocal,
+    // within an ulp: the refinement works from this factor in FP64)
     float minpiv = 3.0e38f;
+    float Dinv[M];
 #pragma unroll
     for (int k = 0; k < N; ++k) {
         const int mk = k >> 2, ko = k & 3;
         const float d = qb(L[off(mk) + k], ko);
         if (k < c.n) minpiv = fminf(minpiv, d);
-        const float s = sqrtf(fmaxf(d, 1.0e-30f));
-        const float inv = 1.0f / s;
+        const float s = __builtin_amdgcn_sqrtf(fmaxf(d, 1.0e-30f));
+        const float inv = __builtin_amdgcn_rcpf(s);
+        if (q == ko) Dinv[mk] = inv;
 #pragma unroll
         for (int m = 0; m < M; ++m)
             if (4 * m + 3 >= k) {
@@ -357,8 +364,8 @@ __global__ __launch_bounds__(64) void k_solve_quad_mixed(SolveArgs a) {
         v1[m] = (float)b[m];
         v2[m] = (two && i < c.n) ? 1.0f : 0.0f;
     }
-    q_solve32<N>(L, q, v1);
-    if (two) q_solve32<N>(L, q, v2);
+    q_solve32<N>(L, Dinv, q, v1);
+    if (two) q_solve32<N>(L, Dinv, q, v2);
 #pragma unroll
     for (int m = 0; m < M; ++m) {
         x1[m] = v1[m];
@@ -428,8 +435,8 @@ __global__ __launch_bounds__(64) void k_solve_quad_mixed(SolveArgs a) {
             v1[m] = (float)r1[m];
             v2[m] = (float)r2[m];
         }
-        q_solve32<N>(L, q, v1);
-        if (two) q_solve32<N>(L, q, v2);
+        q_solve32<N>(L, Dinv, q, v1);
+        if (two) q_solve32<N>(L, Dinv, q, v2);
         double dmax = 0.0, xmax = 0.0;
 #pragma unroll
         for (int m = 0; m < M; ++m) {
@@ -518,6 +525,7 @@ __global__ __launch_bounds__(64) void k_solve_quad_fp64(SolveArgs a) {
     // broadcast, the pivot is A_jj - sum_k L_jk^2 (sequential), and every
     // row below gets (A_ij + (-1) * sum_k L_ik L_jk) * (1 / L_jj).
     int info = 0;
+    double dinv[M];  // 1 / L(i, i) of my rows, for the triangular solves (div_rn)
 #pragma unroll
     for (int j = 0; j < N; ++j) {
         const int mj = j >> 2, jo = j & 3;
@@ -540,6 +548,7 @@ __global__ __launch_bounds__(64) void k_solve_quad_fp64(SolveArgs a) {
         if (!(ajj > 0.0)) ajj = 1.0;  // failed system or padding: keep the arithmetic finite
         ajj = sqrt(ajj);
         const double rcp = 1.0 / ajj;
+        if (q == jo) dinv[mj] = rcp;
 #pragma unroll
         for (int m = 0; m < M; ++m)
             if (4 * m + 3 >= j) {
@@ -562,8 +571,8 @@ __global__ __launch_bounds__(64) void k_solve_quad_fp64(SolveArgs a) {
     for (int k = 0; k < N; ++k) {
         const int mk = k >> 2, ko = k & 3;
         if (q == ko) {
-            y1[mk] = y1[mk] / L[off(mk) + k];
-            y2[mk] = y2[mk] / L[off(mk) + k];
+            y1[mk] = div_rn(y1[mk], L[off(mk) + k], dinv[mk]);
+            if (two) y2[mk] = div_rn(y2[mk], L[off(mk) + k], dinv[mk]);
         }
         const double a1 = qb(y1[mk], ko), a2 = qb(y2[mk], ko);
 #pragma unroll
@@ -582,8 +591,8 @@ __global__ __launch_bounds__(64) void k_solve_quad_fp64(SolveArgs a) {
         if (q == ko) {
             nz1 = y1[mk] != 0.0;
             nz2 = y2[mk] != 0.0;
-            if (nz1) y1[mk] = y1[mk] / L[off(mk) + k];
-            if (nz2) y2[mk] = y2[mk] / L[off(mk) + k];
+            if (nz1) y1[mk] = div_rn(y1[mk], L[off(mk) + k], dinv[mk]);
+            if (nz2) y2[mk] = div_rn(y2[mk], L[off(mk) + k], dinv[mk]);
         }
         nz1 = qb(nz1, ko);
         nz2 = qb(nz2, ko);
