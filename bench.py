@@ -32,6 +32,8 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0
+CU_COUNT = 256
+CLOCK_HZ = 2.4e9  # peak engine clock
 FP32_PEAK_TFLOPS = 157.3
 FP64_PEAK_TFLOPS = 78.6
 
@@ -1008,8 +1010,27 @@ def main():
         for line in open(tpath):
             if line.startswith("bench-traffic %s bytes_per_launch=" % dom_name):
                 traffic = int(line.split("=", 1)[1])
+    # the cell-binned search is limited by the vector L1 / address unit, not
+    # by HBM: line accesses per target and the address unit's busy fraction
+    # come from the committed counter summary, the launch time from this run
+    roofline_l1 = None
+    lpath = os.path.join(ROOT, "profiles", "l1_sk_ik_256x256x128.md")
+    if dom_name == "k_cluster" and os.path.exists(lpath):
+        for line in open(lpath):
+            if line.startswith("bench-l1 k_cluster "):
+                kv = dict(f.split("=", 1) for f in line.split()[2:])
+                lines_per_target = float(kv["line_accesses_per_target"])
+                per_launch = lines_per_target * dom_nodes / launches
+                achieved_l1 = per_launch / (avg_ms * 1e-3) / (CU_COUNT * CLOCK_HZ) if avg_ms > 0 else 0.0
+                roofline_l1 = {"bound": "vector L1 / address unit (TA)", "unit": "cache-line accesses per CU-cycle",
+                               "line_accesses_per_target": lines_per_target,
+                               "achieved": round(achieved_l1, 4),
+                               "ta_busy_frac_measured": float(kv["ta_busy_frac"]),
+                               "source": "profiles/l1_sk_ik_256x256x128.md (counters), launch time from this run"}
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic, "kernel": dom_name,
+                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
+                "traffic_source": "profiles/traffic_sk_ik_256x256x128.md (FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+                "kernel": dom_name,
                 "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": int(alg_bytes),
                 "bytes_per_node": round(bytes_per_node, 2)}
     n = n_avg
@@ -1041,6 +1062,7 @@ def main():
                    "precision": args.precision},
         "roofline": roofline,
         "roofline_solve": roofline_solve,
+        "roofline_l1": roofline_l1,
         "kernel_ms_per_step": {k: round(acc.get(k, 0.0) / args.steps, 3) for k in kernel_keys},
         "solves": {"fp32_accepted": int(acc.get("solves_fp32", 0)), "fp64": int(acc.get("solves_fp64", 0)),
                    "singular": int(acc.get("nodes_singular", 0)),
