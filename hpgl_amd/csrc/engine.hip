@@ -336,12 +336,16 @@ __global__ void k_iota(int *p, int n) {
     if (i < n) p[i] = i;
 }
 
-// Adds the number of non-zero bytes of m[0..n) to *out (one atomic per
-// wavefront).
+// Adds the number of non-zero bytes of m[0..n) to *out: a grid-stride count
+// per lane, summed over the wavefront, one atomic per wavefront (a few
+// thousand in all, so the shared counter does not serialise the launch).
 __global__ void k_count_nonzero(const uint8_t *m, long long n, unsigned long long *out) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const unsigned long long b = __ballot(i < n && m[i] != 0);
-    if ((threadIdx.x & 63) == 0 && b) atomicAdd(out, (unsigned long long)__popcll(b));
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    unsigned int c = 0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) c += m[i] != 0;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) c += __shfl_xor(c, o, 64);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(out, (unsigned long long)c);
 }
 
 // Sort key of path position i (target cell targets[i]) for the template
@@ -1282,7 +1286,8 @@ void sim_batch(long long N, const SimPaths &p, int r0, int rc, const uint8_t *ha
         // hard-data cells (a simulation mask leaves other cells uninformed)
         DBuf<unsigned long long> d_hard(1);
         hip_check(hipMemsetAsync(d_hard.p, 0, sizeof(unsigned long long), s), "memset");
-        hipLaunchKernelGGL(k_count_nonzero, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, hard, N, d_hard.p);
+        hipLaunchKernelGGL(k_count_nonzero, dim3((unsigned)std::min<long long>((N + 255) / 256, 1024)), dim3(256), 0,
+                           s, hard, N, d_hard.p);
         hip_check(hipGetLastError(), "count hard data");
         unsigned long long h_hard = 0;
         hip_check(hipMemcpyAsync(&h_hard, d_hard.p, sizeof(h_hard), hipMemcpyDeviceToHost, s), "copy count");
