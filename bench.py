@@ -193,6 +193,18 @@ def _worker_headline(O, spec, arrays):
     return {"stride": stride, "seconds": med, "runs_s": all_t, "kriged_nodes": n_unk / stride * 6}
 
 
+def _worker_cfg1(O, spec, arrays):
+    g, data, mask, cov, radii, max_n = make_cfg1()
+    res = {}
+
+    def run():
+        res["ok"] = O.ordinary_kriging((data, mask), g, radii, max_n, cov)
+
+    med, all_t, _ = median_time(run, spec["runs"])
+    arrays.update(d=res["ok"][0].ravel(order="F"), m=res["ok"][1].ravel(order="F"))
+    return {"seconds": med, "runs_s": all_t}
+
+
 def _worker_cfg3(O, spec, arrays):
     g, data, mask, twins, cov, radii, max_n = make_cfg3()
     res = {}
@@ -265,7 +277,7 @@ def cpu_worker(spec):
 
     O.set_threads(spec["threads"])
     arrays = {}
-    fn = {"headline": _worker_headline, "cfg3": _worker_cfg3, "sgs": _worker_sgs,
+    fn = {"headline": _worker_headline, "cfg1": _worker_cfg1, "cfg3": _worker_cfg3, "sgs": _worker_sgs,
           "serving": _worker_serving}[spec["leg"]]
     out = fn(O, spec, arrays)
     out["omp_max_threads"] = O.lib.oracle_get_threads()
@@ -388,6 +400,78 @@ def cfg3_bench(torch, lib, abi, hpgl_wrap, steps=1, cpu=True, budget_s=5.0, runs
             both = sel & (om == 1)
             par[prec + "_vs_oracle"] = err_stats(d[both], od[both], 1e-3)
             par[prec + "_bit_exact_cells"] = float(np.mean(d[both] == od[both])) if both.any() else 1.0
+        res["parity"] = par
+    return res
+
+
+def make_cfg1(seed=1001):
+    """Ordinary kriging on 100x100x50 (500,000 cells), 2% of cells informed
+    uniformly, values lognormal(0, 1); spherical covariance, ranges
+    20/15/5, sill 1, nugget 0.1; search radii 20/20/10; 12 neighbours."""
+    from cases import Cov, Grid, scattered
+
+    g = Grid(100, 100, 50)
+    data, mask = scattered(g.shape, 0.02, seed)
+    return g, data, mask, Cov(0, (20, 15, 5), sill=1.0, nugget=0.1), (20, 20, 10), 12
+
+
+def cfg1_bench(torch, lib, abi, hpgl_wrap, steps=5, cpu=True, runs=3):
+    """The CPU-sized ordinary-kriging workload, inputs resident in HBM:
+    kriged nodes/s in mixed and FP64 mode.  The CPU leg runs the oracle on
+    the whole grid (median of `runs`) and compares both GPU results with it
+    cell for cell."""
+    g, data, mask, cov, radii, max_n = make_cfg1()
+    p = abi._HPGL_OK_PARAMS(radiuses=(C.c_int * 3)(*radii), max_neighbours=max_n, covariance_type=cov.type,
+                            ranges=(C.c_double * 3)(*cov.ranges), angles=(C.c_double * 3)(*cov.angles),
+                            sill=cov.sill, nugget=cov.nugget)
+    dev = torch.device("cuda:0")
+    d_in = torch.from_numpy(data.ravel(order="F").copy()).to(dev)
+    d_m = torch.from_numpy(mask.ravel(order="F").copy()).to(dev)
+    d_out = torch.empty_like(d_in)
+    d_om = torch.empty_like(d_m)
+    shape = abi._HPGL_SHAPE(data=(C.c_int * 3)(*g.shape), strides=(C.c_int * 3)(1, g.x, g.x * g.y))
+    sp = C.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    n_unk = int(np.sum(mask == 0))
+
+    def run():
+        d_out.copy_(d_in)
+        d_om.copy_(d_m)
+        lib.hpgl_amd_ordinary_kriging_dev(d_in.data_ptr(), d_m.data_ptr(), C.byref(shape), C.byref(p),
+                                          d_out.data_ptr(), d_om.data_ptr(), sp)
+        hpgl_wrap.check_last_error()
+
+    res = {"workload": "OK, 100x100x50 grid, 2% informed (lognormal), spherical 20/15/5, nugget 0.1, "
+                       "radii 20/20/10, max_neighbours 12", "kriged_nodes_per_job": n_unk}
+    outs = {}
+    for prec in ("mixed", "fp64"):
+        hpgl_wrap.set_precision(prec)
+        run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            run()
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / steps
+        st = hpgl_wrap.last_stats()
+        res[prec] = {"value": n_unk / wall, "unit": "kriged nodes/s", "ms_per_job": wall * 1e3,
+                     "solves": {"fp32_accepted": int(st["solves_fp32"]), "fp64": int(st["solves_fp64"]),
+                                "singular": int(st["nodes_singular"])}}
+        outs[prec] = (d_out.cpu().numpy(), d_om.cpu().numpy())
+    hpgl_wrap.set_precision("mixed")
+    if cpu:
+        info, arr = cpu_leg({"leg": "cfg1", "runs": runs})
+        od, om = arr["d"], arr["m"]
+        res["cpu_baseline"] = {"value": n_unk / info["seconds"], "unit": "kriged nodes/s", "cores": CPU["threads"],
+                               "kind": "port", "sample": "the whole grid, median of %d runs %s s"
+                                                         % (runs, info["runs_s"])}
+        unk = mask.ravel(order="F") == 0
+        par = {}
+        for prec in ("mixed", "fp64"):
+            d, m = outs[prec]
+            both = unk & (om == 1)
+            par[prec + "_masks_equal"] = bool(np.array_equal(m, om))
+            par[prec + "_vs_oracle"] = err_stats(d[both], od[both], 1e-3)
+            par[prec + "_bit_exact_cells"] = float(np.mean(d[both] == od[both]))
         res["parity"] = par
     return res
 
@@ -854,6 +938,7 @@ def main():
     ap.add_argument("--no-cokriging", action="store_true", help="skip the cokriging measurement")
     ap.add_argument("--cpu-worker", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--no-host-path", action="store_true", help="skip the host-pointer geo_bsd timing")
+    ap.add_argument("--no-cfg1", action="store_true", help="skip the 100x100x50 ordinary-kriging measurement")
     ap.add_argument("--no-cfg3", action="store_true", help="skip the precision-stress (uneven OK) measurement")
     ap.add_argument("--cfg3-only", action="store_true", help="run only the precision-stress OK measurement")
     ap.add_argument("--no-variogram", action="store_true", help="skip the experimental variogram measurement")
@@ -1112,6 +1197,9 @@ def main():
                                     "note": "geo_bsd.simple_kriging + indicator_kriging on numpy arrays: the same "
                                             "workload with host-device copies and the input clone"}
 
+    if not args.no_cfg1:
+        progress("ordinary kriging (100x100x50)")
+        out["cfg1"] = cfg1_bench(torch, lib, abi, hpgl_wrap, cpu=not args.no_cpu_baseline)
     if not args.no_cfg3:
         progress("precision-stress OK (300x300x100)")
         out["cfg3"] = cfg3_bench(torch, lib, abi, hpgl_wrap, cpu=not args.no_cpu_baseline)
