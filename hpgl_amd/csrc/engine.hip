@@ -792,12 +792,19 @@ void launch_fp64_subset(const SolveArgs &b, int order, hipStream_t s, Stats &st)
     st.n_k_solve_fp64 += 1;
 }
 
+// Orders below this go to the FP64 quad kernel in mixed mode too: there it
+// is both exact and faster than the FP32 factor with FP64 refinement
+// (490,000 systems of order 4 / 8 / 12 on MI355X: 0.047 / 0.112 / 0.219 ms
+// in FP64 against 0.063 / 0.157 / 0.290 ms mixed; from order 16 on the
+// mixed kernels are faster, 0.57 against 0.69 ms at 16).
+constexpr int kMixedMinOrder = 16;
+
 // Solve all targets, bucketed by neighbour count.  Mixed mode: the quad
-// kernels (orders <= 32 with a covariance table) in FP32 + FP64 refinement,
-// then FP64 for the systems they hand over; wider systems and the
-// analytic-covariance fallback go to the FP64 kernels directly.  FP64 mode:
-// every system through the FP64 kernels.  Correlogram systems always take
-// the FP64 kernels.
+// kernels (orders 16..32 with a covariance table) in FP32 + FP64
+// refinement, then FP64 for the systems they hand over; smaller and wider
+// systems and the analytic-covariance fallback go to the FP64 kernels
+// directly.  FP64 mode: every system through the FP64 kernels.
+// Correlogram systems always take the FP64 kernels.
 void solve(SolveArgs sa, int epi, int precision, DBuf<uint8_t> &status, hipStream_t s, Stats &st) {
     sa.status = status.p;
     sa.subset = nullptr;
@@ -811,7 +818,7 @@ void solve(SolveArgs sa, int epi, int precision, DBuf<uint8_t> &status, hipStrea
     const int kmax = maxn > kQuadMaxMixed ? 8 : std::max(0, (maxn + 3) / 4 - 1);
     for (int k = 0; k <= kmax; ++k) {
         const int order = k == 8 ? 64 : 4 * (k + 1);
-        const bool mixed = precision == 0 && k < 8 && sa.cov.table && !sa.correlogram;
+        const bool mixed = precision == 0 && order >= kMixedMinOrder && k < 8 && sa.cov.table && !sa.correlogram;
         if (!mixed) {
             solve_fp64_bucket(sa, keys, (uint8_t)k, false, status, s, st);
             continue;
