@@ -939,6 +939,7 @@ def main():
     ap.add_argument("--cpu-worker", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--no-host-path", action="store_true", help="skip the host-pointer geo_bsd timing")
     ap.add_argument("--no-cfg1", action="store_true", help="skip the 100x100x50 ordinary-kriging measurement")
+    ap.add_argument("--cfg1-only", action="store_true", help="run only the 100x100x50 ordinary-kriging measurement")
     ap.add_argument("--no-cfg3", action="store_true", help="skip the precision-stress (uneven OK) measurement")
     ap.add_argument("--cfg3-only", action="store_true", help="run only the precision-stress OK measurement")
     ap.add_argument("--no-variogram", action="store_true", help="skip the experimental variogram measurement")
@@ -990,6 +991,10 @@ def main():
 
     if args.cfg3_only:
         print(json.dumps({"cfg3": cfg3_bench(torch, lib, abi, hpgl_wrap, steps=max(1, args.steps),
+                                             cpu=not args.no_cpu_baseline)}), flush=True)
+        return
+    if args.cfg1_only:
+        print(json.dumps({"cfg1": cfg1_bench(torch, lib, abi, hpgl_wrap, steps=max(1, args.steps),
                                              cpu=not args.no_cpu_baseline)}), flush=True)
         return
 

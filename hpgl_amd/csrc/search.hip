@@ -17,6 +17,12 @@ namespace hpgl_amd {
 
 namespace {
 
+// Template offsets a wavefront probes per round: their loads are issued
+// together, then the results are taken in template order, so a lane keeps
+// exactly the offsets the sequential scan keeps and ignores those past its
+// max_n-th find.
+constexpr int kScanUnroll = 8;
+
 template <int PRED>
 __global__ __launch_bounds__(256) void k_template_scan(ScanArgs a) {
     const int li = blockIdx.x * blockDim.x + threadIdx.x;
@@ -31,25 +37,38 @@ __global__ __launch_bounds__(256) void k_template_scan(ScanArgs a) {
     const int max_n = a.max_n;
     bool want = active && count < max_n;
     uint64_t sim = 0;
-    for (int k = 0; k < a.tsize; ++k) {
-        if ((k & 7) == 0 && !__any(want)) break;
-        const int4 o = a.tmpl[k];
-        if (want) {
-            const int px = x + o.x, py = y + o.y, pz = z + o.z;
-            if ((unsigned)px < (unsigned)X && (unsigned)py < (unsigned)Y && (unsigned)pz < (unsigned)Z) {
-                const int idx = (pz * Y + py) * X + px;
-                bool inf;
-                if (PRED == 0) {
-                    inf = a.mask[idx] != 0;
-                } else {
-                    const int32_t kc = a.key[idx];
-                    inf = kc < my_key;
-                    if (inf && kc >= 0) sim |= 1ull << count;
-                }
-                if (inf) {
-                    out[count++] = idx;
-                    want = count < max_n;
-                }
+    for (int k0 = 0; k0 < a.tsize; k0 += kScanUnroll) {
+        if (!__any(want)) break;
+        // the round's template entries (wave-uniform) and every lane's
+        // probe; lanes that do not probe read their own cell instead, so
+        // all loads are issued before the first is used
+        int4 o[kScanUnroll];
+#pragma unroll
+        for (int u = 0; u < kScanUnroll; ++u) o[u] = a.tmpl[__builtin_amdgcn_readfirstlane(min(k0 + u, a.tsize - 1))];
+        int idx[kScanUnroll];
+        bool probe[kScanUnroll];
+#pragma unroll
+        for (int u = 0; u < kScanUnroll; ++u) {
+            const int px = x + o[u].x, py = y + o[u].y, pz = z + o[u].z;
+            probe[u] = want & (k0 + u < a.tsize) & ((unsigned)px < (unsigned)X) & ((unsigned)py < (unsigned)Y) &
+                       ((unsigned)pz < (unsigned)Z);
+            idx[u] = probe[u] ? (pz * Y + py) * X + px : node;
+        }
+        int32_t v[kScanUnroll];
+#pragma unroll
+        for (int u = 0; u < kScanUnroll; ++u) v[u] = PRED == 0 ? (int32_t)a.mask[idx[u]] : a.key[idx[u]];
+#pragma unroll
+        for (int u = 0; u < kScanUnroll; ++u) {
+            bool inf;
+            if (PRED == 0) {
+                inf = v[u] != 0;
+            } else {
+                inf = v[u] < my_key;
+                if (want && probe[u] && inf && v[u] >= 0) sim |= 1ull << count;
+            }
+            if (want && probe[u] && inf) {
+                out[count++] = idx[u];
+                want = count < max_n;
             }
         }
     }
