@@ -346,6 +346,33 @@ def test_ordinary_kriging_wide_systems_fp64_bit_exact(geo):
             assert rel_err(out.data[om == 1], od[om == 1], 1.0) <= TOL
 
 
+@pytest.mark.parametrize("max_n", [24, 64])
+def test_ordinary_kriging_device_evaluated_covariance(geo, max_n):
+    """OK with radii 42: the pair table over the offsets neighbour pairs can
+    take would exceed 2^25 entries, so the FP64 kernels evaluate the model
+    on the device (cov_eval) while building each system, for 17..32 and
+    33..64 neighbours.  The neighbour lists come from the host-built
+    template and rank tables as always, so the masks equal the oracle's;
+    the device's exp and square root may differ from the host's in the last
+    bit, so values agree within 1e-5 (parity of the device evaluation is
+    within tolerance, not bit-exact)."""
+    shape = (40, 36, 12)
+    data, mask = scattered(shape, 0.05, 91)
+    g = Grid(*shape)
+    r = (42, 42, 42)
+    assert (2 * (4 * r[0] - 2) + 1) ** 3 > 1 << 25
+    od, om = O.ordinary_kriging((data, mask), g, r, max_n, Cov(1, (30, 25, 8), angles=(30.0, 0.0, 0.0), nugget=0.1))
+    for precision in ("fp64", "mixed"):
+        geo.set_precision(precision)
+        out = geo.ordinary_kriging((data, mask), g, r, max_n, Cov(1, (30, 25, 8), angles=(30.0, 0.0, 0.0),
+                                                                   nugget=0.1))
+        st = geo.last_stats()
+        assert st["solves_fp64"] > 0 and st["solves_fp32"] == 0
+        assert np.array_equal(out.mask, om)
+        assert rel_err(out.data[om == 1], od[om == 1], 1.0) <= TOL
+    geo.set_precision("mixed")
+
+
 @pytest.mark.parametrize("case_idx", [0, 1, 2])
 def test_cell_binned_ties_and_partial_overflow(geo, case_idx):
     """Lattice data with heavy covariance ties and a few overfull bins, so a
