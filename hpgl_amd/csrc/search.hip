@@ -2,13 +2,18 @@
 //
 // Template scan (sugarbox_neighbour_lookup.h:36-61): one lane per target
 // cell; the lanes of a wavefront hold neighbouring cells and walk the same
-// template offset in lockstep, so the template entry is a scalar load and
-// the 64 mask probes of one step hit one or two cache lines.  A lane stops
-// when it has max_n neighbours; the wave stops when all lanes have.
+// template offsets in lockstep, so the 64 mask probes of one offset hit one
+// or two cache lines.  A lane stops when it has max_n neighbours.  Once at
+// most kTailLanes lanes of a wavefront are still searching, those targets
+// are handed, with their count and next offset, to a tail pass that gives
+// each one a whole wavefront (64 offsets per round), so no wavefront idles
+// behind one lane in a sparse neighbourhood.  Both passes take the finds in
+// template order, so the lists are those of the sequential scan.
 //
 // Cell bins (clusterizer.cpp:70-184): counts per bin of informed cells and,
 // per target, whether any of its 27 bins holds more than `limit` cells (the
 // reference then uses the template scan, sugarbox_indexed_neighbour_lookup.h:68).
+#include "engine.h"
 #include "kernels.h"
 
 #include <hip/hip_runtime.h>
@@ -23,8 +28,17 @@ namespace {
 // max_n-th find.
 constexpr int kScanUnroll = 8;
 
+// Hand-over to the tail pass: (list position, next template offset) per
+// target, and their count.
+struct ScanTail {
+    int2 *cont;
+    unsigned *n_cont;
+};
+// Lanes still searching at which a wavefront hands them to the tail pass.
+constexpr int kTailLanes = 8;
+
 template <int PRED>
-__global__ __launch_bounds__(256) void k_template_scan(ScanArgs a) {
+__global__ __launch_bounds__(256) void k_template_scan(ScanArgs a, ScanTail tl) {
     const int li = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = li < a.n_list;
     const int t = active ? (a.list ? a.list[li] : li) : 0;
@@ -37,8 +51,18 @@ __global__ __launch_bounds__(256) void k_template_scan(ScanArgs a) {
     const int max_n = a.max_n;
     bool want = active && count < max_n;
     uint64_t sim = 0;
+    const int lane = threadIdx.x & 63;
     for (int k0 = 0; k0 < a.tsize; k0 += kScanUnroll) {
-        if (!__any(want)) break;
+        const uint64_t wm = __ballot(want);
+        if (wm == 0) break;
+        if (__popcll(wm) <= kTailLanes) {
+            // the lanes still searching continue in the tail pass from k0
+            unsigned base = 0;
+            if (lane == 0) base = atomicAdd(tl.n_cont, (unsigned)__popcll(wm));
+            base = __shfl(base, 0);
+            if (want) tl.cont[base + __builtin_amdgcn_mbcnt_hi((uint32_t)(wm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)wm, 0u))] = make_int2(li, k0);
+            break;
+        }
         // the round's template entries (wave-uniform) and every lane's
         // probe; lanes that do not probe read their own cell instead, so
         // all loads are issued before the first is used
@@ -74,6 +98,57 @@ __global__ __launch_bounds__(256) void k_template_scan(ScanArgs a) {
     }
     if (active) a.cnt[t] = count;
     if (PRED == 1 && active && a.simbits) a.simbits[t] = sim;
+}
+
+// Tail pass: one wavefront per handed-over target, 64 consecutive template
+// offsets per round.  The round's finds are taken in offset order (lane
+// order) up to max_n, so the list continues exactly as the sequential scan
+// would; the count and, for the key predicate, the simulated-neighbour
+// bits continue from what the first pass stored.
+template <int PRED>
+__global__ __launch_bounds__(256) void k_template_scan_tail(ScanArgs a, ScanTail tl) {
+    const int lane = threadIdx.x & 63;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int n_waves = (gridDim.x * blockDim.x) >> 6;
+    const int n_cont = (int)*tl.n_cont;
+    const int X = a.X, Y = a.Y, Z = a.Z;
+    const int max_n = a.max_n;
+    for (int e = wave; e < n_cont; e += n_waves) {
+        const int2 c = tl.cont[e];
+        const int t = a.list ? a.list[c.x] : c.x;
+        const int node = a.targets[t];
+        const int x = node % X, y = node % (X * Y) / X, z = node / (X * Y);
+        const int32_t my_key = PRED == 1 ? a.key[node] : 0;
+        int count = a.cnt[t];
+        uint64_t sim = (PRED == 1 && a.simbits) ? a.simbits[t] : 0ull;
+        int *out = a.nbr + (size_t)t * max_n;
+        for (int k0 = c.y; k0 < a.tsize && count < max_n; k0 += 64) {
+            const int k = k0 + lane;
+            const int4 o = a.tmpl[k < a.tsize ? k : 0];
+            const int px = x + o.x, py = y + o.y, pz = z + o.z;
+            const bool probe = (k < a.tsize) & ((unsigned)px < (unsigned)X) & ((unsigned)py < (unsigned)Y) &
+                               ((unsigned)pz < (unsigned)Z);
+            const int idx = probe ? (pz * Y + py) * X + px : node;
+            const int32_t v = PRED == 0 ? (int32_t)a.mask[idx] : a.key[idx];
+            const bool inf = probe && (PRED == 0 ? v != 0 : v < my_key);
+            const uint64_t m = __ballot(inf);
+            const int pos = count + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            const bool take = inf && pos < max_n;
+            if (take) out[pos] = idx;
+            if (PRED == 1) {
+                uint64_t bit = (take && v >= 0) ? (1ull << pos) : 0ull;
+#pragma unroll
+                for (int d = 1; d < 64; d <<= 1) bit |= __shfl_xor(bit, d, 64);
+                sim |= bit;
+            }
+            count = min(max_n, count + (int)__popcll(m));
+        }
+        if (lane == 0) {
+            a.cnt[t] = count;
+            if (PRED == 1 && a.simbits) a.simbits[t] = sim;
+        }
+    }
 }
 
 __global__ void k_bin_ids(const int *cells, int n, BinArgs b, int *bin_of, uint32_t *bin_count) {
@@ -151,10 +226,22 @@ inline unsigned blocks(long long n, int bs) { return (unsigned)((n + bs - 1) / b
 
 void launch_template_scan(const ScanArgs &a, int pred_kind, hipStream_t s) {
     if (a.n_list <= 0) return;
-    if (pred_kind == 0)
-        hipLaunchKernelGGL(k_template_scan<0>, dim3(blocks(a.n_list, 256)), dim3(256), 0, s, a);
-    else
-        hipLaunchKernelGGL(k_template_scan<1>, dim3(blocks(a.n_list, 256)), dim3(256), 0, s, a);
+    // the hand-over list (at most one entry per target) and its count, in
+    // stream-ordered pool memory
+    ScanTail tl;
+    tl.cont = (int2 *)pool_alloc((size_t)a.n_list * sizeof(int2));
+    tl.n_cont = (unsigned *)pool_alloc(sizeof(unsigned));
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, (uint32_t *)tl.n_cont, 0u, 1ll);
+    const unsigned tail_blocks = std::min<unsigned>(blocks(a.n_list, 256), 512u);
+    if (pred_kind == 0) {
+        hipLaunchKernelGGL(k_template_scan<0>, dim3(blocks(a.n_list, 256)), dim3(256), 0, s, a, tl);
+        hipLaunchKernelGGL(k_template_scan_tail<0>, dim3(tail_blocks), dim3(256), 0, s, a, tl);
+    } else {
+        hipLaunchKernelGGL(k_template_scan<1>, dim3(blocks(a.n_list, 256)), dim3(256), 0, s, a, tl);
+        hipLaunchKernelGGL(k_template_scan_tail<1>, dim3(tail_blocks), dim3(256), 0, s, a, tl);
+    }
+    pool_free(tl.cont);
+    pool_free(tl.n_cont);
 }
 
 void launch_bin_ids(const int *cells, int n, const BinArgs &b, int *bin_of, uint32_t *bin_count, hipStream_t s) {
