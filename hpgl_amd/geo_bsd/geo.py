@@ -136,7 +136,12 @@ class IndProperty:
         self.data = numpy.require(data, "uint8", "F")
         self.mask = numpy.require(mask, "uint8", "F")
         self.indicator_count = indicator_count
-        if numpy.sum(numpy.bitwise_and(numpy.asarray(mask) > 0, numpy.asarray(data) >= indicator_count)) > 0:
+        # the reference's range check (geo.py:114); on uint8 inputs the
+        # converted Fortran-order arrays hold the same values, and checking
+        # them avoids strided passes over differently ordered arrays
+        d0, m0 = numpy.asarray(data), numpy.asarray(mask)
+        d, m = (self.data, self.mask) if (d0.dtype == numpy.uint8 and m0.dtype == numpy.uint8) else (d0, m0)
+        if numpy.any(numpy.logical_and(m > 0, d >= indicator_count)):
             raise RuntimeError("Property contains some indicators outside of [0..%s] range." % (indicator_count - 1))
         assert numpy.shape(data) == numpy.shape(mask)
 
@@ -218,7 +223,10 @@ def _clone_prop(prop):
     data2 = prop.data.copy("F")
     mask2 = prop.mask.copy("F")
     if isinstance(prop, IndProperty):
-        return IndProperty(data2, mask2, prop.indicator_count)
+        # a copy of a checked property: the range check need not run again
+        out = IndProperty.__new__(IndProperty)
+        out.data, out.mask, out.indicator_count = data2, mask2, prop.indicator_count
+        return out
     elif isinstance(prop, ContProperty):
         return ContProperty(data2, mask2)
     assert False, "Unknown prop type"

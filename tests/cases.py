@@ -50,7 +50,7 @@ def categories(data, mask, quantiles=(0.1, 0.3, 0.5, 0.7, 0.9), k=5):
     min(searchsorted(t, v), k - 1); marginals are the category frequencies."""
     v = data[mask == 1]
     t = np.quantile(v, quantiles)
-    cat = np.zeros(data.shape, np.uint8)
+    cat = np.zeros(data.shape, np.uint8, order="F")  # the grids' Fortran order
     c = np.minimum(np.searchsorted(t, data[mask == 1]), k - 1).astype(np.uint8)
     cat[mask == 1] = c
     marg = [float(np.mean(c == i)) for i in range(k)]

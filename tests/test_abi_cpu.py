@@ -4,6 +4,8 @@ import ctypes as C
 import os
 import re
 
+import pytest
+
 import oracle_binding as O
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -139,3 +141,37 @@ def test_python_api_surface():
         assert False
     except Exception as e:
         assert "Nugget" in str(e)
+
+
+def test_indicator_property_range_check():
+    """IndProperty raises exactly when the reference's expression
+    sum((mask > 0) & (data >= indicator_count)) > 0 holds (geo.py:114),
+    whatever the inputs' dtype and memory order; the clone of a checked
+    property keeps its arrays and count."""
+    import numpy as np
+
+    from hpgl_amd.geo_bsd import geo
+
+    rng = np.random.default_rng(3)
+    shape = (7, 5, 4)
+    for order in ("C", "F"):
+        for ddt, mdt in (("uint8", "uint8"), ("int32", "uint8"), ("uint8", "float32"), ("int64", "int16")):
+            data = np.asarray(rng.integers(0, 4, shape), dtype=ddt, order=order)
+            mask = np.asarray(rng.integers(0, 2, shape), dtype=mdt, order=order)
+            p = geo.IndProperty(data, mask, 4)
+            assert p.data.flags["F_CONTIGUOUS"] and p.mask.flags["F_CONTIGUOUS"]
+            c = geo._clone_prop(p)
+            assert c.indicator_count == 4 and np.array_equal(c.data, p.data) and np.array_equal(c.mask, p.mask)
+            assert c.data is not p.data
+            bad = data.copy(order=order)
+            i = np.flatnonzero(np.asarray(mask).ravel() > 0)[0]
+            bad[np.unravel_index(i, shape)] = 9
+            want = np.sum(np.bitwise_and(np.asarray(mask) > 0, np.asarray(bad) >= 4)) > 0
+            assert want
+            with pytest.raises(RuntimeError):
+                geo.IndProperty(bad, mask, 4)
+            # an out-of-range value where the mask is 0 is accepted
+            ok = data.copy(order=order)
+            j = np.flatnonzero(np.asarray(mask).ravel() == 0)[0]
+            ok[np.unravel_index(j, shape)] = 9
+            geo.IndProperty(ok, mask, 4)
