@@ -165,9 +165,10 @@ __device__ inline double group_bcast(double v, int j, int g) {
 // the column back as broadcasts (rows r + 8a and c + 8b) and adds
 // L(i,k) L(j,k) into every trailing accumulator.  Each accumulator starts at
 // 0 and receives its products in k order, rounded separately, so element
-// (i, j) sees exactly dpotf2's sum; the pivot and the column scaling are the
-// same expressions as k_solve_fp64, and the triangular solves run lane per
-// row as there.  Accumulators of finished columns and of the upper half of
+// (i, j) sees exactly dpotf2's sum: AJJ = A(j,j) - sum, L(i,j) =
+// (1 / L(j,j)) (A(i,j) + (-1) sum).  The forward substitution is fused into
+// the steps and the backward substitution runs lane per row, both in
+// dtrsm's order.  Accumulators of finished columns and of the upper half of
 // the diagonal blocks take updates that are never read.
 __host__ __device__ constexpr int tri_col(int m, int j) { return m * j - j * (j - 1) / 2; }
 
