@@ -111,6 +111,25 @@ def test_sgs_simple_kriging_conditional(geo, precision):
     assert rel_err(out.data, od, 1.0) <= TOL
 
 
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_sgs_wide_neighbourhoods(geo, precision):
+    """SK SGS with up to 48 neighbours: systems of 33..48 unknowns take the
+    FP64 block-cyclic kernel's SGS epilogue (weights and variance for the
+    ordered sweep), and the simulated-neighbour bits reach past bit 31 in
+    both template-scan passes.  Lists bit-exact, values within 1e-5."""
+    geo.set_precision(precision)
+    shape = (30, 26, 10)
+    g = Grid(*shape)
+    data, mask = _normal_scores(shape, 0.02, 515)
+    cov = Cov(1, (10, 9, 5), sill=1.0, nugget=0.0)
+    out, (od, om), pd, od_ = _run_pair(geo, (data, mask), g, _cdf_of(data, mask), (10, 9, 5), 48, cov, 4711)
+    _same_lists(*pd, *od_)
+    assert int(np.max(pd[1])) > 32
+    assert np.array_equal(out.mask, om)
+    assert rel_err(out.data, od, 1.0) <= TOL
+    geo.set_precision("mixed")
+
+
 def test_sgs_ordinary_kriging_without_transform(geo):
     """OK SGS (kriging kind 0), no cdf, with a nugget (normalised model)."""
     shape = (30, 30, 8)
