@@ -794,13 +794,14 @@ void launch_fp64_subset(const SolveArgs &b, int order, hipStream_t s, Stats &st)
 
 // Orders below this go to the FP64 quad kernel in mixed mode too: there it
 // is both exact and faster than the FP32 factor with FP64 refinement
-// (490,000 systems of order 4 / 8 / 12 on MI355X: 0.047 / 0.112 / 0.219 ms
-// in FP64 against 0.063 / 0.157 / 0.290 ms mixed; from order 16 on the
-// mixed kernels are faster, 0.57 against 0.69 ms at 16).
-constexpr int kMixedMinOrder = 16;
+// (490,000 ordinary-kriging systems of order 4 / 8 / 12 / 16 on MI355X:
+// 0.047 / 0.112 / 0.217 / 0.446 ms in FP64 against 0.063 / 0.157 / 0.290 /
+// 0.573 ms mixed); from order 20 on the mixed kernels are faster (0.82
+// against 3.1 ms at 20, 2.8 against 4.6 ms at 32).
+constexpr int kMixedMinOrder = 20;
 
 // Solve all targets, bucketed by neighbour count.  Mixed mode: the quad
-// kernels (orders 16..32 with a covariance table) in FP32 + FP64
+// kernels (orders 20..32 with a covariance table) in FP32 + FP64
 // refinement, then FP64 for the systems they hand over; smaller and wider
 // systems and the analytic-covariance fallback go to the FP64 kernels
 // directly.  FP64 mode: every system through the FP64 kernels.

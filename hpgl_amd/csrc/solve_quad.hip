@@ -299,8 +299,20 @@ __device__ inline void q_solve32(const float *L, const float *Dinv, int q, float
     }
 }
 
+// Wavefronts per SIMD that the register budget of k_solve_quad_mixed<N,
+// TWO> must allow.  Left alone, orders 28 and 32 and the two-right-hand-side
+// kernels from order 20 take more than 256 registers (one wavefront per
+// SIMD); capped at two wavefronts they spill a few hundred bytes per lane
+// and still run faster (order 32, one right-hand side: 20.9 -> 15.2 ms per
+// SGS realisation; two right-hand sides, 490,000 systems: order 20 1.31 ->
+// 0.82 ms, 24 1.82 -> 1.18 ms, 32 3.40 -> 2.79 ms).  The order-24 single
+// right-hand-side kernel stays at its natural two (three spilled and took
+// 123 against 89 ms per headline step).
 template <int N, bool TWO>
-__global__ __launch_bounds__(64) void k_solve_quad_mixed(SolveArgs a) {
+constexpr int kQuadMixedWaves = (N >= 28 || (N >= 20 && TWO)) ? 2 : 1;
+
+template <int N, bool TWO>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kQuadMixedWaves<N, TWO>, 8))) void k_solve_quad_mixed(SolveArgs a) {
     const int EPI = a.epi;
     constexpr int M = N / 4;
     __shared__ QShared<N> sh;
@@ -500,7 +512,7 @@ __global__ __launch_bounds__(64) void k_solve_quad_mixed(SolveArgs a) {
 // FP64 in the reference's operation order.
 // ---------------------------------------------------------------------------
 template <int N>
-__global__ __launch_bounds__(64) void k_solve_quad_fp64(SolveArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(N >= 16 ? 2 : 1, 8))) void k_solve_quad_fp64(SolveArgs a) {
     const int EPI = a.epi;
     constexpr int M = N / 4;
     __shared__ QShared<N> sh;
