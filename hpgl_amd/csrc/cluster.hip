@@ -440,11 +440,9 @@ __device__ void finish_(const E *e, int n, int K, int *out, const S &sh, const i
     }
 }
 
-constexpr int kRankPad = kClusterRankPad;
-
-// G consecutive rank-table entries from entry idx (idx >= 0 counts from the
-// start of the padded allocation), one vector load: G = 2 and 4 need only
-// dword alignment for global loads.
+// G consecutive rank-table entries from byte offset off, one vector load
+// (offsets are dword aligned; G = 2 and 4 need only dword alignment for
+// global loads).
 template <int G>
 struct RankVec {
     uint32_t v[G];
@@ -452,9 +450,9 @@ struct RankVec {
 typedef uint32_t u32x2_a4 __attribute__((ext_vector_type(2), aligned(4)));
 typedef uint32_t u32x4_a4 __attribute__((ext_vector_type(4), aligned(4)));
 template <int G>
-__device__ inline RankVec<G> load_ranks(const uint32_t *base, uint32_t idx) {
+__device__ inline RankVec<G> load_ranks(const uint32_t *base, uint32_t off) {
     RankVec<G> r;
-    const char *p = reinterpret_cast<const char *>(base) + (idx << 2);
+    const char *p = reinterpret_cast<const char *>(base) + off;
     if constexpr (G == 1) {
         r.v[0] = *reinterpret_cast<const uint32_t *>(p);
     } else if constexpr (G == 2) {
@@ -472,24 +470,35 @@ __device__ inline RankVec<G> load_ranks(const uint32_t *base, uint32_t idx) {
     return r;
 }
 
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
 // Gather of one run of nt <= G targets that share their 27 bins: the same
 // y, z and x bin, at x = x0 .. x0 + nt - 1.  Their candidates are the same
-// cells, and a candidate's rank-table offsets for the nt targets are nt
+// cells, and a candidate's rank-table entries for the nt targets are nt
 // consecutive entries (the x offset drops by one per target), so one
 // coordinate load and one G-entry rank read per lane serve all of them.
+//
+// The candidates are numbered densely in gather order (raw index j = the
+// position in the concatenation of the 21 segments), 64 to a group; a
+// group that straddles segment boundaries takes each lane's offset into the
+// bin-sorted arrays from the segment its j falls in.  A candidate's entry
+// is its packed table coordinate plus the target's base; candidates outside
+// the covariance box in x or y (the z trim keeps only the box's layers) are
+// dropped by one packed 16-bit test, and read the zero entries before the
+// table instead.  Run members whose own offset leaves the box read the
+// table's zero columns in x.
+//
 // Target g keeps the candidates with rank > 0 in raw order at e0 + g cap
 // (LDS up to cap, then the spill scratch ge, if any); returns the list
 // lengths in pos[0 .. nt).
-template <bool PACKED, int G>
+template <int G>
 __device__ __forceinline__ void gather_run_(const ClusterArgs &a, WaveLds &sh, int x, int y, int z, int nt,
                                             uint32_t *e0, int cap, uint32_t *ge, int *pos) {
     const int lane = threadIdx.x & 63;
-    const int X = a.X, Y = a.Y;
     const BinArgs &b = a.bins;
     const int lb = a.local_bits;
     const int bx = x / b.r[0], by = y / b.r[1], bz = z / b.r[2];
-    const int s0 = a.span[0], s1 = a.span[1], s2 = a.span[2];
-    const int rd0 = a.rd[0], rd1 = a.rd[1], rd2 = a.rd[2];
+    const int s2 = a.span[2];
     // lane g < kSegs: segment g's first position in the bin-sorted arrays
     // and its length (bins outside the bin box hold no cells; bx + 1 <=
     // m0 - 1 always holds, bx - 1 may be -1)
@@ -517,90 +526,96 @@ __device__ __forceinline__ void gather_run_(const ClusterArgs &a, WaveLds &sh, i
             cnt = max(end - start, 0);
         }
     }
-    // raw indices: segments padded to multiples of 64, so that every group
-    // of 64 lies in one segment
-    const int padded = (cnt + 63) & ~63;
-    int incl = padded;
+    // segment g holds raw indices [d0, d0 + cnt)
+    int incl = cnt;
 #pragma unroll
     for (int d = 1; d < kMaxSegs; d <<= 1) {
         const int v = __shfl_up(incl, d, 64);
         if (lane >= d) incl += v;
     }
-    const int raw0 = incl - padded;
-    const int raw_total = __builtin_amdgcn_readlane(incl, kSegs - 1);
-    // segment ends for the cursor; lane kSegs is a sentinel that no group
-    // passes (its cells end at raw_total)
-    const int seg_end = lane < kSegs ? incl : 0x7fffffff;
+    const int d0 = incl - cnt;
+    const int total = __builtin_amdgcn_readlane(incl, kSegs - 1);
+    // lanes from kSegs on are sentinels that no raw index reaches
+    const int seg_d = lane < kSegs ? d0 : 0x7fffffff;
+    const int seg_o = start - d0;  // raw index -> position in the bin-sorted arrays
     if (lane < kMaxSegs) {
-        sh.seg_raw[lane] = lane < kSegs ? raw0 : 0x7fffffff;
-        sh.seg_off[lane] = start - raw0;
+        sh.seg_raw[lane] = seg_d;
+        sh.seg_off[lane] = seg_o;
     }
-    const int seg_lim = raw0 + cnt;  // lane g: end of segment g's cells
-    // gather: each pass issues the loads of kGatherUnroll groups of 64
-    // before compacting; a scalar cursor follows the segment of each group
-    int seg = -1, g_end = 0, g_off = 0, g_lim = 0;
-    // offsets from target 0 as rank-table indices: o = w - (t0 - span)
-    const int xs = x - s0, ys = y - s1, zs = z - s2;
+    // byte offset of target 0's entries, less G - 1 entries: the load for
+    // the run starts at the entry of its last member (x offset smaller by
+    // G - 1); x + w0 y + w01 z modulo 2^32, as bin_lin
+    const uint32_t w0 = (uint32_t)a.w[0], w01 = (uint32_t)a.w[0] * (uint32_t)a.w[1];
+    const uint32_t tb = a.centre4 - 4u * ((uint32_t)x + w0 * (uint32_t)y + w01 * (uint32_t)z) - 4u * (G - 1);
+    // box test: (cx - (x - span0), cy - (y - span1)) modulo 2^16 must not
+    // exceed (2 span0 + nt - 1, 2 span1) (offsets of a run's union)
+    const u16x2 lo_xy = {(unsigned short)(x - a.span[0]), (unsigned short)(y - a.span[1])};
+    const u16x2 box_xy = {(unsigned short)(2 * a.span[0] + nt - 1), (unsigned short)(2 * a.span[1])};
+    const uint32_t lane8 = 8u * (uint32_t)lane;
 #pragma unroll
     for (int g = 0; g < G; ++g) pos[g] = 0;
-    for (int base = 0; base < raw_total; base += 64 * kGatherUnroll) {
-        int gi[kGatherUnroll];
-        bool valid[kGatherUnroll];
+    // the segment cursor: cur = offset of the segment holding the next
+    // group's first raw index; (dn, on) = start and offset of the next
+    // segment not yet entered
+    int nx = 1;
+    int cur = __builtin_amdgcn_readlane(seg_o, 0);
+    int dn = __builtin_amdgcn_readlane(seg_d, 1), on = __builtin_amdgcn_readlane(seg_o, 1);
+    for (int base = 0; base < total; base += 64 * kGatherUnroll) {
+        uint32_t boff[kGatherUnroll];
 #pragma unroll
         for (int u = 0; u < kGatherUnroll; ++u) {
             const int gb = base + u * 64;
-            while (gb >= g_end) {
-                ++seg;
-                g_end = __builtin_amdgcn_readlane(seg_end, seg);
-                g_off = __builtin_amdgcn_readlane(start - raw0, seg);
-                g_lim = __builtin_amdgcn_readlane(seg_lim, seg);
+            while (dn <= gb) {
+                cur = on;
+                ++nx;
+                dn = __builtin_amdgcn_readlane(seg_d, nx);
+                on = __builtin_amdgcn_readlane(seg_o, nx);
             }
-            const int p = gb + lane;
-            valid[u] = p < g_lim;
-            gi[u] = valid[u] ? g_off + p : 0;
+            int off = cur;
+            while (dn < gb + 64) {  // segments that start inside this group
+                off = gb + lane >= dn ? on : off;
+                cur = on;
+                ++nx;
+                dn = __builtin_amdgcn_readlane(seg_d, nx);
+                on = __builtin_amdgcn_readlane(seg_o, nx);
+            }
+            // position off + j of the bin-sorted arrays, in bytes; lanes past
+            // the last candidate read at most kClusterLinPad - 1 entries past
+            // the end
+            boff[u] = 8u * (uint32_t)(off + gb) + lane8;
         }
-        int tx[kGatherUnroll], ty[kGatherUnroll], tz[kGatherUnroll];
+        uint2 lin[kGatherUnroll];
+#pragma unroll
+        for (int u = 0; u < kGatherUnroll; ++u)
+            lin[u] = *reinterpret_cast<const uint2 *>(reinterpret_cast<const char *>(a.bin_lin) + boff[u]);
+        // both words in registers once all the groups' loads are issued
+        // (left alone, the compiler splits the load and defers the first
+        // word behind a branch on the box test)
+#pragma unroll
+        for (int u = 0; u < kGatherUnroll; ++u) asm volatile("" : "+v"(lin[u].x), "+v"(lin[u].y));
+        RankVec<G> rv[kGatherUnroll];
 #pragma unroll
         for (int u = 0; u < kGatherUnroll; ++u) {
-            if (PACKED) {
-                const uint32_t w = load_u32(a.bin_xyz, (uint32_t)gi[u]);
-                tx[u] = (int)(w & 2047u) - xs;
-                ty[u] = (int)((w >> 11) & 2047u) - ys;
-                tz[u] = (int)(w >> 22) - zs;
-            } else {
-                const int c = a.bin_cells[gi[u]];
-                tx[u] = c % X - xs;
-                ty[u] = c % (X * Y) / X - ys;
-                tz[u] = c / (X * Y) - zs;
-            }
-        }
-        uint32_t rk[kGatherUnroll][G];
-#pragma unroll
-        for (int u = 0; u < kGatherUnroll; ++u) {
-            // offsets within [-span, span] per axis index the rank table
-            // ((2 span[1] + 1)(2 span[2] + 1) < 2^24, checked on the host);
-            // target g sees the x offset tx - g
-            const int ox = tx[u];
-            const unsigned oy = (unsigned)ty[u], oz = (unsigned)tz[u];
-            const bool yz = valid[u] && oy < (unsigned)rd1 && oz < (unsigned)rd2;
-            const bool any = yz && ox >= 0 && ox - (nt - 1) < rd0;
-            const unsigned full = __umul24(__umul24(oz, (unsigned)rd1) + oy, (unsigned)rd0) + (unsigned)ox;
-            const RankVec<G> r = load_ranks<G>(a.rank, any ? full + kRankPad - (G - 1) : 0u);
-#pragma unroll
-            for (int g = 0; g < G; ++g) {
-                const bool in = yz && g < nt && (unsigned)(ox - g) < (unsigned)rd0;
-                rk[u][g] = in ? r.v[G - 1 - g] : 0u;
-            }
+            const u16x2 cxy = __builtin_bit_cast(u16x2, lin[u].y);
+            const u16x2 d = cxy - lo_xy;
+            const bool in_box =
+                __builtin_bit_cast(uint32_t, __builtin_elementwise_min(d, box_xy)) == __builtin_bit_cast(uint32_t, d);
+            // lanes past the last candidate or outside the box read the zero
+            // entries before the table
+            const bool valid = base + u * 64 + lane < total && in_box;
+            rv[u] = load_ranks<G>(a.rank, valid ? lin[u].x + tb : 0u);
         }
 #pragma unroll
         for (int u = 0; u < kGatherUnroll; ++u) {
+            const uint32_t j = (uint32_t)(base + u * 64 + lane);
 #pragma unroll
             for (int g = 0; g < G; ++g) {
                 if (g >= nt) continue;
-                const uint64_t mk = ballot(rk[u][g] != 0);
+                const uint32_t rk = rv[u].v[G - 1 - g];
+                const uint64_t mk = ballot(rk != 0);
                 const int pp = pos[g] + bits_below(mk);
-                if (rk[u][g] != 0) {
-                    const uint32_t ev = (rk[u][g] << lb) | (uint32_t)(base + u * 64 + lane);
+                if (rk != 0) {
+                    const uint32_t ev = (rk << lb) | j;
                     if (pp < cap)
                         e0[g * cap + pp] = ev;
                     else if (ge && pp < a.scratch_per_wave)
@@ -612,9 +627,7 @@ __device__ __forceinline__ void gather_run_(const ClusterArgs &a, WaveLds &sh, i
     }
 }
 
-// One wavefront per workgroup; the wavefronts loop over the target list.
-// PACKED: candidate coordinates come from bin_xyz (x | y << 11 | z << 22),
-// otherwise they are decoded from the cell index.
+// W wavefronts per workgroup; the wavefronts loop over the target list.
 //
 // SPILLED = false: the grid holds about n_list / per_wave wavefronts, far
 // more than are resident, so the hardware balances the uneven per-target
@@ -633,7 +646,7 @@ __device__ __forceinline__ void gather_run_(const ClusterArgs &a, WaveLds &sh, i
 // W wavefronts per workgroup take consecutive chunks (neighbouring targets
 // along x): their rank-table reads, offsets shifted by G, share the CU's L1
 // lines.
-template <bool PACKED, bool SPILLED, bool WAVES8, int W, int G>
+template <bool SPILLED, bool WAVES8, int W, int G>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WAVES8 ? 8 : 1, 8))) void k_cluster(
     ClusterArgs a) {
     static_assert(!SPILLED || G == 1, "the spill pass sorts one target per wavefront");
@@ -676,9 +689,9 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WAVES8 ?
             }
             int pos[G];
             if (G > 1 && nt > 1)
-                gather_run_<PACKED, G>(a, sh, x, y, z, nt, e0, cap, ge, pos);
+                gather_run_<G>(a, sh, x, y, z, nt, e0, cap, ge, pos);
             else
-                gather_run_<PACKED, 1>(a, sh, x, y, z, 1, e0, cap, ge, pos);
+                gather_run_<1>(a, sh, x, y, z, 1, e0, cap, ge, pos);
             wave_sync();
             for (int g = 0; g < nt; ++g) {
                 const int t = __builtin_amdgcn_readlane(my_t, r0 + g);
@@ -766,8 +779,11 @@ static int waves_per_cu(int G, int cap) {
 // room in LDS for at least kMinWavesForRuns wavefronts per CU, single
 // targets otherwise; the eight-wavefront register budget when 32
 // wavefronts fit.
-template <bool PACKED>
-static void launch_cap(const ClusterArgs &a, hipStream_t s) {
+void launch_cluster_search(const ClusterArgs &a, hipStream_t s) {
+    if (a.n_list <= 0) return;
+    if (a.per_wave < 1 || a.cap < kMinCap || a.cap > kMaxCap || a.cap % 64 != 0 || !a.bin_lin ||
+        (a.spill_list && (a.n_spill_waves < 1 || !a.spill_count || !a.scratch)))
+        throw std::runtime_error("cluster search: invalid launch arguments");
     constexpr int W = kWavesPerGroup;
     const int grid = (int)(((long long)a.n_list + (long long)a.per_wave * W - 1) / ((long long)a.per_wave * W));
     const size_t lds1 = (size_t)a.cap * sizeof(uint32_t);
@@ -778,33 +794,23 @@ static void launch_cap(const ClusterArgs &a, hipStream_t s) {
     const size_t lds = W * G * lds1;
     if (G == 4) {
         if (w8)
-            hipLaunchKernelGGL((k_cluster<PACKED, false, true, W, 4>), dim3(grid), dim3(64 * W), lds, s, a);
+            hipLaunchKernelGGL((k_cluster<false, true, W, 4>), dim3(grid), dim3(64 * W), lds, s, a);
         else
-            hipLaunchKernelGGL((k_cluster<PACKED, false, false, W, 4>), dim3(grid), dim3(64 * W), lds, s, a);
+            hipLaunchKernelGGL((k_cluster<false, false, W, 4>), dim3(grid), dim3(64 * W), lds, s, a);
     } else if (G == 2) {
         if (w8)
-            hipLaunchKernelGGL((k_cluster<PACKED, false, true, W, 2>), dim3(grid), dim3(64 * W), lds, s, a);
+            hipLaunchKernelGGL((k_cluster<false, true, W, 2>), dim3(grid), dim3(64 * W), lds, s, a);
         else
-            hipLaunchKernelGGL((k_cluster<PACKED, false, false, W, 2>), dim3(grid), dim3(64 * W), lds, s, a);
+            hipLaunchKernelGGL((k_cluster<false, false, W, 2>), dim3(grid), dim3(64 * W), lds, s, a);
     } else {
         if (w8)
-            hipLaunchKernelGGL((k_cluster<PACKED, false, true, W, 1>), dim3(grid), dim3(64 * W), lds, s, a);
+            hipLaunchKernelGGL((k_cluster<false, true, W, 1>), dim3(grid), dim3(64 * W), lds, s, a);
         else
-            hipLaunchKernelGGL((k_cluster<PACKED, false, false, W, 1>), dim3(grid), dim3(64 * W), lds, s, a);
+            hipLaunchKernelGGL((k_cluster<false, false, W, 1>), dim3(grid), dim3(64 * W), lds, s, a);
     }
     if (a.spill_list)
-        hipLaunchKernelGGL((k_cluster<PACKED, true, false, 1, 1>), dim3(a.n_spill_waves), dim3(64), 0, s, a);
+        hipLaunchKernelGGL((k_cluster<true, false, 1, 1>), dim3(a.n_spill_waves), dim3(64), 0, s, a);
 }
 
-void launch_cluster_search(const ClusterArgs &a, hipStream_t s) {
-    if (a.n_list <= 0) return;
-    if (a.per_wave < 1 || a.cap < kMinCap || a.cap > kMaxCap || a.cap % 64 != 0 ||
-        (a.spill_list && (a.n_spill_waves < 1 || !a.spill_count || !a.scratch)))
-        throw std::runtime_error("cluster search: invalid launch arguments");
-    if (a.bin_xyz)
-        launch_cap<true>(a, s);
-    else
-        launch_cap<false>(a, s);
-}
 
 }  // namespace hpgl_amd

@@ -436,6 +436,10 @@ struct CovAssets {
     bool have_rank = false;
     RankTable rank;
     long long rank_positive = 0;  // offsets with rank > 0
+    uint32_t max_rank = 0;
+    // half-widths of the device table: the box, widened in x by
+    // kClusterRunSlack zero columns on each side
+    int wh[3] = {0, 0, 0};
     DBuf<uint32_t> d_rank;
     bool have_pair = false;
     BoxTable pair;
@@ -502,24 +506,43 @@ std::shared_ptr<CovAssets> get_assets(const CovSpec &cov, const int radii[3], bo
 
 // Rank table of the cluster path over [-span, span]^3: span = r with a
 // table, 2r - 1 with the analytic model (candidates lie in the 27 bins).
+// The device copy has kClusterRunSlack zero columns on either side in x, so
+// a run of neighbouring targets reads its members' entries in one load.
 const CovAssets &assets_rank(CovAssets &a, hipStream_t s, Stats &st) {
     std::lock_guard<std::mutex> g(a.lazy);
     if (a.have_rank) return a;
     int span[3];
     for (int i = 0; i < 3; ++i) span[i] = std::max(a.use_table ? a.radii[i] : 2 * a.radii[i] - 1, 0);
-    // the search kernel indexes the table with 32-bit byte offsets and
-    // 24-bit multiplies: (2 span[1] + 1)(2 span[2] + 1) < 2^24
-    if ((2ll * span[0] + 1) * (2ll * span[1] + 1) * (2ll * span[2] + 1) >= (1ll << 30) ||
-        (2ll * span[1] + 1) * (2ll * span[2] + 1) >= (1ll << 24))
+    // the search's box test takes offsets modulo 2^16
+    if (span[0] + kClusterRunSlack >= (1 << 14) || span[1] >= (1 << 14))
+        throw std::runtime_error("search radiuses too large for the cell-binned search");
+    int wd[3];
+    for (int i = 0; i < 3; ++i) {
+        a.wh[i] = span[i] + (i == 0 ? kClusterRunSlack : 0);
+        wd[i] = 2 * a.wh[i] + 1;
+    }
+    // the search kernel indexes the table with 32-bit byte offsets
+    if ((long long)wd[0] * wd[1] * wd[2] + 2ll * kClusterRankPad >= (1ll << 30))
         throw std::runtime_error("search radiuses too large for the cell-binned search");
     const double t0 = now_ms();
     a.rank.build(a.model, a.use_table ? &a.table : nullptr, span);
     a.rank_positive = 0;
-    for (uint32_t r : a.rank.rank) a.rank_positive += r != 0;
+    a.max_rank = 0;
+    for (uint32_t r : a.rank.rank) {
+        a.rank_positive += r != 0;
+        a.max_rank = std::max(a.max_rank, r);
+    }
+    std::vector<uint32_t> wide((size_t)wd[0] * wd[1] * wd[2] + 2 * kClusterRankPad, 0u);
+    const RankTable &rt = a.rank;
+    for (int z = 0; z < rt.d[2]; ++z)
+        for (int y = 0; y < rt.d[1]; ++y) {
+            const size_t src = ((size_t)z * rt.d[1] + y) * rt.d[0];
+            const size_t dst = kClusterRankPad + ((size_t)(z - span[2] + a.wh[2]) * wd[1] + (y - span[1] + a.wh[1])) *
+                                                     wd[0] + (size_t)(a.wh[0] - span[0]);
+            std::copy(rt.rank.begin() + src, rt.rank.begin() + src + rt.d[0], wide.begin() + dst);
+        }
     st.ms_setup_host += now_ms() - t0;
-    std::vector<uint32_t> padded(a.rank.rank.size() + 2 * kClusterRankPad, 0u);
-    std::copy(a.rank.rank.begin(), a.rank.rank.end(), padded.begin() + kClusterRankPad);
-    upload(a.d_rank, padded, s);
+    upload(a.d_rank, wide, s);
     hip_check(hipStreamSynchronize(s), "sync");
     a.have_rank = true;
     return a;
@@ -530,6 +553,9 @@ const CovAssets &assets_rank(CovAssets &a, hipStream_t s, Stats &st) {
 void search(CovAssets &as, const int grid[3], const int radii[3], int max_n, const uint8_t *d_mask,
             const int *d_targets, int T, const int *d_inf, int I, int *d_nbr, int *d_cnt, hipStream_t s, Stats &st) {
     if (T == 0) return;
+    // the cluster path's rank table and its gather trim follow these radii
+    for (int i = 0; i < 3; ++i)
+        if (radii[i] != as.radii[i]) throw std::runtime_error("search: radiuses differ from the covariance assets'");
     struct {
         BinGeometry bins;
     } ss;
@@ -619,18 +645,16 @@ void search(CovAssets &as, const int grid[3], const int radii[3], int max_n, con
         DBuf<uint32_t> zoff(n_zoff);
         launch_bin_zoff(bin_start.p, bin_count.p, cells_out.p, ba, ss.bins.nbins, zoff.p, s);
         ca.zoff = zoff.p;
-        DBuf<uint32_t> xyz;
-        ca.bin_xyz = nullptr;
-        if (grid[0] <= 2048 && grid[1] <= 2048 && grid[2] <= 1024 && I < (1 << 30)) {
-            xyz.alloc(std::max(I, 1));
-            launch_pack_xyz(cells_out.p, I, grid[0], grid[1], xyz.p, s);
-            ca.bin_xyz = xyz.p;
-        }
         ca.rank = ra.d_rank.p;
         for (int i = 0; i < 3; ++i) {
             ca.span[i] = rt.span[i];
-            ca.rd[i] = rt.d[i];
+            ca.w[i] = 2 * ra.wh[i] + 1;
         }
+        ca.centre4 = 4u * (uint32_t)(kClusterRankPad + ra.wh[0] + ca.w[0] * (ra.wh[1] + (long long)ca.w[1] * ra.wh[2]));
+        DBuf<uint2> lin((size_t)I + kClusterLinPad);
+        launch_pack_lin(cells_out.p, I, grid[0], grid[1], (uint32_t)ca.w[0], (uint32_t)ca.w[0] * (uint32_t)ca.w[1],
+                        lin.p, s);
+        ca.bin_lin = lin.p;
         ca.max_n = max_n;
         ca.targets = d_targets;
         ca.list = clist.p;
@@ -643,8 +667,7 @@ void search(CovAssets &as, const int grid[3], const int radii[3], int max_n, con
         const long long max_raw = cluster_raw_bound(std::max(ss.bins.limit, 0));
         int lb = 1;
         while ((1ll << lb) < max_raw) ++lb;
-        uint32_t max_rank = 0;
-        for (uint32_t r : rt.rank) max_rank = std::max(max_rank, r);
+        const uint32_t max_rank = ra.max_rank;
         if (lb >= 32 || (unsigned long long)max_rank >= (1ull << (32 - lb)))
             throw std::runtime_error("cluster search: covariance has too many distinct values for 32-bit sort keys");
         ca.local_bits = lb;

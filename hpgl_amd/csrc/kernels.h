@@ -128,11 +128,20 @@ struct ClusterArgs {
     const uint32_t *bin_start;
     const uint32_t *bin_count;
     const int *bin_cells;     // informed cells sorted by (bin, cell)
-    const uint32_t *bin_xyz;  // their coordinates packed x | y << 11 | z << 22 (nullptr: decode the cell)
+    // per cell of bin_cells: x = its rank-table coordinate
+    // 4 (x + w[0] y + w[0] w[1] z) mod 2^32, y = x mod 2^16 | (y mod 2^16) << 16
+    // (launch_pack_lin); then kClusterLinPad zero entries
+    const uint2 *bin_lin;
     const uint32_t *zoff;     // per bin, cells below each z layer (launch_bin_zoff)
-    const uint32_t *rank;     // rank table over [-span, span]^3, after kClusterRankPad zero entries
+    // rank table over the covariance box [-span, span]^3, widened in x by
+    // kClusterRunSlack zero entries on each side, after kClusterRankPad zero
+    // entries: w = (2 span[0] + 1 + 2 kClusterRunSlack, 2 span[1] + 1,
+    // 2 span[2] + 1) entries, offset (dx, dy, dz) at byte
+    // centre4 + 4 (dx + w[0] dy + w[0] w[1] dz)
+    const uint32_t *rank;
     int span[3];
-    int rd[3];
+    int w[3];
+    uint32_t centre4;
     int max_n;
     const int *targets;
     const int *list;
@@ -159,13 +168,24 @@ void launch_cluster_search(const ClusterArgs &a, hipStream_t s);
 // search that reads the entries of several neighbouring targets with one
 // vector load may start up to three entries before the first one.
 constexpr int kClusterRankPad = 4;
+// Zero rank-table columns on either side of the box in x: a run of up to
+// kClusterRunSlack + 1 targets reads its members' entries together.
+constexpr int kClusterRunSlack = 3;
+// Entries after the bin-sorted rank-table coordinates: a gather pass reads
+// up to 255 positions past the last candidate of a target.
+constexpr int kClusterLinPad = 256;
+// out[i] = (4 (x + w0 y + w01 z) mod 2^32, x mod 2^16 | (y mod 2^16) << 16)
+// for the cell cells[i], i < n, and (0, 0) for the kClusterLinPad entries
+// after them.
+void launch_pack_lin(const int *cells, int n, int X, int Y, uint32_t w0, uint32_t w01, uint2 *out, hipStream_t s);
 // zoff[b (r[2] + 1) + d]: cells of bin b whose z lies below the bin's first
 // layer + d, for d = 0 .. r[2].
 void launch_bin_zoff(const uint32_t *bin_start, const uint32_t *bin_count, const int *bin_cells, const BinArgs &b,
                      long long nbins, uint32_t *zoff, hipStream_t s);
-// Raw candidate indices of a target whose 27 bins hold at most limit cells
-// each lie below this bound (21 segments, each padded to a multiple of 64).
-inline long long cluster_raw_bound(int limit) { return 27ll * limit + 21ll * 64; }
+// Raw candidate indices (positions in the concatenation of a target's 27
+// bins) of a target whose bins hold at most limit cells each lie below
+// this bound.
+inline long long cluster_raw_bound(int limit) { return 27ll * limit + 1; }
 // Scratch words per wave for lists of up to n candidates (the elements,
 // rounded up to a multiple of 64 words).
 __host__ __device__ inline long long cluster_scratch_words(int n) { return ((long long)n + 63) / 64 * 64; }
@@ -279,6 +299,5 @@ void launch_copy_informed(const float *in, const uint8_t *in_mask, long long n, 
                           hipStream_t s);
 void launch_ik_argmax(const float *probs, int K, long long n, uint8_t *out, uint8_t *out_mask, hipStream_t s);
 void launch_fill_u32(uint32_t *p, uint32_t v, long long n, hipStream_t s);
-void launch_pack_xyz(const int *cells, int n, int X, int Y, uint32_t *out, hipStream_t s);
 
 }  // namespace hpgl_amd

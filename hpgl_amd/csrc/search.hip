@@ -212,12 +212,19 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, long long n) {
     if (i < n) p[i] = v;
 }
 
-__global__ void k_pack_xyz(const int *cells, int n, int X, int Y, uint32_t *out) {
+// Differences of the first words are offsets in the rank table's strides
+// (modulo 2^32, exact for offsets inside the table); the second word holds
+// x and y modulo 2^16 for the search's box test.
+__global__ void k_pack_lin(const int *cells, int n, int X, int Y, uint32_t w0, uint32_t w01, uint2 *out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n + kClusterLinPad) return;
+    if (i >= n) {
+        out[i] = make_uint2(0u, 0u);
+        return;
+    }
     const int c = cells[i];
     const uint32_t x = c % X, y = c % (X * Y) / X, z = c / (X * Y);
-    out[i] = x | (y << 11) | (z << 22);
+    out[i] = make_uint2(4u * (x + w0 * y + w01 * z), (x & 0xffffu) | (y << 16));
 }
 
 inline unsigned blocks(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
@@ -266,9 +273,10 @@ void launch_ik_argmax(const float *probs, int K, long long n, uint8_t *out, uint
     hipLaunchKernelGGL(k_ik_argmax, dim3(blocks(n, 256)), dim3(256), 0, s, probs, K, n, out, out_mask);
 }
 
-void launch_pack_xyz(const int *cells, int n, int X, int Y, uint32_t *out, hipStream_t s) {
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_pack_xyz, dim3(blocks(n, 256)), dim3(256), 0, s, cells, n, X, Y, out);
+void launch_pack_lin(const int *cells, int n, int X, int Y, uint32_t w0, uint32_t w01, uint2 *out, hipStream_t s) {
+    if (n < 0) return;
+    hipLaunchKernelGGL(k_pack_lin, dim3(blocks((long long)n + kClusterLinPad, 256)), dim3(256), 0, s, cells, n, X, Y,
+                       w0, w01, out);
 }
 
 void launch_fill_u32(uint32_t *p, uint32_t v, long long n, hipStream_t s) {
