@@ -629,15 +629,13 @@ __device__ __forceinline__ void gather_run_(const ClusterArgs &a, WaveLds &sh, i
 
 // W wavefronts per workgroup; the wavefronts loop over the target list.
 //
-// SPILLED = false: the grid holds about n_list / per_wave wavefronts, far
-// more than are resident, so the hardware balances the uneven per-target
-// work; wavefronts stride over the list in chunks of G consecutive
-// entries, gather each run of targets that share their 27 bins (same y, z
-// and x bin, consecutive x) together, then sort the runs' lists one by one;
-// a list longer than a.cap is not sorted but its target appended to
-// spill_list.  SPILLED = true (G = 1): the wavefronts stride over the
-// spill_count targets of spill_list, sorting all their lists in their
-// global scratch.
+// SPILLED = false: as many wavefronts as are resident at once take chunks
+// of per_wave list entries from the work counters, gather each run of
+// targets that share their 27 bins (same y, z and x bin, consecutive x)
+// together, then sort the runs' lists one by one; a list longer than a.cap
+// is not sorted but its target appended to spill_list.  SPILLED = true
+// (G = 1): the wavefronts stride over the spill_count targets of
+// spill_list, sorting all their lists in their global scratch.
 //
 // WAVES8: the register budget of eight wavefronts per SIMD (a few SGPRs
 // move to VGPR lanes), used when the LDS capacity allows 32 wavefronts per
@@ -665,10 +663,8 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WAVES8 ?
     uint32_t *ge = SPILLED ? a.scratch + (size_t)wave_id * cluster_scratch_words(a.scratch_per_wave) : nullptr;
     unsigned long long cand_acc = 0, spill_acc = 0;
     const int li_end = SPILLED ? *a.spill_count : a.n_list;
-    const int n_chunks = (li_end + G - 1) / G;
-    for (int ch = wave_id; ch < n_chunks; ch += gridDim.x * W) {
-        const int ent0 = ch * G;
-        const int ne = min(G, li_end - ent0);
+    // list entries [ent0, ent0 + ne), ne <= G
+    auto entries = [&](int ent0, int ne) {
         // lane g < ne: entry ent0 + g's target and cell
         int my_t = 0, my_node = 0;
         if (lane < ne) {
@@ -727,6 +723,36 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WAVES8 ?
             }
             r0 += nt;
         }
+    };
+    if constexpr (SPILLED) {
+        for (int li = wave_id; li < li_end; li += gridDim.x * W) entries(li, 1);
+    } else {
+        // chunks of per_wave entries (a multiple of G) from the work
+        // counters: part p holds chunks [n_ch p / P, n_ch (p + 1) / P); a
+        // wavefront starts at part blockIdx mod P and moves to the next part
+        // when one is exhausted, so all finish within about one chunk
+        const int chunk = a.per_wave;
+        const int n_ch = (li_end + chunk - 1) / chunk;
+        int tries = 0;
+        for (;;) {
+            int ch = -1;
+            while (tries < kClusterWorkParts) {
+                const int p = ((int)blockIdx.x + tries) % kClusterWorkParts;
+                const int c0 = (int)((long long)n_ch * p / kClusterWorkParts);
+                const int c1 = (int)((long long)n_ch * (p + 1) / kClusterWorkParts);
+                int c = 0;
+                if (lane == 0) c = atomicAdd(a.work + p * kClusterWorkStride, 1);
+                c = __builtin_amdgcn_readfirstlane(c);
+                if (c < c1 - c0) {
+                    ch = c0 + c;
+                    break;
+                }
+                ++tries;
+            }
+            if (ch < 0) break;
+            const int end = min(li_end, (ch + 1) * chunk);
+            for (int ent0 = ch * chunk; ent0 < end; ent0 += G) entries(ent0, min(G, end - ent0));
+        }
     }
     if (lane == 0 && a.cand_total && cand_acc != 0) atomicAdd(counter_line(a.cand_total), cand_acc);
     if (lane == 0 && a.cand_total && spill_acc != 0) atomicAdd(counter_line(a.cand_total) + 1, spill_acc);
@@ -775,42 +801,56 @@ static int waves_per_cu(int G, int cap) {
     return (int)(kLdsPerCu / ((size_t)G * cap * sizeof(uint32_t) + sizeof(WaveLds)));
 }
 
+// Main pass with the grid the device holds at once (at most one workgroup
+// per chunk of the list): its wavefronts stay resident and take chunks
+// from the work counters.
+template <bool W8, int G>
+static void launch_main(const ClusterArgs &a, size_t lds, hipStream_t s) {
+    constexpr int W = kWavesPerGroup;
+    const auto kern = k_cluster<false, W8, W, G>;
+    int per_cu = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * W, lds) != hipSuccess ||
+        hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        throw std::runtime_error("cluster search: occupancy query failed");
+    const long long chunks = ((long long)a.n_list + a.per_wave - 1) / a.per_wave;
+    const long long grid = std::max(1ll, std::min((long long)std::max(per_cu, 1) * cus, (chunks + W - 1) / W));
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64 * W), lds, s, a);
+}
+
 // Main pass: runs of up to G = 4 (or 2) targets when their lists leave
 // room in LDS for at least kMinWavesForRuns wavefronts per CU, single
 // targets otherwise; the eight-wavefront register budget when 32
 // wavefronts fit.
 void launch_cluster_search(const ClusterArgs &a, hipStream_t s) {
     if (a.n_list <= 0) return;
-    if (a.per_wave < 1 || a.cap < kMinCap || a.cap > kMaxCap || a.cap % 64 != 0 || !a.bin_lin ||
-        (a.spill_list && (a.n_spill_waves < 1 || !a.spill_count || !a.scratch)))
+    if (a.per_wave < 1 || a.cap < kMinCap || a.cap > kMaxCap || a.cap % 64 != 0 || !a.bin_lin || !a.work ||
+        a.per_wave % kTargetsPerRun != 0 || (a.spill_list && (a.n_spill_waves < 1 || !a.spill_count || !a.scratch)))
         throw std::runtime_error("cluster search: invalid launch arguments");
-    constexpr int W = kWavesPerGroup;
-    const int grid = (int)(((long long)a.n_list + (long long)a.per_wave * W - 1) / ((long long)a.per_wave * W));
     const size_t lds1 = (size_t)a.cap * sizeof(uint32_t);
     const int G = (kTargetsPerRun >= 4 && waves_per_cu(4, a.cap) >= kMinWavesForRuns)   ? 4
                   : (kTargetsPerRun >= 2 && waves_per_cu(2, a.cap) >= kMinWavesForRuns) ? 2
                                                                                          : 1;
     const bool w8 = waves_per_cu(G, a.cap) >= 32;
-    const size_t lds = W * G * lds1;
+    const size_t lds = kWavesPerGroup * G * lds1;
     if (G == 4) {
         if (w8)
-            hipLaunchKernelGGL((k_cluster<false, true, W, 4>), dim3(grid), dim3(64 * W), lds, s, a);
+            launch_main<true, 4>(a, lds, s);
         else
-            hipLaunchKernelGGL((k_cluster<false, false, W, 4>), dim3(grid), dim3(64 * W), lds, s, a);
+            launch_main<false, 4>(a, lds, s);
     } else if (G == 2) {
         if (w8)
-            hipLaunchKernelGGL((k_cluster<false, true, W, 2>), dim3(grid), dim3(64 * W), lds, s, a);
+            launch_main<true, 2>(a, lds, s);
         else
-            hipLaunchKernelGGL((k_cluster<false, false, W, 2>), dim3(grid), dim3(64 * W), lds, s, a);
+            launch_main<false, 2>(a, lds, s);
     } else {
         if (w8)
-            hipLaunchKernelGGL((k_cluster<false, true, W, 1>), dim3(grid), dim3(64 * W), lds, s, a);
+            launch_main<true, 1>(a, lds, s);
         else
-            hipLaunchKernelGGL((k_cluster<false, false, W, 1>), dim3(grid), dim3(64 * W), lds, s, a);
+            launch_main<false, 1>(a, lds, s);
     }
     if (a.spill_list)
         hipLaunchKernelGGL((k_cluster<true, false, 1, 1>), dim3(a.n_spill_waves), dim3(64), 0, s, a);
 }
-
 
 }  // namespace hpgl_amd

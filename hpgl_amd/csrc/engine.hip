@@ -677,6 +677,9 @@ void search(CovAssets &as, const int grid[3], const int radii[3], int max_n, con
         const double expect = (double)I / ((double)grid[0] * grid[1] * grid[2]) * (double)n_pos;
         ca.cap = cluster_capacity((long long)(expect + 6.0 * std::sqrt(expect) + 64.0));
         ca.per_wave = kClusterPerWave;
+        DBuf<int> work(kClusterWorkParts * kClusterWorkStride);
+        launch_fill_u32(reinterpret_cast<uint32_t *>(work.p), 0u, kClusterWorkParts * kClusterWorkStride, s);
+        ca.work = work.p;
         DBuf<uint32_t> scratch;
         DBuf<int> spill_list, spill_count;
         if (max_cand > ca.cap) {

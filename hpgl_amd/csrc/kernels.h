@@ -150,7 +150,12 @@ struct ClusterArgs {
     int *cnt;
     int local_bits;           // element = (rank << local_bits) | raw index (< cluster_raw_bound)
     int cap;                  // LDS list capacity, a value returned by cluster_capacity
-    int per_wave;             // main pass grid: ceil(n_list / per_wave) wavefronts
+    // main pass: wavefronts resident for the whole launch take chunks of
+    // per_wave list entries from kClusterWorkParts counters (work, one
+    // 64-byte line each, zero before the launch), each over a contiguous
+    // part of the list
+    int per_wave;
+    int *work;
     // lists longer than cap: the main pass appends their targets to
     // spill_list (spill_count entries; nullptr: an overflow error) and a
     // second pass of n_spill_waves wavefronts sorts them in global scratch,
@@ -168,6 +173,10 @@ void launch_cluster_search(const ClusterArgs &a, hipStream_t s);
 // search that reads the entries of several neighbouring targets with one
 // vector load may start up to three entries before the first one.
 constexpr int kClusterRankPad = 4;
+// Work counters of the cell-binned search's main pass (kClusterWorkStride
+// ints apart).
+constexpr int kClusterWorkParts = 8;
+constexpr int kClusterWorkStride = 16;
 // Zero rank-table columns on either side of the box in x: a run of up to
 // kClusterRunSlack + 1 targets reads its members' entries together.
 constexpr int kClusterRunSlack = 3;
