@@ -88,10 +88,8 @@ struct WaveLds {
 };
 
 constexpr int kGatherUnroll = 4;  // 64-candidate groups whose loads are in flight together
-// Targets gathered together when consecutive list entries share their bins,
-// and the fewest wavefronts per CU whose LDS lists that may cost.
-constexpr int kTargetsPerRun = 4;
-constexpr int kMinWavesForRuns = 32;
+// Targets gathered together when consecutive list entries share their bins.
+constexpr int kTargetsPerRun = kClusterRunSlack + 1;
 
 struct Keyed {
     int lb;
@@ -488,12 +486,13 @@ typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 // table instead.  Run members whose own offset leaves the box read the
 // table's zero columns in x.
 //
-// Target g keeps the candidates with rank > 0 in raw order at e0 + g cap
-// (LDS up to cap, then the spill scratch ge, if any); returns the list
-// lengths in pos[0 .. nt).
+// Target 0 keeps the candidates with rank > 0 in raw order at e0 (LDS up
+// to cap, then the spill scratch ge, if any), target g > 0 at
+// gs + (g - 1) cap (global scratch, up to cap); returns the list lengths in
+// pos[0 .. nt).
 template <int G>
 __device__ __forceinline__ void gather_run_(const ClusterArgs &a, WaveLds &sh, int x, int y, int z, int nt,
-                                            uint32_t *e0, int cap, uint32_t *ge, int *pos) {
+                                            uint32_t *e0, int cap, uint32_t *ge, uint32_t *gs, int *pos) {
     const int lane = threadIdx.x & 63;
     const BinArgs &b = a.bins;
     const int lb = a.local_bits;
@@ -616,10 +615,14 @@ __device__ __forceinline__ void gather_run_(const ClusterArgs &a, WaveLds &sh, i
                 const int pp = pos[g] + bits_below(mk);
                 if (rk != 0) {
                     const uint32_t ev = (rk << lb) | j;
-                    if (pp < cap)
-                        e0[g * cap + pp] = ev;
-                    else if (ge && pp < a.scratch_per_wave)
-                        ge[pp] = ev;
+                    if (g == 0) {
+                        if (pp < cap)
+                            e0[pp] = ev;
+                        else if (ge && pp < a.scratch_per_wave)
+                            ge[pp] = ev;
+                    } else if (pp < cap) {
+                        gs[(g - 1) * cap + pp] = ev;
+                    }
                 }
                 pos[g] += __popcll(mk);
             }
@@ -654,8 +657,13 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WAVES8 ?
     const int cap = SPILLED ? 0 : a.cap;
     const int w = W == 1 ? 0 : (int)(threadIdx.x >> 6);
     WaveLds &sh = sh_all[w];
-    uint32_t *const e0 = lds_all + (size_t)w * G * cap;
+    uint32_t *const e0 = lds_all + (size_t)w * cap;
     const int wave_id = blockIdx.x * W + w;
+    // the later members' lists of a run; a wavefront without a scratch slot
+    // takes its targets one at a time
+    uint32_t *const gs = !SPILLED && G > 1 && wave_id < a.run_slots
+                             ? a.run_scratch + (size_t)wave_id * (G - 1) * (size_t)cap
+                             : nullptr;
     const int lane = threadIdx.x & 63;
     const int X = a.X, Y = a.Y;
     const Keyed key{a.local_bits};
@@ -677,7 +685,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WAVES8 ?
             const int x = node0 % X, y = node0 % (X * Y) / X, z = node0 / (X * Y);
             const int bx = x / a.bins.r[0];
             int nt = 1;
-            while (r0 + nt < ne) {
+            while (gs && r0 + nt < ne) {
                 const int xn = x + nt;
                 if (__builtin_amdgcn_readlane(my_node, r0 + nt) != node0 + nt || xn >= X || xn / a.bins.r[0] != bx)
                     break;
@@ -685,9 +693,9 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WAVES8 ?
             }
             int pos[G];
             if (G > 1 && nt > 1)
-                gather_run_<G>(a, sh, x, y, z, nt, e0, cap, ge, pos);
+                gather_run_<G>(a, sh, x, y, z, nt, e0, cap, ge, gs, pos);
             else
-                gather_run_<1>(a, sh, x, y, z, 1, e0, cap, ge, pos);
+                gather_run_<1>(a, sh, x, y, z, 1, e0, cap, ge, gs, pos);
             wave_sync();
             for (int g = 0; g < nt; ++g) {
                 const int t = __builtin_amdgcn_readlane(my_t, r0 + g);
@@ -695,7 +703,13 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WAVES8 ?
 #pragma unroll
                 for (int k = 1; k < G; ++k)
                     if (k == g) total = pos[k];
-                uint32_t *const eg = e0 + g * cap;
+                if (g > 0 && total <= cap) {
+                    // a later member's list from the run scratch into LDS
+                    const uint32_t *src = gs + (size_t)(g - 1) * cap;
+                    for (int i = lane; i < total; i += 64) e0[i] = src[i];
+                    wave_sync();
+                }
+                uint32_t *const eg = e0;
                 int *out = a.nbr + (size_t)t * a.max_n;
                 bool sane = true;
                 bool spilled = false;
@@ -795,60 +809,54 @@ int cluster_capacity(long long expected_max) {
     return (int)std::min<long long>(kMaxCap, std::max<long long>(kMinCap, c));
 }
 
-// Wavefronts per CU that the LDS holds with lists of G targets of cap
-// elements each.
-static int waves_per_cu(int G, int cap) {
-    return (int)(kLdsPerCu / ((size_t)G * cap * sizeof(uint32_t) + sizeof(WaveLds)));
+// Wavefronts per CU that the LDS holds with one list of cap elements each.
+static int waves_per_cu(int cap) { return (int)(kLdsPerCu / ((size_t)cap * sizeof(uint32_t) + sizeof(WaveLds))); }
+
+// The main pass's kernel and dynamic LDS for these arguments: runs of up to
+// kTargetsPerRun targets, the eight-wavefront register budget when 32
+// wavefronts fit the LDS.
+struct MainPass {
+    const void *kern;
+    size_t lds;
+    bool w8;
+};
+static MainPass main_pass(const ClusterArgs &a) {
+    MainPass m;
+    m.w8 = waves_per_cu(a.cap) >= 32;
+    m.lds = kWavesPerGroup * (size_t)a.cap * sizeof(uint32_t);
+    m.kern = m.w8 ? reinterpret_cast<const void *>(k_cluster<false, true, kWavesPerGroup, kTargetsPerRun>)
+                  : reinterpret_cast<const void *>(k_cluster<false, false, kWavesPerGroup, kTargetsPerRun>);
+    return m;
 }
 
-// Main pass with the grid the device holds at once (at most one workgroup
-// per chunk of the list): its wavefronts stay resident and take chunks
-// from the work counters.
-template <bool W8, int G>
-static void launch_main(const ClusterArgs &a, size_t lds, hipStream_t s) {
+// The grid the device holds at once, and at most one workgroup per chunk
+// of the list.
+static long long main_grid(const ClusterArgs &a, const MainPass &m) {
     constexpr int W = kWavesPerGroup;
-    const auto kern = k_cluster<false, W8, W, G>;
     int per_cu = 0, dev = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * W, lds) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, m.kern, 64 * W, m.lds) != hipSuccess ||
         hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         throw std::runtime_error("cluster search: occupancy query failed");
     const long long chunks = ((long long)a.n_list + a.per_wave - 1) / a.per_wave;
-    const long long grid = std::max(1ll, std::min((long long)std::max(per_cu, 1) * cus, (chunks + W - 1) / W));
-    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64 * W), lds, s, a);
+    return std::max(1ll, std::min((long long)std::max(per_cu, 1) * cus, (chunks + W - 1) / W));
 }
 
-// Main pass: runs of up to G = 4 (or 2) targets when their lists leave
-// room in LDS for at least kMinWavesForRuns wavefronts per CU, single
-// targets otherwise; the eight-wavefront register budget when 32
-// wavefronts fit.
+long long cluster_main_waves(const ClusterArgs &a) { return main_grid(a, main_pass(a)) * kWavesPerGroup; }
+
 void launch_cluster_search(const ClusterArgs &a, hipStream_t s) {
     if (a.n_list <= 0) return;
     if (a.per_wave < 1 || a.cap < kMinCap || a.cap > kMaxCap || a.cap % 64 != 0 || !a.bin_lin || !a.work ||
-        a.per_wave % kTargetsPerRun != 0 || (a.spill_list && (a.n_spill_waves < 1 || !a.spill_count || !a.scratch)))
+        a.per_wave % kTargetsPerRun != 0 || (a.run_slots > 0 && !a.run_scratch) ||
+        (a.spill_list && (a.n_spill_waves < 1 || !a.spill_count || !a.scratch)))
         throw std::runtime_error("cluster search: invalid launch arguments");
-    const size_t lds1 = (size_t)a.cap * sizeof(uint32_t);
-    const int G = (kTargetsPerRun >= 4 && waves_per_cu(4, a.cap) >= kMinWavesForRuns)   ? 4
-                  : (kTargetsPerRun >= 2 && waves_per_cu(2, a.cap) >= kMinWavesForRuns) ? 2
-                                                                                         : 1;
-    const bool w8 = waves_per_cu(G, a.cap) >= 32;
-    const size_t lds = kWavesPerGroup * G * lds1;
-    if (G == 4) {
-        if (w8)
-            launch_main<true, 4>(a, lds, s);
-        else
-            launch_main<false, 4>(a, lds, s);
-    } else if (G == 2) {
-        if (w8)
-            launch_main<true, 2>(a, lds, s);
-        else
-            launch_main<false, 2>(a, lds, s);
-    } else {
-        if (w8)
-            launch_main<true, 1>(a, lds, s);
-        else
-            launch_main<false, 1>(a, lds, s);
-    }
+    const MainPass m = main_pass(a);
+    const long long grid = main_grid(a, m);
+    constexpr int W = kWavesPerGroup, G = kTargetsPerRun;
+    if (m.w8)
+        hipLaunchKernelGGL((k_cluster<false, true, W, G>), dim3((unsigned)grid), dim3(64 * W), m.lds, s, a);
+    else
+        hipLaunchKernelGGL((k_cluster<false, false, W, G>), dim3((unsigned)grid), dim3(64 * W), m.lds, s, a);
     if (a.spill_list)
         hipLaunchKernelGGL((k_cluster<true, false, 1, 1>), dim3(a.n_spill_waves), dim3(64), 0, s, a);
 }

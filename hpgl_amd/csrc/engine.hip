@@ -680,6 +680,13 @@ void search(CovAssets &as, const int grid[3], const int radii[3], int max_n, con
         DBuf<int> work(kClusterWorkParts * kClusterWorkStride);
         launch_fill_u32(reinterpret_cast<uint32_t *>(work.p), 0u, kClusterWorkParts * kClusterWorkStride, s);
         ca.work = work.p;
+        // the later members' lists of runs of neighbouring targets
+        ca.run_slots = 0;
+        ca.run_scratch = nullptr;
+        const long long slots = cluster_main_waves(ca);
+        DBuf<uint32_t> run_scratch((size_t)slots * kClusterRunSlack * (size_t)ca.cap);
+        ca.run_scratch = run_scratch.p;
+        ca.run_slots = (int)slots;
         DBuf<uint32_t> scratch;
         DBuf<int> spill_list, spill_count;
         if (max_cand > ca.cap) {

@@ -156,6 +156,11 @@ struct ClusterArgs {
     // part of the list
     int per_wave;
     int *work;
+    // runs of neighbouring targets: the first member's list is gathered
+    // into LDS, the others' into run_scratch, (kClusterRunSlack) cap words
+    // per main-pass wavefront, run_slots wavefronts (cluster_main_waves)
+    uint32_t *run_scratch;
+    int run_slots;
     // lists longer than cap: the main pass appends their targets to
     // spill_list (spill_count entries; nullptr: an overflow error) and a
     // second pass of n_spill_waves wavefronts sorts them in global scratch,
@@ -169,6 +174,9 @@ struct ClusterArgs {
     unsigned long long *cand_total;  // counter lines: word 0 candidates gathered, word 1 lists sorted in scratch
 };
 void launch_cluster_search(const ClusterArgs &a, hipStream_t s);
+// Wavefronts of the cell-binned search's main pass for these arguments
+// (the grid the device holds at once, at most one workgroup per chunk).
+long long cluster_main_waves(const ClusterArgs &a);
 // Zero entries before and after the rank table in its device allocation: a
 // search that reads the entries of several neighbouring targets with one
 // vector load may start up to three entries before the first one.
