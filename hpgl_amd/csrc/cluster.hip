@@ -87,7 +87,7 @@ struct WaveLds {
     int seg_off[kMaxSegs];
 };
 
-constexpr int kGatherUnroll = 4;  // 64-candidate groups whose loads are in flight together
+constexpr int kGatherUnrollMax = 4;  // 64-candidate groups whose loads are in flight together
 // Targets gathered together when consecutive list entries share their bins.
 constexpr int kTargetsPerRun = kClusterRunSlack + 1;
 
@@ -458,12 +458,15 @@ __device__ inline RankVec<G> load_ranks(const uint32_t *base, uint32_t off) {
         r.v[0] = w.x;
         r.v[1] = w.y;
     } else {
-        static_assert(G == 4, "groups of 1, 2 or 4 targets");
-        const u32x4_a4 w = *reinterpret_cast<const u32x4_a4 *>(p);
-        r.v[0] = w.x;
-        r.v[1] = w.y;
-        r.v[2] = w.z;
-        r.v[3] = w.w;
+        static_assert(G % 4 == 0, "groups of 1, 2 or a multiple of 4 targets");
+#pragma unroll
+        for (int k = 0; k < G; k += 4) {
+            const u32x4_a4 w = *reinterpret_cast<const u32x4_a4 *>(p + 4 * k);
+            r.v[k] = w.x;
+            r.v[k + 1] = w.y;
+            r.v[k + 2] = w.z;
+            r.v[k + 3] = w.w;
+        }
     }
     return r;
 }
@@ -556,6 +559,9 @@ __device__ __forceinline__ void gather_run_(const ClusterArgs &a, WaveLds &sh, i
     // the segment cursor: cur = offset of the segment holding the next
     // group's first raw index; (dn, on) = start and offset of the next
     // segment not yet entered
+    // groups whose loads are in flight together (fewer for long runs, whose
+    // rank reads take more registers)
+    constexpr int kGatherUnroll = G >= 8 ? 2 : kGatherUnrollMax;
     int nx = 1;
     int cur = __builtin_amdgcn_readlane(seg_o, 0);
     int dn = __builtin_amdgcn_readlane(seg_d, 1), on = __builtin_amdgcn_readlane(seg_o, 1);
@@ -621,7 +627,9 @@ __device__ __forceinline__ void gather_run_(const ClusterArgs &a, WaveLds &sh, i
                         else if (ge && pp < a.scratch_per_wave)
                             ge[pp] = ev;
                     } else if (pp < cap) {
-                        gs[(g - 1) * cap + pp] = ev;
+                        // 32-bit byte offset from the wavefront's scratch base
+                        *reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(gs) +
+                                                      (((uint32_t)((g - 1) * cap) + (uint32_t)pp) << 2)) = ev;
                     }
                 }
                 pos[g] += __popcll(mk);
@@ -652,7 +660,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WAVES8 ?
     ClusterArgs a) {
     static_assert(!SPILLED || G == 1, "the spill pass sorts one target per wavefront");
     __shared__ WaveLds sh_all[W];
-    extern __shared__ uint32_t lds_all[];
+    extern __shared__ __attribute__((aligned(16))) uint32_t lds_all[];
     // the spill pass gathers straight into its global scratch
     const int cap = SPILLED ? 0 : a.cap;
     const int w = W == 1 ? 0 : (int)(threadIdx.x >> 6);
@@ -705,8 +713,12 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WAVES8 ?
                     if (k == g) total = pos[k];
                 if (g > 0 && total <= cap) {
                     // a later member's list from the run scratch into LDS
+                    // four elements per lane and instruction: both lists start
+                    // 16-byte aligned and hold cap (a multiple of 64) words,
+                    // so the last quadruple stays inside them
                     const uint32_t *src = gs + (size_t)(g - 1) * cap;
-                    for (int i = lane; i < total; i += 64) e0[i] = src[i];
+                    for (int i = 4 * lane; i < total; i += 256)
+                        *reinterpret_cast<uint4 *>(e0 + i) = *reinterpret_cast<const uint4 *>(src + i);
                     wave_sync();
                 }
                 uint32_t *const eg = e0;

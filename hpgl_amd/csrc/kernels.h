@@ -177,10 +177,10 @@ void launch_cluster_search(const ClusterArgs &a, hipStream_t s);
 // Wavefronts of the cell-binned search's main pass for these arguments
 // (the grid the device holds at once, at most one workgroup per chunk).
 long long cluster_main_waves(const ClusterArgs &a);
-// Zero entries before and after the rank table in its device allocation: a
-// search that reads the entries of several neighbouring targets with one
-// vector load may start up to three entries before the first one.
-constexpr int kClusterRankPad = 4;
+// Zero entries before and after the rank table in its device allocation:
+// the search's lanes with no entry to read take the run's entries from the
+// start of the allocation.
+constexpr int kClusterRankPad = 8;
 // Work counters of the cell-binned search's main pass (kClusterWorkStride
 // ints apart).
 constexpr int kClusterWorkParts = 8;
