@@ -409,31 +409,56 @@ __device__ __forceinline__ bool topk_sort_(uint32_t *e, S &sh, int n, int K, Key
 
 // Stable rank sort of e[0, P) by descending key; writes the first K cells,
 // decoding each kept element's raw candidate index back to its cell.
+// P = min(n, K + 16) <= 80: lane i holds positions i and 64 + i.  When the
+// keys leave 7 bits free (local bits >= 7), e[0, P) is overwritten with
+// (key << 7) | (127 - position), so that one unsigned compare orders by
+// descending key with ties in position order, as the stable sort does.
 template <typename E, typename S>
-__device__ void finish_(const E *e, int n, int K, int *out, const S &sh, const int *bin_cells, Keyed key) {
+__device__ void finish_(E *e, int n, int K, int *out, const S &sh, const int *bin_cells, Keyed key) {
     const int lane = threadIdx.x & 63;
     const int P = min(n, K + 16);
-    const uint32_t lmask = (1u << key.lb) - 1u;
-    for (int base = 0; base < P; base += 64) {
-        const int p = base + lane;
-        if (p < P) {
-            const uint32_t ep = e[p];
-            const uint32_t kp = key(ep);
-            int r = 0;
+    const bool two = P > 64;
+    const uint32_t el0 = lane < P ? e[lane] : 0u;
+    const uint32_t el1 = two && 64 + lane < P ? e[64 + lane] : 0u;
+    int r0 = 0, r1 = 0;
+    if (key.lb >= 7) {
+        const uint32_t c0 = (key(el0) << 7) | (127u - (uint32_t)lane);
+        const uint32_t c1 = (key(el1) << 7) | (63u - (uint32_t)lane);
+        // each lane rewrites only its own positions, then all read them
+        if (lane < P) e[lane] = c0;
+        if (two && 64 + lane < P) e[64 + lane] = c1;
+        wave_sync();
+        if (two) {
             for (int q = 0; q < P; ++q) {
-                const uint32_t kq = key(e[q]);
-                r += (kq > kp) || (kq == kp && q < p);
+                const uint32_t cq = e[q];
+                r0 += cq > c0;
+                r1 += cq > c1;
             }
-            if (r < K) {
-                // the segment of raw index j: the last one starting at or
-                // before j (empty segments start where the next one does)
-                const int j = (int)(ep & lmask);
-                int g = 0;
+        } else {
+            for (int q = 0; q < P; ++q) r0 += e[q] > c0;
+        }
+    } else {
+        const uint32_t k0 = key(el0), k1 = key(el1);
+        for (int q = 0; q < P; ++q) {
+            const uint32_t kq = key(e[q]);
+            r0 += (kq > k0) || (kq == k0 && q < lane);
+            r1 += (kq > k1) || (kq == k1 && q < 64 + lane);
+        }
+    }
+    const uint32_t lmask = (1u << key.lb) - 1u;
 #pragma unroll
-                for (int step = kMaxSegs / 2; step >= 1; step >>= 1)
-                    if (sh.seg_raw[g + step] <= j) g += step;
-                out[r] = bin_cells[sh.seg_off[g] + j];
-            }
+    for (int h = 0; h < 2; ++h) {
+        const int p = 64 * h + lane;
+        const int r = h ? r1 : r0;
+        if (p < P && r < K) {
+            // the segment of raw index j: the last one starting at or
+            // before j (empty segments start where the next one does)
+            const int j = (int)((h ? el1 : el0) & lmask);
+            int g = 0;
+#pragma unroll
+            for (int step = kMaxSegs / 2; step >= 1; step >>= 1)
+                if (sh.seg_raw[g + step] <= j) g += step;
+            out[r] = bin_cells[sh.seg_off[g] + j];
         }
     }
 }
