@@ -159,9 +159,9 @@ __device__ inline uint32_t load_u32(const uint32_t *base, uint32_t idx) {
     return *reinterpret_cast<const uint32_t *>(reinterpret_cast<const char *>(base) + (idx << 2));
 }
 
-// Number of set bits of the wave-uniform mask m below this lane.
-__device__ inline int bits_below(uint64_t m) {
-    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+// Number of set bits of the wave-uniform mask m below this lane (plus base).
+__device__ inline int bits_below(uint64_t m, int base = 0) {
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, (uint32_t)base));
 }
 
 // std::__unguarded_partition_pivot on e[lo, hi) (e in LDS or in global
@@ -523,7 +523,6 @@ __device__ __forceinline__ void gather_run_(const ClusterArgs &a, WaveLds &sh, i
                                             uint32_t *e0, int cap, uint32_t *ge, uint32_t *gs, int *pos) {
     const int lane = threadIdx.x & 63;
     const BinArgs &b = a.bins;
-    const int lb = a.local_bits;
     const int bx = x / b.r[0], by = y / b.r[1], bz = z / b.r[2];
     const int s2 = a.span[2];
     // lane g < kSegs: segment g's first position in the bin-sorted arrays
@@ -641,11 +640,12 @@ __device__ __forceinline__ void gather_run_(const ClusterArgs &a, WaveLds &sh, i
 #pragma unroll
             for (int g = 0; g < G; ++g) {
                 if (g >= nt) continue;
+                // the table holds the ranks shifted into place
                 const uint32_t rk = rv[u].v[G - 1 - g];
                 const uint64_t mk = ballot(rk != 0);
-                const int pp = pos[g] + bits_below(mk);
+                const int pp = bits_below(mk, pos[g]);
                 if (rk != 0) {
-                    const uint32_t ev = (rk << lb) | j;
+                    const uint32_t ev = rk | j;
                     if (g == 0) {
                         if (pp < cap)
                             e0[pp] = ev;

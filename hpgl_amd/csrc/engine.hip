@@ -437,6 +437,9 @@ struct CovAssets {
     RankTable rank;
     long long rank_positive = 0;  // offsets with rank > 0
     uint32_t max_rank = 0;
+    // the device table's ranks are stored shifted left by the sort elements'
+    // index bits (fixed by the template size); -1 when they do not fit
+    int rank_lb = -1;
     // half-widths of the device table: the box, widened in x by
     // kClusterRunSlack zero columns on each side
     int wh[3] = {0, 0, 0};
@@ -532,6 +535,13 @@ const CovAssets &assets_rank(CovAssets &a, hipStream_t s, Stats &st) {
         a.rank_positive += r != 0;
         a.max_rank = std::max(a.max_rank, r);
     }
+    // elements of the search's sort pack (rank << lb) | raw index, lb the
+    // bits of the raw indices of bins of at most template/250 cells
+    const long long max_raw = cluster_raw_bound((int)(a.tmpl.size() / 250));
+    int lb = 1;
+    while ((1ll << lb) < max_raw) ++lb;
+    a.rank_lb = (lb < 32 && (unsigned long long)a.max_rank < (1ull << (32 - lb))) ? lb : -1;
+    const int shift = std::max(a.rank_lb, 0);
     std::vector<uint32_t> wide((size_t)wd[0] * wd[1] * wd[2] + 2 * kClusterRankPad, 0u);
     const RankTable &rt = a.rank;
     for (int z = 0; z < rt.d[2]; ++z)
@@ -539,7 +549,7 @@ const CovAssets &assets_rank(CovAssets &a, hipStream_t s, Stats &st) {
             const size_t src = ((size_t)z * rt.d[1] + y) * rt.d[0];
             const size_t dst = kClusterRankPad + ((size_t)(z - span[2] + a.wh[2]) * wd[1] + (y - span[1] + a.wh[1])) *
                                                      wd[0] + (size_t)(a.wh[0] - span[0]);
-            std::copy(rt.rank.begin() + src, rt.rank.begin() + src + rt.d[0], wide.begin() + dst);
+            for (int x = 0; x < rt.d[0]; ++x) wide[dst + x] = rt.rank[src + x] << shift;
         }
     st.ms_setup_host += now_ms() - t0;
     upload(a.d_rank, wide, s);
@@ -670,6 +680,7 @@ void search(CovAssets &as, const int grid[3], const int radii[3], int max_n, con
         const uint32_t max_rank = ra.max_rank;
         if (lb >= 32 || (unsigned long long)max_rank >= (1ull << (32 - lb)))
             throw std::runtime_error("cluster search: covariance has too many distinct values for 32-bit sort keys");
+        if (lb != ra.rank_lb) throw std::runtime_error("cluster search: rank table shifted for other bins");
         ca.local_bits = lb;
         // expected list length: informed density x offsets with rank > 0,
         // plus six standard deviations of a Poisson count

@@ -133,7 +133,8 @@ struct ClusterArgs {
     // (launch_pack_lin); then kClusterLinPad zero entries
     const uint2 *bin_lin;
     const uint32_t *zoff;     // per bin, cells below each z layer (launch_bin_zoff)
-    // rank table over the covariance box [-span, span]^3, widened in x by
+    // rank table (ranks shifted left by local_bits) over the covariance box
+    // [-span, span]^3, widened in x by
     // kClusterRunSlack zero entries on each side, after kClusterRankPad zero
     // entries: w = (2 span[0] + 1 + 2 kClusterRunSlack, 2 span[1] + 1,
     // 2 span[2] + 1) entries, offset (dx, dy, dz) at byte
