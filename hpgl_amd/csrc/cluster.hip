@@ -210,10 +210,14 @@ __device__ __forceinline__ int partition_(uint32_t *e, S &sh, int lo, int hi, Ke
         }
         wave_sync();
     }
+    // whole elements against thresholds: key <= kp iff e <= t_le, key >= kp
+    // iff e >= t_ge
+    const uint32_t t_ge = kp << key.lb, t_le = t_ge | ((1u << key.lb) - 1u);
     const int first = lo + 1;
     const int nch = (hi - first + 63) >> 6;
     const int p0 = first + lane;  // this lane's position in chunk 0
     int cl = -1, cr = nch;        // chunks the cursors are in
+    int pcl = p0 - 64, pcr = p0 + (nch << 6);  // this lane's position in them
     uint64_t lrem = 0, rrem = 0;  // their stops not yet paired
     int r_prev = 0x7fffffff;      // R_{s-1} of the rounds before
     for (;;) {
@@ -221,16 +225,15 @@ __device__ __forceinline__ int partition_(uint32_t *e, S &sh, int lo, int hi, Ke
         // use the stops left in the current chunk, then enter the next one
         int fl = 0;
         for (;;) {
-            const int idx = fl + bits_below(lrem);
+            const int idx = bits_below(lrem, fl);
             const uint64_t took = lrem & ballot((unsigned)idx < 64u);
-            sh.pairs[lane_select(took, idx, 128 + lane)] = p0 + (cl << 6);
+            sh.pairs[lane_select(took, idx, 128 + lane)] = pcl;
             lrem ^= took;
             fl += __popcll(took);
             if (fl == 64 || cl + 1 >= nch) break;
             ++cl;
-            const int p = p0 + (cl << 6);
-            const uint32_t k = key(e[min(p, hi - 1)]);
-            lrem = ballot(k <= kp) & ballot(p < hi);
+            pcl += 64;
+            lrem = ballot(e[min(pcl, hi - 1)] <= t_le) & ballot(pcl < hi);
         }
         // right stops R_{s} .. R_{s+63} into pairs[64, 128), in descending
         // position order
@@ -238,14 +241,13 @@ __device__ __forceinline__ int partition_(uint32_t *e, S &sh, int lo, int hi, Ke
         for (;;) {
             const int idx = fr + __popcll(rrem) - 1 - bits_below(rrem);
             const uint64_t took = rrem & ballot((unsigned)idx < 64u);
-            sh.pairs[lane_select(took, 64 + idx, 128 + lane)] = p0 + (cr << 6);
+            sh.pairs[lane_select(took, 64 + idx, 128 + lane)] = pcr;
             rrem ^= took;
             fr += __popcll(took);
             if (fr == 64 || cr <= 0) break;
             --cr;
-            const int p = p0 + (cr << 6);
-            const uint32_t k = key(e[min(p, hi - 1)]);
-            rrem = ballot(k >= kp) & ballot(p < hi);
+            pcr -= 64;
+            rrem = ballot(e[min(pcr, hi - 1)] >= t_ge) & ballot(pcr < hi);
         }
         wave_sync();
         const int m = min(fl, fr);
@@ -307,10 +309,9 @@ __device__ __forceinline__ bool small_sort_(uint32_t *e, S &sh, int lo, int hi, 
             }
             --dd;
             // __move_median_to_first(l, l+1, mid, h-1)
-            const uint32_t kv = key(v);
             const int a = l + 1, bm = l + (h - l) / 2, c = h - 1;
-            const uint32_t ka = __builtin_amdgcn_readlane(kv, a), kb = __builtin_amdgcn_readlane(kv, bm),
-                           kc = __builtin_amdgcn_readlane(kv, c);
+            const uint32_t ka = key(__builtin_amdgcn_readlane(v, a)), kb = key(__builtin_amdgcn_readlane(v, bm)),
+                           kc = key(__builtin_amdgcn_readlane(v, c));
             int sw;
             if (ka > kb) {
                 if (kb > kc)
@@ -329,10 +330,11 @@ __device__ __forceinline__ bool small_sort_(uint32_t *e, S &sh, int lo, int hi, 
             const uint32_t kp = sw == a ? ka : sw == bm ? kb : kc;
             const uint32_t vl = __builtin_amdgcn_readlane(v, l), vs = __builtin_amdgcn_readlane(v, sw);
             v = lane == l ? vs : lane == sw ? vl : v;
-            const uint32_t k = key(v);
-            // left stops key <= kp, right stops key >= kp, in [l + 1, h)
+            // left stops key <= kp, right stops key >= kp, in [l + 1, h),
+            // by whole elements against thresholds
+            const uint32_t t_ge = kp << key.lb, t_le = t_ge | ((1u << key.lb) - 1u);
             const uint64_t range = (h >= 64 ? ~0ull : (1ull << h) - 1) & ~((1ull << (l + 1)) - 1);
-            const uint64_t lm = ballot(k <= kp) & range, rm = ballot(k >= kp) & range;
+            const uint64_t lm = ballot(v <= t_le) & range, rm = ballot(v >= t_ge) & range;
             const int nl = __popcll(lm), nr = __popcll(rm);
             const int bl = bits_below(lm), br = bits_below(rm);
             const bool in_l = (lm >> lane) & 1u, in_r = (rm >> lane) & 1u;
@@ -429,12 +431,14 @@ __device__ void finish_(E *e, int n, int K, int *out, const S &sh, const int *bi
         if (two && 64 + lane < P) e[64 + lane] = c1;
         wave_sync();
         if (two) {
+#pragma unroll 8
             for (int q = 0; q < P; ++q) {
                 const uint32_t cq = e[q];
                 r0 += cq > c0;
                 r1 += cq > c1;
             }
         } else {
+#pragma unroll 8
             for (int q = 0; q < P; ++q) r0 += e[q] > c0;
         }
     } else {
