@@ -522,7 +522,7 @@ typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 // to cap, then the spill scratch ge, if any), target g > 0 at
 // gs + (g - 1) cap (global scratch, up to cap); returns the list lengths in
 // pos[0 .. nt).
-template <int G>
+template <int G, bool SPILLED>
 __device__ __forceinline__ void gather_run_(const ClusterArgs &a, WaveLds &sh, int x, int y, int z, int nt,
                                             uint32_t *e0, int cap, uint32_t *ge, uint32_t *gs, int *pos) {
     const int lane = threadIdx.x & 63;
@@ -648,17 +648,23 @@ __device__ __forceinline__ void gather_run_(const ClusterArgs &a, WaveLds &sh, i
                 const uint32_t rk = rv[u].v[G - 1 - g];
                 const uint64_t mk = ballot(rk != 0);
                 const int pp = bits_below(mk, pos[g]);
-                if (rk != 0) {
-                    const uint32_t ev = rk | j;
+                const uint32_t ev = rk | j;
+                if (SPILLED) {
+                    if (rk != 0 && pp < a.scratch_per_wave) ge[pp] = ev;
+                } else {
+                    // every lane stores: lanes that keep nothing, or past the
+                    // capacity (their list is spilled), store to their own
+                    // dead slot, so the stores need no branch
+                    const bool keep = rk != 0 && pp < cap;
                     if (g == 0) {
-                        if (pp < cap)
-                            e0[pp] = ev;
-                        else if (ge && pp < a.scratch_per_wave)
-                            ge[pp] = ev;
-                    } else if (pp < cap) {
-                        // 32-bit byte offset from the wavefront's scratch base
-                        *reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(gs) +
-                                                      (((uint32_t)((g - 1) * cap) + (uint32_t)pp) << 2)) = ev;
+                        uint32_t *dead = reinterpret_cast<uint32_t *>(sh.pairs) + 128 + lane;
+                        *(keep ? e0 + pp : dead) = ev;
+                    } else {
+                        // 32-bit byte offsets from the wavefront's scratch
+                        // base; the dead slots follow its lists
+                        const uint32_t w = keep ? (uint32_t)((g - 1) * cap + pp)
+                                                : (uint32_t)(kClusterRunSlack * cap + lane);
+                        *reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(gs) + (w << 2)) = ev;
                     }
                 }
                 pos[g] += __popcll(mk);
@@ -699,7 +705,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WAVES8 ?
     // the later members' lists of a run; a wavefront without a scratch slot
     // takes its targets one at a time
     uint32_t *const gs = !SPILLED && G > 1 && wave_id < a.run_slots
-                             ? a.run_scratch + (size_t)wave_id * (G - 1) * (size_t)cap
+                             ? a.run_scratch + (size_t)wave_id * cluster_run_words(cap)
                              : nullptr;
     const int lane = threadIdx.x & 63;
     const int X = a.X, Y = a.Y;
@@ -730,9 +736,9 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WAVES8 ?
             }
             int pos[G];
             if (G > 1 && nt > 1)
-                gather_run_<G>(a, sh, x, y, z, nt, e0, cap, ge, gs, pos);
+                gather_run_<G, SPILLED>(a, sh, x, y, z, nt, e0, cap, ge, gs, pos);
             else
-                gather_run_<1>(a, sh, x, y, z, 1, e0, cap, ge, gs, pos);
+                gather_run_<1, SPILLED>(a, sh, x, y, z, 1, e0, cap, ge, gs, pos);
             wave_sync();
             for (int g = 0; g < nt; ++g) {
                 const int t = __builtin_amdgcn_readlane(my_t, r0 + g);

@@ -695,7 +695,7 @@ void search(CovAssets &as, const int grid[3], const int radii[3], int max_n, con
         ca.run_slots = 0;
         ca.run_scratch = nullptr;
         const long long slots = cluster_main_waves(ca);
-        DBuf<uint32_t> run_scratch((size_t)slots * kClusterRunSlack * (size_t)ca.cap);
+        DBuf<uint32_t> run_scratch((size_t)slots * cluster_run_words(ca.cap));
         ca.run_scratch = run_scratch.p;
         ca.run_slots = (int)slots;
         DBuf<uint32_t> scratch;

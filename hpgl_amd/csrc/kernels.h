@@ -158,7 +158,7 @@ struct ClusterArgs {
     int per_wave;
     int *work;
     // runs of neighbouring targets: the first member's list is gathered
-    // into LDS, the others' into run_scratch, (kClusterRunSlack) cap words
+    // into LDS, the others' into run_scratch, cluster_run_words(cap) words
     // per main-pass wavefront, run_slots wavefronts (cluster_main_waves)
     uint32_t *run_scratch;
     int run_slots;
@@ -189,6 +189,9 @@ constexpr int kClusterWorkStride = 16;
 // Zero rank-table columns on either side of the box in x: a run of up to
 // kClusterRunSlack + 1 targets reads its members' entries together.
 constexpr int kClusterRunSlack = 3;
+// Run scratch words per wavefront: the later members' lists, then one dead
+// slot per lane for the stores of lanes that keep nothing.
+__host__ __device__ inline size_t cluster_run_words(int cap) { return (size_t)kClusterRunSlack * cap + 64; }
 // Entries after the bin-sorted rank-table coordinates: a gather pass reads
 // up to 255 positions past the last candidate of a target.
 constexpr int kClusterLinPad = 256;
