@@ -652,19 +652,18 @@ __device__ __forceinline__ void gather_run_(const ClusterArgs &a, WaveLds &sh, i
                 if (SPILLED) {
                     if (rk != 0 && pp < a.scratch_per_wave) ge[pp] = ev;
                 } else {
-                    // every lane stores: lanes that keep nothing, or past the
-                    // capacity (their list is spilled), store to their own
-                    // dead slot, so the stores need no branch
+                    // every lane stores to LDS: lanes that keep nothing, or
+                    // past the capacity (their list is spilled), store to
+                    // their own dead slot, so the store needs no branch; the
+                    // scratch stores stay masked (fewer lanes, fewer lines)
                     const bool keep = rk != 0 && pp < cap;
                     if (g == 0) {
                         uint32_t *dead = reinterpret_cast<uint32_t *>(sh.pairs) + 128 + lane;
                         *(keep ? e0 + pp : dead) = ev;
-                    } else {
-                        // 32-bit byte offsets from the wavefront's scratch
-                        // base; the dead slots follow its lists
-                        const uint32_t w = keep ? (uint32_t)((g - 1) * cap + pp)
-                                                : (uint32_t)(kClusterRunSlack * cap + lane);
-                        *reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(gs) + (w << 2)) = ev;
+                    } else if (keep) {
+                        // 32-bit byte offsets from the wavefront's scratch base
+                        *reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(gs) +
+                                                      (((uint32_t)((g - 1) * cap) + (uint32_t)pp) << 2)) = ev;
                     }
                 }
                 pos[g] += __popcll(mk);

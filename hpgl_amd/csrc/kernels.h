@@ -189,9 +189,8 @@ constexpr int kClusterWorkStride = 16;
 // Zero rank-table columns on either side of the box in x: a run of up to
 // kClusterRunSlack + 1 targets reads its members' entries together.
 constexpr int kClusterRunSlack = 3;
-// Run scratch words per wavefront: the later members' lists, then one dead
-// slot per lane for the stores of lanes that keep nothing.
-__host__ __device__ inline size_t cluster_run_words(int cap) { return (size_t)kClusterRunSlack * cap + 64; }
+// Run scratch words per wavefront: the later members' lists.
+__host__ __device__ inline size_t cluster_run_words(int cap) { return (size_t)kClusterRunSlack * cap; }
 // Entries after the bin-sorted rank-table coordinates: a gather pass reads
 // up to 255 positions past the last candidate of a target.
 constexpr int kClusterLinPad = 256;
