@@ -389,10 +389,11 @@ int solve_group(int max_n) {
     return g;
 }
 
-DevCov make_devcov(const CovModel &m, const BoxTable *table, const double *d_table) {
+DevCov make_devcov(const CovModel &m, const BoxTable *table, const double *d_table, const double *d_fold = nullptr) {
     DevCov c;
     std::memset(&c, 0, sizeof(c));
     c.table = table ? d_table : nullptr;
+    c.fold = table ? d_fold : nullptr;
     for (int i = 0; i < 3; ++i) {
         c.tr[i] = table ? table->r[i] : 0;
         c.td[i] = table ? table->d[i] : 1;
@@ -431,6 +432,8 @@ struct CovAssets {
     BoxTable table;
     Template tmpl;
     DBuf<double> d_table;
+    std::vector<double> fold;  // the table's octant when it is symmetric (BoxTable::fold)
+    DBuf<double> d_fold;
     DBuf<int32_t> d_tmpl;
     std::mutex lazy;  // guards the members below
     bool have_rank = false;
@@ -447,6 +450,7 @@ struct CovAssets {
     bool have_pair = false;
     BoxTable pair;
     DBuf<double> d_pair;
+    DBuf<double> d_pair_fold;
     explicit CovAssets(const CovSpec &c) : model(c) {}
     // Jobs on other streams may have read these buffers after the last
     // holder's stream did: wait for the device before they are freed.
@@ -497,8 +501,12 @@ std::shared_ptr<CovAssets> get_assets(const CovSpec &cov, const int radii[3], bo
     const double t0 = now_ms();
     if (use_table) e->table.build(e->model, radii);
     build_template(e->model, use_table ? &e->table : nullptr, radii, e->tmpl);
+    const bool folds = use_table && e->table.fold(e->fold);
     st.ms_setup_host += now_ms() - t0;
-    if (use_table) upload(e->d_table, e->table.v, s);
+    if (use_table) {
+        upload(e->d_table, e->table.v, s);
+        if (folds) upload(e->d_fold, e->fold, s);
+    }
     upload(e->d_tmpl, e->tmpl.xyz, s);
     hip_check(hipStreamSynchronize(s), "sync");
     std::lock_guard<std::mutex> g(c.mu);
@@ -954,12 +962,15 @@ DevCov analytic_cov(CovAssets &as, const int radii[3], hipStream_t s, Stats &st)
     if (!as.have_pair) {
         const double t0 = now_ms();
         as.pair.build(as.model, pr);
+        std::vector<double> oct;
+        const bool folds = as.pair.fold(oct);
         st.ms_setup_host += now_ms() - t0;
         upload(as.d_pair, as.pair.v, s);
+        if (folds) upload(as.d_pair_fold, oct, s);
         hip_check(hipStreamSynchronize(s), "sync");
         as.have_pair = true;
     }
-    return make_devcov(as.model, &as.pair, as.d_pair.p);
+    return make_devcov(as.model, &as.pair, as.d_pair.p, as.d_pair_fold.p);
 }
 
 void run_cont(const ContJob &job, hipStream_t s, int precision, const Dump *dump, Stats &st) {
@@ -1002,7 +1013,7 @@ void run_cont(const ContJob &job, hipStream_t s, int precision, const Dump *dump
     if (job.ordinary) {
         sa.cov = analytic_cov(*as, job.radii, s, st);
     } else {
-        sa.cov = make_devcov(as->model, &as->table, as->d_table.p);
+        sa.cov = make_devcov(as->model, &as->table, as->d_table.p, as->d_fold.p);
     }
     sa.ok_system = job.ordinary ? 1 : 0;
     sa.values_f = job.d_in;
@@ -1074,7 +1085,7 @@ void run_ind(const IndJob &job, hipStream_t s, int precision, const Dump *dump, 
         sa.X = job.grid[0];
         sa.Y = job.grid[1];
         sa.Z = job.grid[2];
-        sa.cov = make_devcov(as->model, &as->table, as->d_table.p);
+        sa.cov = make_devcov(as->model, &as->table, as->d_table.p, as->d_fold.p);
         sa.ok_system = 0;
         sa.values_u8 = job.d_in;
         sa.category = job.median ? 1 : k;
@@ -1479,7 +1490,7 @@ void run_sgs(const SgsJob &job, hipStream_t s, int precision, const Dump *dump, 
             sa.X = job.grid[0];
             sa.Y = job.grid[1];
             sa.Z = job.grid[2];
-            sa.cov = make_devcov(as->model, &as->table, as->d_table.p);
+            sa.cov = make_devcov(as->model, &as->table, as->d_table.p, as->d_fold.p);
             sa.ok_system = job.ordinary ? 1 : 0;
             sa.mean_kind = 0;
             sa.out_var = var.p;
@@ -1650,7 +1661,7 @@ void run_sis(const SisJob &job, hipStream_t s, int precision, Stats &st) {
                 sa.X = job.grid[0];
                 sa.Y = job.grid[1];
                 sa.Z = job.grid[2];
-                sa.cov = make_devcov(as[c]->model, &as[c]->table, as[c]->d_table.p);
+                sa.cov = make_devcov(as[c]->model, &as[c]->table, as[c]->d_table.p, as[c]->d_fold.p);
                 sa.ok_system = 0;
                 sa.mean_kind = lvm ? 2 : 0;
                 sa.mean_cells = lvm ? job.d_lvm[c] : nullptr;
@@ -1797,7 +1808,7 @@ void run_cokriging(const CokrigingJob &job, hipStream_t s, const Dump *dump, Sta
         // its box table over the search radii
         as2 = get_assets(job.secondary, job.radii, true, s, st);
         const double c2 = as2->model.c0();
-        a.cross = make_devcov(as2->model, &as2->table, as2->d_table.p);
+        a.cross = make_devcov(as2->model, &as2->table, as2->d_table.p, as2->d_fold.p);
         a.sec_var = c2;
         a.coef = job.correlation * sqrt(c2 / c1);
         a.cross0 = a.coef * c2;

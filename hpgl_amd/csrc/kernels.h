@@ -27,6 +27,11 @@ struct DevCov {
     const double *table;  // nullptr => analytic
     int tr[3];            // table radii
     int td[3];            // table extents 2r+1
+    // the table's non-negative octant, entry (|dx|, |dy|, |dz|) at
+    // (|dz| (tr[1] + 1) + |dy|) (tr[0] + 1) + |dx|, when every entry of the
+    // table equals the octant's bit for bit (models without rotation);
+    // nullptr otherwise
+    const double *fold;
     double t[9];          // transform (row-major)
     double range0;
     double c0;            // sill - nugget

@@ -1,6 +1,9 @@
 // model.cpp - host-side job set-up (see model.h).
 #include "model.h"
 
+#include <cstdlib>
+#include <cstring>
+
 #include <algorithm>
 #include <cmath>
 #include <stdexcept>
@@ -103,6 +106,21 @@ void BoxTable::build(const CovModel &m, const int radii[3]) {
         for (int y = 0; y < d[1]; ++y)
             for (int x = 0; x < d[0]; ++x)
                 v[((size_t)z * d[1] + y) * d[0] + x] = m.at(-(x - r[0]), -(y - r[1]), -(z - r[2]));
+}
+
+bool BoxTable::fold(std::vector<double> &out) const {
+    const size_t f0 = (size_t)r[0] + 1, f1 = (size_t)r[1] + 1, f2 = (size_t)r[2] + 1;
+    for (int z = -r[2]; z <= r[2]; ++z)
+        for (int y = -r[1]; y <= r[1]; ++y)
+            for (int x = -r[0]; x <= r[0]; ++x) {
+                const double a = at(x, y, z), b = at(std::abs(x), std::abs(y), std::abs(z));
+                if (std::memcmp(&a, &b, sizeof(double)) != 0) return false;
+            }
+    out.resize(f0 * f1 * f2);
+    for (size_t z = 0; z < f2; ++z)
+        for (size_t y = 0; y < f1; ++y)
+            for (size_t x = 0; x < f0; ++x) out[(z * f1 + y) * f0 + x] = at((int)x, (int)y, (int)z);
+    return true;
 }
 
 void build_template(const CovModel &m, const BoxTable *table, const int radii[3], Template &out) {

@@ -41,6 +41,10 @@ struct BoxTable {
     int d[3] = {1, 1, 1};  // 2r+1
     std::vector<double> v;  // index ((z+rz)*dy + (y+ry))*dx + (x+rx)
     void build(const CovModel &m, const int radii[3]);
+    // The non-negative octant, index (z*(ry+1) + y)*(rx+1) + x, into out
+    // when every entry equals its octant entry bit for bit (true for models
+    // without rotation); false (out untouched) otherwise.
+    bool fold(std::vector<double> &out) const;
     double at(int dx, int dy, int dz) const {
         if (dx < -r[0] || dx > r[0] || dy < -r[1] || dy > r[1] || dz < -r[2] || dz > r[2]) return 0;
         return v[((size_t)(dz + r[2]) * d[1] + (dy + r[1])) * d[0] + (dx + r[0])];

@@ -102,12 +102,23 @@ __device__ inline int qmaxi(int v) {
 // neighbour pairs can take).  Branch-free: out-of-box offsets read entry 0
 // and are replaced by 0, so the unrolled build loops issue their loads
 // back to back.
+// FOLD: the table's octant (DevCov::fold), an eighth of the lines, so more
+// of a wavefront's reads hit the vector L1.
+template <bool FOLD>
 __device__ inline double cov_at(const DevCov &c, int dx, int dy, int dz) {
-    const bool in = dx >= -c.tr[0] && dx <= c.tr[0] && dy >= -c.tr[1] && dy <= c.tr[1] && dz >= -c.tr[2] &&
-                    dz <= c.tr[2];
-    const int idx = ((dz + c.tr[2]) * c.td[1] + (dy + c.tr[1])) * c.td[0] + (dx + c.tr[0]);
-    const double v = c.table[in ? idx : 0];
-    return in ? v : 0.0;
+    if constexpr (FOLD) {
+        const int ax = abs(dx), ay = abs(dy), az = abs(dz);
+        const bool in = ax <= c.tr[0] && ay <= c.tr[1] && az <= c.tr[2];
+        const int idx = (az * (c.tr[1] + 1) + ay) * (c.tr[0] + 1) + ax;
+        const double v = c.fold[in ? idx : 0];
+        return in ? v : 0.0;
+    } else {
+        const bool in = dx >= -c.tr[0] && dx <= c.tr[0] && dy >= -c.tr[1] && dy <= c.tr[1] && dz >= -c.tr[2] &&
+                        dz <= c.tr[2];
+        const int idx = ((dz + c.tr[2]) * c.td[1] + (dy + c.tr[1])) * c.td[0] + (dx + c.tr[0]);
+        const double v = c.table[in ? idx : 0];
+        return in ? v : 0.0;
+    }
 }
 
 constexpr int off(int m) { return 2 * m * (m + 1); }  // start of my m-th row (row length 4m+4)
@@ -165,17 +176,17 @@ __device__ inline void q_setup(const SolveArgs &a, QShared<N> &sh, QCtx<N> &c) {
 // A(i, j) of the padded system.  Branch-free: padding rows hold
 // coordinates 0 in LDS, so the table read is always in bounds and the
 // result is selected afterwards.
-template <int N>
+template <int N, bool FOLD>
 __device__ inline double q_aij(const DevCov &cv, const QShared<N> &sh, int slot, int n, int i, int j) {
-    const double v = cov_at(cv, sh.xyz[slot][0][j] - sh.xyz[slot][0][i], sh.xyz[slot][1][j] - sh.xyz[slot][1][i],
+    const double v = cov_at<FOLD>(cv, sh.xyz[slot][0][j] - sh.xyz[slot][0][i], sh.xyz[slot][1][j] - sh.xyz[slot][1][i],
                             sh.xyz[slot][2][j] - sh.xyz[slot][2][i]);
     return (i < n && j < n) ? v : (i == j ? 1.0 : 0.0);
 }
 
-template <int N>
+template <int N, bool FOLD>
 __device__ inline double q_bi(const SolveArgs &a, const QShared<N> &sh, const QCtx<N> &c, int i) {
     const double v =
-        cov_at(a.cov, c.cx - sh.xyz[c.slot][0][i], c.cy - sh.xyz[c.slot][1][i], c.cz - sh.xyz[c.slot][2][i]);
+        cov_at<FOLD>(a.cov, c.cx - sh.xyz[c.slot][0][i], c.cy - sh.xyz[c.slot][1][i], c.cz - sh.xyz[c.slot][2][i]);
     return i < c.n ? v : 0.0;
 }
 
@@ -311,7 +322,7 @@ __device__ inline void q_solve32(const float *L, const float *Dinv, int q, float
 template <int N, bool TWO>
 constexpr int kQuadMixedWaves = (N >= 28 || (N >= 20 && TWO)) ? 2 : 1;
 
-template <int N, bool TWO>
+template <int N, bool TWO, bool FOLD>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kQuadMixedWaves<N, TWO>, 8))) void k_solve_quad_mixed(SolveArgs a) {
     const int EPI = a.epi;
     constexpr int M = N / 4;
@@ -327,10 +338,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kQuadMixedWa
 #pragma unroll
     for (int m = 0; m < M; ++m) {
         const int i = 4 * m + q;
-        b[m] = q_bi<N>(a, sh, c, i);
+        b[m] = q_bi<N, FOLD>(a, sh, c, i);
 #pragma unroll
         for (int j = 0; j < 4 * m + 4; ++j) {
-            const double e = q_aij<N>(a.cov, sh, c.slot, c.n, i, j);
+            const double e = q_aij<N, FOLD>(a.cov, sh, c.slot, c.n, i, j);
             const double aij = (j <= i) ? e : 0.0;
             L[off(m) + j] = (float)aij;
             if (j == i && i < c.n) maxdiag = fmax(maxdiag, aij);
@@ -390,7 +401,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kQuadMixedWa
         // the table reads of a step are not kept for the next one (that
         // would hold A in registers): the pointer is opaque per step
         DevCov cv = a.cov;
-        asm volatile("" : "+s"(cv.table));
+        asm volatile("" : "+s"(cv.table), "+s"(cv.fold));
         // r = b - A x in FP64 for my rows; x goes through LDS
 #pragma unroll
         for (int m = 0; m < M; ++m) {
@@ -424,7 +435,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kQuadMixedWa
                 const int i = 4 * m + q;
                 double e[4];
 #pragma unroll
-                for (int cc = 0; cc < 4; ++cc) e[cc] = q_aij<N>(cv, sh, c.slot, c.n, i, 4 * mb + cc);
+                for (int cc = 0; cc < 4; ++cc) e[cc] = q_aij<N, FOLD>(cv, sh, c.slot, c.n, i, 4 * mb + cc);
 #pragma unroll
                 for (int cc = 0; cc < 4; ++cc) {
                     r1[m] = fma(-e[cc], xc1[cc], r1[m]);
@@ -511,7 +522,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kQuadMixedWa
 // ---------------------------------------------------------------------------
 // FP64 in the reference's operation order.
 // ---------------------------------------------------------------------------
-template <int N>
+template <int N, bool FOLD>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(N >= 16 ? 2 : 1, 8))) void k_solve_quad_fp64(SolveArgs a) {
     const int EPI = a.epi;
     constexpr int M = N / 4;
@@ -525,10 +536,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(N >= 16 ? 2 
 #pragma unroll
     for (int m = 0; m < M; ++m) {
         const int i = 4 * m + q;
-        b[m] = q_bi<N>(a, sh, c, i);
+        b[m] = q_bi<N, FOLD>(a, sh, c, i);
 #pragma unroll
         for (int j = 0; j < 4 * m + 4; ++j) {
-            const double e = q_aij<N>(a.cov, sh, c.slot, c.n, i, j);
+            const double e = q_aij<N, FOLD>(a.cov, sh, c.slot, c.n, i, j);
             L[off(m) + j] = (j <= i) ? e : 0.0;
         }
     }
@@ -665,12 +676,23 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(N >= 16 ? 2 
 
 }  // namespace
 
-#define HPGL_Q(K, NN) hipLaunchKernelGGL(K<NN>, grid, dim3(64), 0, s, a)
-#define HPGL_QM(NN)                                                                     \
-    if (a.ok_system)                                                                    \
-        hipLaunchKernelGGL((k_solve_quad_mixed<NN, true>), grid, dim3(64), 0, s, a);   \
-    else                                                                                \
-        hipLaunchKernelGGL((k_solve_quad_mixed<NN, false>), grid, dim3(64), 0, s, a)
+// FOLD when the covariance has a symmetric octant table (DevCov::fold)
+#define HPGL_Q(K, NN)                                                      \
+    if (a.cov.fold)                                                        \
+        hipLaunchKernelGGL((K<NN, true>), grid, dim3(64), 0, s, a);        \
+    else                                                                   \
+        hipLaunchKernelGGL((K<NN, false>), grid, dim3(64), 0, s, a)
+#define HPGL_QM2(NN, F)                                                                    \
+    if (a.ok_system)                                                                       \
+        hipLaunchKernelGGL((k_solve_quad_mixed<NN, true, F>), grid, dim3(64), 0, s, a);   \
+    else                                                                                   \
+        hipLaunchKernelGGL((k_solve_quad_mixed<NN, false, F>), grid, dim3(64), 0, s, a)
+#define HPGL_QM(NN)            \
+    if (a.cov.fold) {          \
+        HPGL_QM2(NN, true);    \
+    } else {                   \
+        HPGL_QM2(NN, false);   \
+    }
 
 void launch_solve_quad(const SolveArgs &a, int order, bool fp64, hipStream_t s) {
     const int items = a.subset ? a.n_subset : a.T;
