@@ -66,4 +66,14 @@ __device__ inline double cov_tab(const DevCov &c, int dx, int dy, int dz) {
     return in ? v : 0.0;
 }
 
+// The same from the table's octant (DevCov::fold), for tables that equal it
+// bit for bit.
+__device__ inline double cov_fold(const DevCov &c, int dx, int dy, int dz) {
+    const int ax = abs(dx), ay = abs(dy), az = abs(dz);
+    const bool in = ax <= c.tr[0] && ay <= c.tr[1] && az <= c.tr[2];
+    const int idx = (az * (c.tr[1] + 1) + ay) * (c.tr[0] + 1) + ax;
+    const double v = c.fold[in ? idx : 0];
+    return in ? v : 0.0;
+}
+
 }  // namespace hpgl_amd

@@ -318,7 +318,8 @@ struct BcBlock {
     }
 };
 
-template <int NB>
+// FOLD: table reads from the octant (DevCov::fold), an eighth of the lines.
+template <int NB, bool FOLD>
 __global__ __launch_bounds__(64) void k_solve_fp64_bc(SolveArgs a) {
     constexpr int G = 64;
     constexpr int M = 8 * NB;
@@ -339,7 +340,9 @@ __global__ __launch_bounds__(64) void k_solve_fp64_bc(SolveArgs a) {
 
     // b (my_kriging_weights.h:156-165) on lane i = l
     double bi;
-    if (a.cov.table)
+    if (FOLD)
+        bi = (l < n) ? cov_fold(a.cov, cx - xi, cy - yi, cz - zi) : 0.0;
+    else if (a.cov.table)
         bi = (l < n) ? cov_tab(a.cov, cx - xi, cy - yi, cz - zi) : 0.0;
     else
         bi = (l < n) ? cov_eval(a.cov, cx - xi, cy - yi, cz - zi) : 0.0;
@@ -366,7 +369,8 @@ __global__ __launch_bounds__(64) void k_solve_fp64_bc(SolveArgs a) {
                 const int i = r + 8 * ab, j = c + 8 * bb;
                 const int q = ab * (ab + 1) / 2 + bb;
                 if (i >= j && i < n && j < n)
-                    e[q] = cov_tab(a.cov, xc[0][bb] - xr[0][ab], xc[1][bb] - xr[1][ab], xc[2][bb] - xr[2][ab]);
+                    e[q] = FOLD ? cov_fold(a.cov, xc[0][bb] - xr[0][ab], xc[1][bb] - xr[1][ab], xc[2][bb] - xr[2][ab])
+                                : cov_tab(a.cov, xc[0][bb] - xr[0][ab], xc[1][bb] - xr[1][ab], xc[2][bb] - xr[2][ab]);
                 else
                     e[q] = (i == j) ? 1.0 : 0.0;
             }
@@ -493,10 +497,18 @@ void launch_solve_fp64(const SolveArgs &a, int group, hipStream_t s) {
     if (items <= 0) return;
     // one system per wavefront
     const dim3 grid(items);
-    if (group <= 32)
-        hipLaunchKernelGGL(k_solve_fp64_bc<4>, grid, dim3(64), 0, s, a);
-    else
-        hipLaunchKernelGGL(k_solve_fp64_bc<8>, grid, dim3(64), 0, s, a);
+    const bool fold = a.cov.table && a.cov.fold;
+    if (group <= 32) {
+        if (fold)
+            hipLaunchKernelGGL((k_solve_fp64_bc<4, true>), grid, dim3(64), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_solve_fp64_bc<4, false>), grid, dim3(64), 0, s, a);
+    } else {
+        if (fold)
+            hipLaunchKernelGGL((k_solve_fp64_bc<8, true>), grid, dim3(64), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_solve_fp64_bc<8, false>), grid, dim3(64), 0, s, a);
+    }
 }
 
 }  // namespace hpgl_amd
