@@ -104,6 +104,23 @@ def err_stats(got, want, floor):
             "floor": floor}
 
 
+def output_rounding(got, want):
+    """How far float32 outputs of the mixed path sit from the FP64 path's:
+    cells whose floats differ, the largest difference in absolute terms, in
+    units of the data's standard deviation (1 for the standard-normal
+    workloads) and in float32 spacings of the FP64 path's value.  Weights
+    that differ in their last digits flip the rounding of single float32
+    terms of the combine sum (as the reference forms it), so a difference of
+    a few spacings is rounding, not solve error; relative to a small floor
+    such a flip on an estimate near zero reads large."""
+    got = np.asarray(got, np.float32).ravel()
+    want = np.asarray(want, np.float32).ravel()
+    d = np.abs(got.astype(np.float64) - want.astype(np.float64))
+    sp = np.spacing(np.maximum(np.abs(want), np.float32(1e-30))).astype(np.float64)
+    return {"cells_differing": int(np.count_nonzero(got != want)), "max_abs_err": float(d.max()) if d.size else 0.0,
+            "max_err_in_float_spacings": float((d / sp).max()) if d.size else 0.0}
+
+
 def make_cfg2(seed=1002):
     from cases import Cov, Grid, categories, scattered
 
@@ -371,7 +388,8 @@ def cfg3_bench(torch, lib, abi, hpgl_wrap, steps=1, cpu=True, budget_s=5.0, runs
         wall = (time.perf_counter() - t0) / steps
         st = sts[-1]
         res[prec] = {"value": n_unk / wall, "unit": "kriged nodes/s", "ms_per_job": wall * 1e3,
-                     "solves": {"fp32_accepted": int(st["solves_fp32"]), "fp64": int(st["solves_fp64"]),
+                     "solves": {"fp32_accepted": int(st["solves_fp32"]), "fp32_two_steps": int(st["solves_fp32_two_steps"]),
+                                "fp64": int(st["solves_fp64"]),
                                 "singular": int(st["nodes_singular"]),
                                 "no_neighbours": int(st["nodes_no_neighbours"])},
                      "kernel_ms": {k: round(st[k], 3) for k in ("ms_k_template", "ms_k_cluster", "ms_k_solve_mixed",
@@ -454,7 +472,8 @@ def cfg1_bench(torch, lib, abi, hpgl_wrap, steps=5, cpu=True, runs=3):
         wall = (time.perf_counter() - t0) / steps
         st = hpgl_wrap.last_stats()
         res[prec] = {"value": n_unk / wall, "unit": "kriged nodes/s", "ms_per_job": wall * 1e3,
-                     "solves": {"fp32_accepted": int(st["solves_fp32"]), "fp64": int(st["solves_fp64"]),
+                     "solves": {"fp32_accepted": int(st["solves_fp32"]), "fp32_two_steps": int(st["solves_fp32_two_steps"]),
+                                "fp64": int(st["solves_fp64"]),
                                 "singular": int(st["nodes_singular"])}}
         outs[prec] = (d_out.cpu().numpy(), d_om.cpu().numpy())
     hpgl_wrap.set_precision("mixed")
@@ -743,7 +762,8 @@ def sgs_bench(torch, lib, abi, hpgl_wrap, R=64, cpu=True):
            "simulated_cells": n_sim, "value": n_sim / wall, "unit": "simulated cells/s",
            "ms_per_call": wall * 1e3, "ms_per_realisation": wall * 1e3 / R,
            "ms_template_scan": st["ms_k_template"], "ms_solve": st["ms_solve"],
-           "solves": {"fp32_accepted": int(st["solves_fp32"]), "fp64": int(st["solves_fp64"])}}
+           "solves": {"fp32_accepted": int(st["solves_fp32"]), "fp32_two_steps": int(st["solves_fp32_two_steps"]),
+                      "fp64": int(st["solves_fp64"])}}
     if cpu:
         info, arr = cpu_leg({"leg": "sgs"})
         P = info["batch_realisations"]
@@ -1154,7 +1174,8 @@ def main():
         "roofline_solve": roofline_solve,
         "roofline_l1": roofline_l1,
         "kernel_ms_per_step": {k: round(acc.get(k, 0.0) / args.steps, 3) for k in kernel_keys},
-        "solves": {"fp32_accepted": int(acc.get("solves_fp32", 0)), "fp64": int(acc.get("solves_fp64", 0)),
+        "solves": {"fp32_accepted": int(acc.get("solves_fp32", 0)),
+                   "fp32_two_steps": int(acc.get("solves_fp32_two_steps", 0)), "fp64": int(acc.get("solves_fp64", 0)),
                    "singular": int(acc.get("nodes_singular", 0)),
                    "no_neighbours": int(acc.get("nodes_no_neighbours", 0))},
         "paths": {"template": int(acc.get("nodes_template_path", 0)), "cluster": int(acc.get("nodes_cluster_path", 0)),
@@ -1187,6 +1208,7 @@ def main():
                                "sk_masks_equal": bool(np.array_equal(gpu_out["sk_mixed_mask"],
                                                                      gpu_out["sk_fp64_mask"])),
                                "ik_category_agreement": float(np.mean(gpu_out["ik_mixed"] == gpu_out["ik_fp64"]))}
+        out["fp64_compare"].update(output_rounding(gpu_out["sk_mixed"][m], gpu_out["sk_fp64"][m]))
     if not args.no_host_path:
         # the drop-in path a geo_bsd user calls: host arrays, PCIe copies,
         # the front end's clone of the input

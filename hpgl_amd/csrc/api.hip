@@ -286,6 +286,7 @@ API void hpgl_amd_get_last_stats(hpgl_amd_stats_t *o) {
     o->cluster_candidates = t_stats.cluster_candidates;
     o->neighbours_total = t_stats.neighbours_total;
     o->cluster_lists_spilled = t_stats.cluster_lists_spilled;
+    o->solves_fp32_two_steps = t_stats.solves_fp32_two_steps;
 }
 
 API void hpgl_amd_set_neighbour_dump(int *indices, int *counts, long long n_cells, int max_n) {

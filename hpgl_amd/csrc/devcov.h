@@ -67,13 +67,10 @@ __device__ inline double cov_tab(const DevCov &c, int dx, int dy, int dz) {
 }
 
 // The same from the table's octant (DevCov::fold), for tables that equal it
-// bit for bit.
+// bit for bit: offsets outside the box clamp into the octant's zero layer.
 __device__ inline double cov_fold(const DevCov &c, int dx, int dy, int dz) {
-    const int ax = abs(dx), ay = abs(dy), az = abs(dz);
-    const bool in = ax <= c.tr[0] && ay <= c.tr[1] && az <= c.tr[2];
-    const int idx = (az * (c.tr[1] + 1) + ay) * (c.tr[0] + 1) + ax;
-    const double v = c.fold[in ? idx : 0];
-    return in ? v : 0.0;
+    const int ax = min(abs(dx), c.tr[0] + 1), ay = min(abs(dy), c.tr[1] + 1), az = min(abs(dz), c.tr[2] + 1);
+    return c.fold[(az * (c.tr[1] + 2) + ay) * (c.tr[0] + 2) + ax];
 }
 
 }  // namespace hpgl_amd

@@ -18,6 +18,7 @@ struct Stats {
     long long nodes_singular = 0;
     long long solves_fp32 = 0;
     long long solves_fp64 = 0;
+    long long solves_fp32_two_steps = 0;
     long long nodes_template_path = 0;
     long long nodes_cluster_path = 0;
     double ms_setup_host = 0;

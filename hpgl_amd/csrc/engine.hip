@@ -914,11 +914,12 @@ void collect_counters(const unsigned long long *d_counters, hipStream_t s, Stats
     hip_check(hipMemcpyAsync(h.data(), d_counters, kCounterWords * sizeof(unsigned long long), hipMemcpyDeviceToHost, s),
               "counters");
     hip_check(hipStreamSynchronize(s), "sync");
-    unsigned long long c[4] = {0, 0, 0, 0};
+    unsigned long long c[5] = {0, 0, 0, 0, 0};
     for (int l = 0; l < kCounterSlots; ++l)
-        for (int w = 0; w < 4; ++w) c[w] += h[(size_t)l * kCounterStride + w];
+        for (int w = 0; w < 5; ++w) c[w] += h[(size_t)l * kCounterStride + w];
     st.solves_fp32 += c[0];
     st.solves_fp64 += c[1];
+    st.solves_fp32_two_steps += c[4];
     st.nodes_kriged += c[0] + c[1];
     st.nodes_singular += c[2];
     st.nodes_no_neighbours += c[3];

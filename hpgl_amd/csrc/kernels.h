@@ -27,10 +27,11 @@ struct DevCov {
     const double *table;  // nullptr => analytic
     int tr[3];            // table radii
     int td[3];            // table extents 2r+1
-    // the table's non-negative octant, entry (|dx|, |dy|, |dz|) at
-    // (|dz| (tr[1] + 1) + |dy|) (tr[0] + 1) + |dx|, when every entry of the
-    // table equals the octant's bit for bit (models without rotation);
-    // nullptr otherwise
+    // the table's non-negative octant with a layer of zeros appended in each
+    // axis (BoxTable::fold), entry (ax, ay, az) at (az (tr[1] + 2) + ay)
+    // (tr[0] + 2) + ax for ax = min(|dx|, tr[0] + 1) and so on, when every
+    // entry of the table equals the octant's bit for bit (models without
+    // rotation); nullptr otherwise
     const double *fold;
     double t[9];          // transform (row-major)
     double range0;

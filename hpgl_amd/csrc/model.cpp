@@ -109,7 +109,7 @@ void BoxTable::build(const CovModel &m, const int radii[3]) {
 }
 
 bool BoxTable::fold(std::vector<double> &out) const {
-    const size_t f0 = (size_t)r[0] + 1, f1 = (size_t)r[1] + 1, f2 = (size_t)r[2] + 1;
+    const size_t f0 = (size_t)r[0] + 2, f1 = (size_t)r[1] + 2, f2 = (size_t)r[2] + 2;
     for (int z = -r[2]; z <= r[2]; ++z)
         for (int y = -r[1]; y <= r[1]; ++y)
             for (int x = -r[0]; x <= r[0]; ++x) {

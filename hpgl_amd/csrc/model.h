@@ -41,9 +41,12 @@ struct BoxTable {
     int d[3] = {1, 1, 1};  // 2r+1
     std::vector<double> v;  // index ((z+rz)*dy + (y+ry))*dx + (x+rx)
     void build(const CovModel &m, const int radii[3]);
-    // The non-negative octant, index (z*(ry+1) + y)*(rx+1) + x, into out
-    // when every entry equals its octant entry bit for bit (true for models
-    // without rotation); false (out untouched) otherwise.
+    // The non-negative octant with one layer of zeros appended in each axis
+    // (x = rx+1, y = ry+1, z = rz+1), index (z*(ry+2) + y)*(rx+2) + x, into
+    // out when every entry equals its octant entry bit for bit (true for
+    // models without rotation); false (out untouched) otherwise.  An offset
+    // whose components are clamped to r+1 then reads its covariance, 0
+    // outside the box, with no test.
     bool fold(std::vector<double> &out) const;
     double at(int dx, int dy, int dz) const {
         if (dx < -r[0] || dx > r[0] || dy < -r[1] || dy > r[1] || dz < -r[2] || dz > r[2]) return 0;

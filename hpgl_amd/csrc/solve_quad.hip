@@ -10,12 +10,27 @@
 //
 // k_solve_quad_mixed: FP32 right-looking Cholesky, forward and backward
 //   substitution, one or two refinement steps whose residual b - A x is
-//   formed in FP64 with A re-evaluated from the covariance (table lookup or
-//   analytic model).  A system is handed to the FP64 kernel when a pivot is
-//   not positive, when the pivot-based condition estimate max(A_kk) /
-//   min(pivot) exceeds 1e4 (then FP32 rounding could decide the sign of a
-//   pivot the FP64 factorisation sees), or when the last correction is
-//   larger than 1e-9 of the solution.
+//   formed in FP64 with A re-evaluated from the covariance table.  A system
+//   is handed to the FP64 kernel when a pivot is not positive, when the
+//   pivot-based condition estimate max(A_kk) / min(pivot) exceeds 1e4 (then
+//   FP32 rounding could decide the sign of a pivot the FP64 factorisation
+//   sees), or when its solution is certified by neither test below.
+//   One step is enough when the error bound of iterative refinement
+//   certifies it: each step contracts the error by rho <= c n u kappa(A)
+//   (u = 2^-24; Higham, Accuracy and Stability of Numerical Algorithms,
+//   2nd ed., sections 10.1 and 12.1), so after the first correction d0,
+//   |x - x1| <= rho / (1 - rho) |d0|.  The kernel estimates kappa as
+//   ||A||_F ||A^-1||_2 (the second from the FP32 factor, below), takes
+//   rho = n u kappa and accepts x1 when rho <= 1/2 and 2 rho |d0| / |x1| <=
+//   5e-8.  It does so only where the outputs are categories (indicator
+//   kriging's argmax, median IK's threshold): a continuous estimate is stored
+//   as float32 from a float sum of w_i (v_i - m_i) terms, and weights that
+//   differ from the FP64 path's in their twelfth digit flip the rounding of
+//   single terms (on 8.3 M simple-kriging nodes: 150 floats differ from the
+//   FP64 path's after one step, 1 after two).  A wavefront with a system
+//   that is neither certified nor corrected below 1e-9 of its solution runs
+//   a second step for all its systems, and a system is accepted after it
+//   when the second correction is at most 1e-9 of the solution.
 // k_solve_quad_fp64: the reference's dpotf2 + dpotrs operation order
 //   (sequential dot products, reciprocal scaling, division in the
 //   triangular solves, no fused multiply-add; the translation unit is
@@ -103,16 +118,28 @@ __device__ inline int qmaxi(int v) {
 // and are replaced by 0, so the unrolled build loops issue their loads
 // back to back.
 // FOLD: the table's octant (DevCov::fold), an eighth of the lines, so more
-// of a wavefront's reads hit the vector L1.
+// of a wavefront's reads hit the vector L1.  Its coordinates arrive scaled
+// by the octant's strides (q_setup), so the entry of the offset between two
+// points is the sum of their clamped absolute coordinate differences (an
+// offset outside the box clamps into the octant's zero layer): no test, no
+// multiply.
+// (v_sad_u32 gives |a - b| in one instruction; the compiler forms it only
+// from a sum with a third operand)
+__device__ inline unsigned absdiff_clamp(unsigned a, unsigned b, unsigned lim) {
+    unsigned d;
+    asm("v_sad_u32 %0, %1, %2, 0" : "=v"(d) : "v"(a), "v"(b));
+    return min(d, lim);
+}
+// covariance of the offset from point i to point j
 template <bool FOLD>
-__device__ inline double cov_at(const DevCov &c, int dx, int dy, int dz) {
+__device__ inline double cov_at(const DevCov &c, int xi, int yi, int zi, int xj, int yj, int zj) {
     if constexpr (FOLD) {
-        const int ax = abs(dx), ay = abs(dy), az = abs(dz);
-        const bool in = ax <= c.tr[0] && ay <= c.tr[1] && az <= c.tr[2];
-        const int idx = (az * (c.tr[1] + 1) + ay) * (c.tr[0] + 1) + ax;
-        const double v = c.fold[in ? idx : 0];
-        return in ? v : 0.0;
+        const unsigned sy = c.tr[0] + 2, sz = sy * (c.tr[1] + 2);
+        const unsigned idx = absdiff_clamp(xi, xj, c.tr[0] + 1) + absdiff_clamp(yi, yj, (c.tr[1] + 1) * sy) +
+                             absdiff_clamp(zi, zj, (c.tr[2] + 1) * sz);
+        return c.fold[idx];
     } else {
+        const int dx = xj - xi, dy = yj - yi, dz = zj - zi;
         const bool in = dx >= -c.tr[0] && dx <= c.tr[0] && dy >= -c.tr[1] && dy <= c.tr[1] && dz >= -c.tr[2] &&
                         dz <= c.tr[2];
         const int idx = ((dz + c.tr[2]) * c.td[1] + (dy + c.tr[1])) * c.td[0] + (dx + c.tr[0]);
@@ -141,8 +168,11 @@ struct QCtx {
     int nb[N / 4];
 };
 
-template <int N>
+// FOLD: y and z are stored scaled by the octant's strides, tr[0] + 2 and
+// (tr[0] + 2)(tr[1] + 2) (cov_at; the launchers check that they fit).
+template <int N, bool FOLD>
 __device__ inline void q_setup(const SolveArgs &a, QShared<N> &sh, QCtx<N> &c) {
+    const int sy = FOLD ? a.cov.tr[0] + 2 : 1, sz = FOLD ? sy * (a.cov.tr[1] + 2) : 1;
     const int lane = threadIdx.x & 63;
     c.q = lane & 3;
     c.slot = lane >> 2;
@@ -153,8 +183,8 @@ __device__ inline void q_setup(const SolveArgs &a, QShared<N> &sh, QCtx<N> &c) {
     c.n = c.valid ? min(a.cnt[c.t], N) : 0;
     c.node = a.targets[c.t];
     c.cx = c.node % a.X;
-    c.cy = c.node % (a.X * a.Y) / a.X;
-    c.cz = c.node / (a.X * a.Y);
+    c.cy = c.node % (a.X * a.Y) / a.X * sy;
+    c.cz = c.node / (a.X * a.Y) * sz;
 #pragma unroll
     for (int m = 0; m < N / 4; ++m) {
         const int i = 4 * m + c.q;
@@ -162,8 +192,8 @@ __device__ inline void q_setup(const SolveArgs &a, QShared<N> &sh, QCtx<N> &c) {
         if (i < c.n) {
             cell = a.nbr[(size_t)c.t * a.max_n + i];
             x = cell % a.X;
-            y = cell % (a.X * a.Y) / a.X;
-            z = cell / (a.X * a.Y);
+            y = cell % (a.X * a.Y) / a.X * sy;
+            z = cell / (a.X * a.Y) * sz;
         }
         c.nb[m] = cell;
         sh.xyz[c.slot][0][i] = x;
@@ -178,15 +208,15 @@ __device__ inline void q_setup(const SolveArgs &a, QShared<N> &sh, QCtx<N> &c) {
 // result is selected afterwards.
 template <int N, bool FOLD>
 __device__ inline double q_aij(const DevCov &cv, const QShared<N> &sh, int slot, int n, int i, int j) {
-    const double v = cov_at<FOLD>(cv, sh.xyz[slot][0][j] - sh.xyz[slot][0][i], sh.xyz[slot][1][j] - sh.xyz[slot][1][i],
-                            sh.xyz[slot][2][j] - sh.xyz[slot][2][i]);
+    const double v = cov_at<FOLD>(cv, sh.xyz[slot][0][i], sh.xyz[slot][1][i], sh.xyz[slot][2][i], sh.xyz[slot][0][j],
+                                  sh.xyz[slot][1][j], sh.xyz[slot][2][j]);
     return (i < n && j < n) ? v : (i == j ? 1.0 : 0.0);
 }
 
 template <int N, bool FOLD>
 __device__ inline double q_bi(const SolveArgs &a, const QShared<N> &sh, const QCtx<N> &c, int i) {
-    const double v =
-        cov_at<FOLD>(a.cov, c.cx - sh.xyz[c.slot][0][i], c.cy - sh.xyz[c.slot][1][i], c.cz - sh.xyz[c.slot][2][i]);
+    const double v = cov_at<FOLD>(a.cov, sh.xyz[c.slot][0][i], sh.xyz[c.slot][1][i], sh.xyz[c.slot][2][i], c.cx, c.cy,
+                                  c.cz);
     return i < c.n ? v : 0.0;
 }
 
@@ -310,6 +340,56 @@ __device__ inline void q_solve32(const float *L, const float *Dinv, int q, float
     }
 }
 
+// Estimate of ||A^-1||_2 from the FP32 factor (the idea of LINPACK's
+// dpoco): L y = e with e_k = +-1 chosen during the forward sweep so that
+// |y_k| grows, then L^T z = y; ||z|| / ||e|| (a lower bound that the sign
+// choice makes close: y leans towards the dominant singular direction of
+// L^-1, which L^-T then amplifies again).  Rows n..N-1 take e_k = 0.
+template <int N>
+__device__ inline double q_inv_norm_est(const float *L, const float *Dinv, int q, int n) {
+    constexpr int M = N / 4;
+    float v[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) v[m] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        const int mk = k >> 2, ko = k & 3;
+        if (q == ko) {
+            const float s = v[mk];
+            const float e = k < n ? (s >= 0.0f ? 1.0f : -1.0f) : 0.0f;
+            v[mk] = (s + e) * Dinv[mk];
+        }
+        const float yk = qb(v[mk], ko);
+#pragma unroll
+        for (int m = 0; m < M; ++m)
+            if (4 * m + 3 > k) {
+                if (4 * m + q > k) v[m] = fmaf(-L[off(m) + k], yk, v[m]);
+            }
+    }
+#pragma unroll
+    for (int i = N - 1; i >= 0; --i) {
+        const int mi = i >> 2, io = i & 3;
+        float p = 0.0f;
+#pragma unroll
+        for (int m = 0; m < M; ++m)
+            if (4 * m + 3 > i) {
+                if (4 * m + q > i) p = fmaf(L[off(m) + i], v[m], p);
+            }
+        p += qswap1(p);
+        p += qswap2(p);
+        if (q == io) v[mi] = (v[mi] - p) * Dinv[mi];
+    }
+    double zz = 0.0;
+#pragma unroll
+    for (int m = 0; m < M; ++m) zz += (double)v[m] * (double)v[m];
+    zz = qsum(zz);
+    return n > 0 ? sqrt(zz / (double)n) : 0.0;
+}
+
+// Tolerance of the one-step certificate: the bound on |x - x1| / |x1|,
+// twenty times below the 1e-6 relative error the mixed path is held to.
+constexpr double kOneStepTol = 5.0e-8;
+
 // Wavefronts per SIMD that the register budget of k_solve_quad_mixed<N,
 // TWO> must allow.  Left alone, orders 28 and 32 and the two-right-hand-side
 // kernels from order 20 take more than 256 registers (one wavefront per
@@ -317,10 +397,10 @@ __device__ inline void q_solve32(const float *L, const float *Dinv, int q, float
 // and still run faster (order 32, one right-hand side: 20.9 -> 15.2 ms per
 // SGS realisation; two right-hand sides, 490,000 systems: order 20 1.31 ->
 // 0.82 ms, 24 1.82 -> 1.18 ms, 32 3.40 -> 2.79 ms).  The order-24 single
-// right-hand-side kernel stays at its natural two (three spilled and took
-// 123 against 89 ms per headline step).
+// right-hand-side kernel needs about 256 and is held to two as well (three
+// spilled and took 123 against 89 ms per headline step).
 template <int N, bool TWO>
-constexpr int kQuadMixedWaves = (N >= 28 || (N >= 20 && TWO)) ? 2 : 1;
+constexpr int kQuadMixedWaves = (N >= 24 || (N >= 20 && TWO)) ? 2 : 1;
 
 template <int N, bool TWO, bool FOLD>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kQuadMixedWaves<N, TWO>, 8))) void k_solve_quad_mixed(SolveArgs a) {
@@ -328,13 +408,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kQuadMixedWa
     constexpr int M = N / 4;
     __shared__ QShared<N> sh;
     QCtx<N> c;
-    q_setup<N>(a, sh, c);
+    q_setup<N, FOLD>(a, sh, c);
     const int q = c.q;
 
     // build L <- A (FP32), b (FP64)
     float L[off(M)];
     double b[M];
     double maxdiag = 0.0;
+    float low2 = 0.0f;  // sum of squares of my rows' entries j <= i
 #pragma unroll
     for (int m = 0; m < M; ++m) {
         const int i = 4 * m + q;
@@ -344,10 +425,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kQuadMixedWa
             const double e = q_aij<N, FOLD>(a.cov, sh, c.slot, c.n, i, j);
             const double aij = (j <= i) ? e : 0.0;
             L[off(m) + j] = (float)aij;
+            low2 = fmaf(L[off(m) + j], L[off(m) + j], low2);
             if (j == i && i < c.n) maxdiag = fmax(maxdiag, aij);
         }
     }
     maxdiag = qmax(maxdiag);
+    // the quad's sum, parked in LDS until the first refinement step
+    // (sh.tf is free until the epilogue)
+    sh.tf[c.slot][q] = (float)qsum((double)low2);
 
     // right-looking Cholesky in place (hardware square root and reciprocal,
     // within an ulp: the refinement works from this factor in FP64)
@@ -394,9 +479,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kQuadMixedWa
         x1[m] = v1[m];
         x2[m] = v2[m];
     }
-    // Refinement: a second step runs unless every system of the wave already
-    // has a first correction below 1e-9 of its solution.
-    bool conv = false;
+    // Refinement: a second step runs unless every system of the wave has a
+    // first correction below 1e-9 of its solution or, where the outputs are
+    // categories, is certified after the first (header comment).
+    const bool categorical = EPI == EPI_IK || EPI == EPI_MEDIAN;
+    bool conv = false, twice = false;
     for (int step = 0; step < 2; ++step) {
         // the table reads of a step are not kept for the next one (that
         // would hold A in registers): the pointer is opaque per step
@@ -473,8 +560,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kQuadMixedWa
         const double rel = dmax / (xmax > 0 ? xmax : 1.0);
         if (step == 0) {
             conv = rel <= 1.0e-9;
+            if (categorical) {
+                // contraction bound of a refinement step, n u ||A||_F
+                // ||A^-1||_2, with ||A||_F^2 <= 2 (sum of squares over the
+                // lower triangle) less the padding rows' unit diagonal
+                const double fro = sqrt(fmax(0.0, 2.0 * ((double)sh.tf[c.slot][q] - (double)(N - c.n))));
+                const double rho = (double)c.n * 0x1p-24 * fro * q_inv_norm_est<N>(L, Dinv, q, c.n);
+                conv = conv || (rho <= 0.5 && 2.0 * rho * rel <= kOneStepTol);
+            }
             if (__all(conv || !c.valid)) break;
         } else {
+            twice = true;
             conv = conv || rel <= 1.0e-9;
         }
     }
@@ -516,6 +612,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kQuadMixedWa
     if (c.valid && q == 0) a.status[c.t] = st;
     wave_count(a.counters, 3, c.valid && q == 0 && st == ST_NO_NEIGHBOURS);
     wave_count(a.counters, 0, c.valid && q == 0 && st == ST_OK);
+    wave_count(a.counters, 4, c.valid && q == 0 && st == ST_OK && twice);
     q_epilogue<N>(a, &sh, c, c.valid && (c.n == 0 || !bad), c.n > 0, w, variance);
 }
 
@@ -528,7 +625,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(N >= 16 ? 2 
     constexpr int M = N / 4;
     __shared__ QShared<N> sh;
     QCtx<N> c;
-    q_setup<N>(a, sh, c);
+    q_setup<N, FOLD>(a, sh, c);
     const int q = c.q;
 
     double L[off(M)];
@@ -678,7 +775,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(N >= 16 ? 2 
 
 // FOLD when the covariance has a symmetric octant table (DevCov::fold)
 #define HPGL_Q(K, NN)                                                      \
-    if (a.cov.fold)                                                        \
+    if (fold)                                                              \
         hipLaunchKernelGGL((K<NN, true>), grid, dim3(64), 0, s, a);        \
     else                                                                   \
         hipLaunchKernelGGL((K<NN, false>), grid, dim3(64), 0, s, a)
@@ -688,7 +785,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(N >= 16 ? 2 
     else                                                                                   \
         hipLaunchKernelGGL((k_solve_quad_mixed<NN, false, F>), grid, dim3(64), 0, s, a)
 #define HPGL_QM(NN)            \
-    if (a.cov.fold) {          \
+    if (fold) {                \
         HPGL_QM2(NN, true);    \
     } else {                   \
         HPGL_QM2(NN, false);   \
@@ -698,6 +795,9 @@ void launch_solve_quad(const SolveArgs &a, int order, bool fp64, hipStream_t s) 
     const int items = a.subset ? a.n_subset : a.T;
     if (items <= 0) return;
     dim3 grid((items + 15) / 16);
+    // the octant path stores y and z scaled by its strides in 32-bit ints
+    const long long sy = a.cov.tr[0] + 2, sz = sy * (a.cov.tr[1] + 2);
+    const bool fold = a.cov.fold && (long long)(a.Y - 1) * sy <= INT32_MAX && (long long)(a.Z - 1) * sz <= INT32_MAX;
     if (fp64) {
         switch (order) {
             case 4: HPGL_Q(k_solve_quad_fp64, 4); break;

@@ -134,6 +134,7 @@ class _HPGL_AMD_STATS(C.Structure):
         ("cluster_candidates", C.c_longlong),
         ("neighbours_total", C.c_longlong),
         ("cluster_lists_spilled", C.c_longlong),
+        ("solves_fp32_two_steps", C.c_longlong),
     ]
 
 

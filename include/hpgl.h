@@ -351,6 +351,7 @@ struct hpgl_amd_stats_t {
     long long cluster_candidates;   /* candidate cells gathered by the cell-binned search */
     long long neighbours_total;     /* neighbours over all searched targets */
     long long cluster_lists_spilled; /* cell-binned candidate lists sorted in global scratch */
+    long long solves_fp32_two_steps; /* FP32 solves accepted after a second refinement step */
 };
 void hpgl_amd_get_last_stats(struct hpgl_amd_stats_t *out);
 

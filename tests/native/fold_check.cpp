@@ -3,9 +3,11 @@
 // table is symmetric in every axis:
 //
 //   * models without rotation (all three types, anisotropic ranges, with and
-//     without nugget) fold, and the octant entry (|dx|, |dy|, |dz|) at
-//     (|dz| (ry + 1) + |dy|) (rx + 1) + |dx| equals the full table's entry
-//     for every offset of the box, bit for bit;
+//     without nugget) fold, and the octant entry (ax, ay, az) at
+//     (az (ry + 2) + ay) (rx + 2) + ax, with ax = min(|dx|, rx + 1) and so
+//     on, equals the full table's entry (0 outside the box) for every offset
+//     of the box grown by two in each axis, bit for bit: the octant's
+//     appended layer holds zeros, so offsets past the box read 0;
 //   * rotated models do not fold (their table is not symmetric per axis);
 //   * a table with one entry changed (a planted asymmetry) does not fold.
 //
@@ -13,6 +15,7 @@
 // failure otherwise.
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -60,10 +63,15 @@ int main() {
                 }
                 ++folded;
                 const int rx = t.r[0], ry = t.r[1], rz = t.r[2];
-                for (int z = -rz; z <= rz; ++z)
-                    for (int y = -ry; y <= ry; ++y)
-                        for (int x = -rx; x <= rx; ++x) {
-                            const size_t i = ((size_t)std::abs(z) * (ry + 1) + std::abs(y)) * (rx + 1) + std::abs(x);
+                if (oct.size() != (size_t)(rx + 2) * (ry + 2) * (rz + 2)) {
+                    std::printf("octant size %zu, type %d case %d\n", oct.size(), type, k);
+                    return 1;
+                }
+                auto cl = [](int v, int r) { return std::min(std::abs(v), r + 1); };
+                for (int z = -rz - 2; z <= rz + 2; ++z)
+                    for (int y = -ry - 2; y <= ry + 2; ++y)
+                        for (int x = -rx - 2; x <= rx + 2; ++x) {
+                            const size_t i = ((size_t)cl(z, rz) * (ry + 2) + cl(y, ry)) * (rx + 2) + cl(x, rx);
                             if (!same(oct[i], t.at(x, y, z))) {
                                 std::printf("octant entry differs at (%d,%d,%d), type %d case %d\n", x, y, z, type,
                                             k);
