@@ -301,6 +301,57 @@ struct KTimer {
     }
 };
 
+// Device times of launches on stream s without a host wait per launch: each
+// start/stop pair records two events, read by collect() once the stream has
+// been synchronised past them for another reason.
+class KernelClock {
+  public:
+    explicit KernelClock(hipStream_t s) : s_(s) {}
+    KernelClock(const KernelClock &) = delete;
+    KernelClock &operator=(const KernelClock &) = delete;
+    ~KernelClock() {
+        for (const Mark &m : marks_) {
+            (void)hipEventDestroy(m.a);
+            (void)hipEventDestroy(m.b);
+        }
+    }
+    void start() {
+        Mark m{};
+        hip_check(hipEventCreate(&m.a), "hipEventCreate");
+        hip_check(hipEventCreate(&m.b), "hipEventCreate");
+        marks_.push_back(m);
+        hip_check(hipEventRecord(m.a, s_), "hipEventRecord");
+    }
+    // ends the last start(); its time goes to ms and the launch to count
+    void stop(double &ms, long long &count) {
+        Mark &m = marks_.back();
+        hip_check(hipEventRecord(m.b, s_), "hipEventRecord");
+        m.ms = &ms;
+        m.count = &count;
+    }
+    // the stream must have passed every recorded event
+    void collect() {
+        for (const Mark &m : marks_) {
+            float ms = 0;
+            hip_check(hipEventElapsedTime(&ms, m.a, m.b), "hipEventElapsedTime");
+            if (m.ms) *m.ms += ms;
+            if (m.count) *m.count += 1;
+            (void)hipEventDestroy(m.a);
+            (void)hipEventDestroy(m.b);
+        }
+        marks_.clear();
+    }
+
+  private:
+    struct Mark {
+        hipEvent_t a, b;
+        double *ms;
+        long long *count;
+    };
+    hipStream_t s_;
+    std::vector<Mark> marks_;
+};
+
 struct IsZero {
     const uint8_t *m;
     __host__ __device__ bool operator()(const int &i) const { return m[i] == 0; }
@@ -315,16 +366,23 @@ struct FlagEq {
     __host__ __device__ bool operator()(const int &i) const { return f[i] == v; }
 };
 
-// Ascending list of i in [0, n) with pred(i), returned with its length.
+// Ascending list of i in [0, n) with pred(i) into out, its length into
+// *d_num on the device (no host wait).
 template <typename Pred>
-int select_indices(Pred pred, int n, DBuf<int> &out, hipStream_t s) {
+void select_indices_async(Pred pred, int n, DBuf<int> &out, int *d_num, hipStream_t s) {
     out.alloc(n);
-    DBuf<int> d_num(1);
     hipcub::CountingInputIterator<int> it(0);
     size_t bytes = 0;
-    hip_check(hipcub::DeviceSelect::If(nullptr, bytes, it, out.p, d_num.p, n, pred, s), "select");
+    hip_check(hipcub::DeviceSelect::If(nullptr, bytes, it, out.p, d_num, n, pred, s), "select");
     DBuf<uint8_t> tmp(bytes);
-    hip_check(hipcub::DeviceSelect::If(tmp.p, bytes, it, out.p, d_num.p, n, pred, s), "select");
+    hip_check(hipcub::DeviceSelect::If(tmp.p, bytes, it, out.p, d_num, n, pred, s), "select");
+}
+
+// The same, returning the length.
+template <typename Pred>
+int select_indices(Pred pred, int n, DBuf<int> &out, hipStream_t s) {
+    DBuf<int> d_num(1);
+    select_indices_async(pred, n, out, d_num.p, s);
     int num = 0;
     hip_check(hipMemcpyAsync(&num, d_num.p, sizeof(int), hipMemcpyDeviceToHost, s), "copy count");
     hip_check(hipStreamSynchronize(s), "sync");
@@ -596,9 +654,17 @@ void search(CovAssets &as, const int grid[3], const int radii[3], int max_n, con
     launch_path_flags(d_targets, T, ba, bin_count.p, flag.p, s);
     hip_check(hipGetLastError(), "bin kernels");
 
-    DBuf<int> tlist, clist;
-    const int n_tmpl = select_indices(FlagEq{flag.p, 0}, T, tlist, s);
-    const int n_clu = T - n_tmpl;
+    // both paths' target lists, their lengths read with one host wait
+    DBuf<int> tlist, clist, d_nums(2);
+    select_indices_async(FlagEq{flag.p, 0}, T, tlist, d_nums.p, s);
+    select_indices_async(FlagEq{flag.p, 1}, T, clist, d_nums.p + 1, s);
+    int nums[2] = {0, 0};
+    hip_check(hipMemcpyAsync(nums, d_nums.p, sizeof(nums), hipMemcpyDeviceToHost, s), "copy counts");
+    hip_check(hipStreamSynchronize(s), "sync");
+    const int n_tmpl = nums[0];
+    const int n_clu = nums[1];
+    if (n_tmpl + n_clu != T) throw std::runtime_error("search: path lists do not add up");
+    KernelClock clk(s);
     st.nodes_template_path += n_tmpl;
     st.nodes_cluster_path += n_clu;
 
@@ -618,15 +684,16 @@ void search(CovAssets &as, const int grid[3], const int radii[3], int max_n, con
     sa.cnt = d_cnt;
     sa.simbits = nullptr;
     if (n_tmpl > 0) {
-        KTimer kt(s);
+        clk.start();
         launch_template_scan(sa, 0, s);
         hip_check(hipGetLastError(), "template scan");
-        st.ms_k_template += kt.stop();
-        st.n_k_template += 1;
+        clk.stop(st.ms_k_template, st.n_k_template);
     }
 
+    // neighbour total, read with the search's counters
+    DBuf<int> total(1);
+    unsigned int ov = 0;
     if (n_clu > 0) {
-        select_indices(FlagEq{flag.p, 1}, T, clist, s);
         // bin members: informed cells sorted by (bin, cell); radix sort is stable
         DBuf<int> keys_out(std::max(I, 1)), cells_out(std::max(I, 1));
         size_t bytes = 0;
@@ -731,28 +798,31 @@ void search(CovAssets &as, const int grid[3], const int radii[3], int max_n, con
         hip_check(hipMemsetAsync(cand.p, 0, kCounterWords * sizeof(unsigned long long), s), "memset");
         ca.overflow = overflow.p;
         ca.cand_total = cand.p;
-        {
-            KTimer kt(s);
-            launch_cluster_search(ca, s);
-            hip_check(hipGetLastError(), "cluster search");
-            st.ms_k_cluster += kt.stop();
-            st.n_k_cluster += 1;
-        }
+        clk.start();
+        launch_cluster_search(ca, s);
+        hip_check(hipGetLastError(), "cluster search");
+        clk.stop(st.ms_k_cluster, st.n_k_cluster);
         std::vector<unsigned long long> hc(kCounterWords);
+        {
+            size_t bytes = 0;
+            hip_check(hipcub::DeviceReduce::Sum(nullptr, bytes, d_cnt, total.p, T, s), "reduce");
+            DBuf<uint8_t> tmp(bytes);
+            hip_check(hipcub::DeviceReduce::Sum(tmp.p, bytes, d_cnt, total.p, T, s), "reduce");
+        }
         hip_check(hipMemcpyAsync(hc.data(), cand.p, kCounterWords * sizeof(unsigned long long), hipMemcpyDeviceToHost, s),
                   "copy cand");
-        unsigned int ov = 0;
         hip_check(hipMemcpyAsync(&ov, overflow.p, sizeof(ov), hipMemcpyDeviceToHost, s), "copy overflow");
+        int h = 0;
+        hip_check(hipMemcpyAsync(&h, total.p, sizeof(int), hipMemcpyDeviceToHost, s), "copy");
         hip_check(hipStreamSynchronize(s), "sync");
+        st.neighbours_total += h;
         for (int l = 0; l < kCounterSlots; ++l) {
             st.cluster_candidates += (long long)hc[(size_t)l * kCounterStride];
             st.cluster_lists_spilled += (long long)hc[(size_t)l * kCounterStride + 1];
         }
         if (ov & 1u) throw std::runtime_error("cluster search: candidate list exceeded the scratch capacity");
         if (ov & 2u) throw std::runtime_error("cluster search: internal error in the sort replay");
-    }
-    {
-        DBuf<int> total(1);
+    } else {
         size_t bytes = 0;
         hip_check(hipcub::DeviceReduce::Sum(nullptr, bytes, d_cnt, total.p, T, s), "reduce");
         DBuf<uint8_t> tmp(bytes);
@@ -762,6 +832,7 @@ void search(CovAssets &as, const int grid[3], const int radii[3], int max_n, con
         hip_check(hipStreamSynchronize(s), "sync");
         st.neighbours_total += h;
     }
+    clk.collect();
 }
 
 void write_dump(const Dump *dump, long long cell_offset, const int *d_targets, int T, const int *d_nbr,
@@ -781,51 +852,24 @@ void write_dump(const Dump *dump, long long cell_offset, const int *d_targets, i
     }
 }
 
-__global__ void k_bucket_keys(const int *cnt, int T, uint8_t *key) {
+// Bucket key per target (0..7: quad buckets of order 4, 8, ..., 32; 8:
+// wider systems) and the buckets' sizes (counted per workgroup in LDS, one
+// atomic per non-empty bucket and workgroup).
+constexpr int kBuckets = 9;
+__global__ void k_bucket_keys(const int *cnt, int T, uint8_t *key, int *count) {
+    __shared__ int h[kBuckets];
+    if (threadIdx.x < kBuckets) h[threadIdx.x] = 0;
+    __syncthreads();
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= T) return;
-    const int n = cnt[t];
-    // 0..7: quad buckets of order 4, 8, ..., 32; 8: wider systems
-    key[t] = n > kQuadMaxMixed ? 8 : (uint8_t)(n <= 4 ? 0 : (n + 3) / 4 - 1);
+    if (t < T) {
+        const int n = cnt[t];
+        const int k = n > kQuadMaxMixed ? 8 : (n <= 4 ? 0 : (n + 3) / 4 - 1);
+        key[t] = (uint8_t)k;
+        atomicAdd(&h[k], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < kBuckets && h[threadIdx.x] != 0) atomicAdd(&count[threadIdx.x], h[threadIdx.x]);
 }
-
-struct KeyEq {
-    const uint8_t *k;
-    uint8_t v;
-    __host__ __device__ bool operator()(const int &i) const { return k[i] == v; }
-};
-struct KeyEqStatus {
-    const uint8_t *k;
-    uint8_t v;
-    const uint8_t *st;
-    uint8_t sv;
-    __host__ __device__ bool operator()(const int &i) const { return k[i] == v && st[i] == sv; }
-};
-
-// FP64 solves of targets whose bucket key is k (and, when `only_fallback`,
-// whose status is ST_FALLBACK): quad kernel up to order 16, lane-per-row
-// kernels above.
-void launch_fp64_subset(const SolveArgs &b, int order, hipStream_t s, Stats &st);
-void solve_fp64_bucket(SolveArgs sa, const DBuf<uint8_t> &keys, uint8_t k, bool only_fallback,
-                       const DBuf<uint8_t> &status, hipStream_t s, Stats &st) {
-    DBuf<int> list;
-    int n;
-    if (only_fallback)
-        n = select_indices(KeyEqStatus{keys.p, k, status.p, (uint8_t)ST_FALLBACK}, sa.T, list, s);
-    else
-        n = select_indices(KeyEq{keys.p, k}, sa.T, list, s);
-    if (n == 0) return;
-    sa.subset = list.p;
-    sa.n_subset = n;
-    launch_fp64_subset(sa, k == 8 ? 64 : 4 * (k + 1), s, st);
-}
-
-struct SubsetStatusEq {
-    const int *list;
-    const uint8_t *st;
-    uint8_t v;
-    __host__ __device__ bool operator()(const int &i) const { return st[list[i]] == v; }
-};
 
 // Systems per bucket solved first in mixed precision to decide whether the
 // bucket's model is too ill-conditioned for FP32 (Gaussian covariance with a
@@ -833,15 +877,23 @@ struct SubsetStatusEq {
 // handed to FP64, the rest of the bucket goes to FP64 directly.
 constexpr int kMixedProbe = 16384;
 
-void launch_fp64_subset(const SolveArgs &b, int order, hipStream_t s, Stats &st) {
-    KTimer kt(s);
+// FP64 solves of the targets sa.subset[0 .. n_subset): the quad kernel up
+// to order 16, lane-per-row kernels above.
+void launch_fp64_subset(const SolveArgs &b, int order, hipStream_t s, KernelClock &clk, Stats &st) {
+    clk.start();
     if (order <= kQuadMaxFp64 && b.cov.table && !b.correlogram)
         launch_solve_quad(b, order, true, s);
     else
         launch_solve_fp64(b, order <= 32 ? 32 : 64, s);
     hip_check(hipGetLastError(), "solve fp64");
-    st.ms_k_solve_fp64 += kt.stop();
-    st.n_k_solve_fp64 += 1;
+    clk.stop(st.ms_k_solve_fp64, st.n_k_solve_fp64);
+}
+
+void launch_mixed_subset(const SolveArgs &b, int order, hipStream_t s, KernelClock &clk, Stats &st) {
+    clk.start();
+    launch_solve_quad(b, order, false, s);
+    hip_check(hipGetLastError(), "solve mixed");
+    clk.stop(st.ms_k_solve_mixed, st.n_k_solve_mixed);
 }
 
 // Orders below this go to the FP64 quad kernel in mixed mode too: there it
@@ -852,61 +904,102 @@ void launch_fp64_subset(const SolveArgs &b, int order, hipStream_t s, Stats &st)
 // against 3.1 ms at 20, 2.8 against 4.6 ms at 32).
 constexpr int kMixedMinOrder = 20;
 
-// Solve all targets, bucketed by neighbour count.  Mixed mode: the quad
-// kernels (orders 20..32 with a covariance table) in FP32 + FP64
-// refinement, then FP64 for the systems they hand over; smaller and wider
-// systems and the analytic-covariance fallback go to the FP64 kernels
-// directly.  FP64 mode: every system through the FP64 kernels.
-// Correlogram systems always take the FP64 kernels.
+// Solve all targets, bucketed by neighbour count.  The targets are sorted
+// by bucket key once (stable, so each bucket lists its targets in ascending
+// order).  Mixed mode: the quad kernels (orders 20..32 with a covariance
+// table) in FP32 + FP64 refinement, appending the systems they hand over to
+// a per-bucket list; those go through the FP64 kernels after every bucket
+// has been launched.  Smaller and wider systems and the analytic-covariance
+// fallback go to the FP64 kernels directly.  FP64 mode: every system
+// through the FP64 kernels.  Correlogram systems always take the FP64
+// kernels.  The host waits for the stream three times plus once per large
+// mixed bucket (bucket sizes, each bucket's probe, the fallback counts,
+// the end), not once per launch.
 void solve(SolveArgs sa, int epi, int precision, DBuf<uint8_t> &status, hipStream_t s, Stats &st) {
     sa.status = status.p;
     sa.subset = nullptr;
     sa.n_subset = 0;
     sa.epi = epi;
-    DBuf<uint8_t> keys(sa.T);
-    hipLaunchKernelGGL(k_bucket_keys, dim3((sa.T + 255) / 256), dim3(256), 0, s, sa.cnt, sa.T, keys.p);
+    sa.fb_list = nullptr;
+    sa.fb_count = nullptr;
+    if (sa.T <= 0) return;
+    KernelClock clk(s);
+    DBuf<uint8_t> keys(sa.T), keys_sorted(sa.T);
+    DBuf<int> counts(2 * kBuckets), iota(sa.T), order(sa.T);
+    launch_fill_u32(reinterpret_cast<uint32_t *>(counts.p), 0u, 2 * kBuckets, s);
+    hipLaunchKernelGGL(k_bucket_keys, dim3((sa.T + 255) / 256), dim3(256), 0, s, sa.cnt, sa.T, keys.p, counts.p);
+    hipLaunchKernelGGL(k_iota, dim3((sa.T + 255) / 256), dim3(256), 0, s, iota.p, sa.T);
     hip_check(hipGetLastError(), "bucket keys");
-    // the neighbour limit bounds the buckets that can be non-empty
-    const int maxn = sa.max_n;
-    const int kmax = maxn > kQuadMaxMixed ? 8 : std::max(0, (maxn + 3) / 4 - 1);
-    for (int k = 0; k <= kmax; ++k) {
-        const int order = k == 8 ? 64 : 4 * (k + 1);
-        const bool mixed = precision == 0 && order >= kMixedMinOrder && k < 8 && sa.cov.table && !sa.correlogram;
+    {
+        size_t bytes = 0;
+        hip_check(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, keys.p, keys_sorted.p, iota.p, order.p, sa.T, 0, 4,
+                                                     s),
+                  "bucket sort");
+        DBuf<uint8_t> tmp(bytes);
+        hip_check(hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, keys.p, keys_sorted.p, iota.p, order.p, sa.T, 0, 4,
+                                                     s),
+                  "bucket sort");
+    }
+    int n_k[kBuckets];
+    hip_check(hipMemcpyAsync(n_k, counts.p, sizeof(n_k), hipMemcpyDeviceToHost, s), "copy bucket sizes");
+    hip_check(hipStreamSynchronize(s), "sync");
+    int off[kBuckets + 1];
+    off[0] = 0;
+    for (int k = 0; k < kBuckets; ++k) off[k + 1] = off[k] + n_k[k];
+    if (off[kBuckets] != sa.T) throw std::runtime_error("solve: bucket sizes do not add up");
+
+    // per-bucket fallback lists share the bucket layout of `order`
+    DBuf<int> fb_list(sa.T);
+    int *fb_count = counts.p + kBuckets;  // zeroed above
+    bool any_mixed = false;
+    for (int k = 0; k < kBuckets; ++k) {
+        const int n = n_k[k];
+        if (n == 0) continue;
+        const int ord = k == 8 ? 64 : 4 * (k + 1);
+        SolveArgs b = sa;
+        b.subset = order.p + off[k];
+        const bool mixed = precision == 0 && ord >= kMixedMinOrder && k < 8 && sa.cov.table && !sa.correlogram;
         if (!mixed) {
-            solve_fp64_bucket(sa, keys, (uint8_t)k, false, status, s, st);
+            b.n_subset = n;
+            launch_fp64_subset(b, ord, s, clk, st);
             continue;
         }
-        SolveArgs b = sa;
-        DBuf<int> list;
-        const int n = select_indices(KeyEq{keys.p, (uint8_t)k}, sa.T, list, s);
-        if (n == 0) continue;
+        any_mixed = true;
+        b.fb_list = fb_list.p + off[k];
+        b.fb_count = fb_count + k;
         const int probe = n > 4 * kMixedProbe ? kMixedProbe : n;
-        b.subset = list.p;
         b.n_subset = probe;
-        {
-            KTimer kt(s);
-            launch_solve_quad(b, order, false, s);
-            hip_check(hipGetLastError(), "solve mixed");
-            st.ms_k_solve_mixed += kt.stop();
-            st.n_k_solve_mixed += 1;
-        }
+        launch_mixed_subset(b, ord, s, clk, st);
         if (probe < n) {
-            DBuf<int> tmp;
-            const int fb = select_indices(SubsetStatusEq{list.p, status.p, (uint8_t)ST_FALLBACK}, probe, tmp, s);
-            b.subset = list.p + probe;
+            int fb = 0;
+            hip_check(hipMemcpyAsync(&fb, fb_count + k, sizeof(int), hipMemcpyDeviceToHost, s), "copy fallbacks");
+            hip_check(hipStreamSynchronize(s), "sync");
+            b.subset = order.p + off[k] + probe;
             b.n_subset = n - probe;
             if (2 * fb > probe) {
-                launch_fp64_subset(b, order, s, st);
+                b.fb_list = nullptr;
+                b.fb_count = nullptr;
+                launch_fp64_subset(b, ord, s, clk, st);
             } else {
-                KTimer kt(s);
-                launch_solve_quad(b, order, false, s);
-                hip_check(hipGetLastError(), "solve mixed");
-                st.ms_k_solve_mixed += kt.stop();
-                st.n_k_solve_mixed += 1;
+                launch_mixed_subset(b, ord, s, clk, st);
             }
         }
-        solve_fp64_bucket(sa, keys, (uint8_t)k, true, status, s, st);
     }
+    if (any_mixed) {
+        int fbn[kBuckets];
+        hip_check(hipMemcpyAsync(fbn, fb_count, sizeof(fbn), hipMemcpyDeviceToHost, s), "copy fallbacks");
+        hip_check(hipStreamSynchronize(s), "sync");
+        for (int k = 0; k < kBuckets; ++k) {
+            if (fbn[k] == 0) continue;
+            if (fbn[k] < 0 || fbn[k] > n_k[k]) throw std::runtime_error("solve: fallback count out of range");
+            SolveArgs b = sa;
+            b.subset = fb_list.p + off[k];
+            b.n_subset = fbn[k];
+            launch_fp64_subset(b, k == 8 ? 64 : 4 * (k + 1), s, clk, st);
+        }
+    }
+    hip_check(hipStreamSynchronize(s), "sync");
+    clk.collect();
 }
 
 void collect_counters(const unsigned long long *d_counters, hipStream_t s, Stats &st) {

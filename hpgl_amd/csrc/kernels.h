@@ -78,6 +78,10 @@ struct SolveArgs {
     const int *subset;
     int n_subset;
     int epi;  // EpilogueKind
+    // mixed kernels: targets handed to FP64 (ST_FALLBACK) are appended to
+    // fb_list at positions taken from *fb_count (nullptr = not recorded)
+    int *fb_list;
+    int *fb_count;
 };
 
 // FP64 solves (solve.hip): group <= 32 two systems per wavefront with rows

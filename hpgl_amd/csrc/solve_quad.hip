@@ -613,6 +613,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kQuadMixedWa
     wave_count(a.counters, 3, c.valid && q == 0 && st == ST_NO_NEIGHBOURS);
     wave_count(a.counters, 0, c.valid && q == 0 && st == ST_OK);
     wave_count(a.counters, 4, c.valid && q == 0 && st == ST_OK && twice);
+    if (a.fb_list) {
+        // one atomic per wavefront, then each handed-over target at its rank
+        const bool fb = c.valid && q == 0 && st == ST_FALLBACK;
+        const unsigned long long m = __ballot(fb);
+        int base = 0;
+        if ((threadIdx.x & 63) == 0 && m) base = atomicAdd(a.fb_count, (int)__popcll(m));
+        base = __shfl(base, 0, 64);
+        if (fb) a.fb_list[base + (int)__popcll(m & ((1ull << (threadIdx.x & 63)) - 1ull))] = c.t;
+    }
     q_epilogue<N>(a, &sh, c, c.valid && (c.n == 0 || !bad), c.n > 0, w, variance);
 }
 
