@@ -506,7 +506,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kQuadMixedWa
         // its rows i = 4m+q only; an off-diagonal block also gives the rows
         // 4mb+cc their terms A(4mb+cc, i) x_i, summed over the quad.  The
         // column-block loop stays rolled (an unrolled one keeps too many
-        // values live).
+        // values live).  Entries of padding rows and columns are read as
+        // the covariance of their zero coordinates, unselected: x is zero
+        // there, and the padding rows' residuals are not used.
 #pragma unroll 1
         for (int mb = 0; mb < M; ++mb) {
             double xc1[4], xc2[4];
@@ -522,7 +524,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kQuadMixedWa
                 const int i = 4 * m + q;
                 double e[4];
 #pragma unroll
-                for (int cc = 0; cc < 4; ++cc) e[cc] = q_aij<N, FOLD>(cv, sh, c.slot, c.n, i, 4 * mb + cc);
+                for (int cc = 0; cc < 4; ++cc) {
+                    const int j = 4 * mb + cc;
+                    e[cc] = cov_at<FOLD>(cv, sh.xyz[c.slot][0][i], sh.xyz[c.slot][1][i], sh.xyz[c.slot][2][i],
+                                         sh.xyz[c.slot][0][j], sh.xyz[c.slot][1][j], sh.xyz[c.slot][2][j]);
+                }
 #pragma unroll
                 for (int cc = 0; cc < 4; ++cc) {
                     r1[m] = fma(-e[cc], xc1[cc], r1[m]);
@@ -550,9 +556,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kQuadMixedWa
         double dmax = 0.0, xmax = 0.0;
 #pragma unroll
         for (int m = 0; m < M; ++m) {
-            x1[m] += (double)v1[m];
-            x2[m] += (double)v2[m];
-            dmax = fmax(dmax, fmax(fabs((double)v1[m]), fabs((double)v2[m])));
+            // padding rows keep x = 0 (their residuals are not formed)
+            const bool row = 4 * m + q < c.n;
+            const double d1 = row ? (double)v1[m] : 0.0, d2 = row ? (double)v2[m] : 0.0;
+            x1[m] += d1;
+            x2[m] += d2;
+            dmax = fmax(dmax, fmax(fabs(d1), fabs(d2)));
             xmax = fmax(xmax, fmax(fabs(x1[m]), fabs(x2[m])));
         }
         dmax = qmax(dmax);
