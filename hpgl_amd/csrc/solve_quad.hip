@@ -422,11 +422,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kQuadMixedWa
         b[m] = q_bi<N, FOLD>(a, sh, c, i);
 #pragma unroll
         for (int j = 0; j < 4 * m + 4; ++j) {
-            const double e = q_aij<N, FOLD>(a.cov, sh, c.slot, c.n, i, j);
-            const double aij = (j <= i) ? e : 0.0;
-            L[off(m) + j] = (float)aij;
-            low2 = fmaf(L[off(m) + j], L[off(m) + j], low2);
-            if (j == i && i < c.n) maxdiag = fmax(maxdiag, aij);
+            // lower triangle of the padded system, selected in FP32: A(i, j)
+            // on rows i < n (then j <= i < n too), the identity on padding
+            // rows, 0 above the diagonal
+            const double e = cov_at<FOLD>(a.cov, sh.xyz[c.slot][0][i], sh.xyz[c.slot][1][i], sh.xyz[c.slot][2][i],
+                                          sh.xyz[c.slot][0][j], sh.xyz[c.slot][1][j], sh.xyz[c.slot][2][j]);
+            const float l = j > i ? 0.0f : (i < c.n ? (float)e : (i == j ? 1.0f : 0.0f));
+            L[off(m) + j] = l;
+            low2 = fmaf(l, l, low2);
+            if (j == i && i < c.n) maxdiag = fmax(maxdiag, e);
         }
     }
     maxdiag = qmax(maxdiag);
