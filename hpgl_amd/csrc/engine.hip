@@ -853,19 +853,26 @@ void write_dump(const Dump *dump, long long cell_offset, const int *d_targets, i
 }
 
 // Bucket key per target (0..7: quad buckets of order 4, 8, ..., 32; 8:
-// wider systems) and the buckets' sizes (counted per workgroup in LDS, one
-// atomic per non-empty bucket and workgroup).
+// wider systems) and the buckets' sizes: each wavefront counts a bucket's
+// lanes with one ballot and adds them to the workgroup's count in LDS (most
+// targets share one bucket, so per-lane atomics would serialise), then one
+// global atomic per non-empty bucket and workgroup.
 constexpr int kBuckets = 9;
 __global__ void k_bucket_keys(const int *cnt, int T, uint8_t *key, int *count) {
     __shared__ int h[kBuckets];
     if (threadIdx.x < kBuckets) h[threadIdx.x] = 0;
     __syncthreads();
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int k = -1;
     if (t < T) {
         const int n = cnt[t];
-        const int k = n > kQuadMaxMixed ? 8 : (n <= 4 ? 0 : (n + 3) / 4 - 1);
+        k = n > kQuadMaxMixed ? 8 : (n <= 4 ? 0 : (n + 3) / 4 - 1);
         key[t] = (uint8_t)k;
-        atomicAdd(&h[k], 1);
+    }
+#pragma unroll
+    for (int b = 0; b < kBuckets; ++b) {
+        const unsigned long long m = __ballot(k == b);
+        if ((threadIdx.x & 63) == 0 && m != 0) atomicAdd(&h[b], (int)__popcll(m));
     }
     __syncthreads();
     if (threadIdx.x < kBuckets && h[threadIdx.x] != 0) atomicAdd(&count[threadIdx.x], h[threadIdx.x]);
