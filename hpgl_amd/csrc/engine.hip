@@ -853,26 +853,31 @@ void write_dump(const Dump *dump, long long cell_offset, const int *d_targets, i
 }
 
 // Bucket key per target (0..7: quad buckets of order 4, 8, ..., 32; 8:
-// wider systems) and the buckets' sizes: each wavefront counts a bucket's
-// lanes with one ballot and adds them to the workgroup's count in LDS (most
-// targets share one bucket, so per-lane atomics would serialise), then one
-// global atomic per non-empty bucket and workgroup.
+// wider systems) and the buckets' sizes.  A bounded grid strides over the
+// targets; each wavefront counts a bucket's lanes with one ballot into the
+// workgroup's LDS counts, and each workgroup adds those with one global
+// atomic per non-empty bucket (most targets share one bucket, so atomics
+// per lane or per small workgroup would serialise on one address).
 constexpr int kBuckets = 9;
+constexpr int kBucketGrid = 1024;
 __global__ void k_bucket_keys(const int *cnt, int T, uint8_t *key, int *count) {
     __shared__ int h[kBuckets];
     if (threadIdx.x < kBuckets) h[threadIdx.x] = 0;
     __syncthreads();
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    int k = -1;
-    if (t < T) {
-        const int n = cnt[t];
-        k = n > kQuadMaxMixed ? 8 : (n <= 4 ? 0 : (n + 3) / 4 - 1);
-        key[t] = (uint8_t)k;
-    }
+    const int stride = gridDim.x * blockDim.x;
+    for (int base = blockIdx.x * blockDim.x; base < T; base += stride) {
+        const int t = base + threadIdx.x;
+        int k = -1;
+        if (t < T) {
+            const int n = cnt[t];
+            k = n > kQuadMaxMixed ? 8 : (n <= 4 ? 0 : (n + 3) / 4 - 1);
+            key[t] = (uint8_t)k;
+        }
 #pragma unroll
-    for (int b = 0; b < kBuckets; ++b) {
-        const unsigned long long m = __ballot(k == b);
-        if ((threadIdx.x & 63) == 0 && m != 0) atomicAdd(&h[b], (int)__popcll(m));
+        for (int b = 0; b < kBuckets; ++b) {
+            const unsigned long long m = __ballot(k == b);
+            if ((threadIdx.x & 63) == 0 && m != 0) atomicAdd(&h[b], (int)__popcll(m));
+        }
     }
     __syncthreads();
     if (threadIdx.x < kBuckets && h[threadIdx.x] != 0) atomicAdd(&count[threadIdx.x], h[threadIdx.x]);
@@ -934,7 +939,8 @@ void solve(SolveArgs sa, int epi, int precision, DBuf<uint8_t> &status, hipStrea
     DBuf<uint8_t> keys(sa.T), keys_sorted(sa.T);
     DBuf<int> counts(2 * kBuckets), iota(sa.T), order(sa.T);
     launch_fill_u32(reinterpret_cast<uint32_t *>(counts.p), 0u, 2 * kBuckets, s);
-    hipLaunchKernelGGL(k_bucket_keys, dim3((sa.T + 255) / 256), dim3(256), 0, s, sa.cnt, sa.T, keys.p, counts.p);
+    hipLaunchKernelGGL(k_bucket_keys, dim3(std::min(kBucketGrid, (sa.T + 255) / 256)), dim3(256), 0, s, sa.cnt, sa.T,
+                       keys.p, counts.p);
     hipLaunchKernelGGL(k_iota, dim3((sa.T + 255) / 256), dim3(256), 0, s, iota.p, sa.T);
     hip_check(hipGetLastError(), "bucket keys");
     {
