@@ -10,6 +10,9 @@ take every such decision as the oracle does, in both precision modes:
   near-duplicate twins, Gaussian covariance of range 25 with nugget 1e-4,
   max_neighbours 64) on a 100x100x50 sub-grid: neighbour lists and masks
   bit-exact, FP64 values bit-exact, mixed values within 1e-5;
+* indicator kriging with Gaussian models at 24 neighbours, whose systems
+  take every path of the mixed kernel (one refinement step, two, FP64,
+  singular): categories equal to the oracle's;
 * Gaussian jobs whose FP64 factorisation fails for part of the nodes (SK at
   24 and 32 neighbours, which the mixed FP32 kernels handle first, and OK
   at 64): the singular counts equal the oracle's, mixed mode must send those
@@ -20,7 +23,7 @@ import numpy as np
 import pytest
 
 import oracle_binding as O
-from cases import Cov, Grid, rel_err, scattered, uneven_twins
+from cases import Cov, Grid, categories, rel_err, scattered, uneven_twins
 
 pytestmark = pytest.mark.gpu
 
@@ -96,6 +99,32 @@ def test_precision_stress_subgrid(geo, stress_case, precision):
         # systems of 33..64 unknowns are factorised in FP64 in both modes
         assert st["solves_fp64"] > 0
         assert rel_err(out.data[ok], c["od"][ok], 1.0) <= TOL
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_indicator_kriging_gaussian_refinement_paths(geo, precision):
+    """Indicator kriging with Gaussian models at 24 neighbours: in mixed mode
+    the order-24 systems take every path of the mixed kernel (accepted after
+    one refinement step by the certificate, after two steps, handed to FP64,
+    and singular in FP64), and the categories still equal the oracle's."""
+    g = Grid(48, 48, 16)
+    data, mask = scattered(g.shape, 0.04, 77, law="normal")
+    cat, marg = categories(data, mask, quantiles=(0.3, 0.7), k=3)
+    params = [{"cov_model": Cov(2, (10, 10, 4), nugget=0.05), "radiuses": (10, 10, 4), "max_neighbours": 24}
+              for _ in range(3)]
+    oc, om = O.indicator_kriging((cat, mask, 3), g, [dict(p) for p in params], marg)
+    geo.set_precision(precision)
+    out = geo.indicator_kriging((cat, mask, 3), g, [dict(p) for p in params], marg)
+    st = geo.last_stats()
+    assert np.array_equal(out.mask, om)
+    flips = int(np.sum(out.data != oc))
+    if precision == "fp64":
+        assert flips == 0, "%d categories differ in FP64 mode" % flips
+    else:
+        one_step = st["solves_fp32"] - st["solves_fp32_two_steps"]
+        assert one_step > 0 and st["solves_fp32_two_steps"] > 0, "both refinement lengths are exercised"
+        assert st["solves_fp64"] > 0 and st["nodes_singular"] > 0, "FP64 hand-over and singular systems occur"
+        assert flips <= 2, "%d of %d categories differ in mixed mode" % (flips, out.data.size)
 
 
 SINGULAR_CASES = [
