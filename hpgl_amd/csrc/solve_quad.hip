@@ -150,13 +150,21 @@ __device__ inline double cov_at(const DevCov &c, int xi, int yi, int zi, int xj,
 
 constexpr int off(int m) { return 2 * m * (m + 1); }  // start of my m-th row (row length 4m+4)
 
+// Per-wavefront LDS of the quad kernels.  The epilogue's terms reuse the
+// refinement's solution vectors (dead by then).
 template <int N>
 struct QShared {
     int xyz[16][3][N];
-    double x1[16][N];
-    double x2[16][N];
-    float tf[16][N];
-    double td[16][N];
+    union {
+        double x1[16][N];
+        float tf[16][N];
+    };
+    union {
+        double x2[16][N];
+        double td[16][N];
+    };
+    double b[16][N];    // right-hand side (mixed kernel)
+    float fro2[16][4];  // the quad's sum of squares (mixed kernel)
 };
 
 // Common set-up of one quad: target, neighbour count, coordinates in LDS.
@@ -165,7 +173,6 @@ struct QCtx {
     int t, n, node, q, slot;
     bool valid;
     int cx, cy, cz;
-    int nb[N / 4];
 };
 
 // FOLD: y and z are stored scaled by the octant's strides, tr[0] + 2 and
@@ -195,7 +202,6 @@ __device__ inline void q_setup(const SolveArgs &a, QShared<N> &sh, QCtx<N> &c) {
             y = cell % (a.X * a.Y) / a.X * sy;
             z = cell / (a.X * a.Y) * sz;
         }
-        c.nb[m] = cell;
         sh.xyz[c.slot][0][i] = x;
         sh.xyz[c.slot][1][i] = y;
         sh.xyz[c.slot][2][i] = z;
@@ -255,7 +261,7 @@ __device__ inline void q_epilogue(const SolveArgs &a, QShared<N> *shp, const QCt
         tf[m] = 0.0f;
         td[m] = 0.0;
         if (ok && i < c.n) {
-            const int nb = c.nb[m];
+            const int nb = a.nbr[(size_t)c.t * a.max_n + i];
             const float mi = a.mean_kind == 2 ? a.mean_cells[nb] : (a.mean_kind == 1 ? a.mean_single : 0.0f);
             float v;
             if (EPI == EPI_CONT)
@@ -411,15 +417,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kQuadMixedWa
     q_setup<N, FOLD>(a, sh, c);
     const int q = c.q;
 
-    // build L <- A (FP32), b (FP64)
+    // build L <- A (FP32), b (FP64, kept in LDS: read back for the first
+    // solve, each residual and the SGS variance, not held in registers
+    // through the factorisation)
     float L[off(M)];
-    double b[M];
     double maxdiag = 0.0;
     float low2 = 0.0f;  // sum of squares of my rows' entries j <= i
 #pragma unroll
     for (int m = 0; m < M; ++m) {
         const int i = 4 * m + q;
-        b[m] = q_bi<N, FOLD>(a, sh, c, i);
+        sh.b[c.slot][i] = q_bi<N, FOLD>(a, sh, c, i);
 #pragma unroll
         for (int j = 0; j < 4 * m + 4; ++j) {
             // lower triangle of the padded system, selected in FP32: A(i, j)
@@ -435,8 +442,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kQuadMixedWa
     }
     maxdiag = qmax(maxdiag);
     // the quad's sum, parked in LDS until the first refinement step
-    // (sh.tf is free until the epilogue)
-    sh.tf[c.slot][q] = (float)qsum((double)low2);
+    sh.fro2[c.slot][q] = (float)qsum((double)low2);
 
     // right-looking Cholesky in place (hardware square root and reciprocal,
     // within an ulp: the refinement works from this factor in FP64)
@@ -473,7 +479,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kQuadMixedWa
 #pragma unroll
     for (int m = 0; m < M; ++m) {
         const int i = 4 * m + q;
-        v1[m] = (float)b[m];
+        v1[m] = (float)sh.b[c.slot][i];
         v2[m] = (two && i < c.n) ? 1.0f : 0.0f;
     }
     q_solve32<N>(L, Dinv, q, v1);
@@ -503,7 +509,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kQuadMixedWa
         double r1[M], r2[M];
 #pragma unroll
         for (int m = 0; m < M; ++m) {
-            r1[m] = b[m];
+            r1[m] = sh.b[c.slot][4 * m + q];
             r2[m] = (TWO && 4 * m + q < c.n) ? 1.0 : 0.0;
         }
         // A is symmetric: lane q reads the 4x4 blocks (m, mb), mb <= m, of
@@ -577,7 +583,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kQuadMixedWa
                 // contraction bound of a refinement step, n u ||A||_F
                 // ||A^-1||_2, with ||A||_F^2 <= 2 (sum of squares over the
                 // lower triangle) less the padding rows' unit diagonal
-                const double fro = sqrt(fmax(0.0, 2.0 * ((double)sh.tf[c.slot][q] - (double)(N - c.n))));
+                const double fro = sqrt(fmax(0.0, 2.0 * ((double)sh.fro2[c.slot][q] - (double)(N - c.n))));
                 const double rho = (double)c.n * 0x1p-24 * fro * q_inv_norm_est<N>(L, Dinv, q, c.n);
                 conv = conv || (rho <= 0.5 && 2.0 * rho * rel <= kOneStepTol);
             }
@@ -614,7 +620,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kQuadMixedWa
 #pragma unroll
         for (int m = 0; m < M; ++m) {
             const int i = 4 * m + q;
-            if (i < c.n) sv += w[m] * b[m];
+            if (i < c.n) sv += w[m] * sh.b[c.slot][i];
             if (i == c.n - 1) wl = w[m];
         }
         variance = a.cov.c0 - qsum(sv);
@@ -650,12 +656,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(N >= 16 ? 2 
     q_setup<N, FOLD>(a, sh, c);
     const int q = c.q;
 
-    double L[off(M)];
-    double b[M];
+    double L[off(M)];  // b is kept in LDS through the factorisation
 #pragma unroll
     for (int m = 0; m < M; ++m) {
         const int i = 4 * m + q;
-        b[m] = q_bi<N, FOLD>(a, sh, c, i);
+        sh.b[c.slot][i] = q_bi<N, FOLD>(a, sh, c, i);
 #pragma unroll
         for (int j = 0; j < 4 * m + 4; ++j) {
             const double e = q_aij<N, FOLD>(a.cov, sh, c.slot, c.n, i, j);
@@ -705,7 +710,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(N >= 16 ? 2 
 #pragma unroll
     for (int m = 0; m < M; ++m) {
         const int i = 4 * m + q;
-        y1[m] = b[m];
+        y1[m] = sh.b[c.slot][i];
         y2[m] = (two && i < c.n) ? 1.0 : 0.0;
     }
     // dtrsm('L','U','T','N'): temp = b_i; temp -= U(k,i) b_k for k < i; temp / U(i,i)
@@ -777,7 +782,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(N >= 16 ? 2 
         double wl = 0.0;
 #pragma unroll
         for (int i = 0; i < N; ++i) {
-            const double wi = qb(w[i >> 2], i & 3), bi = qb(b[i >> 2], i & 3);
+            const double wi = qb(w[i >> 2], i & 3), bi = sh.b[c.slot][i];
             if (i < c.n) v -= wi * bi;
             if (i == c.n - 1) wl = wi;
         }
