@@ -151,7 +151,10 @@ __device__ inline double cov_at(const DevCov &c, int xi, int yi, int zi, int xj,
 constexpr int off(int m) { return 2 * m * (m + 1); }  // start of my m-th row (row length 4m+4)
 
 // Per-wavefront LDS of the quad kernels.  The epilogue's terms reuse the
-// refinement's solution vectors (dead by then).
+// refinement's solution vector and the right-hand side, both dead by then
+// (the SGS variance reads b before the epilogue, which writes no terms for
+// SGS).  The second right-hand side's solution of ordinary kriging has its
+// own array in the mixed kernel.
 template <int N>
 struct QShared {
     int xyz[16][3][N];
@@ -160,10 +163,9 @@ struct QShared {
         float tf[16][N];
     };
     union {
-        double x2[16][N];
+        double b[16][N];  // right-hand side
         double td[16][N];
     };
-    double b[16][N];    // right-hand side (mixed kernel)
     float fro2[16][4];  // the quad's sum of squares (mixed kernel)
 };
 
@@ -403,8 +405,10 @@ constexpr double kOneStepTol = 5.0e-8;
 // and still run faster (order 32, one right-hand side: 20.9 -> 15.2 ms per
 // SGS realisation; two right-hand sides, 490,000 systems: order 20 1.31 ->
 // 0.82 ms, 24 1.82 -> 1.18 ms, 32 3.40 -> 2.79 ms).  The order-24 single
-// right-hand-side kernel needs about 256 and is held to two as well (three
-// spilled and took 123 against 89 ms per headline step).
+// right-hand-side kernel is held to two as well (at three it spills: 332
+// bytes per lane and 123 against 89 ms per headline step with an earlier
+// build, 196 bytes and 61.0 against 52.1 ms with the right-hand side in
+// LDS).
 template <int N, bool TWO>
 constexpr int kQuadMixedWaves = (N >= 24 || (N >= 20 && TWO)) ? 2 : 1;
 
@@ -413,6 +417,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kQuadMixedWa
     const int EPI = a.epi;
     constexpr int M = N / 4;
     __shared__ QShared<N> sh;
+    __shared__ double sh_x2[TWO ? 16 : 1][N];
     QCtx<N> c;
     q_setup<N, FOLD>(a, sh, c);
     const int q = c.q;
@@ -503,7 +508,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kQuadMixedWa
 #pragma unroll
         for (int m = 0; m < M; ++m) {
             sh.x1[c.slot][4 * m + q] = x1[m];
-            if (TWO) sh.x2[c.slot][4 * m + q] = x2[m];
+            if (TWO) sh_x2[c.slot][4 * m + q] = x2[m];
         }
         __syncthreads();
         double r1[M], r2[M];
@@ -525,7 +530,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kQuadMixedWa
 #pragma unroll
             for (int cc = 0; cc < 4; ++cc) {
                 xc1[cc] = sh.x1[c.slot][4 * mb + cc];
-                xc2[cc] = TWO ? sh.x2[c.slot][4 * mb + cc] : 0.0;
+                xc2[cc] = TWO ? sh_x2[c.slot][4 * mb + cc] : 0.0;
             }
             double s1 = 0.0, s2 = 0.0;  // terms of row 4mb+q from the blocks below
 #pragma unroll
