@@ -1552,7 +1552,9 @@ void run_sgs(const SgsJob &job, hipStream_t s, int precision, const Dump *dump, 
     }
 
     std::shared_ptr<CovAssets> as = get_assets(job.cov, job.radii, true, s, st);
-    const int RC = sim_batch_size(R, 17 + (12 * (size_t)max_n + 32), N);
+    // per target: lists, weights, counts, draws and the solve's bucket sort
+    // and hand-over lists (about 20 bytes)
+    const int RC = sim_batch_size(R, 17 + (12 * (size_t)max_n + 52), N);
 
     DBuf<unsigned long long> counters(kCounterWords);
     hip_check(hipMemsetAsync(counters.p, 0, kCounterWords * sizeof(unsigned long long), s), "memset");
@@ -1712,7 +1714,7 @@ void run_sis(const SisJob &job, hipStream_t s, int precision, Stats &st) {
     SideStream side;
 
     std::vector<std::shared_ptr<CovAssets>> as(kriged);
-    size_t per_cell = 21;
+    size_t per_cell = 21 + 20;  // and the solve's bucket sort and hand-over lists
     for (int c = 0; c < kriged; ++c) {
         as[c] = get_assets(job.cats[c].cov, job.cats[c].radii, true, s, st);
         per_cell += 12 * (size_t)std::max(job.cats[c].max_n, 1) + 32;

@@ -43,12 +43,13 @@ long long cells(const hpgl_shape_t &s) { return (long long)s.m_data[0] * s.m_dat
 long long box(const int r[3]) { return (2ll * r[0] + 1) * (2ll * r[1] + 1) * (2ll * r[2] + 1); }
 
 // Device bytes a job may hold at once: its arrays, neighbour lists and
-// solve state per target, bins, and the covariance / rank tables
+// solve state per target (including the solve's bucket sort and hand-over
+// lists, about 20 bytes), bins, and the covariance / rank tables
 // (cell-binned rank table over twice the radii), with 25% headroom.
 long long footprint(const hpgl_amd_job_t &j) {
     const long long N = cells(j.shape);
     auto search = [&](const int r[3], int max_n, long long T) {
-        return T * (4ll * max_n + 24) + N * 12 + box(r) * 8 * 10;
+        return T * (4ll * max_n + 44) + N * 12 + box(r) * 8 * 10;
     };
     long long b = 0;
     switch (j.kind) {
