@@ -6,6 +6,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
@@ -16,6 +17,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "kernels.h"
@@ -505,6 +507,10 @@ struct CovAssets {
     // kClusterRunSlack zero columns on each side
     int wh[3] = {0, 0, 0};
     DBuf<uint32_t> d_rank;
+    // host copy of the device rank table, built ahead by prefetch_assets
+    // (rank_built) and freed once uploaded
+    bool rank_built = false;
+    std::vector<uint32_t> rank_wide;
     bool have_pair = false;
     BoxTable pair;
     DBuf<double> d_pair;
@@ -531,8 +537,7 @@ AssetCache &asset_cache() {
 // Assets of (cov, radii, use_table) on the current device; builds them on a
 // miss (host time counted in ms_setup_host) and waits for the upload before
 // other streams may use them.
-std::shared_ptr<CovAssets> get_assets(const CovSpec &cov, const int radii[3], bool use_table, hipStream_t s,
-                                      Stats &st) {
+std::string asset_key(const CovSpec &cov, const int radii[3], bool use_table) {
     int dev = 0;
     hip_check(hipGetDevice(&dev), "hipGetDevice");
     std::string key(reinterpret_cast<const char *>(&dev), sizeof(dev));
@@ -543,43 +548,114 @@ std::shared_ptr<CovAssets> get_assets(const CovSpec &cov, const int radii[3], bo
     key.append(reinterpret_cast<const char *>(&cov.nugget), sizeof(cov.nugget));
     key.append(reinterpret_cast<const char *>(radii), 3 * sizeof(int));
     key.push_back(use_table ? 1 : 0);
+    return key;
+}
+
+std::shared_ptr<CovAssets> find_assets(const std::string &key) {
     AssetCache &c = asset_cache();
-    {
-        std::lock_guard<std::mutex> g(c.mu);
-        for (auto it = c.lru.begin(); it != c.lru.end(); ++it)
-            if (it->first == key) {
-                auto e = it->second;
-                c.lru.splice(c.lru.begin(), c.lru, it);
-                return e;
-            }
+    std::lock_guard<std::mutex> g(c.mu);
+    for (auto it = c.lru.begin(); it != c.lru.end(); ++it)
+        if (it->first == key) {
+            auto e = it->second;
+            c.lru.splice(c.lru.begin(), c.lru, it);
+            return e;
+        }
+    return nullptr;
+}
+
+void insert_assets(const std::string &key, const std::shared_ptr<CovAssets> &e) {
+    AssetCache &c = asset_cache();
+    std::lock_guard<std::mutex> g(c.mu);
+    c.lru.emplace_front(key, e);
+    while (c.lru.size() > kAssetCacheEntries) c.lru.pop_back();
+}
+
+// Host part of the assets (no HIP calls): box table, template, octant.
+void build_assets_host(CovAssets &e) {
+    if (e.use_table) e.table.build(e.model, e.radii);
+    build_template(e.model, e.use_table ? &e.table : nullptr, e.radii, e.tmpl);
+    if (!(e.use_table && e.table.fold(e.fold))) e.fold.clear();
+}
+
+void upload_assets(CovAssets &e, hipStream_t s) {
+    if (e.use_table) {
+        upload(e.d_table, e.table.v, s);
+        if (!e.fold.empty()) upload(e.d_fold, e.fold, s);
     }
+    upload(e.d_tmpl, e.tmpl.xyz, s);
+    hip_check(hipStreamSynchronize(s), "sync");
+}
+
+std::shared_ptr<CovAssets> get_assets(const CovSpec &cov, const int radii[3], bool use_table, hipStream_t s,
+                                      Stats &st) {
+    const std::string key = asset_key(cov, radii, use_table);
+    if (auto e = find_assets(key)) return e;
     auto e = std::make_shared<CovAssets>(cov);
     e->use_table = use_table;
     for (int i = 0; i < 3; ++i) e->radii[i] = radii[i];
     const double t0 = now_ms();
-    if (use_table) e->table.build(e->model, radii);
-    build_template(e->model, use_table ? &e->table : nullptr, radii, e->tmpl);
-    const bool folds = use_table && e->table.fold(e->fold);
+    build_assets_host(*e);
     st.ms_setup_host += now_ms() - t0;
-    if (use_table) {
-        upload(e->d_table, e->table.v, s);
-        if (folds) upload(e->d_fold, e->fold, s);
-    }
-    upload(e->d_tmpl, e->tmpl.xyz, s);
-    hip_check(hipStreamSynchronize(s), "sync");
-    std::lock_guard<std::mutex> g(c.mu);
-    c.lru.emplace_front(key, e);
-    while (c.lru.size() > kAssetCacheEntries) c.lru.pop_back();
+    upload_assets(*e, s);
+    insert_assets(key, e);
     return e;
 }
 
-// Rank table of the cluster path over [-span, span]^3: span = r with a
-// table, 2r - 1 with the analytic model (candidates lie in the 27 bins).
-// The device copy has kClusterRunSlack zero columns on either side in x, so
-// a run of neighbouring targets reads its members' entries in one load.
-const CovAssets &assets_rank(CovAssets &a, hipStream_t s, Stats &st) {
-    std::lock_guard<std::mutex> g(a.lazy);
-    if (a.have_rank) return a;
+bool build_rank_host(CovAssets &a);
+
+// Builds the host parts of the assets of several covariance models at once,
+// one thread per model not yet cached (the box tables, the template sorts
+// and, for tables, the cell-binned search's rank tables are independent),
+// then uploads them and enters them in the cache, so the jobs' calls of
+// get_assets find them.  A rank table that cannot be built here is left to
+// assets_rank, which reports the error if the search needs it.
+void prefetch_assets(const std::vector<std::pair<CovSpec, std::array<int, 3>>> &specs, bool use_table,
+                     hipStream_t s, Stats &st) {
+    std::vector<std::string> keys;
+    std::vector<std::shared_ptr<CovAssets>> todo;
+    for (const auto &sp : specs) {
+        const std::string key = asset_key(sp.first, sp.second.data(), use_table);
+        if (std::find(keys.begin(), keys.end(), key) != keys.end() || find_assets(key)) continue;
+        auto e = std::make_shared<CovAssets>(sp.first);
+        e->use_table = use_table;
+        for (int i = 0; i < 3; ++i) e->radii[i] = sp.second[i];
+        keys.push_back(key);
+        todo.push_back(e);
+    }
+    if (todo.size() < 2) return;  // get_assets builds a single model as fast
+    const double t0 = now_ms();
+    std::vector<std::string> errors(todo.size());
+    {
+        std::vector<std::thread> th;
+        for (size_t k = 0; k < todo.size(); ++k)
+            th.emplace_back([&, k] {
+                try {
+                    build_assets_host(*todo[k]);
+                } catch (const std::exception &ex) {
+                    errors[k] = ex.what();
+                    return;
+                }
+                try {
+                    if (use_table) build_rank_host(*todo[k]);
+                } catch (const std::exception &) {
+                    todo[k]->rank_built = false;
+                }
+            });
+        for (auto &t : th) t.join();
+    }
+    st.ms_setup_host += now_ms() - t0;
+    for (const auto &m : errors)
+        if (!m.empty()) throw std::runtime_error(m);
+    for (size_t k = 0; k < todo.size(); ++k) {
+        upload_assets(*todo[k], s);
+        insert_assets(keys[k], todo[k]);
+    }
+}
+
+// Host part of the rank table: the ranks, their packing shift and the
+// device layout in a.rank_wide (no HIP calls).  Throws when the radii are
+// too large for the cell-binned search.
+bool build_rank_host(CovAssets &a) {
     int span[3];
     for (int i = 0; i < 3; ++i) span[i] = std::max(a.use_table ? a.radii[i] : 2 * a.radii[i] - 1, 0);
     // the search's box test takes offsets modulo 2^16
@@ -593,7 +669,6 @@ const CovAssets &assets_rank(CovAssets &a, hipStream_t s, Stats &st) {
     // the search kernel indexes the table with 32-bit byte offsets
     if ((long long)wd[0] * wd[1] * wd[2] + 2ll * kClusterRankPad >= (1ll << 30))
         throw std::runtime_error("search radiuses too large for the cell-binned search");
-    const double t0 = now_ms();
     a.rank.build(a.model, a.use_table ? &a.table : nullptr, span);
     a.rank_positive = 0;
     a.max_rank = 0;
@@ -608,18 +683,34 @@ const CovAssets &assets_rank(CovAssets &a, hipStream_t s, Stats &st) {
     while ((1ll << lb) < max_raw) ++lb;
     a.rank_lb = (lb < 32 && (unsigned long long)a.max_rank < (1ull << (32 - lb))) ? lb : -1;
     const int shift = std::max(a.rank_lb, 0);
-    std::vector<uint32_t> wide((size_t)wd[0] * wd[1] * wd[2] + 2 * kClusterRankPad, 0u);
+    a.rank_wide.assign((size_t)wd[0] * wd[1] * wd[2] + 2 * kClusterRankPad, 0u);
     const RankTable &rt = a.rank;
     for (int z = 0; z < rt.d[2]; ++z)
         for (int y = 0; y < rt.d[1]; ++y) {
             const size_t src = ((size_t)z * rt.d[1] + y) * rt.d[0];
             const size_t dst = kClusterRankPad + ((size_t)(z - span[2] + a.wh[2]) * wd[1] + (y - span[1] + a.wh[1])) *
                                                      wd[0] + (size_t)(a.wh[0] - span[0]);
-            for (int x = 0; x < rt.d[0]; ++x) wide[dst + x] = rt.rank[src + x] << shift;
+            for (int x = 0; x < rt.d[0]; ++x) a.rank_wide[dst + x] = rt.rank[src + x] << shift;
         }
-    st.ms_setup_host += now_ms() - t0;
-    upload(a.d_rank, wide, s);
+    a.rank_built = true;
+    return true;
+}
+
+// Rank table of the cluster path over [-span, span]^3: span = r with a
+// table, 2r - 1 with the analytic model (candidates lie in the 27 bins).
+// The device copy has kClusterRunSlack zero columns on either side in x, so
+// a run of neighbouring targets reads its members' entries in one load.
+const CovAssets &assets_rank(CovAssets &a, hipStream_t s, Stats &st) {
+    std::lock_guard<std::mutex> g(a.lazy);
+    if (a.have_rank) return a;
+    if (!a.rank_built) {
+        const double t0 = now_ms();
+        build_rank_host(a);
+        st.ms_setup_host += now_ms() - t0;
+    }
+    upload(a.d_rank, a.rank_wide, s);
     hip_check(hipStreamSynchronize(s), "sync");
+    std::vector<uint32_t>().swap(a.rank_wide);
     a.have_rank = true;
     return a;
 }
@@ -1169,6 +1260,12 @@ void run_ind(const IndJob &job, hipStream_t s, int precision, const Dump *dump, 
     DBuf<uint8_t> status(T);
     int maxn_all = 1;
     for (const auto &c : job.cats) maxn_all = std::max(maxn_all, c.max_n);
+    {
+        // the categories' covariance assets, built on the host side by side
+        std::vector<std::pair<CovSpec, std::array<int, 3>>> specs;
+        for (const auto &c : job.cats) specs.push_back({c.cov, {c.radii[0], c.radii[1], c.radii[2]}});
+        prefetch_assets(specs, true, s, st);
+    }
     DBuf<int> nbr((size_t)T * maxn_all), cnt(T);
     for (int k = 0; k < K; ++k) {
         const IndicatorParams &c = job.cats[k];
@@ -1714,6 +1811,12 @@ void run_sis(const SisJob &job, hipStream_t s, int precision, Stats &st) {
     SideStream side;
 
     std::vector<std::shared_ptr<CovAssets>> as(kriged);
+    {
+        std::vector<std::pair<CovSpec, std::array<int, 3>>> specs;
+        for (int c = 0; c < kriged; ++c)
+            specs.push_back({job.cats[c].cov, {job.cats[c].radii[0], job.cats[c].radii[1], job.cats[c].radii[2]}});
+        prefetch_assets(specs, true, s, st);
+    }
     size_t per_cell = 21 + 20;  // and the solve's bucket sort and hand-over lists
     for (int c = 0; c < kriged; ++c) {
         as[c] = get_assets(job.cats[c].cov, job.cats[c].radii, true, s, st);
