@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
@@ -17,6 +18,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -626,21 +628,32 @@ void prefetch_assets(const std::vector<std::pair<CovSpec, std::array<int, 3>>> &
     const double t0 = now_ms();
     std::vector<std::string> errors(todo.size());
     {
-        std::vector<std::thread> th;
-        for (size_t k = 0; k < todo.size(); ++k)
-            th.emplace_back([&, k] {
+        // at most 16 threads, each taking the next model
+        std::atomic<size_t> next{0};
+        auto work = [&] {
+            for (size_t k = next++; k < todo.size(); k = next++) {
                 try {
                     build_assets_host(*todo[k]);
                 } catch (const std::exception &ex) {
                     errors[k] = ex.what();
-                    return;
+                    continue;
                 }
                 try {
                     if (use_table) build_rank_host(*todo[k]);
                 } catch (const std::exception &) {
                     todo[k]->rank_built = false;
                 }
-            });
+            }
+        };
+        // the calling thread works too, so a thread that cannot be started
+        // only costs parallelism
+        std::vector<std::thread> th;
+        const size_t n_threads = std::min<size_t>(todo.size(), 16) - 1;
+        try {
+            for (size_t t = 0; t < n_threads; ++t) th.emplace_back(work);
+        } catch (const std::system_error &) {
+        }
+        work();
         for (auto &t : th) t.join();
     }
     st.ms_setup_host += now_ms() - t0;
