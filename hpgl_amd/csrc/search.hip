@@ -36,9 +36,18 @@ struct ScanTail {
 };
 // Lanes still searching at which a wavefront hands them to the tail pass.
 constexpr int kTailLanes = 8;
+// Largest max_n whose finds the first pass stages in LDS (256 x max_n
+// words per workgroup) and writes out as whole rows at its end; above it
+// each find is stored as it is taken.
+constexpr int kScanStageMax = 32;
 
-template <int PRED>
+// STAGE: find c of thread i is kept at s_found[c * 256 + i] (consecutive
+// lanes, consecutive banks) until the wavefront leaves the walk; then the
+// wavefront writes its 64 lists in row-major order, 64 consecutive list
+// words per store, instead of one scattered word per lane and find.
+template <int PRED, bool STAGE>
 __global__ __launch_bounds__(256) void k_template_scan(ScanArgs a, ScanTail tl) {
+    extern __shared__ int s_found[];
     const int li = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = li < a.n_list;
     const int t = active ? (a.list ? a.list[li] : li) : 0;
@@ -91,9 +100,33 @@ __global__ __launch_bounds__(256) void k_template_scan(ScanArgs a, ScanTail tl) 
                 if (want && probe[u] && inf && v[u] >= 0) sim |= 1ull << count;
             }
             if (want && probe[u] && inf) {
-                out[count++] = idx[u];
+                if (STAGE)
+                    s_found[count * 256 + threadIdx.x] = idx[u];
+                else
+                    out[count] = idx[u];
+                ++count;
                 want = count < max_n;
             }
+        }
+    }
+    if (STAGE) {
+        // every exit of the walk above is wave-uniform, so the whole
+        // wavefront is here; it reads back only its own lanes' finds
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        const int wbase = threadIdx.x & ~63;
+        const float inv = 1.0f / (float)max_n;
+        for (int e = lane; e < 64 * max_n; e += 64) {
+            // row r = e / max_n (e < 4096 is exact in float; one step corrects the rounding)
+            int r = (int)((float)e * inv);
+            if (r * max_n > e)
+                --r;
+            else if ((r + 1) * max_n <= e)
+                ++r;
+            const int col = e - r * max_n;
+            const int tr = __shfl(t, r, 64);
+            const int cr = __shfl(count, r, 64);
+            if (col < cr) a.nbr[(size_t)tr * max_n + col] = s_found[col * 256 + wbase + r];
         }
     }
     if (active) a.cnt[t] = count;
@@ -240,11 +273,20 @@ void launch_template_scan(const ScanArgs &a, int pred_kind, hipStream_t s) {
     tl.n_cont = (unsigned *)pool_alloc(sizeof(unsigned));
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, (uint32_t *)tl.n_cont, 0u, 1ll);
     const unsigned tail_blocks = std::min<unsigned>(blocks(a.n_list, 256), 512u);
+    const bool stage = a.max_n >= 1 && a.max_n <= kScanStageMax;
+    const size_t lds = stage ? (size_t)256 * a.max_n * sizeof(int) : 0;
+    const dim3 grid(blocks(a.n_list, 256));
     if (pred_kind == 0) {
-        hipLaunchKernelGGL(k_template_scan<0>, dim3(blocks(a.n_list, 256)), dim3(256), 0, s, a, tl);
+        if (stage)
+            hipLaunchKernelGGL((k_template_scan<0, true>), grid, dim3(256), lds, s, a, tl);
+        else
+            hipLaunchKernelGGL((k_template_scan<0, false>), grid, dim3(256), 0, s, a, tl);
         hipLaunchKernelGGL(k_template_scan_tail<0>, dim3(tail_blocks), dim3(256), 0, s, a, tl);
     } else {
-        hipLaunchKernelGGL(k_template_scan<1>, dim3(blocks(a.n_list, 256)), dim3(256), 0, s, a, tl);
+        if (stage)
+            hipLaunchKernelGGL((k_template_scan<1, true>), grid, dim3(256), lds, s, a, tl);
+        else
+            hipLaunchKernelGGL((k_template_scan<1, false>), grid, dim3(256), 0, s, a, tl);
         hipLaunchKernelGGL(k_template_scan_tail<1>, dim3(tail_blocks), dim3(256), 0, s, a, tl);
     }
     pool_free(tl.cont);
